@@ -1,0 +1,137 @@
+"""ctypes binding of include/artia.h (libartia.so, built for gfx950).
+
+The library is loaded from this package directory.  A missing library is an error: there is
+no CPU or PyTorch fallback for any operator.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libartia.so"
+
+ARTIA_OK = 0
+ARTIA_NUMERICS_EXACT = 0
+ARTIA_NUMERICS_REFERENCE = 1
+ARTIA_FILTER_POINT = 0
+ARTIA_FILTER_LINEAR = 1
+PROF_KERNELS = {"fp": 0, "bp": 1, "compare": 2, "crop": 3, "fill": 4}
+
+_F3 = C.c_float * 3
+
+
+class TiltGeom(C.Structure):
+    """artia_tilt_geom: c_detektor, c_uPitch, c_vPitch, c_projNorm, c_DetectorMatrix,
+    c_magAniso, c_magAnisoInv (DeviceVariables.cuh:13-34)."""
+
+    _fields_ = [
+        ("d", _F3), ("u", _F3), ("v", _F3), ("n", _F3),
+        ("det_matrix", C.c_float * 16),
+        ("mag_aniso", C.c_float * 9), ("mag_aniso_inv", C.c_float * 9),
+    ]
+
+
+class VolGeom(C.Structure):
+    """artia_vol_geom: c_bBoxMin/Max, c_volumeBBoxRcp, c_volumeDim, c_voxelSize, zShift."""
+
+    _fields_ = [
+        ("bbox_min", _F3), ("bbox_max", _F3), ("bbox_rcp", _F3), ("dim", _F3),
+        ("voxel_size", _F3), ("z_shift", C.c_float),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+    ]
+
+
+class TiltParams(C.Structure):
+    """artia_tilt_params: MarkerFile items of one projection."""
+
+    _fields_ = [("tilt_deg", C.c_float), ("psi_deg", C.c_float), ("mag", C.c_float),
+                ("shift_x", C.c_float), ("shift_y", C.c_float)]
+
+
+class GlobalParams(C.Structure):
+    """artia_global_params: AddTiltAngle, AddTiltXAngle, PhiAngle, UseFixPsiAngle, PsiAngle."""
+
+    _fields_ = [("add_tilt_deg", C.c_float), ("add_tilt_x_deg", C.c_float), ("phi_deg", C.c_float),
+                ("use_fix_psi", C.c_int), ("fix_psi_deg", C.c_float)]
+
+
+class SartParams(C.Structure):
+    """artia_sart_params: SIRTCount, Lambda, OverSampling, Crop, CropDim, CutLength,
+    DimLength, ProjectionScaleFactor."""
+
+    _fields_ = [("sirt_count", C.c_int), ("lambda_", C.c_float), ("oversampling", C.c_int),
+                ("crop", C.c_float * 4), ("crop_dim", C.c_float * 4),
+                ("cut_length", C.c_float * 2), ("dim_length", C.c_float * 2),
+                ("proj_scale", C.c_float)]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+_SZ = C.c_size_t
+_PF = C.POINTER(C.c_float)
+_PI = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "artia_create": (_I, [_I, C.POINTER(_P)]),
+    "artia_destroy": (_I, [_P]),
+    "artia_set_stream": (_I, [_P, _P]),
+    "artia_get_stream": (_I, [_P, C.POINTER(_P)]),
+    "artia_set_numerics": (_I, [_P, _I]),
+    "artia_sync": (_I, [_P]),
+    "artia_last_error": (C.c_char_p, [_P]),
+    "artia_version": (C.c_char_p, []),
+    "artia_geom_tilt": (_I, [C.POINTER(TiltParams), C.POINTER(GlobalParams), _I, _I, _F, _F,
+                             C.POINTER(TiltGeom)]),
+    "artia_geom_volume": (_I, [_I, _I, _I, _PF, _PF, C.POINTER(VolGeom)]),
+    "artia_geom_subvolume": (_I, [_I, _F, _PF, C.POINTER(VolGeom), _PF, _PF, C.POINTER(VolGeom)]),
+    "artia_geom_hitpoints": (_I, [C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI]),
+    "artia_geom_real_length": (_I, [C.POINTER(VolGeom), C.POINTER(TiltGeom), _PF]),
+    "artia_projection_index_list": (_I, [_PF, _I, _PI]),
+    "artia_fp_march": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _SZ, _P, _P, _PI, _PF]),
+    "artia_fp_slicer": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _SZ, _P, _F, _F, _PI, _PF]),
+    "artia_vol_traversal": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _I, _I, _SZ, _P, _PI, _PF]),
+    "artia_compare": (_I, [_P, _I, _I, _SZ, _P, _P, _P, _F, _PF, _PF, _F, _PF]),
+    "artia_crop_border": (_I, [_P, _I, _I, _SZ, _P, _PF, _PF, _PI, _PF]),
+    "artia_bp": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _P, _I, _I, _SZ, _F, _I, _F, _F, _I, _PF]),
+    "artia_make_square": (_I, [_P, _I, _I, _SZ, _P, _I, _P, _I, _I, _I, _I, _PF]),
+    "artia_profile_enable": (_I, [_P, _I]),
+    "artia_profile_read": (_I, [_P, _I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "artia_sart_run": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _I, _PI, _P, _I, _I, _P,
+                            C.POINTER(SartParams), _I, _PI, _PF]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads libartia.so and declares every entry point; raises if the library is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m artiatomi_amd.build` "
+            "(there is no fallback implementation)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def farr(values, n=None):
+    vals = list(values)
+    if n is not None and len(vals) != n:
+        raise ValueError(f"expected {n} values, got {len(vals)}")
+    return (C.c_float * len(vals))(*vals)
+
+
+def iarr(values, n=None):
+    vals = list(values)
+    if n is not None and len(vals) != n:
+        raise ValueError(f"expected {n} values, got {len(vals)}")
+    return (C.c_int * len(vals))(*vals)

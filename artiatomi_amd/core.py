@@ -1,0 +1,260 @@
+"""Host-side mirror of the EmSART reconstruction-core operator interface.
+
+`ArtiaContext` exposes one method per reference functor of src/EmSART/Kernels.h:37-245
+(FPKernel -> fp_march, SlicerKernel -> fp_slicer, VolTravLengthKernel -> vol_traversal,
+CompKernel -> compare, CropBorderKernel -> crop_border, BPKernel -> bp,
+CopyToSquareKernel -> make_square) over the C ABI in include/artia.h, taking torch CUDA
+tensors as device memory.  Like the reference functors, each call returns the elapsed
+milliseconds when `timed=True` (the call then synchronises); otherwise it returns None and
+does not block.  Errors raise `ArtiaError` carrying the library's message (the reference
+throws CudaException, CudaException.h:34-36).
+
+Geometry helpers mirror Projection / Volume / Reconstructor host code through the ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+from . import _abi
+from ._abi import GlobalParams, SartParams, TiltGeom, TiltParams, VolGeom
+
+
+class ArtiaError(RuntimeError):
+    pass
+
+
+def _check(status: int, ctx=None, what: str = "") -> None:
+    if status != _abi.ARTIA_OK:
+        msg = ""
+        if ctx is not None:
+            msg = _abi.load().artia_last_error(ctx).decode(errors="replace")
+        raise ArtiaError(f"{what} failed with status {status}: {msg}")
+
+
+# ------------------------------------------------------------------ geometry
+
+def tilt_geometry(tilt_deg: float, width: int, height: int, psi_deg: float = 0.0, mag: float = 1.0,
+                  shift: Sequence[float] = (0.0, 0.0), glob: Optional[GlobalParams] = None,
+                  mag_aniso: Sequence[float] = (1.0, 0.0)) -> TiltGeom:
+    """Projection::GetPosition/GetPixelU,VPitch/GetNormalVector/GetDetectorMatrix for one tilt."""
+    lib = _abi.load()
+    tp = TiltParams(tilt_deg, psi_deg, mag, shift[0], shift[1])
+    gp = glob if glob is not None else GlobalParams(0.0, 0.0, 0.0, 0, 0.0)
+    out = TiltGeom()
+    _check(lib.artia_geom_tilt(C.byref(tp), C.byref(gp), int(width), int(height), float(mag_aniso[0]),
+                               float(mag_aniso[1]), C.byref(out)), what="artia_geom_tilt")
+    return out
+
+
+def volume_geometry(nx: int, ny: int, nz: int, voxel_size: Sequence[float] = (1.0, 1.0, 1.0),
+                    shift: Sequence[float] = (0.0, 0.0, 0.0)) -> VolGeom:
+    """Volume::PositionInSpace for the whole volume."""
+    lib = _abi.load()
+    out = VolGeom()
+    _check(lib.artia_geom_volume(nx, ny, nz, _abi.farr(voxel_size, 3), _abi.farr(shift, 3), C.byref(out)),
+           what="artia_geom_volume")
+    return out
+
+
+def subvolume_geometry(n: int, voxel_size_sub: float, voxel_size_full: Sequence[float], full: VolGeom,
+                       position: Sequence[float], shift: Sequence[float] = (0.0, 0.0, 0.0)) -> VolGeom:
+    """Volume::PositionInSpace for a motive-list sub-volume (1-based position)."""
+    lib = _abi.load()
+    out = VolGeom()
+    _check(lib.artia_geom_subvolume(n, voxel_size_sub, _abi.farr(voxel_size_full, 3), C.byref(full),
+                                    _abi.farr(position, 3), _abi.farr(shift, 3), C.byref(out)),
+           what="artia_geom_subvolume")
+    return out
+
+
+def hitpoints(full: VolGeom, tg: TiltGeom) -> list:
+    lib = _abi.load()
+    out = (C.c_int * 8)()
+    _check(lib.artia_geom_hitpoints(C.byref(full), C.byref(tg), out), what="artia_geom_hitpoints")
+    return list(out)
+
+
+def real_length(full: VolGeom, tg: TiltGeom) -> float:
+    lib = _abi.load()
+    out = C.c_float()
+    _check(lib.artia_geom_real_length(C.byref(full), C.byref(tg), C.byref(out)), what="artia_geom_real_length")
+    return out.value
+
+
+def projection_index_list(tilts_deg: Sequence[float]) -> list:
+    lib = _abi.load()
+    n = len(tilts_deg)
+    out = (C.c_int * n)()
+    _check(lib.artia_projection_index_list(_abi.farr(tilts_deg), n, out), what="artia_projection_index_list")
+    return list(out)
+
+
+@dataclass
+class Acquisition:
+    """Per-tilt geometry of a tilt series, as EmSART builds it from the marker file."""
+
+    width: int
+    height: int
+    tilts_deg: list
+    psis_deg: list
+    shifts: list
+    geoms: list = field(default_factory=list)
+    hitpoints: list = field(default_factory=list)
+    real_lengths: list = field(default_factory=list)
+    index_list: list = field(default_factory=list)
+
+    @classmethod
+    def build(cls, full: VolGeom, width: int, height: int, tilts_deg, psis_deg=None, shifts=None,
+              glob: Optional[GlobalParams] = None) -> "Acquisition":
+        n = len(tilts_deg)
+        psis = list(psis_deg) if psis_deg is not None else [0.0] * n
+        sh = [tuple(s) for s in shifts] if shifts is not None else [(0.0, 0.0)] * n
+        acq = cls(width, height, [float(t) for t in tilts_deg], [float(p) for p in psis], sh)
+        for t, p, s in zip(acq.tilts_deg, psis, sh):
+            g = tilt_geometry(t, width, height, psi_deg=p, shift=s, glob=glob)
+            acq.geoms.append(g)
+            acq.hitpoints.append(hitpoints(full, g))
+            acq.real_lengths.append(real_length(full, g))
+        acq.index_list = projection_index_list(acq.tilts_deg)
+        return acq
+
+    def c_arrays(self):
+        n = len(self.geoms)
+        tg = (TiltGeom * n)(*self.geoms)
+        hp = (C.c_int * (8 * n))(*[v for h in self.hitpoints for v in h])
+        rl = (C.c_float * n)(*self.real_lengths)
+        il = (C.c_int * n)(*self.index_list)
+        return tg, hp, rl, il
+
+
+def sart_params(lambda_: float = 0.3, sirt_count: int = 1, oversampling: int = 1, crop=(0, 0, 0, 0),
+                crop_dim=(0, 0, 0, 0), cut_length=(0, 0), dim_length=(0, 0), proj_scale: float = 1.0) -> SartParams:
+    return SartParams(sirt_count, lambda_, oversampling, _abi.farr(crop, 4), _abi.farr(crop_dim, 4),
+                      _abi.farr(cut_length, 2), _abi.farr(dim_length, 2), proj_scale)
+
+
+# ------------------------------------------------------------------ device operators
+
+def _ptr(t, name: str, numel: Optional[int] = None) -> int:
+    import torch
+
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float32 or not t.is_contiguous():
+        raise ArtiaError(f"{name} must be a contiguous float32 CUDA tensor")
+    if numel is not None and t.numel() < numel:
+        raise ArtiaError(f"{name} holds {t.numel()} elements, needs {numel}")
+    return t.data_ptr()
+
+
+class ArtiaContext:
+    """One libartia context on one GPU, bound to torch's current stream of that device."""
+
+    def __init__(self, device: int = 0, numerics: str = "exact"):
+        import torch
+
+        self.lib = _abi.load()
+        self.device = device
+        h = C.c_void_p()
+        _check(self.lib.artia_create(device, C.byref(h)), what="artia_create")
+        self.h = h
+        stream = torch.cuda.current_stream(device)
+        _check(self.lib.artia_set_stream(self.h, C.c_void_p(stream.cuda_stream)), self.h, "artia_set_stream")
+        self.set_numerics(numerics)
+
+    def set_numerics(self, numerics: str) -> None:
+        mode = {"exact": _abi.ARTIA_NUMERICS_EXACT, "reference": _abi.ARTIA_NUMERICS_REFERENCE}[numerics]
+        _check(self.lib.artia_set_numerics(self.h, mode), self.h, "artia_set_numerics")
+        self.numerics = numerics
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.artia_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self) -> None:
+        _check(self.lib.artia_sync(self.h), self.h, "artia_sync")
+
+    def _call(self, name, *args, timed: bool = False):
+        ms = C.c_float() if timed else None
+        st = getattr(self.lib, name)(self.h, *args, C.byref(ms) if timed else None)
+        _check(st, self.h, name)
+        return ms.value if timed else None
+
+    @staticmethod
+    def _roi(roi):
+        return None if roi is None else _abi.iarr(roi, 4)
+
+    def fp_march(self, vg: VolGeom, tg: TiltGeom, vol, proj, dist, roi=None, timed=False):
+        H, W = proj.shape
+        return self._call("artia_fp_march", C.byref(vg), C.byref(tg), _ptr(vol, "vol", vg.nx * vg.ny * vg.nz), W, H,
+                          W, _ptr(proj, "proj"), _ptr(dist, "dist", W * H), self._roi(roi), timed=timed)
+
+    def fp_slicer(self, vg: VolGeom, tg: TiltGeom, vol, proj, tmin, tmax, roi=None, timed=False):
+        H, W = proj.shape
+        return self._call("artia_fp_slicer", C.byref(vg), C.byref(tg), _ptr(vol, "vol", vg.nx * vg.ny * vg.nz), W, H,
+                          W, _ptr(proj, "proj"), float(tmin), float(tmax), self._roi(roi), timed=timed)
+
+    def vol_traversal(self, vg: VolGeom, tg: TiltGeom, dist, roi=None, timed=False):
+        H, W = dist.shape
+        return self._call("artia_vol_traversal", C.byref(vg), C.byref(tg), W, H, W, _ptr(dist, "dist"),
+                          self._roi(roi), timed=timed)
+
+    def compare(self, real, virt, dist, real_length, crop=(0, 0, 0, 0), crop_dim=(0, 0, 0, 0), scale=1.0,
+                timed=False):
+        H, W = virt.shape
+        return self._call("artia_compare", W, H, W, _ptr(real, "real", W * H), _ptr(virt, "virt"),
+                          _ptr(dist, "dist", W * H), float(real_length), _abi.farr(crop, 4),
+                          _abi.farr(crop_dim, 4), float(scale), timed=timed)
+
+    def crop_border(self, img, points, cut_length=(0, 0), dim_length=(0, 0), timed=False):
+        H, W = img.shape
+        return self._call("artia_crop_border", W, H, W, _ptr(img, "img"), _abi.farr(cut_length, 2),
+                          _abi.farr(dim_length, 2), _abi.iarr(points, 8), timed=timed)
+
+    def bp(self, vg: VolGeom, tg: TiltGeom, vol, img, lambda_, oversampling=1, dist_min=0.0,
+           dist_max=9999999999999.0, filter="point", timed=False):
+        H, W = img.shape
+        f = {"point": _abi.ARTIA_FILTER_POINT, "linear": _abi.ARTIA_FILTER_LINEAR}[filter]
+        return self._call("artia_bp", C.byref(vg), C.byref(tg), _ptr(vol, "vol", vg.nx * vg.ny * vg.nz),
+                          _ptr(img, "img"), W, H, W, float(lambda_), int(oversampling), float(dist_min),
+                          float(dist_max), f, timed=timed)
+
+    def make_square(self, img, out, border_x, border_y, mirror_y=False, fill_zero=True, timed=False):
+        H, W = img.shape
+        N = out.shape[0]
+        return self._call("artia_make_square", W, H, W, _ptr(img, "img"), N, _ptr(out, "out", N * N),
+                          int(border_x), int(border_y), int(bool(mirror_y)), int(bool(fill_zero)), timed=timed)
+
+    def profile(self, enable: bool = True) -> None:
+        """Enables (and resets) per-kernel-class event timing of driver launches."""
+        _check(self.lib.artia_profile_enable(self.h, int(enable)), self.h, "artia_profile_enable")
+
+    def profile_read(self) -> dict:
+        """{kernel class: (total device ms, launches)} accumulated since profile(True)."""
+        out = {}
+        for name, k in _abi.PROF_KERNELS.items():
+            ms, n = C.c_double(), C.c_longlong()
+            _check(self.lib.artia_profile_read(self.h, k, C.byref(ms), C.byref(n)), self.h, "artia_profile_read")
+            out[name] = (ms.value, n.value)
+        return out
+
+    def sart_run(self, vg: VolGeom, acq: Acquisition, projections, vol, params: SartParams, iterations=1,
+                 volume_is_empty: bool = True, timed=False):
+        """Runs `iterations` SART iterations; returns (ms or None, volume_is_empty)."""
+        n = len(acq.geoms)
+        tg, hp, rl, il = acq.c_arrays()
+        empty = C.c_int(1 if volume_is_empty else 0)
+        ms = C.c_float() if timed else None
+        st = self.lib.artia_sart_run(self.h, C.byref(vg), tg, hp, rl, n, il,
+                                     _ptr(projections, "projections", n * acq.width * acq.height), acq.width,
+                                     acq.height, _ptr(vol, "vol", vg.nx * vg.ny * vg.nz), C.byref(params),
+                                     int(iterations), C.byref(empty), C.byref(ms) if timed else None)
+        _check(st, self.h, "artia_sart_run")
+        return (ms.value if timed else None), bool(empty.value)

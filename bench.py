@@ -1,0 +1,254 @@
+#!/usr/bin/env python3
+"""SART throughput benchmark on one MI355X.
+
+Workload (default): plain SART, fp32 volume 1024x1024x256 of 400 seeded ellipsoids + 2000 gold
+beads (r = 3 voxels, density 5); 61 tilts -60..60 step 2 deg, 1024x1024 projections, tilt-axis
+rotation psi ~ U[-5, 5] deg, per-tilt shifts ~ U[-5, 5] px, Gaussian noise at SNR 0.5,
+oversampling 2, ordered-subset size 1, lambda 0.3.  A step is one SART iteration (61 x
+[forward projection, compare, cropBorder, backprojection]) with all data resident in HBM.
+
+Output: one JSON line.  value = voxel updates (V * T per iteration) per second, in G.
+roofline: the dominant kernel's algorithmic bytes per launch / its average launch time (HIP events
+on the driver's stream, inside the timed region) against MI355X's 8 TB/s HBM peak.
+cpu_baseline: the float CPU oracle (reference operation order, OpenMP over the host cores) on a
+bounded sample of the same workload, extrapolated to one iteration.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: dims, W, H, tilt step, n ellipsoids, n beads, psi range, shift range, OS, seed
+    "sart_1024x1024x256_61tilts": dict(dims=(1024, 1024, 256), W=1024, H=1024, step=2.0, n_ell=400, n_beads=2000,
+                                       psi=5.0, shift=5.0, os=2, seed=2, noise_snr=0.5),
+    "sart_256x256x64_41tilts": dict(dims=(256, 256, 64), W=256, H=256, step=3.0, n_ell=50, n_beads=0, psi=0.0,
+                                    shift=0.0, os=1, seed=1, noise_rel=0.05),
+}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(name: str, ctx):
+    import numpy as np
+    import torch
+
+    import artiatomi_amd as A
+    from artiatomi_amd import phantom
+
+    w = WORKLOADS[name]
+    nx, ny, nz = w["dims"]
+    W, H = w["W"], w["H"]
+    tilts = phantom.tilt_series(-60.0, 60.0, w["step"])
+    rng = np.random.default_rng(w["seed"])
+    psis = rng.uniform(-w["psi"], w["psi"], len(tilts)) if w["psi"] else np.zeros(len(tilts))
+    shifts = rng.uniform(-w["shift"], w["shift"], (len(tilts), 2)) if w["shift"] else np.zeros((len(tilts), 2))
+    vg = A.volume_geometry(nx, ny, nz)
+    acq = A.Acquisition.build(vg, W, H, tilts, psis_deg=psis.tolist(), shifts=shifts.tolist())
+    t0 = time.time()
+    shapes = phantom.make_shapes(w["dims"], w["n_ell"], w["seed"], n_beads=w["n_beads"])
+    truth = phantom.rasterize(w["dims"], shapes, device="cuda")
+    projs = torch.zeros((len(tilts), H, W), device="cuda")
+    dist = torch.zeros((H, W), device="cuda")
+    for i, g in enumerate(acq.geoms):
+        ctx.fp_march(vg, g, truth, projs[i], dist)
+    torch.cuda.synchronize()
+    gen = torch.Generator(device="cuda").manual_seed(w["seed"])
+    if "noise_snr" in w:
+        sigma = float(torch.sqrt(projs.var() / w["noise_snr"]))
+    else:
+        sigma = w["noise_rel"] * float(projs.mean())
+    projs += torch.randn(projs.shape, generator=gen, device="cuda") * sigma
+    torch.cuda.synchronize()
+    log(f"workload {name}: phantom + {len(tilts)} projections in {time.time() - t0:.1f} s (noise sigma {sigma:.3g})")
+    del truth
+    return vg, acq, projs, w
+
+
+def fp_samples_per_launch(ctx, vg, acq, vol):
+    """Approximate trilinear samples per forward projection (4 sub-rays x (chord / 0.15 + 0.5)),
+    averaged over the tilt series, from the mean chord image of each tilt."""
+    import torch
+
+    H, W = acq.height, acq.width
+    total = 0.0
+    dist = torch.zeros((H, W), device="cuda")
+    proj = torch.zeros((H, W), device="cuda")
+    for g in acq.geoms[:: max(1, len(acq.geoms) // 8)]:
+        ctx.vol_traversal(vg, g, dist.zero_())
+        hit = dist > 0
+        total += float((4 * (dist[hit] / (0.15 * vg.voxel_size[0]) + 0.5)).sum())
+    del proj
+    return total / len(acq.geoms[:: max(1, len(acq.geoms) // 8)])
+
+
+def cpu_baseline(vg, acq, vol_dev, projs_dev, w):
+    """Float CPU oracle on a bounded sample: one tilt's forward projection over a band of rows and
+    its backprojection over a slab of z slices, scaled to the full image / volume and to all
+    tilts (compare and cropBorder run on the full image)."""
+    import ctypes as C
+
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+
+    lib = O.load()
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    O.set_threads(cores)
+    cores = lib.oracle_get_threads()
+    nx, ny, nz = w["dims"]
+    W, H = acq.width, acq.height
+    ti = len(acq.geoms) // 4   # a tilt of about -30 deg
+    g = acq.geoms[ti]
+    vol = vol_dev.detach().cpu().numpy()
+    rows = 8 if W >= 1024 else 32
+    y0 = H // 2 - rows // 2
+    t0 = time.perf_counter()
+    proj, dist = O.march(vg, g, vol, W, H, dtype=np.float32, roi=(0, y0, W, y0 + rows))
+    t_fp = (time.perf_counter() - t0) * (H / rows)
+    real = projs_dev[ti].detach().cpu().numpy()
+    dist_full = np.ones((H, W), np.float32) * 2.0
+    t0 = time.perf_counter()
+    res = O.compare(real, np.zeros((H, W), np.float32), dist_full, acq.real_lengths[ti])
+    res = O.crop_border(res, acq.hitpoints[ti])
+    t_cmp = time.perf_counter() - t0
+    zs = 2 if nz >= 128 else 8
+    z0 = nz // 2
+    slab = type(vg)()
+    C.memmove(C.byref(slab), C.byref(vg), C.sizeof(vg))
+    slab.bbox_min[2] = vg.bbox_min[2] + z0 * vg.voxel_size[2]
+    slab.bbox_max[2] = slab.bbox_min[2] + zs * vg.voxel_size[2]
+    slab.dim[2] = float(zs)
+    slab.nz = zs
+    t0 = time.perf_counter()
+    O.back_project(slab, g, vol[z0:z0 + zs], res, 0.3, os_=w["os"], dtype=np.float32)
+    t_bp = (time.perf_counter() - t0) * (nz / zs)
+    t_iter = (t_fp + t_cmp + t_bp) * len(acq.geoms)
+    V = nx * ny * nz
+    return {
+        "value": V * len(acq.geoms) / t_iter / 1e9,
+        "unit": "Gvoxel-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"float oracle, tilt {acq.tilts_deg[ti]:+.0f} deg: forward projection of {rows} of {H} rows "
+                   f"({t_fp * rows / H:.2f} s), backprojection of {zs} of {nz} z-slices "
+                   f"({t_bp * zs / nz:.2f} s), compare+cropBorder full image ({t_cmp:.3f} s); extrapolated "
+                   f"x{H // rows} rows, x{nz // zs} slices, x{len(acq.geoms)} tilts -> {t_iter:.0f} s/iteration"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="sart_1024x1024x256_61tilts", choices=sorted(WORKLOADS))
+    ap.add_argument("--numerics", default="exact", choices=["exact", "reference"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this benchmark measures one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}), flush=True)
+        return 2
+
+    import torch
+
+    import artiatomi_amd as A
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    ctx = A.ArtiaContext(0, numerics=args.numerics)
+    vg, acq, projs, w = build_workload(args.workload, ctx)
+    nx, ny, nz = w["dims"]
+    V, T = nx * ny * nz, len(acq.geoms)
+    sp = A.sart_params(lambda_=0.3, sirt_count=1, oversampling=w["os"])
+    vol = torch.zeros((nz, ny, nx), device="cuda")
+
+    empty = True
+    for i in range(args.warmup):
+        _, empty = ctx.sart_run(vg, acq, projs, vol, sp, iterations=1, volume_is_empty=empty)
+        torch.cuda.synchronize()
+        log(f"warmup iteration {i + 1}/{args.warmup} done")
+
+    ctx.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        _, empty = ctx.sart_run(vg, acq, projs, vol, sp, iterations=1, volume_is_empty=empty)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    log(f"timed {args.steps} iterations in {elapsed:.3f} s; per-kernel totals {prof}")
+
+    H, W = acq.height, acq.width
+    per_kernel = {}
+    for name, (ms, n) in prof.items():
+        if n:
+            per_kernel[name] = {"avg_ms": ms / n, "launches": n, "share": ms / (elapsed * 1e3)}
+    bytes_bp = 8.0 * V + 4.0 * W * H
+    bytes_fp = 4.0 * V + 8.0 * W * H
+    bp_avg = prof["bp"][0] / max(1, prof["bp"][1]) / 1e3
+    fp_avg = prof["fp"][0] / max(1, prof["fp"][1]) / 1e3
+    samples = fp_samples_per_launch(ctx, vg, acq, vol)
+    per_kernel["bp"].update({"algorithmic_bytes": bytes_bp, "achieved_GBs": bytes_bp / bp_avg / 1e9,
+                             "hbm_frac": bytes_bp / bp_avg / 1e9 / HBM_PEAK_GBS,
+                             "Gvoxel_updates_per_s": V / bp_avg / 1e9})
+    per_kernel["fp"].update({"algorithmic_bytes": bytes_fp, "achieved_GBs": bytes_fp / fp_avg / 1e9,
+                             "hbm_frac": bytes_fp / fp_avg / 1e9 / HBM_PEAK_GBS,
+                             "Gsamples_per_s_approx": samples / fp_avg / 1e9})
+    dom = "fp" if prof["fp"][0] >= prof["bp"][0] else "bp"
+    dom_bytes, dom_avg = (bytes_fp, fp_avg) if dom == "fp" else (bytes_bp, bp_avg)
+    achieved = dom_bytes / dom_avg / 1e9
+    roofline = {"kernel": "march" if dom == "fp" else "backProjection", "bound": "hbm", "achieved": achieved,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(vg, acq, vol, projs, w)
+        except Exception as exc:   # reported, never silently replaced
+            log(f"cpu baseline failed: {exc!r}")
+            cpu = {"error": repr(exc)}
+
+    value = V * T * args.steps / elapsed / 1e9
+    out = {
+        "metric": "Gvoxel-updates/s per SART iteration",
+        "value": value,
+        "unit": "Gvoxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid + gold-bead phantom, projections from the forward projector + noise)",
+        "config": {"workload": args.workload, "volume": list(w["dims"]), "projection": [W, H], "tilts": T,
+                   "tilt_range_deg": [-60, 60], "oversampling": w["os"], "sirt_count": 1, "lambda": 0.3,
+                   "numerics": args.numerics, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "kernels": per_kernel,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,207 @@
+/*
+ * artia.h - C ABI of the MI355X (gfx950) EmSART reconstruction core.
+ *
+ * The entry points replace the reference's operator boundary, the C++ functor classes of
+ * src/EmSART/Kernels.h:37-245 plus the cuFFT/NPP glue between them inside
+ * src/EmSART/Reconstructor.cpp, with plain-pointer C functions:
+ *   - no exceptions cross the ABI: every call returns an artia_status, and
+ *     artia_last_error() holds the message (the C++ adapter in artia_kernels.hpp rethrows,
+ *     keeping the reference's exception convention of CudaException.h:34-36);
+ *   - geometry is passed per call in artia_tilt_geom / artia_vol_geom instead of the
+ *     module __constant__ globals of DeviceVariables.cuh:13-34 that SetConstantValues
+ *     (Kernels.cpp:949-1132) uploads one cuMemcpyHtoD at a time;
+ *   - images and volumes are caller-owned device pointers: fp32, row-major, x fastest.
+ *     Projections use a pitch counted in elements; volumes are dense [z][y][x];
+ *   - every launch is stream-ordered on the context's stream.  When `ms` is non-NULL the
+ *     call synchronises and returns the elapsed milliseconds, which is the reference
+ *     functors' "float operator()" contract (Kernels.cpp:65-86).  When `ms` is NULL the
+ *     call does not block.
+ *
+ * Numerics (artia_set_numerics):
+ *   ARTIA_NUMERICS_REFERENCE  kernels evaluate the reference kernel bodies in the reference's
+ *                             float operation order (IEEE division and square root, no FMA
+ *                             contraction, exact-float texture filtering).
+ *   ARTIA_NUMERICS_EXACT      (default) kernels use numerically stable formulations of the same
+ *                             algorithm, built for throughput, which track the double-precision
+ *                             evaluation of the reference algorithm.
+ * A context is used by one host thread at a time; several contexts may coexist.
+ */
+#ifndef ARTIA_H
+#define ARTIA_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct artia_ctx artia_ctx;
+
+typedef enum {
+    ARTIA_OK = 0,
+    ARTIA_ERR_INVALID_ARGUMENT = -1,
+    ARTIA_ERR_DEVICE = -2,
+    ARTIA_ERR_OUT_OF_MEMORY = -3,
+    ARTIA_ERR_UNSUPPORTED = -4,
+    ARTIA_ERR_FFT = -5
+} artia_status;
+
+typedef enum { ARTIA_NUMERICS_EXACT = 0, ARTIA_NUMERICS_REFERENCE = 1 } artia_numerics;
+
+/* BP source-image sampling: POINT without CTF, LINEAR with CTF (Reconstructor.cpp:318-327) */
+typedef enum { ARTIA_FILTER_POINT = 0, ARTIA_FILTER_LINEAR = 1 } artia_filter;
+
+/* Per-tilt detector geometry: the values of c_detektor, c_uPitch, c_vPitch, c_projNorm,
+ * c_DetectorMatrix, c_magAniso, c_magAnisoInv (DeviceVariables.cuh:13-34). */
+typedef struct {
+    float d[3];
+    float u[3];
+    float v[3];
+    float n[3];
+    float det_matrix[16]; /* row-major 4x4 inverse of [u; v; n; d] */
+    float mag_aniso[9];   /* row-major 3x3, forward projection */
+    float mag_aniso_inv[9];
+} artia_tilt_geom;
+
+/* Volume placement: c_bBoxMin, c_bBoxMax, c_volumeBBoxRcp, c_volumeDim, c_voxelSize,
+ * c_zShiftForPartialVolume, plus the extent of the stored array. */
+typedef struct {
+    float bbox_min[3];
+    float bbox_max[3];
+    float bbox_rcp[3];
+    float dim[3];
+    float voxel_size[3];
+    float z_shift;
+    int nx, ny, nz;
+} artia_vol_geom;
+
+/* Per-tilt acquisition values (MarkerFile items, MarkerFile.h:34-46) */
+typedef struct {
+    float tilt_deg;
+    float psi_deg;
+    float mag;
+    float shift_x;
+    float shift_y;
+} artia_tilt_params;
+
+/* Global config keys entering the rotation (Config.cpp:303-406) */
+typedef struct {
+    float add_tilt_deg;
+    float add_tilt_x_deg;
+    float phi_deg;
+    int use_fix_psi;
+    float fix_psi_deg;
+} artia_global_params;
+
+/* SART schedule parameters (EmSart.cpp:560-675, Config.cpp:303-406) */
+typedef struct {
+    int sirt_count;       /* ordered-subset size S */
+    float lambda;         /* relaxation; each BP uses lambda / S */
+    int oversampling;     /* BP sub-pixel sampling factor */
+    float crop[4];        /* Crop (compare border) */
+    float crop_dim[4];    /* CropDim */
+    float cut_length[2];  /* CutLength (cropBorder) */
+    float dim_length[2];  /* DimLength */
+    float proj_scale;     /* ProjectionScaleFactor */
+} artia_sart_params;
+
+/* ---------------------------------------------------------------- context */
+artia_status artia_create(int device, artia_ctx** out);
+artia_status artia_destroy(artia_ctx* ctx);
+artia_status artia_set_stream(artia_ctx* ctx, void* hip_stream);
+artia_status artia_get_stream(artia_ctx* ctx, void** hip_stream);
+artia_status artia_set_numerics(artia_ctx* ctx, int numerics);
+artia_status artia_sync(artia_ctx* ctx);
+const char* artia_last_error(artia_ctx* ctx);
+const char* artia_version(void);
+
+/* ---------------------------------------------------------------- host geometry
+ * Projection::RotateMatrix, GetPosition, GetPixelUPitch, GetPixelVPitch, GetNormalVector,
+ * GetDetectorMatrix (Projection.cpp:80-404) and Reconstructor::GetMagAnistropyMatrix
+ * (Reconstructor.cpp:386-426). */
+artia_status artia_geom_tilt(const artia_tilt_params* tp, const artia_global_params* gp, int width,
+                             int height, float mag_aniso_amount, float mag_aniso_angle_deg,
+                             artia_tilt_geom* out);
+/* Volume::PositionInSpace for the whole volume (Volume.cpp:270-286) */
+artia_status artia_geom_volume(int nx, int ny, int nz, const float voxel_size[3],
+                               const float volume_shift[3], artia_vol_geom* out);
+/* Volume::PositionInSpace for a cubic motive-list sub-volume (Volume.cpp:288-306) */
+artia_status artia_geom_subvolume(int n, float voxel_size_sub, const float voxel_size_full[3],
+                                  const artia_vol_geom* full, const float position[3],
+                                  const float shift[3], artia_vol_geom* out);
+/* Projection::ComputeHitPoints (Projection.cpp:850-1049): int corners p1..p4 as 8 ints */
+artia_status artia_geom_hitpoints(const artia_vol_geom* full, const artia_tilt_geom* g, int out[8]);
+/* Reconstructor::Compare's volume traversal length (Reconstructor.cpp:1390-1393) */
+artia_status artia_geom_real_length(const artia_vol_geom* full, const artia_tilt_geom* g, float* out);
+/* Projection::CreateProjectionIndexList(PLT_RANDOM_START_ZERO_TILT) (Projection.cpp:561-596) */
+artia_status artia_projection_index_list(const float* tilt_deg, int count, int* out);
+
+/* ---------------------------------------------------------------- operators */
+/* FPKernel / march (Kernels.cpp:41-87, ForwardProjectionRayMarcher_TL.cu:57-160):
+ * proj += ray integral, dist = mean chord, over pixels in roi = {x0, y0, x1, y1}
+ * (roi NULL = whole image). */
+artia_status artia_fp_march(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                            const float* vol, int width, int height, size_t pitch, float* proj,
+                            float* dist, const int* roi, float* ms);
+/* SlicerKernel / slicer (Kernels.cpp:103-156, ForwardProjectionSlicer.cu:56-220) */
+artia_status artia_fp_slicer(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                             const float* vol, int width, int height, size_t pitch, float* proj,
+                             float tmin, float tmax, const int* roi, float* ms);
+/* VolTravLengthKernel / volTraversalLength (ForwardProjectionSlicer.cu:223-288) */
+artia_status artia_vol_traversal(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                                 int width, int height, size_t pitch, float* dist, const int* roi,
+                                 float* ms);
+/* CompKernel / compare (Kernels.cpp:216-270, Compare.cu:49-130): virt <- residual */
+artia_status artia_compare(artia_ctx* ctx, int width, int height, size_t pitch, const float* real,
+                           float* virt, const float* dist, float real_length, const float crop[4],
+                           const float crop_dim[4], float proj_scale, float* ms);
+/* CropBorderKernel / cropBorder (Kernels.cpp:276-320, Compare.cu:170-277); p = p1..p4 */
+artia_status artia_crop_border(artia_ctx* ctx, int width, int height, size_t pitch, float* img,
+                               const float cut_length[2], const float dim_length[2], const int p[8],
+                               float* ms);
+/* BPKernel / backProjection (Kernels.cpp:339-378, BackProjectionSquareOS.cu:263-782):
+ * vol += lambda * chord-weighted mean of img over each voxel's sub-rays; quads of 4 voxels
+ * along x (nx % 4 == 0) whose first corner distance lies in [dist_min, dist_max). */
+artia_status artia_bp(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, float* vol,
+                      const float* img, int width, int height, size_t pitch, float lambda,
+                      int oversampling, float dist_min, float dist_max, int filter, float* ms);
+/* CopyToSquareKernel / makeSquare (Kernels.cpp:804-830, CopyToSquare.cu:37-92) */
+artia_status artia_make_square(artia_ctx* ctx, int width, int height, size_t pitch, const float* in,
+                               int n, float* out, int border_x, int border_y, int mirror_y,
+                               int fill_zero, float* ms);
+
+/* ---------------------------------------------------------------- drivers
+ * EmSART's ordered-subset SART loop (EmSart.cpp:579-675) without CTF, device resident:
+ * for every subset, per tilt FP (volTraversalLength while the volume is empty) -> compare
+ * against projections[tilt] -> residual kept on the device; then per tilt cropBorder with
+ * hitpoints[8*tilt] -> BP with lambda / S.  tilts/hitpoints/real_lengths/index_list are host
+ * arrays of length ntilts; projections is a device array of ntilts * height * width floats
+ * (dense rows).  *volume_is_empty carries EmSART's volumeIsEmpty flag across calls. */
+artia_status artia_sart_run(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                            const int* hitpoints, const float* real_lengths, int ntilts,
+                            const int* index_list, const float* projections, int width, int height,
+                            float* vol, const artia_sart_params* sp, int iterations,
+                            int* volume_is_empty, float* ms);
+
+/* ---------------------------------------------------------------- profiling
+ * With profiling enabled, artia_sart_run brackets each kernel launch with HIP events on the
+ * context stream (no extra synchronisation inside the run) and accumulates, per kernel class,
+ * the summed device time and the launch count; enabling resets the counters.  This replaces
+ * the per-functor millisecond return values of Kernels.cpp:65-86, which the reference computes
+ * and drops. */
+typedef enum {
+    ARTIA_PROF_FP = 0,        /* march / volTraversalLength */
+    ARTIA_PROF_BP = 1,        /* backProjection */
+    ARTIA_PROF_COMPARE = 2,   /* compare */
+    ARTIA_PROF_CROP = 3,      /* cropBorder */
+    ARTIA_PROF_FILL = 4,      /* projection / distance resets */
+    ARTIA_PROF_COUNT = 5
+} artia_prof_kernel;
+artia_status artia_profile_enable(artia_ctx* ctx, int enable);
+artia_status artia_profile_read(artia_ctx* ctx, int kernel, double* total_ms, long long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARTIA_H */

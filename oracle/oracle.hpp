@@ -1,0 +1,217 @@
+// CPU oracle for the EmSART reconstruction core.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product library (artiatomi_amd/) may include,
+// link or call this code; only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+// leg load it, and only as the checker.
+//
+// Every function restates the reference algorithm with the file:line it follows (paths are
+// relative to the uermel/Artiatomi source tree).  Kernel bodies are templated on the
+// arithmetic type T:
+//   T = float  : "float-faithful" - the reference's float operation order, IEEE division and
+//                square root, no FMA contraction (build with -ffp-contract=off), NaN-ignoring
+//                fmin/fmax, and exact-float software texture filtering.
+//   T = double : the same algorithm evaluated in double ("FP64 oracle"), used to state the
+//                error of every float result against the exact math.
+// Host geometry (Projection / Volume / Reconstructor helpers) is restated in the reference's
+// own float/double mix, because that is what the reference feeds its kernels.
+//
+// Parity pinning (what checks each function; see tests/test_oracle_*.py):
+//   geometry      closed form at tilt 0 / psi 0, orthonormality, detector-matrix inverse
+//                 identity, an independent numpy point-projection (matlab project3D.m form)
+//   march         constant-volume closed form, ellipsoid analytic line integral, linearity,
+//                 FP64 brute force on tiny volumes, float-vs-double agreement bound
+//   volTraversal  closed-form chord through an axis-aligned box
+//   slicer        sum over a partition of slabs equals march minus the first-sample quirk
+//   compare       direct closed-form evaluation, all-zero / all-one inputs, border bands
+//   cropBorder    hand-picked quadrilateral membership, point-segment distance
+//   backProject   constant-residual invariant (every hit voxel gets exactly lambda*r),
+//                 impulse footprint, FP64 brute-force chord lengths on tiny volumes
+//   index list    glibc srand(0)/rand() of the running C library
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace artia_oracle {
+
+// Constants.h:33-34
+constexpr float kDist = -10000.0f;
+
+// ----------------------------------------------------------------------------------------
+// Geometry carriers (the values SetConstantValues uploads, Kernels.cpp:985-1102)
+// ----------------------------------------------------------------------------------------
+struct TiltGeom {
+    float d[3];       // c_detektor      Projection::GetPosition
+    float u[3];       // c_uPitch        Projection::GetPixelUPitch
+    float v[3];       // c_vPitch        Projection::GetPixelVPitch
+    float n[3];       // c_projNorm      Projection::GetNormalVector
+    float M[16];      // c_DetectorMatrix Projection::GetDetectorMatrix (row-major 4x4)
+    float A[9];       // c_magAniso      Reconstructor::GetMagAnistropyMatrix (row-major 3x3)
+    float Ainv[9];    // c_magAnisoInv
+};
+
+struct VolGeom {
+    float bbMin[3];   // c_bBoxMin   (sub-volume)
+    float bbMax[3];   // c_bBoxMax
+    float rcp[3];     // c_volumeBBoxRcp
+    float dim[3];     // c_volumeDim (sub-volume dimension, as float)
+    float vs[3];      // c_voxelSize
+    float zshift;     // c_zShiftForPartialVolume
+    int nx, ny, nz;   // stored array extent
+};
+
+// Per-tilt acquisition parameters (MarkerFile item layout, MarkerFile.h:34-46) plus the
+// global config keys that enter Projection::RotateMatrix (Config.cpp:303-406).
+struct TiltParams {
+    float tilt_deg;   // MFI_TiltAngle
+    float psi_deg;    // MFI_RotationPsi
+    float mag;        // MFI_Magnifiaction
+    float shift_x;    // MFI_X_Shift + extra shift
+    float shift_y;    // MFI_Y_Shift + extra shift
+};
+struct GlobalParams {
+    float add_tilt_deg;      // AddTiltAngle
+    float add_tilt_x_deg;    // AddTiltXAngle
+    float phi_deg;           // PhiAngle
+    int use_fix_psi;         // UseFixPsiAngle
+    float fix_psi_deg;       // PsiAngle
+};
+
+// ----------------------------------------------------------------------------------------
+// Small helpers
+// ----------------------------------------------------------------------------------------
+template <class T> inline T fminT(T a, T b) { return std::fmin(a, b); }   // IEEE minNum
+template <class T> inline T fmaxT(T a, T b) { return std::fmax(a, b); }   // IEEE maxNum
+template <class T> inline T lerpT(T p, T q, T w) { return std::fma(w, q - p, p); }
+inline int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+
+// Exact-float emulation of a clamp-addressed, unnormalised-coordinate 3D texture with linear
+// filtering (volume texture flags, EmSart.cpp:421 and CudaTextures.cpp:792-798): voxel j is
+// centred at j+0.5; the two neighbours are index-clamped.  The hardware evaluates the weights
+// in low-precision fixed point; this oracle defines the weights exactly (nested fma lerps).
+// Pre-clamping the coordinate to [-1, n+1] leaves the clamped-index result unchanged and keeps
+// the float->int conversion defined.
+template <class T>
+inline T tex3d_linear(const T* vol, int nx, int ny, int nz, T x, T y, T z) {
+    x = fminT(fmaxT(x, T(-1)), T(nx + 1));
+    y = fminT(fmaxT(y, T(-1)), T(ny + 1));
+    z = fminT(fmaxT(z, T(-1)), T(nz + 1));
+    T xb = x - T(0.5), yb = y - T(0.5), zb = z - T(0.5);
+    T fx = std::floor(xb), fy = std::floor(yb), fz = std::floor(zb);
+    T a = xb - fx, b = yb - fy, c = zb - fz;
+    int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+    int i1 = clampi(i0 + 1, 0, nx - 1), j1 = clampi(j0 + 1, 0, ny - 1), k1 = clampi(k0 + 1, 0, nz - 1);
+    i0 = clampi(i0, 0, nx - 1); j0 = clampi(j0, 0, ny - 1); k0 = clampi(k0, 0, nz - 1);
+    const size_t sx = 1, sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
+    auto at = [&](int i, int j, int k) { return vol[(size_t)k * sz + (size_t)j * sy + (size_t)i * sx]; };
+    T c00 = lerpT(at(i0, j0, k0), at(i1, j0, k0), a);
+    T c10 = lerpT(at(i0, j1, k0), at(i1, j1, k0), a);
+    T c01 = lerpT(at(i0, j0, k1), at(i1, j0, k1), a);
+    T c11 = lerpT(at(i0, j1, k1), at(i1, j1, k1), a);
+    T c0 = lerpT(c00, c10, b);
+    T c1 = lerpT(c01, c11, b);
+    return lerpT(c0, c1, c);
+}
+
+// 2D pitched texture bound to the BP source image (Reconstructor.cpp:318-327): POINT filtering
+// without CTF, LINEAR with CTF; clamp addressing, unnormalised coordinates.
+enum { FILTER_POINT = 0, FILTER_LINEAR = 1 };
+template <class T>
+inline T tex2d(const T* img, int W, int H, size_t pitch, int filter, T x, T y) {
+    x = fminT(fmaxT(x, T(-1)), T(W + 1));
+    y = fminT(fmaxT(y, T(-1)), T(H + 1));
+    if (filter == FILTER_POINT) {
+        int i = clampi((int)std::floor(x), 0, W - 1), j = clampi((int)std::floor(y), 0, H - 1);
+        return img[(size_t)j * pitch + (size_t)i];
+    }
+    T xb = x - T(0.5), yb = y - T(0.5);
+    T fx = std::floor(xb), fy = std::floor(yb);
+    T a = xb - fx, b = yb - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    int i1 = clampi(i0 + 1, 0, W - 1), j1 = clampi(j0 + 1, 0, H - 1);
+    i0 = clampi(i0, 0, W - 1); j0 = clampi(j0, 0, H - 1);
+    T r0 = lerpT(img[(size_t)j0 * pitch + i0], img[(size_t)j0 * pitch + i1], a);
+    T r1 = lerpT(img[(size_t)j1 * pitch + i0], img[(size_t)j1 * pitch + i1], a);
+    return lerpT(r0, r1, b);
+}
+
+// DeviceVariables.cuh:41-48 - mag-anisotropy 3x3 applied to (x, y, 1)
+template <class T>
+inline void mag_apply(const T A[9], T x, T y, T& xo, T& yo) {
+    xo = (A[0] * x + A[1] * y) + A[2] * T(1);
+    yo = (A[3] * x + A[4] * y) + A[5] * T(1);
+}
+
+// ----------------------------------------------------------------------------------------
+// Host geometry (float/double mix exactly as the reference host code)
+// ----------------------------------------------------------------------------------------
+void rotate_vec(const TiltParams& tp, const GlobalParams& gp, const float in[3], float out[3]);
+void tilt_geometry(const TiltParams& tp, const GlobalParams& gp, int W, int H,
+                   float magAmount, float magAngleDeg, TiltGeom& g);
+void mag_aniso_matrix(float amount, float angleDeg, float dimX, float dimY, float out[9]);
+void volume_geometry(int nx, int ny, int nz, const float vs[3], const float volShift[3], VolGeom& vg);
+void subvolume_geometry(int n, float vsSub, const float vsFull[3], const VolGeom& full,
+                        const float pos[3], const float shift[3], VolGeom& vg);
+void hit_points(const VolGeom& full, const TiltGeom& g, int out[8]);
+float real_length(const VolGeom& full, const TiltGeom& g);
+void projection_index_list(const float* tilts, int count, int* out);
+int glibc_rand_sequence(unsigned seed, int count, int* out);
+
+// ----------------------------------------------------------------------------------------
+// Kernel bodies (templated)
+// ----------------------------------------------------------------------------------------
+struct Roi { int x0, y0, x1, y1; };
+
+// ForwardProjectionRayMarcher_TL.cu:57-160 (march)
+template <class T>
+void march(const VolGeom& vg, const TiltGeom& tg, const T* vol, int W, int H, size_t pitch,
+           T* proj, T* dist, Roi roi);
+
+// ForwardProjectionSlicer.cu:56-220 (slicer)
+template <class T>
+void slicer(const VolGeom& vg, const TiltGeom& tg, const T* vol, int W, int H, size_t pitch,
+            T* proj, T tmin, T tmax, Roi roi);
+
+// ForwardProjectionSlicer.cu:223-288 (volTraversalLength)
+template <class T>
+void vol_traversal(const VolGeom& vg, const TiltGeom& tg, int W, int H, size_t pitch, T* dist, Roi roi);
+
+// Compare.cu:49-130 (compare)
+template <class T>
+void compare(int W, int H, size_t pitch, const T* real, T* virt, const T* dist, float realLength,
+             const float cut[4], const float dim[4], float scale);
+
+// Compare.cu:132-277 (GetDistance, cropBorder)
+template <class T>
+void crop_border(int W, int H, size_t pitch, T* img, const float cut[2], const float dim[2], const int p[8]);
+
+// BackProjectionSquareOS.cu:263-782 (backProjection)
+template <class T>
+void back_project(const VolGeom& vg, const TiltGeom& tg, T* vol, const T* img, int W, int H,
+                  size_t pitch, float lambda, int os, float distMin, float distMax, int filter);
+
+// CopyToSquare.cu:37-92 (makeSquare)
+template <class T>
+void make_square(int W, int H, size_t pitch, const T* in, int N, T* out, int bx, int by,
+                 bool mirrorY, bool fillZero);
+
+// EmSart.cpp:579-675 with Reconstructor.cpp:429-490, 871-887, 1386-1405 (SART, no CTF)
+struct SartParams {
+    int iterations;
+    int sirt_count;
+    float lambda;
+    int os;
+    float crop[4], crop_dim[4];
+    float cut[2], dim_len[2];
+    float scale;   // ProjectionScaleFactor
+};
+template <class T>
+void sart(const VolGeom& vg, const VolGeom& full, const std::vector<TiltGeom>& tilts,
+          const std::vector<int>& indexList, const T* projections, int W, int H, T* vol,
+          const SartParams& sp, int iterationsToRun, bool* volumeIsEmpty);
+
+}  // namespace artia_oracle
