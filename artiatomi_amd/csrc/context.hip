@@ -97,6 +97,9 @@ bool valid_geometry(const artia_vol_geom* vg, const artia_tilt_geom* tg) {
             !std::isfinite(tg->n[i]))
             return false;
     }
+    // the ray direction must be a unit vector (up to float rounding) for the sample loops to be bounded
+    const double nn = (double)tg->n[0] * tg->n[0] + (double)tg->n[1] * tg->n[1] + (double)tg->n[2] * tg->n[2];
+    if (!(nn > 0.98 && nn < 1.02)) return false;
     return true;
 }
 

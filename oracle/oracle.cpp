@@ -671,7 +671,11 @@ void sart(const VolGeom& vg, const VolGeom& full, const std::vector<TiltGeom>& t
           const std::vector<int>& indexList, const T* projections, int W, int H, T* vol,
           const SartParams& sp, int iterationsToRun, bool* volumeIsEmpty) {
     const size_t npix = (size_t)W * H;
-    const int S = std::max(1, sp.sirt_count);
+    if (sp.sirt_count <= 0 || sp.sirt_count > (int)indexList.size() || sp.os <= 0 || sp.os > 64) {
+        std::fprintf(stderr, "oracle sart: invalid SIRTCount %d or OverSampling %d\n", sp.sirt_count, sp.os);
+        return;
+    }
+    const int S = sp.sirt_count;
     const int P = (int)indexList.size();
     std::vector<T> proj(npix), dist(npix);
     std::vector<std::vector<T>> residual((size_t)S, std::vector<T>(npix));

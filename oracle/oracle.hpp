@@ -200,8 +200,8 @@ void make_square(int W, int H, size_t pitch, const T* in, int N, T* out, int bx,
                  bool mirrorY, bool fillZero);
 
 // EmSart.cpp:579-675 with Reconstructor.cpp:429-490, 871-887, 1386-1405 (SART, no CTF)
+// Same memory layout as artia_sart_params (include/artia.h).
 struct SartParams {
-    int iterations;
     int sirt_count;
     float lambda;
     int os;
