@@ -127,76 +127,150 @@ __device__ __forceinline__ RayD ray_setup_d(const VolG& v, const TiltG& g, doubl
     return r;
 }
 
-// Sums the samples of one sub-ray between t_first and t_last (inclusive) at spacing step,
-// returning sum(values).  Positions are generated in float voxel coordinates relative to the
-// first sample; the partial sums go to double every 64 samples.
-__device__ __forceinline__ double march_samples_exact(const VolG& v, const TiltG& g, const float* __restrict__ vol,
-                                                      const double c[3], double t_first, int count, double step) {
-    // voxel coordinate = (p - bbMin) * rcp * dim, with the float-rounded rcp the reference uploads
-    float q0[3], dq[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const double scale = (double)v.rcp[i] * (double)v.dim[i];
-        const double p = (c[i] + t_first * (double)g.n[i]) - (double)v.bbMin[i];
-        q0[i] = (float)(p * scale);
-        dq[i] = (float)(step * (double)g.n[i] * scale);
-    }
-    q0[2] -= v.zshift;
+// Pair of x-neighbours vol[o], vol[o+1] (4-byte aligned; the compiler picks the widest legal load).
+template <bool kBig>
+__device__ __forceinline__ float2 load_pair(const float* __restrict__ vol, uint64_t o64, uint32_t o32) {
+    float2 p;
+    const float* a = kBig ? vol + o64 : (const float*)((const char*)vol + (o32 << 2));
+    __builtin_memcpy(&p, a, sizeof(float2));
+    return p;
+}
+
+// Trilinear read at voxel coordinates already shifted by -0.5 (texel j centred at j), clamp
+// addressing, for volumes of at least 2 voxels per axis.  The coordinate is clamped to
+// [0, n-1] and the lower texel to n-2, so each x-neighbour pair is a single load and no
+// per-lane selects are needed; beyond the last texel centre the weight is exactly 1 and the
+// result is the edge texel up to the rounding of one lerp.
+template <bool kBig>
+__device__ __forceinline__ float tri_pairs(const float* __restrict__ vol, int nx, int ny, int nz, float xb, float yb,
+                                           float zb) {
+    const float xc = __builtin_amdgcn_fmed3f(xb, 0.0f, (float)(nx - 1));
+    const float yc = __builtin_amdgcn_fmed3f(yb, 0.0f, (float)(ny - 1));
+    const float zc = __builtin_amdgcn_fmed3f(zb, 0.0f, (float)(nz - 1));
+    const int ib = min((int)xc, nx - 2), jb = min((int)yc, ny - 2), kb = min((int)zc, nz - 2);
+    const float a = xc - (float)ib, b = yc - (float)jb, c = zc - (float)kb;
+    const uint32_t sy = (uint32_t)nx, sz = (uint32_t)nx * (uint32_t)ny;
+    uint32_t o32 = 0;
+    uint64_t o64 = 0;
+    if (kBig)
+        o64 = (uint64_t)kb * sz + (uint64_t)jb * sy + (uint64_t)ib;
+    else
+        o32 = __umul24((uint32_t)kb, sz) + __umul24((uint32_t)jb, sy) + (uint32_t)ib;
+    const float2 p00 = load_pair<kBig>(vol, o64, o32);
+    const float2 p10 = load_pair<kBig>(vol, o64 + sy, o32 + sy);
+    const float2 p01 = load_pair<kBig>(vol, o64 + sz, o32 + sz);
+    const float2 p11 = load_pair<kBig>(vol, o64 + sz + sy, o32 + sz + sy);
+    const float c00 = lerpf(p00.x, p00.y, a), c10 = lerpf(p10.x, p10.y, a);
+    const float c01 = lerpf(p01.x, p01.y, a), c11 = lerpf(p11.x, p11.y, a);
+    return lerpf(lerpf(c00, c10, b), lerpf(c01, c11, b), c);
+}
+
+// Sum of the trilinear samples q0 + k*dq, k = 0..count-1 (coordinates shifted by -0.5).  Two
+// samples are in flight per iteration; float partial sums are folded into double every 256.
+template <bool kBig>
+__device__ __forceinline__ double march_sum(const VolG& v, const float* __restrict__ vol, const float q0[3],
+                                            const float dq[3], int count) {
     double total = 0.0;
-    for (int k0 = 0; k0 < count; k0 += 64) {
-        const int k1 = min(count, k0 + 64);
-        float part = 0.0f;
-        for (int k = k0; k < k1; k++) {
-            const float fk = (float)k;
-            const float qx = __builtin_fmaf(fk, dq[0], q0[0]);
-            const float qy = __builtin_fmaf(fk, dq[1], q0[1]);
-            const float qz = __builtin_fmaf(fk, dq[2], q0[2]);
-            part += tex3d_linear(vol, v.nx, v.ny, v.nz, qx, qy, qz);
+    for (int k0 = 0; k0 < count; k0 += 256) {
+        const int k1 = min(count, k0 + 256);
+        float s0 = 0.0f, s1 = 0.0f;
+        int k = k0;
+        for (; k + 1 < k1; k += 2) {
+            const float f0 = (float)k, f1 = (float)(k + 1);
+            s0 += tri_pairs<kBig>(vol, v.nx, v.ny, v.nz, __builtin_fmaf(f0, dq[0], q0[0]),
+                                  __builtin_fmaf(f0, dq[1], q0[1]), __builtin_fmaf(f0, dq[2], q0[2]));
+            s1 += tri_pairs<kBig>(vol, v.nx, v.ny, v.nz, __builtin_fmaf(f1, dq[0], q0[0]),
+                                  __builtin_fmaf(f1, dq[1], q0[1]), __builtin_fmaf(f1, dq[2], q0[2]));
         }
-        total += (double)part;
+        if (k < k1) {
+            const float f0 = (float)k;
+            s0 += tri_pairs<kBig>(vol, v.nx, v.ny, v.nz, __builtin_fmaf(f0, dq[0], q0[0]),
+                                  __builtin_fmaf(f0, dq[1], q0[1]), __builtin_fmaf(f0, dq[2], q0[2]));
+        }
+        total += (double)s0 + (double)s1;
     }
     return total;
 }
 
-template <bool kSlicer>
+// One lane per sub-ray: a wave covers 4x4 pixels x 4 sub-rays, a 256-thread block 8x8 pixels,
+// so the rays of a wave stay within a few voxels of each other through the whole volume.
+template <bool kSlicer, bool kBig>
 __global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const float* __restrict__ vol, Roi roi,
                                                           size_t pitch, float* __restrict__ proj,
                                                           float* __restrict__ dist, float tminDef, float tmaxDef) {
-    const int x = roi.x0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int y = roi.y0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-    if (x >= roi.x1 || y >= roi.y1) return;
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int sub = lane & 3, pix = lane >> 2;
+    const int x = roi.x0 + (int)blockIdx.x * 8 + (wave & 1) * 4 + (pix & 3);
+    const int y = roi.y0 + (int)blockIdx.y * 8 + (wave >> 1) * 4 + (pix >> 2);
+    const bool active = x < roi.x1 && y < roi.y1;
     const double step = (double)v.vs[0] * 0.15;
-    double acc = 0.0, len = 0.0;
-    int hits = 0;
-#pragma unroll 1
-    for (int s = 0; s < 4; s++) {
-        const double px = (double)x + ((s & 2) ? 0.75 : 0.25);
-        const double py = (double)y + ((s & 1) ? 0.75 : 0.25);
-        RayD r = ray_setup_d(v, g, px, py);
-        if (!((r.t_out - r.t_in) > 0.0)) continue;
-        double t_first = r.t_in, t_in = r.t_in, t_out = r.t_out;
-        if (kSlicer) {
-            t_in = fmax(t_in, (double)tminDef);
-            t_out = fmin(t_out, (double)tmaxDef);
-            if (!(t_in <= t_out)) continue;
-            // first sample at the detector point (ForwardProjectionSlicer.cu:129-133)
-            acc += (double)tex3d_linear(
-                vol, v.nx, v.ny, v.nz, (float)((r.c[0] - (double)v.bbMin[0]) * ((double)v.rcp[0] * (double)v.dim[0])),
-                (float)((r.c[1] - (double)v.bbMin[1]) * ((double)v.rcp[1] * (double)v.dim[1])),
-                (float)((r.c[2] - (double)v.bbMin[2]) * ((double)v.rcp[2] * (double)v.dim[2]) - (double)v.zshift));
-            t_first = t_in + step;
-        } else {
-            hits++;
-            len += t_out - t_in;
+    double acc = 0.0, len = 0.0, hits = 0.0;
+    if (active) {
+        const double px = (double)x + ((sub & 2) ? 0.75 : 0.25);
+        const double py = (double)y + ((sub & 1) ? 0.75 : 0.25);
+        const RayD r = ray_setup_d(v, g, px, py);
+        if ((r.t_out - r.t_in) > 0.0) {
+            double t_first = r.t_in, t_last = r.t_out;
+            bool run = true;
+            if (kSlicer) {
+                const double t_in = fmax(r.t_in, (double)tminDef);
+                t_last = fmin(r.t_out, (double)tmaxDef);
+                run = t_in <= t_last;
+                if (run) {
+                    // first sample at the detector point (ForwardProjectionSlicer.cu:129-133)
+                    acc += (double)tex3d_linear(
+                        vol, v.nx, v.ny, v.nz,
+                        (float)((r.c[0] - (double)v.bbMin[0]) * ((double)v.rcp[0] * (double)v.dim[0])),
+                        (float)((r.c[1] - (double)v.bbMin[1]) * ((double)v.rcp[1] * (double)v.dim[1])),
+                        (float)((r.c[2] - (double)v.bbMin[2]) * ((double)v.rcp[2] * (double)v.dim[2]) -
+                                (double)v.zshift));
+                    t_first = t_in + step;
+                }
+            } else {
+                hits = 1.0;
+                len = r.t_out - r.t_in;
+            }
+            if (run && t_first <= t_last) {
+                const int count = (int)fmin(floor((t_last - t_first) / step) + 1.0, (double)kMaxSamplesPerRay);
+                // voxel coordinate = (p - bbMin) * rcp * dim (the float-rounded rcp the reference uploads),
+                // shifted by -0.5 so that texel j is centred at integer j
+                float q0[3], dq[3];
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const double scale = (double)v.rcp[i] * (double)v.dim[i];
+                    const double p = (r.c[i] + t_first * (double)g.n[i]) - (double)v.bbMin[i];
+                    q0[i] = (float)(p * scale - 0.5 - (i == 2 ? (double)v.zshift : 0.0));
+                    dq[i] = (float)(step * (double)g.n[i] * scale);
+                }
+                if (v.nx >= 2 && v.ny >= 2 && v.nz >= 2) {
+                    acc += march_sum<kBig>(v, vol, q0, dq, count);
+                } else {
+                    double tot = 0.0;
+                    for (int k = 0; k < count; k++) {
+                        const float fk = (float)k;
+                        tot += (double)tex3d_linear(vol, v.nx, v.ny, v.nz, __builtin_fmaf(fk, dq[0], q0[0]) + 0.5f,
+                                                    __builtin_fmaf(fk, dq[1], q0[1]) + 0.5f,
+                                                    __builtin_fmaf(fk, dq[2], q0[2]) + 0.5f);
+                    }
+                    acc += tot;
+                }
+            }
         }
-        if (t_first > t_out) continue;
-        const double span = floor((t_out - t_first) / step);
-        const int count = (int)fmin(span + 1.0, (double)kMaxSamplesPerRay);
-        acc += march_samples_exact(v, g, vol, r.c, t_first, count, step);
     }
-    const size_t i = (size_t)y * pitch + (size_t)x;
-    proj[i] += (float)(acc * step * 0.25);
-    if (!kSlicer) dist[i] = hits ? (float)(len / (double)hits) : 0.0f;
+    // combine the four sub-rays of each pixel (lanes 4p .. 4p+3)
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    if (!kSlicer) {
+        len += __shfl_xor(len, 1);
+        len += __shfl_xor(len, 2);
+        hits += __shfl_xor(hits, 1);
+        hits += __shfl_xor(hits, 2);
+    }
+    if (active && sub == 0) {
+        const size_t i = (size_t)y * pitch + (size_t)x;
+        proj[i] += (float)(acc * step * 0.25);
+        if (!kSlicer) dist[i] = hits > 0.0 ? (float)(len / hits) : 0.0f;
+    }
 }
 
 __global__ void __launch_bounds__(256) vol_traversal_exact_kernel(VolG v, TiltG g, Roi roi, size_t pitch,
@@ -230,21 +304,32 @@ dim3 grid_for(const Roi& r) {
 void launch_march_impl(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vol, const Roi& r, size_t pitch,
                        float* proj, float* dist, bool slicer, float tmin, float tmax) {
     if (r.x1 <= r.x0 || r.y1 <= r.y0) return;
-    const dim3 grid = grid_for(r), block(64, 4);
     if (ctx->numerics == ARTIA_NUMERICS_REFERENCE) {
+        const dim3 grid = grid_for(r), block(64, 4);
         if (slicer)
             hipLaunchKernelGGL(march_ref_kernel<true>, grid, block, 0, ctx->stream, v, g, vol, r, pitch, proj, dist,
                                tmin, tmax);
         else
             hipLaunchKernelGGL(march_ref_kernel<false>, grid, block, 0, ctx->stream, v, g, vol, r, pitch, proj, dist,
                                tmin, tmax);
-    } else {
-        if (slicer)
-            hipLaunchKernelGGL(march_exact_kernel<true>, grid, block, 0, ctx->stream, v, g, vol, r, pitch, proj, dist,
-                               tmin, tmax);
+        return;
+    }
+    const dim3 grid((unsigned)((r.x1 - r.x0 + 7) / 8), (unsigned)((r.y1 - r.y0 + 7) / 8)), block(256);
+    const bool big = (uint64_t)v.nx * (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 30);
+    if (slicer) {
+        if (big)
+            hipLaunchKernelGGL((march_exact_kernel<true, true>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+                               proj, dist, tmin, tmax);
         else
-            hipLaunchKernelGGL(march_exact_kernel<false>, grid, block, 0, ctx->stream, v, g, vol, r, pitch, proj,
-                               dist, tmin, tmax);
+            hipLaunchKernelGGL((march_exact_kernel<true, false>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+                               proj, dist, tmin, tmax);
+    } else {
+        if (big)
+            hipLaunchKernelGGL((march_exact_kernel<false, true>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+                               proj, dist, tmin, tmax);
+        else
+            hipLaunchKernelGGL((march_exact_kernel<false, false>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+                               proj, dist, tmin, tmax);
     }
 }
 
