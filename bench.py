@@ -58,7 +58,8 @@ def build_workload(name: str, ctx):
     acq = A.Acquisition.build(vg, W, H, tilts, psis_deg=psis.tolist(), shifts=shifts.tolist())
     t0 = time.time()
     shapes = phantom.make_shapes(w["dims"], w["n_ell"], w["seed"], n_beads=w["n_beads"])
-    truth = phantom.rasterize(w["dims"], shapes, device="cuda")
+    # rasterised on the host so the GPU sees only the reconstruction kernels (short profiles)
+    truth = phantom.rasterize(w["dims"], shapes, device="cpu").cuda()
     projs = torch.zeros((len(tilts), H, W), device="cuda")
     dist = torch.zeros((H, W), device="cuda")
     for i, g in enumerate(acq.geoms):
@@ -157,7 +158,22 @@ def main() -> int:
     ap.add_argument("--workload", default="sart_1024x1024x256_61tilts", choices=sorted(WORKLOADS))
     ap.add_argument("--numerics", default="exact", choices=["exact", "reference"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only an elementwise copy of 1 GiB (three launches) to calibrate HBM byte counters")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        n = 1 << 28   # 268M floats = 1 GiB read + 1 GiB written per launch
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "elementwise copy", "bytes_read_per_launch": 4 * n,
+                          "bytes_written_per_launch": 4 * n, "launches": 3}), flush=True)
+        return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
