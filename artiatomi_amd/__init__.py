@@ -2,11 +2,14 @@
 backprojection, 3D-CTF slab correction, WBP weighting) as gfx950 HIP kernels behind the C ABI
 of include/artia.h.  This package is the Python view of that library; the compute path is the
 HIP library (libartia.so) and nothing else."""
-from ._abi import GlobalParams, SartParams, TiltGeom, TiltParams, VolGeom, load  # noqa: F401
+from ._abi import CtfParams, GlobalParams, SartParams, TiltDefocus, TiltGeom, TiltParams, VolGeom, load  # noqa: F401
 from .core import (  # noqa: F401
     Acquisition,
     ArtiaContext,
     ArtiaError,
+    ctf_params,
+    defocus_distances,
+    defocus_minmax,
     hitpoints,
     projection_index_list,
     real_length,
@@ -17,7 +20,8 @@ from .core import (  # noqa: F401
 )
 
 __all__ = [
-    "Acquisition", "ArtiaContext", "ArtiaError", "GlobalParams", "SartParams", "TiltGeom", "TiltParams",
+    "Acquisition", "ArtiaContext", "ArtiaError", "CtfParams", "GlobalParams", "TiltDefocus",
+    "ctf_params", "defocus_distances", "defocus_minmax", "SartParams", "TiltGeom", "TiltParams",
     "VolGeom", "hitpoints", "load", "projection_index_list", "real_length", "sart_params",
     "subvolume_geometry", "tilt_geometry", "volume_geometry",
 ]

@@ -15,7 +15,8 @@ ARTIA_NUMERICS_EXACT = 0
 ARTIA_NUMERICS_REFERENCE = 1
 ARTIA_FILTER_POINT = 0
 ARTIA_FILTER_LINEAR = 1
-PROF_KERNELS = {"fp": 0, "bp": 1, "compare": 2, "crop": 3, "fill": 4}
+PROF_KERNELS = {"fp": 0, "bp": 1, "compare": 2, "crop": 3, "fill": 4, "fft": 5}
+WBP_METHODS = {"ramp": 0, "exact": 1, "contrast2": 2, "contrast10": 3, "contrast30": 4}
 
 _F3 = C.c_float * 3
 
@@ -65,6 +66,24 @@ class SartParams(C.Structure):
                 ("proj_scale", C.c_float)]
 
 
+class CtfParams(C.Structure):
+    """artia_ctf_params: Cs, Voltage, pixel size, amplitude contrast, WienerFilterNoiseLevel,
+    CTFBetaFac, PhaseFlipOnly, CTFSliceThickness, VolumeShift.z, IgnoreZShiftForCTF,
+    SwitchCTFDirectionForIMOD."""
+
+    _fields_ = [("cs_mm", C.c_float), ("voltage_kV", C.c_float), ("pixelsize_nm", C.c_float),
+                ("amp_contrast", C.c_float), ("wiener", C.c_float), ("beta_fac", C.c_float * 4),
+                ("phase_flip_only", C.c_int), ("slice_thickness_nm", C.c_float),
+                ("volume_shift_z", C.c_float), ("ignore_z_shift", C.c_int), ("switch_direction", C.c_int)]
+
+
+class TiltDefocus(C.Structure):
+    """artia_tilt_defocus: CtfFile min/max defocus (nm), astigmatism angle (deg), tilt angle."""
+
+    _fields_ = [("defocus_min_nm", C.c_float), ("defocus_max_nm", C.c_float),
+                ("astig_angle_deg", C.c_float), ("tilt_deg", C.c_float)]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _F = C.c_float
@@ -89,6 +108,8 @@ SIGNATURES = {
     "artia_geom_hitpoints": (_I, [C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI]),
     "artia_geom_real_length": (_I, [C.POINTER(VolGeom), C.POINTER(TiltGeom), _PF]),
     "artia_projection_index_list": (_I, [_PF, _I, _PI]),
+    "artia_geom_defocus_distances": (_I, [C.POINTER(VolGeom), C.POINTER(TiltGeom), _PF, _PF]),
+    "artia_geom_defocus_minmax": (_I, [_F, C.POINTER(TiltDefocus), C.POINTER(CtfParams), _PF, _PF]),
     "artia_fp_march": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _SZ, _P, _P, _PI, _PF]),
     "artia_fp_slicer": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _SZ, _P, _F, _F, _PI, _PF]),
     "artia_vol_traversal": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _I, _I, _SZ, _P, _PI, _PF]),
@@ -96,10 +117,18 @@ SIGNATURES = {
     "artia_crop_border": (_I, [_P, _I, _I, _SZ, _P, _PF, _PF, _PI, _PF]),
     "artia_bp": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _P, _I, _I, _SZ, _F, _I, _F, _F, _I, _PF]),
     "artia_make_square": (_I, [_P, _I, _I, _SZ, _P, _I, _P, _I, _I, _I, _I, _PF]),
+    "artia_ctf_apply": (_I, [_P, _P, _I, _SZ, C.POINTER(CtfParams), _F, _F, _F, _I, _PF]),
+    "artia_wbp_weight": (_I, [_P, _P, _I, _SZ, _F, _I, _PF]),
+    "artia_ctf_slab_filter": (_I, [_P, _P, _I, _I, _SZ, _P, _SZ, _I, _PI, _PF, _PF, C.POINTER(CtfParams), _F, _F,
+                                   _F, _I, _PF]),
+    "artia_wbp_filter": (_I, [_P, _P, _I, _I, _SZ, _P, _SZ, _F, _I, _PF]),
     "artia_profile_enable": (_I, [_P, _I]),
     "artia_profile_read": (_I, [_P, _I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "artia_sart_run": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _I, _PI, _P, _I, _I, _P,
                             C.POINTER(SartParams), _I, _PI, _PF]),
+    "artia_sart_ctf_run": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _PF, C.POINTER(TiltDefocus),
+                                _I, _PI, _P, _I, _I, _P, C.POINTER(SartParams), C.POINTER(CtfParams), _I, _PI,
+                                _PF]),
 }
 
 _lib = None

@@ -20,6 +20,7 @@ SOURCES = [
     "fp.hip",
     "bp.hip",
     "image.hip",
+    "spectral.hip",
     "sart.hip",
 ]
 

@@ -3,7 +3,8 @@
 `ArtiaContext` exposes one method per reference functor of src/EmSART/Kernels.h:37-245
 (FPKernel -> fp_march, SlicerKernel -> fp_slicer, VolTravLengthKernel -> vol_traversal,
 CompKernel -> compare, CropBorderKernel -> crop_border, BPKernel -> bp,
-CopyToSquareKernel -> make_square) over the C ABI in include/artia.h, taking torch CUDA
+CopyToSquareKernel -> make_square, CTFKernel -> ctf_apply, WbpWeightingKernel -> wbp_weight,
+plus the FFT slab filters of Reconstructor.cpp -> ctf_slab_filter / wbp_filter) over the C ABI in include/artia.h, taking torch CUDA
 tensors as device memory.  Like the reference functors, each call returns the elapsed
 milliseconds when `timed=True` (the call then synchronises); otherwise it returns None and
 does not block.  Errors raise `ArtiaError` carrying the library's message (the reference
@@ -18,7 +19,7 @@ from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
 from . import _abi
-from ._abi import GlobalParams, SartParams, TiltGeom, TiltParams, VolGeom
+from ._abi import CtfParams, GlobalParams, SartParams, TiltDefocus, TiltGeom, TiltParams, VolGeom
 
 
 class ArtiaError(RuntimeError):
@@ -91,6 +92,33 @@ def projection_index_list(tilts_deg: Sequence[float]) -> list:
     return list(out)
 
 
+def defocus_distances(full: VolGeom, tg: TiltGeom) -> tuple:
+    """[t_in, t_out) of the CTF slab loop along the tilt normal (Reconstructor.cpp:54-173)."""
+    a, b = C.c_float(), C.c_float()
+    _check(_abi.load().artia_geom_defocus_distances(C.byref(full), C.byref(tg), C.byref(a), C.byref(b)),
+           what="artia_geom_defocus_distances")
+    return a.value, b.value
+
+
+def defocus_minmax(ray: float, d: TiltDefocus, p: CtfParams) -> tuple:
+    """Defocus (nm) of the slab starting at `ray` (Reconstructor.cpp:178-201)."""
+    a, b = C.c_float(), C.c_float()
+    _check(_abi.load().artia_geom_defocus_minmax(float(ray), C.byref(d), C.byref(p), C.byref(a), C.byref(b)),
+           what="artia_geom_defocus_minmax")
+    return a.value, b.value
+
+
+def ctf_params(cs_mm: float = 2.7, voltage_kV: float = 300.0, pixelsize_nm: float = 1.0, amp_contrast: float = 0.0,
+               wiener: float = 0.1, beta_fac=(200.0, 0.0, 0.008, 0.0), phase_flip_only: bool = False,
+               slice_thickness_nm: float = 50.0, volume_shift_z: float = 0.0, ignore_z_shift: bool = False,
+               switch_direction: bool = False) -> CtfParams:
+    """CTF settings; the defaults are EmSART's (CTFSliceThickness 50 nm, CTFBetaFac
+    (200, 0, 0.008, 0), WienerFilterNoiseLevel 0.1, amplitude contrast 0)."""
+    return CtfParams(cs_mm, voltage_kV, pixelsize_nm, amp_contrast, wiener, _abi.farr(beta_fac, 4),
+                     int(bool(phase_flip_only)), slice_thickness_nm, volume_shift_z, int(bool(ignore_z_shift)),
+                     int(bool(switch_direction)))
+
+
 @dataclass
 class Acquisition:
     """Per-tilt geometry of a tilt series, as EmSART builds it from the marker file."""
@@ -104,6 +132,7 @@ class Acquisition:
     hitpoints: list = field(default_factory=list)
     real_lengths: list = field(default_factory=list)
     index_list: list = field(default_factory=list)
+    slab_ranges: list = field(default_factory=list)
 
     @classmethod
     def build(cls, full: VolGeom, width: int, height: int, tilts_deg, psis_deg=None, shifts=None,
@@ -118,6 +147,7 @@ class Acquisition:
             acq.hitpoints.append(hitpoints(full, g))
             acq.real_lengths.append(real_length(full, g))
         acq.index_list = projection_index_list(acq.tilts_deg)
+        acq.slab_ranges = [defocus_distances(full, g) for g in acq.geoms]
         return acq
 
     def c_arrays(self):
@@ -136,6 +166,16 @@ def sart_params(lambda_: float = 0.3, sirt_count: int = 1, oversampling: int = 1
 
 
 # ------------------------------------------------------------------ device operators
+
+def _cptr(t, name: str, n: int) -> int:
+    """Device pointer of an (n, n/2+1) complex64 half spectrum."""
+    import torch
+
+    if (not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.complex64 or not t.is_contiguous()
+            or tuple(t.shape) != (n, n // 2 + 1)):
+        raise ArtiaError(f"{name} must be a contiguous ({n}, {n // 2 + 1}) complex64 CUDA tensor")
+    return t.data_ptr()
+
 
 def _ptr(t, name: str, numel: Optional[int] = None) -> int:
     import torch
@@ -232,6 +272,35 @@ class ArtiaContext:
         return self._call("artia_make_square", W, H, W, _ptr(img, "img"), N, _ptr(out, "out", N * N),
                           int(border_x), int(border_y), int(bool(mirror_y)), int(bool(fill_zero)), timed=timed)
 
+    def ctf_apply(self, spec, params: CtfParams, defocus_min_nm, defocus_max_nm, angle_deg, for_fp=True,
+                  timed=False):
+        """In-place CTF multiply (for_fp) or Wiener inverse on an (n, n/2+1) half spectrum."""
+        n = spec.shape[0]
+        return self._call("artia_ctf_apply", _cptr(spec, "spectrum", n), n, n // 2 + 1, C.byref(params),
+                          float(defocus_min_nm), float(defocus_max_nm), float(angle_deg), int(bool(for_fp)),
+                          timed=timed)
+
+    def wbp_weight(self, spec, psi_deg, method="exact", timed=False):
+        """In-place WBP weighting of an (n, n/2+1) half spectrum."""
+        n = spec.shape[0]
+        return self._call("artia_wbp_weight", _cptr(spec, "spectrum", n), n, n // 2 + 1, float(psi_deg),
+                          _abi.WBP_METHODS[method], timed=timed)
+
+    def ctf_slab_filter(self, img, out, corners, params: CtfParams, defocus_min_nm, defocus_max_nm, angle_deg,
+                        for_fp=True, accumulate=False, cut_length=(0, 0), dim_length=(0, 0), timed=False):
+        """cropBorder(img) in place, then out (+)= cropBorder(iFFT(ctf * FFT(square(img))) / n^2)."""
+        H, W = img.shape
+        return self._call("artia_ctf_slab_filter", _ptr(img, "img"), W, H, W, _ptr(out, "out", W * H), W,
+                          int(bool(accumulate)), _abi.iarr(corners, 8), _abi.farr(cut_length, 2),
+                          _abi.farr(dim_length, 2), C.byref(params), float(defocus_min_nm), float(defocus_max_nm),
+                          float(angle_deg), int(bool(for_fp)), timed=timed)
+
+    def wbp_filter(self, img, out, psi_deg, method="exact", timed=False):
+        """out = iFFT(w(psi) * FFT(mirror_square(img))) / n^2, cropped to the image."""
+        H, W = img.shape
+        return self._call("artia_wbp_filter", _ptr(img, "img"), W, H, W, _ptr(out, "out", W * H), W,
+                          float(psi_deg), _abi.WBP_METHODS[method], timed=timed)
+
     def profile(self, enable: bool = True) -> None:
         """Enables (and resets) per-kernel-class event timing of driver launches."""
         _check(self.lib.artia_profile_enable(self.h, int(enable)), self.h, "artia_profile_enable")
@@ -257,4 +326,24 @@ class ArtiaContext:
                                      acq.height, _ptr(vol, "vol", vg.nx * vg.ny * vg.nz), C.byref(params),
                                      int(iterations), C.byref(empty), C.byref(ms) if timed else None)
         _check(st, self.h, "artia_sart_run")
+        return (ms.value if timed else None), bool(empty.value)
+
+    def sart_ctf_run(self, vg: VolGeom, acq: Acquisition, defocus, projections, vol, params: SartParams,
+                     ctf: CtfParams, iterations=1, volume_is_empty: bool = True, timed=False):
+        """SART with 3D-CTF slab correction; `defocus` is one TiltDefocus per tilt.  Returns
+        (ms or None, volume_is_empty)."""
+        n = len(acq.geoms)
+        if len(defocus) != n:
+            raise ArtiaError(f"expected {n} TiltDefocus entries, got {len(defocus)}")
+        tg, hp, rl, il = acq.c_arrays()
+        sr = (C.c_float * (2 * n))(*[v for r in acq.slab_ranges for v in r])
+        df = (TiltDefocus * n)(*defocus)
+        empty = C.c_int(1 if volume_is_empty else 0)
+        ms = C.c_float() if timed else None
+        st = self.lib.artia_sart_ctf_run(self.h, C.byref(vg), tg, hp, rl, sr, df, n, il,
+                                         _ptr(projections, "projections", n * acq.width * acq.height), acq.width,
+                                         acq.height, _ptr(vol, "vol", vg.nx * vg.ny * vg.nz), C.byref(params),
+                                         C.byref(ctf), int(iterations), C.byref(empty),
+                                         C.byref(ms) if timed else None)
+        _check(st, self.h, "artia_sart_ctf_run")
         return (ms.value if timed else None), bool(empty.value)

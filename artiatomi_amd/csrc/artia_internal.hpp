@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <cstdarg>
 #include <cstddef>
@@ -9,6 +10,21 @@
 #include <vector>
 
 #include "../../include/artia.h"
+
+struct artia_ctx;
+
+namespace artia {
+// Square N x N single-precision R2C/C2R plans with their square image and half-spectrum
+// buffers ((N/2+1) x N complex), rebuilt when N changes.
+struct FftPlans {
+    int N = 0;
+    hipfftHandle r2c = 0, c2r = 0;
+    float* square = nullptr;
+    float2* spec = nullptr;
+    artia_status ensure(artia_ctx* ctx, int n);
+    void release();
+};
+}  // namespace artia
 
 struct artia_ctx {
     int device = 0;
@@ -25,6 +41,7 @@ struct artia_ctx {
     std::vector<int> prof_marks;           // kernel class of each recorded start/stop pair
     double prof_ms[ARTIA_PROF_COUNT] = {};
     long long prof_launches[ARTIA_PROF_COUNT] = {};
+    artia::FftPlans fft;
 };
 
 namespace artia {
@@ -79,5 +96,22 @@ void launch_crop_border(artia_ctx* ctx, int W, int H, size_t pitch, float* img, 
 void launch_bp(artia_ctx* ctx, const VolG& vg, const TiltG& tg, float* vol, const float* img, int W, int H,
                size_t pitch, float lambda, int os, float distMin, float distMax, int filter);
 void launch_fill(artia_ctx* ctx, float* p, size_t n, float value);
+void launch_slicer(artia_ctx* ctx, const VolG& vg, const TiltG& tg, const float* vol, int W, int H, size_t pitch,
+                   float* proj, float tmin, float tmax);
+void launch_make_square(artia_ctx* ctx, int W, int H, size_t pitch, const float* in, int N, float* out, int bx, int by,
+                        int mirrorY, int fillZero);
+
+// Fourier-space operators (spectral.hip)
+float ctf_wavelength(float voltage_kV);
+artia_status run_ctf(artia_ctx* ctx, float2* spec, int N, size_t pitch, const artia_ctf_params& p, float dmin_nm,
+                     float dmax_nm, float angle_deg, int forFP);
+artia_status run_wbp_weight(artia_ctx* ctx, float2* spec, int N, size_t pitch, float psi_deg, int method);
+// cropBorder(img) -> makeSquare -> R2C -> ctf -> C2R -> out (+)= cropBorder(square / N^2)
+artia_status ctf_slab_filter(artia_ctx* ctx, float* img, int W, int H, size_t pitch, float* out, size_t out_pitch,
+                             int accumulate, const int corners[8], const float cut[2], const float dim[2],
+                             const artia_ctf_params& p, float dmin_nm, float dmax_nm, float angle_deg, int forFP);
+// makeSquare (reflect) -> R2C -> wbpWeighting -> C2R -> out = square / N^2
+artia_status wbp_filter(artia_ctx* ctx, const float* img, int W, int H, size_t pitch, float* out, size_t out_pitch,
+                        float psi_deg, int method);
 
 }  // namespace artia

@@ -341,6 +341,12 @@ void launch_march(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vo
     launch_march_impl(ctx, v, g, vol, make_roi(roi, W, H), pitch, proj, dist, false, 0.0f, 0.0f);
 }
 
+void launch_slicer(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vol, int W, int H, size_t pitch,
+                   float* proj, float tmin, float tmax) {
+    const int roi[4] = {0, 0, W, H};
+    launch_march_impl(ctx, v, g, vol, make_roi(roi, W, H), pitch, proj, nullptr, true, tmin, tmax);
+}
+
 void launch_vol_traversal(artia_ctx* ctx, const VolG& v, const TiltG& g, int W, int H, size_t pitch, float* dist,
                           int x0, int y0, int x1, int y1) {
     const int roiA[4] = {x0, y0, x1, y1};

@@ -221,6 +221,62 @@ artia_status artia_geom_real_length(const artia_vol_geom* full, const artia_tilt
     return ARTIA_OK;
 }
 
+// The detector pixel nearest to the volume centre's foot point on the detector plane fixes the
+// plane from which the eight bounding-box corners' distances along n are measured.
+artia_status artia_geom_defocus_distances(const artia_vol_geom* full, const artia_tilt_geom* g, float* t_in,
+                                          float* t_out) {
+    if (!full || !g || !t_in || !t_out) return ARTIA_ERR_INVALID_ARGUMENT;
+    const float* n = g->n;
+    float centre[3];
+    for (int i = 0; i < 3; ++i) centre[i] = full->bbox_min[i] + (full->dim[i] * full->voxel_size[i] * 0.5f);
+    const float nd = ((-n[0]) * g->d[0] - n[1] * g->d[1]) - n[2] * g->d[2];
+    float t = (n[0] * centre[0] + n[1] * centre[1]) + n[2] * centre[2];
+    t += nd;
+    t = std::fabs(t);
+    float foot[3];
+    for (int i = 0; i < 3; ++i) foot[i] = t * (-n[i]) + centre[i];
+    int pix[2];
+    for (int r = 0; r < 2; ++r) {
+        const float* m = g->det_matrix + 4 * r;
+        pix[r] = (int)std::round(((m[0] * foot[0] + m[1] * foot[1]) + m[2] * foot[2]) + 1.f * m[3]);
+    }
+    float origin[3];
+    for (int i = 0; i < 3; ++i) origin[i] = (g->d[i] + (float)pix[0] * g->u[i]) + (float)pix[1] * g->v[i];
+    const float no = ((-n[0]) * origin[0] - n[1] * origin[1]) - n[2] * origin[2];
+    float lo = 2 * -kDetectorDistance, hi = 2 * kDetectorDistance;
+    for (int c = 0; c < 8; ++c) {
+        const int bx = (c >> 2) & 1, by = (c >> 1) & 1, bz = c & 1;
+        const float* a = full->bbox_min;
+        const float* b = full->bbox_max;
+        float tc = (n[0] * (a[0] + bx * (b[0] - a[0])) + n[1] * (a[1] + by * (b[1] - a[1]))) +
+                   n[2] * (a[2] + bz * (b[2] - a[2]));
+        tc += no;
+        lo = std::min(lo, tc);
+        hi = std::max(hi, tc);
+    }
+    *t_in = lo;
+    *t_out = hi;
+    return ARTIA_OK;
+}
+
+// Defocus of the slab starting at `ray` (pixels along n): the slab's distance from the
+// detector in nm, quantised to the slice thickness, optionally corrected for the volume's
+// z shift projected on the tilt, and its sign flipped for IMOD's convention.
+artia_status artia_geom_defocus_minmax(float ray, const artia_tilt_defocus* d, const artia_ctf_params* p,
+                                       float* defocus_min_nm, float* defocus_max_nm) {
+    if (!d || !p || !defocus_min_nm || !defocus_max_nm || !(p->slice_thickness_nm > 0.0f))
+        return ARTIA_ERR_INVALID_ARGUMENT;
+    const float tilt = d->tilt_deg / 180.0f * (float)kPi;
+    const float T = p->slice_thickness_nm;
+    float z = ray + kDetectorDistance;
+    z = (std::round(z * p->pixelsize_nm * T) / T) + T / 2.0f;
+    if (!p->ignore_z_shift) z = z - (p->volume_shift_z * p->pixelsize_nm * std::cos(tilt));
+    if (p->switch_direction) z *= -1;
+    *defocus_min_nm = d->defocus_min_nm + z;
+    *defocus_max_nm = d->defocus_max_nm + z;
+    return ARTIA_OK;
+}
+
 // Fisher-Yates with glibc's rand() after srand(0) (additive feedback generator, degree 31,
 // separation 3, 310 warm-up draws), then the smallest |tilt| moved to the front.
 artia_status artia_projection_index_list(const float* tilt_deg, int count, int* out) {

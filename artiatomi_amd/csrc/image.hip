@@ -2,6 +2,7 @@
 // (cropBorder) and pad-to-square (makeSquare).  These are bandwidth-trivial; one thread per
 // pixel, 64x4 blocks so a wave covers 64 consecutive pixels of a row.
 #include "device_common.hpp"
+#include "image_ops.hpp"
 
 namespace artia {
 namespace {
@@ -9,11 +10,6 @@ namespace {
 struct F4 {
     float x, y, z, w;
 };
-struct I8 {
-    int v[8];
-};
-
-__device__ __forceinline__ float border_weight(float w) { return 1.0f - expf(-((w * w) * 9.0f)); }
 
 // Compare.cu:49-130: r = dist >= 1 ? ((b - (p/s)/dist*L)/L)*s : 0, times the border dimming.
 // The right/top branch evaluates a non-positive w and so zeroes a band of CropDim + Crop
@@ -52,58 +48,14 @@ __global__ void __launch_bounds__(256) compare_kernel(int W, int H, size_t pitch
     virt[i] = (((xa * xb) * ya) * yb) * error;
 }
 
-__device__ __forceinline__ float dist2(float ax, float ay, float bx, float by) {
-    return sqrtf((ax - bx) * (ax - bx) + (ay - by) * (ay - by));
-}
-
-// Distance from (x, y) to the segment v-w (Compare.cu:138-167)
-__device__ __forceinline__ float segment_distance(int vx, int vy, int wx, int wy, int x, int y) {
-    const float l2 = (float)((vx - wx) * (vx - wx) + (vy - wy) * (vy - wy));
-    if (l2 == 0) return dist2((float)vx, (float)vy, (float)x, (float)y);
-    const float x1 = (float)(x - vx), y1 = (float)(y - vy);
-    const float x2 = (float)(wx - vx), y2 = (float)(wy - vy);
-    const float t = (x1 * x2 + y1 * y2) / l2;
-    if (t < 0.0f) return dist2((float)x, (float)y, (float)vx, (float)vy);
-    if (t > 1.0f) return dist2((float)x, (float)y, (float)wx, (float)wy);
-    return dist2((float)x, (float)y, (float)vx + t * (float)(wx - vx), (float)vy + t * (float)(wy - vy));
-}
-
-// Compare.cu:170-277: zero outside the quadrilateral p1 p2 p4 p3 (strict integer cross
-// products), dim within cut + dim of its nearest edge inside.
+// Compare.cu:170-277 (weight in image_ops.hpp)
 __global__ void __launch_bounds__(256) crop_border_kernel(int W, int H, size_t pitch, float* __restrict__ img,
-                                                          float cx, float cy, float dx, float dy, I8 p) {
+                                                          CropParams c) {
     const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
     if (x >= W || y >= H) return;
-    const int p1x = p.v[0], p1y = p.v[1], p2x = p.v[2], p2y = p.v[3];
-    const int p3x = p.v[4], p3y = p.v[5], p4x = p.v[6], p4y = p.v[7];
-    float wx = 0.0f, wy = 0.0f;
-    if ((p2x - p1x) * (y - p1y) - (p2y - p1y) * (x - p1x) < 0 && (p3x - p4x) * (y - p4y) - (p3y - p4y) * (x - p4x) < 0 &&
-        (p1x - p3x) * (y - p3y) - (p1y - p3y) * (x - p3x) < 0 && (p4x - p2x) * (y - p2y) - (p4y - p2y) * (x - p2x) < 0) {
-        wx = 1.0f;
-        wy = 1.0f;
-        float mx = (float)(3 * W), my = (float)(3 * H);
-        mx = fminf(mx, segment_distance(p1x, p1y, p2x, p2y, x, y));
-        mx = fminf(mx, segment_distance(p4x, p4y, p3x, p3y, x, y));
-        mx = fminf(mx, (float)x);
-        mx = fminf(mx, (float)(W - x - 1));
-        my = fminf(my, segment_distance(p3x, p3y, p1x, p1y, x, y));
-        my = fminf(my, segment_distance(p2x, p2y, p4x, p4y, x, y));
-        my = fminf(my, (float)y);
-        my = fminf(my, (float)(H - y - 1));
-        if (mx < cx + dx) {
-            float w = (mx - cx) / dx;
-            if (w < 0) w = 0;
-            wx = border_weight(w);
-        }
-        if (my < cy + dy) {
-            float w = (my - cy) / dy;
-            if (w < 0) w = 0;
-            wy = border_weight(w);
-        }
-    }
     const size_t i = (size_t)y * pitch + x;
-    img[i] = (wx * wy) * img[i];
+    img[i] = crop_weight(x, y, W, H, c) * img[i];
 }
 
 // CopyToSquare.cu:37-92: W x H -> N x N, centred zero fill or mirror reflection, optional y flip.
@@ -147,10 +99,14 @@ void launch_compare(artia_ctx* ctx, int W, int H, size_t pitch, const float* rea
 
 void launch_crop_border(artia_ctx* ctx, int W, int H, size_t pitch, float* img, const float cut[2], const float dim[2],
                         const int p[8]) {
-    I8 q;
-    for (int i = 0; i < 8; i++) q.v[i] = p[i];
-    hipLaunchKernelGGL(crop_border_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, W, H, pitch, img, cut[0], cut[1],
-                       dim[0], dim[1], q);
+    hipLaunchKernelGGL(crop_border_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, W, H, pitch, img,
+                       make_crop_params(cut, dim, p));
+}
+
+void launch_make_square(artia_ctx* ctx, int W, int H, size_t pitch, const float* in, int N, float* out, int bx, int by,
+                        int mirrorY, int fillZero) {
+    hipLaunchKernelGGL(make_square_kernel, grid2d(N, N), dim3(64, 4), 0, ctx->stream, W, H, pitch, in, N, out, bx, by,
+                       mirrorY, fillZero);
 }
 
 }  // namespace artia
@@ -186,8 +142,7 @@ artia_status artia_make_square(artia_ctx* ctx, int width, int height, size_t pit
     if (!in || !out || width <= 0 || height <= 0 || pitch < (size_t)width || n <= 0)
         return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_make_square: invalid arguments");
     Timing t(ctx, ms);
-    hipLaunchKernelGGL(make_square_kernel, grid2d(n, n), dim3(64, 4), 0, ctx->stream, width, height, pitch, in, n, out,
-                       border_x, border_y, mirror_y, fill_zero);
+    launch_make_square(ctx, width, height, pitch, in, n, out, border_x, border_y, mirror_y, fill_zero);
     return t.finish();
 }
 

@@ -2,9 +2,16 @@
 // operators of this library, without the per-tilt host round trips of the reference
 // (residual D2H/H2D at EmSart.cpp:632,654 and the projection upload of
 // Reconstructor.cpp:1395).  All launches go to the context stream back to back.
+#include <cmath>
+
 #include "artia_internal.hpp"
 
 using namespace artia;
+
+namespace {
+// Upper bound on CTF slabs per tilt: a slab loop longer than this means a degenerate range/step.
+constexpr double kMaxSlabs = 1 << 16;
+}  // namespace
 
 extern "C" {
 
@@ -75,6 +82,120 @@ artia_status artia_sart_run(artia_ctx* ctx, const artia_vol_geom* vg, const arti
                 launch_bp(ctx, v, g, vol, res, width, height, pitch, lam, sp->oversampling, 0.0f, 9999999999999.0f,
                           ARTIA_FILTER_POINT);
                 prof_stop(ctx, pr);
+            }
+        }
+    }
+    *volume_is_empty = empty ? 1 : 0;
+    st = t.finish();
+    if (st != ARTIA_OK) return st;
+    return prof_collect(ctx);
+}
+
+artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                                const int* hitpoints, const float* real_lengths, const float* slab_range,
+                                const artia_tilt_defocus* defocus, int ntilts, const int* index_list,
+                                const float* projections, int width, int height, float* vol,
+                                const artia_sart_params* sp, const artia_ctf_params* ctf, int iterations,
+                                int* volume_is_empty, float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!vg || !tilts || !hitpoints || !real_lengths || !slab_range || !defocus || !index_list || !projections ||
+        !vol || !sp || !ctf || !volume_is_empty || ntilts <= 0 || width <= 0 || height <= 0 || iterations < 0)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: invalid arguments");
+    if (vg->nx % 4 != 0 || (reinterpret_cast<uintptr_t>(vol) & 15) != 0)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: volume x dimension must be a multiple "
+                                                     "of 4 and the volume 16-byte aligned");
+    if (sp->sirt_count <= 0 || sp->oversampling <= 0 || sp->oversampling > 64)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: bad SIRTCount or OverSampling");
+    if (!(ctf->slice_thickness_nm > 0.0f) || !(ctf->pixelsize_nm > 0.0f) || !(ctf->voltage_kV > 0.0f) ||
+        !(ctf->wiener >= 0.0f))
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: bad CTF parameters");
+    // Reconstructor.cpp:514: the slab step in pixels, accumulated in float as the reference does
+    const float step = ctf->slice_thickness_nm / ctf->pixelsize_nm;
+    for (int i = 0; i < ntilts; i++) {
+        if (index_list[i] < 0 || index_list[i] >= ntilts)
+            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: index list entry %d out of range", i);
+        if (!valid_geometry(vg, &tilts[i]))
+            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: invalid geometry for tilt %d", i);
+        const float a = slab_range[2 * i], b = slab_range[2 * i + 1];
+        if (!std::isfinite(a) || !std::isfinite(b) || !(step > 0.0f) || (double)(b - a) / step > (double)kMaxSlabs ||
+            (b > a && a + step == a))
+            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: slab range of tilt %d holds too many "
+                                                         "slabs",
+                        i);
+    }
+    const int S = sp->sirt_count;
+    const size_t npix = (size_t)width * (size_t)height;
+    artia_status st = ensure_workspace(ctx, npix * (size_t)(1 + S));
+    if (st != ARTIA_OK) return st;
+    float* dist = ctx->ws;
+    float* residual = ctx->ws + npix;
+    const VolG v = to_device(*vg);
+    const size_t pitch = (size_t)width;
+    const float lam = sp->lambda / (float)S;
+    // image corners for the forward slabs (Reconstructor.cpp:557-565): pA, pB, pC, pD
+    const int frame[8] = {0, height - 1, width - 1, height - 1, 0, 0, width - 1, 0};
+    bool empty = *volume_is_empty != 0;
+
+    Timing t(ctx, ms);
+    for (int iter = 0; iter < iterations; iter++) {
+        for (int sstep = 0; sstep < (ntilts + S - 1) / S; sstep++) {
+            for (int i = 0; i < S; i++) {
+                if (sstep * S + i >= ntilts) continue;
+                const int idx = index_list[sstep * S + i];
+                const TiltG g = to_device(tilts[idx]);
+                float* proj = residual + (size_t)i * npix;
+                int pr = prof_start(ctx, ARTIA_PROF_FILL);
+                launch_fill(ctx, proj, npix, 0.0f);
+                launch_fill(ctx, dist, npix, 0.0f);
+                prof_stop(ctx, pr);
+                if (!empty) {
+                    for (float ray = slab_range[2 * idx]; ray < slab_range[2 * idx + 1]; ray += step) {
+                        float dmin, dmax;
+                        artia_geom_defocus_minmax(ray, &defocus[idx], ctf, &dmin, &dmax);
+                        pr = prof_start(ctx, ARTIA_PROF_FILL);
+                        launch_fill(ctx, dist, npix, 0.0f);
+                        prof_stop(ctx, pr);
+                        pr = prof_start(ctx, ARTIA_PROF_FP);
+                        launch_slicer(ctx, v, g, vol, width, height, pitch, dist, ray, ray + step);
+                        prof_stop(ctx, pr);
+                        pr = prof_start(ctx, ARTIA_PROF_FFT);
+                        st = ctf_slab_filter(ctx, dist, width, height, pitch, proj, pitch, 1, frame, sp->cut_length,
+                                             sp->dim_length, *ctf, dmin, dmax, defocus[idx].astig_angle_deg, 1);
+                        prof_stop(ctx, pr);
+                        if (st != ARTIA_OK) return st;
+                    }
+                    pr = prof_start(ctx, ARTIA_PROF_FILL);
+                    launch_fill(ctx, dist, npix, 0.0f);
+                    prof_stop(ctx, pr);
+                }
+                pr = prof_start(ctx, ARTIA_PROF_FP);
+                launch_vol_traversal(ctx, v, g, width, height, pitch, dist, 0, 0, width, height);
+                prof_stop(ctx, pr);
+                pr = prof_start(ctx, ARTIA_PROF_COMPARE);
+                launch_compare(ctx, width, height, pitch, projections + (size_t)idx * npix, proj, dist,
+                               real_lengths[idx], sp->crop, sp->crop_dim, sp->proj_scale);
+                prof_stop(ctx, pr);
+            }
+            for (int i = 0; i < S; i++) {
+                if (sstep * S + i >= ntilts) continue;
+                empty = false;
+                const int idx = index_list[sstep * S + i];
+                const TiltG g = to_device(tilts[idx]);
+                float* res = residual + (size_t)i * npix;
+                const int* hp = hitpoints + 8 * (size_t)idx;
+                for (float ray = slab_range[2 * idx]; ray < slab_range[2 * idx + 1]; ray += step) {
+                    float dmin, dmax;
+                    artia_geom_defocus_minmax(ray, &defocus[idx], ctf, &dmin, &dmax);
+                    int pr = prof_start(ctx, ARTIA_PROF_FFT);
+                    st = ctf_slab_filter(ctx, res, width, height, pitch, dist, pitch, 0, hp, sp->cut_length,
+                                         sp->dim_length, *ctf, dmin, dmax, defocus[idx].astig_angle_deg, 0);
+                    prof_stop(ctx, pr);
+                    if (st != ARTIA_OK) return st;
+                    pr = prof_start(ctx, ARTIA_PROF_BP);
+                    launch_bp(ctx, v, g, vol, dist, width, height, pitch, lam, sp->oversampling, ray, ray + step,
+                              ARTIA_FILTER_LINEAR);
+                    prof_stop(ctx, pr);
+                }
             }
         }
     }

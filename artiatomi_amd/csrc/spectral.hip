@@ -1,0 +1,381 @@
+// Fourier-space operators on gfx950: the CTF multiply / Wiener inverse / phase flip (ctf.cu:80-240),
+// the WBP weighting (wbpWeighting.cu:48-144), and the fused per-slab image filters that replace
+// the reference's cuFFT/NPP glue (Reconstructor.cpp:557-583, 952-965, 1271-1319):
+//   slab filter : cropBorder (in place) -> makeSquare (zero fill) -> R2C -> ctf -> C2R ->
+//                 / N^2 with crop to W x H -> cropBorder -> store or accumulate
+//   WBP filter  : makeSquare (reflect) -> R2C -> wbpWeighting -> C2R -> / N^2 with crop
+// The FFTs are hipFFT single-precision plans cached per size in the context; the spectrum is the
+// (N/2+1) x N half spectrum, rows of N/2+1 complex values.
+#include <cmath>
+
+#include "device_common.hpp"
+#include "image_ops.hpp"
+
+namespace artia {
+namespace {
+
+constexpr float kPiF = 3.14159265358979323846f;
+
+struct CtfK {
+    float cs_m;         // Cs * 0.001 (mm -> m)
+    float lambda;       // electron wavelength (m), the reference's float constants
+    float freq_step;    // (1 / (2 * pixelsize)) / (N / 2)
+    float pc, ac;       // phase and amplitude contrast
+    float halfN;        // c_pixelcount * 0.5
+    float def0, def1, angle;   // metres, metres, radians
+    float wiener;
+    float beta[4];
+    int forFP, phaseFlip, N;
+};
+
+// Reference numerics: the kernel body's float operation order.
+__global__ void __launch_bounds__(256) ctf_ref_kernel(float2* __restrict__ spec, size_t pitch, CtfK k) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= k.N / 2 + 1 || y >= k.N) return;
+    float xpos = (float)x, ypos = (float)y;
+    if (ypos > (float)k.N * 0.5f) ypos = ((float)k.N - ypos) * -1.0f;
+    const float alpha = (xpos == 0) ? (kPiF * 0.5f) : atan2f(ypos, xpos);
+    const float beta = alpha - k.angle;
+    const float defocus = k.def0 + (1 - cosf(2 * beta)) * (k.def1 - k.def0);
+    float len = sqrtf(xpos * xpos + ypos * ypos);
+    len *= k.freq_step;
+    const float lam = k.lambda, Cs = k.cs_m;
+    const float m = (kPiF / 2.0f) * (Cs * lam * lam * lam * len * len * len * len - 2 * defocus * lam * len * len);
+    const float n = k.pc * sinf(m) + k.ac * cosf(m);
+    float2 r = spec[(size_t)y * pitch + x];
+    const float radius = sqrtf(xpos * xpos + ypos * ypos);
+    if (!k.phaseFlip && radius > k.beta[0]) {
+        const float l = len / 100000000.0f;
+        float e = expf((-k.beta[1] * l - k.beta[2] * l * l) - k.beta[3] * l * l * l);
+        if (k.forFP) {
+            e = fmaxf(e, 0.01f);
+            float val = n * e;
+            if (fabsf(val) < 0.0001f && val >= 0) val = 0.0001f;
+            if (fabsf(val) < 0.0001f && val < 0) val = -0.0001f;
+            r.x = r.x * -val;
+            r.y = r.y * -val;
+        } else {
+            e = fmaxf(e, k.wiener);
+            const float val = n * e;
+            r.x = r.x * -val / (val * val + k.wiener);
+            r.y = r.y * -val / (val * val + k.wiener);
+        }
+    }
+    if (k.phaseFlip && n >= 0) {
+        r.x = -r.x;
+        r.y = -r.y;
+    }
+    spec[(size_t)y * pitch + x] = r;
+}
+
+// Exact numerics: the CTF factor in double (the phase reaches hundreds of radians at Nyquist, where
+// a float phase carries ~1e-5 absolute error), applied to the float spectrum.
+__global__ void __launch_bounds__(256) ctf_exact_kernel(float2* __restrict__ spec, size_t pitch, CtfK k) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= k.N / 2 + 1 || y >= k.N) return;
+    double xpos = (double)x, ypos = (double)y;
+    if (ypos > (double)k.N * 0.5) ypos = ((double)k.N - ypos) * -1.0;
+    const double alpha = (xpos == 0) ? (double)(kPiF * 0.5f) : atan2(ypos, xpos);
+    const double beta = alpha - (double)k.angle;
+    const double def0 = k.def0, def1 = k.def1;
+    const double defocus = def0 + (1 - cos(2 * beta)) * (def1 - def0);
+    double len = sqrt(xpos * xpos + ypos * ypos);
+    len *= (double)k.freq_step;
+    const double lam = k.lambda, Cs = k.cs_m;
+    const double m = (double)(kPiF / 2.0f) * (Cs * lam * lam * lam * len * len * len * len - 2 * defocus * lam * len * len);
+    const double n = (double)k.pc * sin(m) + (double)k.ac * cos(m);
+    float2 r = spec[(size_t)y * pitch + x];
+    const double radius = sqrt(xpos * xpos + ypos * ypos);
+    double f = 1.0;
+    if (!k.phaseFlip && radius > (double)k.beta[0]) {
+        const double l = len / 100000000.0;
+        double e = exp((-(double)k.beta[1] * l - (double)k.beta[2] * l * l) - (double)k.beta[3] * l * l * l);
+        if (k.forFP) {
+            e = fmax(e, (double)0.01f);
+            double val = n * e;
+            if (fabs(val) < (double)0.0001f && val >= 0) val = (double)0.0001f;
+            if (fabs(val) < (double)0.0001f && val < 0) val = -(double)0.0001f;
+            f = -val;
+        } else {
+            const double W = k.wiener;
+            e = fmax(e, W);
+            const double val = n * e;
+            f = -val / (val * val + W);
+        }
+    }
+    if (k.phaseFlip && n >= 0) f = -f;
+    r.x = (float)((double)r.x * f);
+    r.y = (float)((double)r.y * f);
+    spec[(size_t)y * pitch + x] = r;
+}
+
+// wbpWeighting.cu:48-144 (sin/cos of the angle as IEEE functions rather than __sinf/__cosf)
+__global__ void __launch_bounds__(256) wbp_weight_kernel(float2* __restrict__ spec, size_t pitch, int N, float ang,
+                                                         int method, int exact) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= N / 2 + 1 || y >= N) return;
+    const unsigned pc = (unsigned)N;
+    float w = 1.0f;
+    if (exact) {
+        double xpos = (double)x, ypos = (double)y;
+        if (ypos > (double)pc * 0.5) ypos = ((double)pc - ypos) * -1.0;
+        const double sn = sin((double)ang), cs = cos((double)ang);
+        const double yr = sn * xpos + cs * ypos;
+        const double len = yr / (double)pc / 2.0;
+        double wd = 1.0;
+        if (method == 0) {
+            wd = fmin(fabs(len), 1.0);
+        } else if (method >= 2 && method <= 4) {
+            const double P[3][4] = {{1.000654227857550f, 0.006008581017124f, 0.004159659493151f, 0.000975903396538f},
+                                    {1.003784816598589f, 0.029016377161629f, 0.019582940715148f, 0.004559409669984f},
+                                    {0.999884616010943f, 0.000307646262566f, 0.004742915272196f, 0.004806551368900f}};
+            const double* q = P[method - 2];
+            if (len == 0) {
+                wd = 0;
+            } else {
+                const double L = log(fabs(len));
+                wd = ((q[0] + q[1] * L) + q[2] * L * L) + q[3] * L * L * L;
+            }
+            wd = fmax(0.0, fmin(wd, 1.0));
+        }
+        w = (float)wd;
+        float2 r = spec[(size_t)y * pitch + x];
+        r.x = (float)((double)r.x * wd);
+        r.y = (float)((double)r.y * wd);
+        spec[(size_t)y * pitch + x] = r;
+        return;
+    }
+    float xpos = (float)x, ypos = (float)y;
+    if (ypos > (float)pc * 0.5f) ypos = ((float)pc - ypos) * -1.0f;
+    const float temp = xpos;
+    const float sn = sinf(ang), cs = cosf(ang);
+    xpos = cs * xpos - sn * ypos;
+    ypos = sn * temp + cs * ypos;
+    const float len = ypos / (float)pc / 2.0f;
+    if (method == 0) {
+        w = fminf(fabsf(len), 1.0f);
+    } else if (method >= 2 && method <= 4) {
+        const float P[3][4] = {{1.000654227857550f, 0.006008581017124f, 0.004159659493151f, 0.000975903396538f},
+                               {1.003784816598589f, 0.029016377161629f, 0.019582940715148f, 0.004559409669984f},
+                               {0.999884616010943f, 0.000307646262566f, 0.004742915272196f, 0.004806551368900f}};
+        const float* q = P[method - 2];
+        if (len == 0) {
+            w = 0;
+        } else {
+            const float L = logf(fabsf(len));
+            w = ((q[0] + q[1] * L) + q[2] * L * L) + q[3] * L * L * L;
+        }
+        w = fmaxf(0.0f, fminf(w, 1.0f));
+    }
+    float2 r = spec[(size_t)y * pitch + x];
+    r.x *= w;
+    r.y *= w;
+    spec[(size_t)y * pitch + x] = r;
+}
+
+// out (+)= crop(x, y) * (square[by + y][bx + x] / N^2)  (nppiDivC + cropBorder + nppiAdd)
+__global__ void __launch_bounds__(256) finish_kernel(const float* __restrict__ sq, int N, int bx, int by, int W, int H,
+                                                     float* __restrict__ out, size_t pitch, float norm, int accumulate,
+                                                     int crop, CropParams cp) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= W || y >= H) return;
+    float v = sq[(size_t)(by + y) * N + (bx + x)] / norm;
+    if (crop) v = crop_weight(x, y, W, H, cp) * v;
+    const size_t i = (size_t)y * pitch + x;
+    out[i] = accumulate ? out[i] + v : v;
+}
+
+dim3 grid2d(int w, int h) { return dim3((unsigned)((w + 63) / 64), (unsigned)((h + 3) / 4)); }
+
+CtfK make_ctf(int N, const artia_ctf_params& p, float dmin_nm, float dmax_nm, float angle_deg, int forFP) {
+    CtfK k;
+    k.cs_m = p.cs_mm * 0.001f;
+    k.lambda = ctf_wavelength(p.voltage_kV);
+    const float ps = p.pixelsize_nm * powf(10.0f, -9.0f);
+    const float maxFreq = (float)(1.0 / (double)(ps * 2.0f));
+    k.freq_step = maxFreq / ((float)N / 2.0f);
+    k.ac = p.amp_contrast;
+    k.pc = sqrtf(1 - p.amp_contrast * p.amp_contrast);
+    k.halfN = (float)N * 0.5f;
+    k.def0 = dmin_nm * 0.000000001f;
+    k.def1 = dmax_nm * 0.000000001f;
+    k.angle = angle_deg / 180.0f * kPiF;
+    k.wiener = p.wiener;
+    for (int i = 0; i < 4; i++) k.beta[i] = p.beta_fac[i];
+    k.forFP = forFP;
+    k.phaseFlip = p.phase_flip_only;
+    k.N = N;
+    return k;
+}
+
+}  // namespace
+
+// ctf.cu:47-63 with the kernel's float constants (h = 6.63e-34, c = 3e8, E0 = 511 keV)
+float ctf_wavelength(float voltage_kV) {
+    const float h = (float)6.63E-34, c = (float)3.00E+08;
+    const int E0 = 511;
+    const float a = (2 * E0 * voltage_kV * 1000.0f * 1000.0f) + (voltage_kV * voltage_kV * 1000.0f * 1000.0f);
+    const double arg = (double)a * 1.602E-19 * 1.602E-19;
+    return (h * c) / sqrtf((float)arg);
+}
+
+artia_status FftPlans::ensure(artia_ctx* ctx, int n) {
+    if (N == n && r2c && c2r) return ARTIA_OK;
+    release();
+    if (hipfftPlan2d(&r2c, n, n, HIPFFT_R2C) != HIPFFT_SUCCESS ||
+        hipfftPlan2d(&c2r, n, n, HIPFFT_C2R) != HIPFFT_SUCCESS) {
+        release();
+        return fail(ctx, ARTIA_ERR_FFT, "hipfftPlan2d(%d x %d) failed", n, n);
+    }
+    artia_status st = check_hip(ctx, hipMalloc(&square, (size_t)n * n * sizeof(float)), "hipMalloc(square)");
+    if (st != ARTIA_OK) {
+        release();
+        return st;
+    }
+    st = check_hip(ctx, hipMalloc(&spec, (size_t)(n / 2 + 1) * n * sizeof(float2)), "hipMalloc(spectrum)");
+    if (st != ARTIA_OK) {
+        release();
+        return st;
+    }
+    N = n;
+    return ARTIA_OK;
+}
+
+void FftPlans::release() {
+    if (r2c) hipfftDestroy(r2c);
+    if (c2r) hipfftDestroy(c2r);
+    if (square) (void)hipFree(square);
+    if (spec) (void)hipFree(spec);
+    r2c = c2r = 0;
+    square = nullptr;
+    spec = nullptr;
+    N = 0;
+}
+
+artia_status run_ctf(artia_ctx* ctx, float2* spec, int N, size_t pitch, const artia_ctf_params& p, float dmin_nm,
+                     float dmax_nm, float angle_deg, int forFP) {
+    const CtfK k = make_ctf(N, p, dmin_nm, dmax_nm, angle_deg, forFP);
+    if (ctx->numerics == ARTIA_NUMERICS_REFERENCE)
+        hipLaunchKernelGGL(ctf_ref_kernel, grid2d(N / 2 + 1, N), dim3(64, 4), 0, ctx->stream, spec, pitch, k);
+    else
+        hipLaunchKernelGGL(ctf_exact_kernel, grid2d(N / 2 + 1, N), dim3(64, 4), 0, ctx->stream, spec, pitch, k);
+    return check_hip(ctx, hipGetLastError(), "ctf launch");
+}
+
+artia_status run_wbp_weight(artia_ctx* ctx, float2* spec, int N, size_t pitch, float psi_deg, int method) {
+    const float ang = -psi_deg / 180.0f * kPiF;
+    hipLaunchKernelGGL(wbp_weight_kernel, grid2d(N / 2 + 1, N), dim3(64, 4), 0, ctx->stream, spec, pitch, N, ang,
+                       method, ctx->numerics == ARTIA_NUMERICS_EXACT ? 1 : 0);
+    return check_hip(ctx, hipGetLastError(), "wbpWeighting launch");
+}
+
+artia_status ctf_slab_filter(artia_ctx* ctx, float* img, int W, int H, size_t pitch, float* out, size_t out_pitch,
+                             int accumulate, const int corners[8], const float cut[2], const float dim[2],
+                             const artia_ctf_params& p, float dmin_nm, float dmax_nm, float angle_deg, int forFP) {
+    const int N = W > H ? W : H;
+    artia_status st = ctx->fft.ensure(ctx, N);
+    if (st != ARTIA_OK) return st;
+    launch_crop_border(ctx, W, H, pitch, img, cut, dim, corners);
+    launch_make_square(ctx, W, H, pitch, img, N, ctx->fft.square, (N - W) / 2, (N - H) / 2, 0, 1);
+    if (hipfftSetStream(ctx->fft.r2c, ctx->stream) != HIPFFT_SUCCESS ||
+        hipfftSetStream(ctx->fft.c2r, ctx->stream) != HIPFFT_SUCCESS)
+        return fail(ctx, ARTIA_ERR_FFT, "hipfftSetStream failed");
+    if (hipfftExecR2C(ctx->fft.r2c, ctx->fft.square, (hipfftComplex*)ctx->fft.spec) != HIPFFT_SUCCESS)
+        return fail(ctx, ARTIA_ERR_FFT, "hipfftExecR2C failed");
+    st = run_ctf(ctx, ctx->fft.spec, N, (size_t)(N / 2 + 1), p, dmin_nm, dmax_nm, angle_deg, forFP);
+    if (st != ARTIA_OK) return st;
+    if (hipfftExecC2R(ctx->fft.c2r, (hipfftComplex*)ctx->fft.spec, ctx->fft.square) != HIPFFT_SUCCESS)
+        return fail(ctx, ARTIA_ERR_FFT, "hipfftExecC2R failed");
+    const CropParams cp = make_crop_params(cut, dim, corners);
+    hipLaunchKernelGGL(finish_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, ctx->fft.square, N, (N - W) / 2,
+                       (N - H) / 2, W, H, out, out_pitch, (float)(N * N), accumulate, 1, cp);
+    return check_hip(ctx, hipGetLastError(), "slab filter");
+}
+
+artia_status wbp_filter(artia_ctx* ctx, const float* img, int W, int H, size_t pitch, float* out, size_t out_pitch,
+                        float psi_deg, int method) {
+    const int N = W > H ? W : H;
+    artia_status st = ctx->fft.ensure(ctx, N);
+    if (st != ARTIA_OK) return st;
+    launch_make_square(ctx, W, H, pitch, img, N, ctx->fft.square, (N - W) / 2, (N - H) / 2, 0, 0);
+    if (hipfftSetStream(ctx->fft.r2c, ctx->stream) != HIPFFT_SUCCESS ||
+        hipfftSetStream(ctx->fft.c2r, ctx->stream) != HIPFFT_SUCCESS)
+        return fail(ctx, ARTIA_ERR_FFT, "hipfftSetStream failed");
+    if (hipfftExecR2C(ctx->fft.r2c, ctx->fft.square, (hipfftComplex*)ctx->fft.spec) != HIPFFT_SUCCESS)
+        return fail(ctx, ARTIA_ERR_FFT, "hipfftExecR2C failed");
+    st = run_wbp_weight(ctx, ctx->fft.spec, N, (size_t)(N / 2 + 1), psi_deg, method);
+    if (st != ARTIA_OK) return st;
+    if (hipfftExecC2R(ctx->fft.c2r, (hipfftComplex*)ctx->fft.spec, ctx->fft.square) != HIPFFT_SUCCESS)
+        return fail(ctx, ARTIA_ERR_FFT, "hipfftExecC2R failed");
+    const CropParams cp{};
+    hipLaunchKernelGGL(finish_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, ctx->fft.square, N, (N - W) / 2,
+                       (N - H) / 2, W, H, out, out_pitch, (float)(N * N), 0, 0, cp);
+    return check_hip(ctx, hipGetLastError(), "wbp filter");
+}
+
+}  // namespace artia
+
+using namespace artia;
+
+namespace {
+bool valid_ctf(const artia_ctf_params* p) {
+    return p && p->voltage_kV > 0.0f && p->pixelsize_nm > 0.0f && p->wiener >= 0.0f;
+}
+}  // namespace
+
+extern "C" {
+
+artia_status artia_ctf_apply(artia_ctx* ctx, void* spectrum, int n, size_t pitch, const artia_ctf_params* p,
+                             float defocus_min_nm, float defocus_max_nm, float angle_deg, int for_fp, float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!spectrum || n <= 1 || pitch < (size_t)(n / 2 + 1) || !valid_ctf(p))
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_ctf_apply: invalid arguments");
+    Timing t(ctx, ms);
+    artia_status st = run_ctf(ctx, (float2*)spectrum, n, pitch, *p, defocus_min_nm, defocus_max_nm, angle_deg, for_fp);
+    if (st != ARTIA_OK) return st;
+    return t.finish();
+}
+
+artia_status artia_wbp_weight(artia_ctx* ctx, void* spectrum, int n, size_t pitch, float psi_deg, int method,
+                              float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!spectrum || n <= 1 || pitch < (size_t)(n / 2 + 1) || method < 0 || method > 4)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_wbp_weight: invalid arguments");
+    Timing t(ctx, ms);
+    artia_status st = run_wbp_weight(ctx, (float2*)spectrum, n, pitch, psi_deg, method);
+    if (st != ARTIA_OK) return st;
+    return t.finish();
+}
+
+artia_status artia_ctf_slab_filter(artia_ctx* ctx, float* img, int width, int height, size_t pitch, float* out,
+                                   size_t out_pitch, int accumulate, const int corners[8], const float cut_length[2],
+                                   const float dim_length[2], const artia_ctf_params* p, float defocus_min_nm,
+                                   float defocus_max_nm, float angle_deg, int for_fp, float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!img || !out || !corners || !cut_length || !dim_length || !valid_ctf(p) || width <= 0 || height <= 0 ||
+        pitch < (size_t)width || out_pitch < (size_t)width)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_ctf_slab_filter: invalid arguments");
+    Timing t(ctx, ms);
+    artia_status st = ctf_slab_filter(ctx, img, width, height, pitch, out, out_pitch, accumulate, corners, cut_length,
+                                      dim_length, *p, defocus_min_nm, defocus_max_nm, angle_deg, for_fp);
+    if (st != ARTIA_OK) return st;
+    return t.finish();
+}
+
+artia_status artia_wbp_filter(artia_ctx* ctx, const float* img, int width, int height, size_t pitch, float* out,
+                              size_t out_pitch, float psi_deg, int method, float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!img || !out || width <= 0 || height <= 0 || pitch < (size_t)width || out_pitch < (size_t)width ||
+        method < 0 || method > 4)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_wbp_filter: invalid arguments");
+    Timing t(ctx, ms);
+    artia_status st = wbp_filter(ctx, img, width, height, pitch, out, out_pitch, psi_deg, method);
+    if (st != ARTIA_OK) return st;
+    return t.finish();
+}
+
+}  // extern "C"

@@ -105,6 +105,32 @@ typedef struct {
     float proj_scale;     /* ProjectionScaleFactor */
 } artia_sart_params;
 
+/* 3D-CTF settings: the CTF kernel constants (SetConstantValues(CTFKernel...),
+ * Kernels.cpp:1104-1132), the functor arguments (Kernels.cpp:395-401) and the slab keys of
+ * Reconstructor::GetDefocusMinMax (Reconstructor.cpp:178-201). */
+typedef struct {
+    float cs_mm;              /* Cs */
+    float voltage_kV;         /* Voltage */
+    float pixelsize_nm;       /* projection pixel size */
+    float amp_contrast;       /* amplitude contrast; the reference hard-codes 0 (Kernels.cpp:1111) */
+    float wiener;             /* WienerFilterNoiseLevel */
+    float beta_fac[4];        /* CTFBetaFac: low-frequency radius, envelope coefficients */
+    int phase_flip_only;      /* PhaseFlipOnly */
+    float slice_thickness_nm; /* CTFSliceThickness */
+    float volume_shift_z;     /* VolumeShift.z (voxels) */
+    int ignore_z_shift;       /* IgnoreZShiftForCTF */
+    int switch_direction;     /* SwitchCTFDirectionForIMOD */
+} artia_ctf_params;
+
+/* Per-tilt defocus (CtfFile rows: min, max, astigmatism angle; CtfFile.cpp:37-56) and the
+ * tilt angle entering the z-shift correction (marker tilt + AddTiltAngle). */
+typedef struct {
+    float defocus_min_nm;
+    float defocus_max_nm;
+    float astig_angle_deg;
+    float tilt_deg;
+} artia_tilt_defocus;
+
 /* ---------------------------------------------------------------- context */
 artia_status artia_create(int device, artia_ctx** out);
 artia_status artia_destroy(artia_ctx* ctx);
@@ -135,6 +161,14 @@ artia_status artia_geom_hitpoints(const artia_vol_geom* full, const artia_tilt_g
 artia_status artia_geom_real_length(const artia_vol_geom* full, const artia_tilt_geom* g, float* out);
 /* Projection::CreateProjectionIndexList(PLT_RANDOM_START_ZERO_TILT) (Projection.cpp:561-596) */
 artia_status artia_projection_index_list(const float* tilt_deg, int count, int* out);
+
+/* Reconstructor::GetDefocusDistances (Reconstructor.cpp:54-173): the range [t_in, t_out) of
+ * the CTF slab loop along the tilt normal, in pixels */
+artia_status artia_geom_defocus_distances(const artia_vol_geom* full, const artia_tilt_geom* g,
+                                          float* t_in, float* t_out);
+/* Reconstructor::GetDefocusMinMax (Reconstructor.cpp:178-201): defocus of the slab at `ray` */
+artia_status artia_geom_defocus_minmax(float ray, const artia_tilt_defocus* d, const artia_ctf_params* p,
+                                       float* defocus_min_nm, float* defocus_max_nm);
 
 /* ---------------------------------------------------------------- operators */
 /* FPKernel / march (Kernels.cpp:41-87, ForwardProjectionRayMarcher_TL.cu:57-160):
@@ -170,6 +204,31 @@ artia_status artia_make_square(artia_ctx* ctx, int width, int height, size_t pit
                                int n, float* out, int border_x, int border_y, int mirror_y,
                                int fill_zero, float* ms);
 
+/* CTFKernel / ctf (Kernels.cpp:395-435, ctf.cu:80-240) on an R2C half spectrum of an n x n
+ * image: (n/2+1) x n complex64 values, row pitch in complex elements.  for_fp selects the
+ * forward multiply by -sin(chi)*E, else the Wiener inverse; phase_flip_only flips the sign
+ * where sin(chi) >= 0.  Defocus in nm, astigmatism angle in degrees. */
+artia_status artia_ctf_apply(artia_ctx* ctx, void* spectrum, int n, size_t pitch, const artia_ctf_params* p,
+                             float defocus_min_nm, float defocus_max_nm, float angle_deg, int for_fp,
+                             float* ms);
+/* WbpWeightingKernel / wbpWeighting (Kernels.cpp:453-488, wbpWeighting.cu:48-144) on an R2C half
+ * spectrum; method 0 ramp, 1 exact (no weighting), 2/3/4 contrast 2/10/30 */
+artia_status artia_wbp_weight(artia_ctx* ctx, void* spectrum, int n, size_t pitch, float psi_deg,
+                              int method, float* ms);
+/* One CTF slab filter (Reconstructor.cpp:557-583 forward, :952-965 backward), n = max(W, H):
+ * cropBorder(img, corners) in place -> makeSquare(zero fill) -> R2C -> ctf -> C2R ->
+ * out = cropBorder(square / n^2) (accumulate = 0) or out += it (accumulate = 1).  The FFT
+ * plans and n x n buffers are cached in the context. */
+artia_status artia_ctf_slab_filter(artia_ctx* ctx, float* img, int width, int height, size_t pitch,
+                                   float* out, size_t out_pitch, int accumulate, const int corners[8],
+                                   const float cut_length[2], const float dim_length[2],
+                                   const artia_ctf_params* p, float defocus_min_nm, float defocus_max_nm,
+                                   float angle_deg, int for_fp, float* ms);
+/* WBP projection filter (Reconstructor.cpp:1271-1319): makeSquare(mirror) -> R2C ->
+ * wbpWeighting(psi, method) -> C2R -> out = square / n^2 cropped to W x H */
+artia_status artia_wbp_filter(artia_ctx* ctx, const float* img, int width, int height, size_t pitch,
+                              float* out, size_t out_pitch, float psi_deg, int method, float* ms);
+
 /* ---------------------------------------------------------------- drivers
  * EmSART's ordered-subset SART loop (EmSart.cpp:579-675) without CTF, device resident:
  * for every subset, per tilt FP (volTraversalLength while the volume is empty) -> compare
@@ -182,6 +241,17 @@ artia_status artia_sart_run(artia_ctx* ctx, const artia_vol_geom* vg, const arti
                             const int* index_list, const float* projections, int width, int height,
                             float* vol, const artia_sart_params* sp, int iterations,
                             int* volume_is_empty, float* ms);
+/* The same loop with 3D-CTF correction (Reconstructor.cpp:496-645 forward, :920-969 backward):
+ * per tilt, slabs ray = t_in, t_in + T/ps, ... < t_out (slab_range[2*tilt] = t_in, t_out from
+ * artia_geom_defocus_distances), each sliced, CTF-filtered and summed into the forward
+ * projection, and each slab of the residual Wiener-filtered and backprojected with LINEAR
+ * sampling inside its distance gate.  defocus[tilt] gives each tilt's defocus. */
+artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                                const int* hitpoints, const float* real_lengths, const float* slab_range,
+                                const artia_tilt_defocus* defocus, int ntilts, const int* index_list,
+                                const float* projections, int width, int height, float* vol,
+                                const artia_sart_params* sp, const artia_ctf_params* ctf, int iterations,
+                                int* volume_is_empty, float* ms);
 
 /* ---------------------------------------------------------------- profiling
  * With profiling enabled, artia_sart_run brackets each kernel launch with HIP events on the
@@ -195,7 +265,8 @@ typedef enum {
     ARTIA_PROF_COMPARE = 2,   /* compare */
     ARTIA_PROF_CROP = 3,      /* cropBorder */
     ARTIA_PROF_FILL = 4,      /* projection / distance resets */
-    ARTIA_PROF_COUNT = 5
+    ARTIA_PROF_FFT = 5,       /* CTF / WBP spectral filters (FFT pair, multiply, crop) */
+    ARTIA_PROF_COUNT = 6
 } artia_prof_kernel;
 artia_status artia_profile_enable(artia_ctx* ctx, int enable);
 artia_status artia_profile_read(artia_ctx* ctx, int kernel, double* total_ms, long long* launches);

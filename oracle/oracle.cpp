@@ -666,6 +666,165 @@ void make_square(int W, int H, size_t pitch, const T* in, int N, T* out, int bx,
     }
 }
 
+// ---------------------------------------------------------------- CTF and WBP weighting
+
+float ctf_wavelength(float voltage_kV) {
+    const float h = (float)6.63E-34, c = (float)3.00E+08;
+    const int E0 = 511;
+    const float V = voltage_kV;
+    const float a = (2 * E0 * V * 1000.0f * 1000.0f) + (V * V * 1000.0f * 1000.0f);
+    const double arg = (double)a * 1.602E-19 * 1.602E-19;   // double literals in the macro
+    return (h * c) / std::sqrt((float)arg);
+}
+
+CtfConsts ctf_consts(float cs_mm, float voltage_kV, float pixelsize_nm, int N, float amp_contrast) {
+    CtfConsts k;
+    k.cs_mm = cs_mm;
+    k.voltage_kV = voltage_kV;
+    k.pixelsize_m = pixelsize_nm * std::pow(10.0f, -9.0f);
+    k.pixelcount = (float)N;
+    const float maxFreq = (float)(1.0 / (double)(k.pixelsize_m * 2.0f));
+    k.freq_step = maxFreq / (k.pixelcount / 2.0f);
+    k.amp_contrast = amp_contrast;
+    k.phase_contrast = std::sqrt(1 - amp_contrast * amp_contrast);
+    return k;
+}
+
+template <class T>
+void ctf_apply(T* spec, int N, size_t pitch, const CtfConsts& k, float defocusMin_nm, float defocusMax_nm,
+               float angle_deg, bool forFP, bool phaseFlipOnly, float wiener, const float betaFac[4]) {
+    const float pi_f = 3.14159265358979323846f;
+    const T def0 = (T)(defocusMin_nm * 0.000000001f), def1 = (T)(defocusMax_nm * 0.000000001f);
+    const T angle = (T)(angle_deg / 180.0f * (float)3.14159265358979323846);
+    const T lam = (T)ctf_wavelength(k.voltage_kV);
+    const T Cs = (T)(k.cs_mm * 0.001f);
+    const T W = wiener;
+    const T pc = k.phase_contrast, ac = k.amp_contrast;
+    const int nx = N / 2 + 1;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < N; y++) {
+        for (int x = 0; x < nx; x++) {
+            T xpos = (T)x, ypos = (T)y;
+            if (ypos > (T)k.pixelcount * T(0.5)) ypos = ((T)k.pixelcount - ypos) * T(-1);
+            T alpha = (xpos == 0) ? (T)(pi_f * 0.5f) : std::atan2(ypos, xpos);
+            T beta = alpha - angle;
+            T defocus = def0 + (T(1) - std::cos(T(2) * beta)) * (def1 - def0);
+            T len = std::sqrt(xpos * xpos + ypos * ypos);
+            len *= (T)k.freq_step;
+            T m = (T)(pi_f / 2.0f) * (Cs * lam * lam * lam * len * len * len * len - T(2) * defocus * lam * len * len);
+            T n = pc * std::sin(m) + ac * std::cos(m);
+            T* r = spec + 2 * ((size_t)y * pitch + (size_t)x);
+            const T radius = std::sqrt(xpos * xpos + ypos * ypos);
+            if (!phaseFlipOnly && radius > (T)betaFac[0]) {
+                T l = len / T(100000000.0f);
+                T e = std::exp((-(T)betaFac[1] * l - (T)betaFac[2] * l * l) - (T)betaFac[3] * l * l * l);
+                if (forFP) {
+                    e = std::max(e, (T)0.01f);
+                    T val = n * e;
+                    if (std::fabs(val) < (T)0.0001f && val >= 0) val = (T)0.0001f;
+                    if (std::fabs(val) < (T)0.0001f && val < 0) val = (T)-0.0001f;
+                    r[0] = r[0] * -val;
+                    r[1] = r[1] * -val;
+                } else {
+                    e = std::max(e, W);
+                    T val = n * e;
+                    r[0] = r[0] * -val / (val * val + W);
+                    r[1] = r[1] * -val / (val * val + W);
+                }
+            }
+            if (phaseFlipOnly && n >= 0) {
+                r[0] = -r[0];
+                r[1] = -r[1];
+            }
+        }
+    }
+}
+
+template <class T>
+void wbp_weight(T* spec, int N, size_t pitch, float psi_deg, int method) {
+    const T ang = (T)(-psi_deg / 180.0f * (float)3.14159265358979323846);
+    const T sn = std::sin(ang), cs = std::cos(ang);
+    const int nx = N / 2 + 1;
+    const unsigned pc = (unsigned)N;
+    struct Poly { float p1, p2, p3, p4; };
+    const Poly polys[3] = {{1.000654227857550f, 0.006008581017124f, 0.004159659493151f, 0.000975903396538f},
+                           {1.003784816598589f, 0.029016377161629f, 0.019582940715148f, 0.004559409669984f},
+                           {0.999884616010943f, 0.000307646262566f, 0.004742915272196f, 0.004806551368900f}};
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < N; y++) {
+        for (int x = 0; x < nx; x++) {
+            T xpos = (T)x, ypos = (T)y;
+            if (ypos > (T)pc * T(0.5)) ypos = ((T)pc - ypos) * T(-1);
+            const T temp = xpos;
+            xpos = cs * xpos - sn * ypos;
+            ypos = sn * temp + cs * ypos;
+            const T len = ypos / (T)pc / T(2);
+            T w = 1;
+            if (method == 0) {
+                w = std::min(std::fabs(len), T(1));
+            } else if (method >= 2 && method <= 4) {
+                const Poly& q = polys[method - 2];
+                if (len == 0) {
+                    w = 0;
+                } else {
+                    const T L = std::log(std::fabs(len));
+                    w = (((T)q.p1 + (T)q.p2 * L) + (T)q.p3 * L * L) + (T)q.p4 * L * L * L;
+                }
+                w = std::max(T(0), std::min(w, T(1)));
+            }
+            T* r = spec + 2 * ((size_t)y * pitch + (size_t)x);
+            r[0] *= w;
+            r[1] *= w;
+        }
+    }
+}
+
+void defocus_distances(const VolGeom& full, const TiltGeom& g, int W, int H, float& t_in, float& t_out) {
+    const float* n = g.n;
+    const float* d = g.d;
+    const float* mn = full.bbMin;
+    const float* mx = full.bbMax;
+    const float cx = mn[0] + (full.dim[0] * full.vs[0] * 0.5f), cy = mn[1] + (full.dim[1] * full.vs[1] * 0.5f),
+                cz = mn[2] + (full.dim[2] * full.vs[2] * 0.5f);
+    float t = (n[0] * cx + n[1] * cy) + n[2] * cz;
+    t += ((-n[0] * d[0]) - n[1] * d[1]) - n[2] * d[2];
+    t = std::fabs(t);
+    const float h[3] = {t * (-n[0]) + cx, t * (-n[1]) + cy, t * (-n[2]) + cz};
+    float hp[2];
+    for (int r = 0; r < 2; r++)
+        hp[r] = ((g.M[4 * r] * h[0] + g.M[4 * r + 1] * h[1]) + g.M[4 * r + 2] * h[2]) + 1.f * g.M[4 * r + 3];
+    const int hitX = (int)std::round(hp[0]), hitY = (int)std::round(hp[1]);
+    float pos[3];
+    for (int i = 0; i < 3; i++) pos[i] = (d[i] + (float)hitX * g.u[i]) + (float)hitY * g.v[i];
+    (void)W;
+    (void)H;
+    t_in = 2 * -kDist;
+    t_out = 2 * kDist;
+    for (int x = 0; x <= 1; x++)
+        for (int y = 0; y <= 1; y++)
+            for (int z = 0; z <= 1; z++) {
+                float tt = (n[0] * (mn[0] + x * (mx[0] - mn[0])) + n[1] * (mn[1] + y * (mx[1] - mn[1]))) +
+                           n[2] * (mn[2] + z * (mx[2] - mn[2]));
+                tt += ((-n[0] * pos[0]) - n[1] * pos[1]) - n[2] * pos[2];
+                if (tt < t_in) t_in = tt;
+                if (tt > t_out) t_out = tt;
+            }
+}
+
+void defocus_min_max(float ray, float tilt_deg, float defMin_nm, float defMax_nm, const CtfSlabParams& p,
+                     float& outMin, float& outMax) {
+    const float tilt = tilt_deg / 180.0f * (float)3.14159265358979323846;
+    float dist0 = ray + kDist;
+    const float T = p.slice_thickness_nm;
+    if (p.ignore_z_shift)
+        dist0 = (std::round(dist0 * p.pixelsize_nm * T) / T) + T / 2.0f;
+    else
+        dist0 = (std::round(dist0 * p.pixelsize_nm * T) / T) + T / 2.0f - (p.volume_shift_z * p.pixelsize_nm * std::cos(tilt));
+    if (p.switch_direction) dist0 *= -1;
+    outMin = defMin_nm + dist0;
+    outMax = defMax_nm + dist0;
+}
+
 template <class T>
 void sart(const VolGeom& vg, const VolGeom& full, const std::vector<TiltGeom>& tilts,
           const std::vector<int>& indexList, const T* projections, int W, int H, T* vol,
@@ -722,6 +881,13 @@ void sart(const VolGeom& vg, const VolGeom& full, const std::vector<TiltGeom>& t
     template void sart<T>(const VolGeom&, const VolGeom&, const std::vector<TiltGeom>&, const std::vector<int>&, \
                           const T*, int, int, T*, const SartParams&, int, bool*);
 
+#define ARTIA_ORACLE_INSTANTIATE_SPEC(T)                                                                  \
+    template void ctf_apply<T>(T*, int, size_t, const CtfConsts&, float, float, float, bool, bool, float,   \
+                               const float*);                                                              \
+    template void wbp_weight<T>(T*, int, size_t, float, int);
+
+ARTIA_ORACLE_INSTANTIATE_SPEC(float)
+ARTIA_ORACLE_INSTANTIATE_SPEC(double)
 ARTIA_ORACLE_INSTANTIATE(float)
 ARTIA_ORACLE_INSTANTIATE(double)
 

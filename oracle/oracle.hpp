@@ -199,6 +199,48 @@ template <class T>
 void make_square(int W, int H, size_t pitch, const T* in, int N, T* out, int bx, int by,
                  bool mirrorY, bool fillZero);
 
+// ctf.cu:47-63 electron wavelength (metres) for an acceleration voltage in kV, evaluated with the
+// kernel's float constants h = 6.63e-34 and c = 3e8.
+float ctf_wavelength(float voltage_kV);
+
+// CTF constants of SetConstantValues(CTFKernel, ...) (Kernels.cpp:1104-1132) and the functor's
+// unit conversions (Kernels.cpp:395-401).
+struct CtfConsts {
+    float cs_mm;             // c_cs
+    float voltage_kV;        // c_voltage
+    float pixelsize_m;       // c_pixelsize = nm * 1e-9
+    float pixelcount;        // c_pixelcount = N
+    float freq_step;         // c_freqStepSize = (1 / (2 * pixelsize)) / (N / 2)
+    float phase_contrast;    // c_phaseContrast = sqrt(1 - ampContrast^2)
+    float amp_contrast;      // c_ampContrast (0 in the reference)
+};
+CtfConsts ctf_consts(float cs_mm, float voltage_kV, float pixelsize_nm, int N, float amp_contrast);
+
+// ctf.cu:80-240: multiplies the R2C half-spectrum (N/2+1 x N complex, row pitch in complex
+// elements) by -(sin chi * envelope) for the forward projection, by its Wiener inverse for the
+// backprojection, or flips the phase where sin chi >= 0.  defocus in nm, angle in degrees.
+template <class T>
+void ctf_apply(T* spec, int N, size_t pitch, const CtfConsts& k, float defocusMin_nm, float defocusMax_nm,
+               float angle_deg, bool forFP, bool phaseFlipOnly, float wiener, const float betaFac[4]);
+
+// wbpWeighting.cu:48-144 with the functor's angle conversion (Kernels.cpp:453-456):
+// method 0 ramp, 1 exact (identity), 2/3/4 contrast 2/10/30.
+template <class T>
+void wbp_weight(T* spec, int N, size_t pitch, float psi_deg, int method);
+
+// Reconstructor::GetDefocusDistances (Reconstructor.cpp:54-173): slab-loop range along n.
+void defocus_distances(const VolGeom& full, const TiltGeom& g, int W, int H, float& t_in, float& t_out);
+// Reconstructor::GetDefocusMinMax (Reconstructor.cpp:178-201).
+struct CtfSlabParams {
+    float slice_thickness_nm;    // CTFSliceThickness
+    float pixelsize_nm;
+    float volume_shift_z;        // VolumeShift.z
+    int ignore_z_shift;          // IgnoreZShiftForCTF
+    int switch_direction;        // SwitchCTFDirectionForIMOD
+};
+void defocus_min_max(float ray, float tilt_deg, float defMin_nm, float defMax_nm, const CtfSlabParams& p,
+                     float& outMin, float& outMax);
+
 // EmSart.cpp:579-675 with Reconstructor.cpp:429-490, 871-887, 1386-1405 (SART, no CTF)
 // Same memory layout as artia_sart_params (include/artia.h).
 struct SartParams {

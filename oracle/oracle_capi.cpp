@@ -74,6 +74,32 @@ void oracle_glibc_rand(unsigned seed, int count, int* out) { glibc_rand_sequence
         *volumeIsEmpty = empty ? 1 : 0;                                                                         \
     }
 
+float oracle_ctf_wavelength(float kV) { return ctf_wavelength(kV); }
+void oracle_ctf_consts(float cs_mm, float kV, float ps_nm, int N, float amp, CtfConsts* out) {
+    *out = ctf_consts(cs_mm, kV, ps_nm, N, amp);
+}
+void oracle_defocus_distances(const VolGeom* full, const TiltGeom* g, int W, int H, float* tin, float* tout) {
+    defocus_distances(*full, *g, W, H, *tin, *tout);
+}
+void oracle_defocus_min_max(float ray, float tilt_deg, float dmin, float dmax, const CtfSlabParams* p, float* omin,
+                            float* omax) {
+    defocus_min_max(ray, tilt_deg, dmin, dmax, *p, *omin, *omax);
+}
+void oracle_ctf_apply_f32(float* spec, int N, size_t pitch, const CtfConsts* k, float dmin, float dmax, float ang,
+                          int fp, int pf, float w, const float* beta) {
+    ctf_apply<float>(spec, N, pitch, *k, dmin, dmax, ang, fp != 0, pf != 0, w, beta);
+}
+void oracle_ctf_apply_f64(double* spec, int N, size_t pitch, const CtfConsts* k, float dmin, float dmax, float ang,
+                          int fp, int pf, float w, const float* beta) {
+    ctf_apply<double>(spec, N, pitch, *k, dmin, dmax, ang, fp != 0, pf != 0, w, beta);
+}
+void oracle_wbp_weight_f32(float* spec, int N, size_t pitch, float psi, int method) {
+    wbp_weight<float>(spec, N, pitch, psi, method);
+}
+void oracle_wbp_weight_f64(double* spec, int N, size_t pitch, float psi, int method) {
+    wbp_weight<double>(spec, N, pitch, psi, method);
+}
+
 ORACLE_KERNELS(f32, float)
 ORACLE_KERNELS(f64, double)
 

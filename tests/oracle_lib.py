@@ -15,6 +15,23 @@ import numpy as np
 from artiatomi_amd._abi import GlobalParams, SartParams, TiltGeom, TiltParams, VolGeom
 
 ROOT = Path(__file__).resolve().parents[1]
+
+
+class CtfConsts(C.Structure):
+    """Oracle CtfConsts: c_cs, c_voltage, c_pixelsize, c_pixelcount, c_freqStepSize,
+    c_phaseContrast, c_ampContrast."""
+
+    _fields_ = [("cs_mm", C.c_float), ("voltage_kV", C.c_float), ("pixelsize_m", C.c_float),
+                ("pixelcount", C.c_float), ("freq_step", C.c_float), ("phase_contrast", C.c_float),
+                ("amp_contrast", C.c_float)]
+
+
+class CtfSlabParams(C.Structure):
+    """Oracle CtfSlabParams: CTFSliceThickness, pixel size, VolumeShift.z, IgnoreZShiftForCTF,
+    SwitchCTFDirectionForIMOD."""
+
+    _fields_ = [("slice_thickness_nm", C.c_float), ("pixelsize_nm", C.c_float), ("volume_shift_z", C.c_float),
+                ("ignore_z_shift", C.c_int), ("switch_direction", C.c_int)]
 ORACLE_SO = ROOT / "oracle" / "liboracle.so"
 
 _lib = None
@@ -42,6 +59,15 @@ def load() -> C.CDLL:
                                                                C.c_int, C.c_int, C.c_int]
         getattr(lib, f"oracle_sart_{suf}").argtypes = [P, P, P, C.c_int, P, P, C.c_int, C.c_int, P, P, C.c_int, P]
     lib.oracle_real_length.restype = C.c_float
+    lib.oracle_ctf_wavelength.restype = C.c_float
+    lib.oracle_ctf_wavelength.argtypes = [C.c_float]
+    lib.oracle_ctf_consts.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_float, P]
+    lib.oracle_defocus_distances.argtypes = [P, P, C.c_int, C.c_int, P, P]
+    lib.oracle_defocus_min_max.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, P, P, P]
+    for suf in ("f32", "f64"):
+        getattr(lib, f"oracle_ctf_apply_{suf}").argtypes = [P, C.c_int, C.c_size_t, P, C.c_float, C.c_float,
+                                                             C.c_float, C.c_int, C.c_int, C.c_float, P]
+        getattr(lib, f"oracle_wbp_weight_{suf}").argtypes = [P, C.c_int, C.c_size_t, C.c_float, C.c_int]
     lib.oracle_set_threads.argtypes = [C.c_int]
     lib.oracle_get_threads.restype = C.c_int
     _lib = lib
@@ -185,3 +211,57 @@ def sart(vg, full, geoms, index_list_, projections, vol, params: SartParams, ite
 
 def set_threads(n: int) -> None:
     load().oracle_set_threads(n)
+
+
+# ------------------------------------------------------------------ CTF / WBP weighting
+
+def ctf_wavelength(kV: float) -> float:
+    return load().oracle_ctf_wavelength(C.c_float(kV))
+
+
+def ctf_consts(cs_mm, kV, ps_nm, N, amp=0.0) -> CtfConsts:
+    k = CtfConsts()
+    load().oracle_ctf_consts(C.c_float(cs_mm), C.c_float(kV), C.c_float(ps_nm), int(N), C.c_float(amp), ref(k))
+    return k
+
+
+def _interleaved(spec, dtype):
+    a = np.ascontiguousarray(spec)
+    out = np.empty(a.shape + (2,), dtype)
+    out[..., 0] = a.real
+    out[..., 1] = a.imag
+    return out
+
+
+def ctf_apply(spec, consts: CtfConsts, dmin_nm, dmax_nm, angle_deg, for_fp=True, phase_flip=False, wiener=0.1,
+              beta=(200.0, 0.0, 0.008, 0.0), dtype=np.float64):
+    """CTF multiply of an (N, N/2+1) half spectrum; returns a complex array."""
+    N = spec.shape[0]
+    assert spec.shape == (N, N // 2 + 1)
+    buf = _interleaved(spec, dtype)
+    getattr(load(), f"oracle_ctf_apply_{_suffix(dtype)}")(_p(buf), N, N // 2 + 1, ref(consts), C.c_float(dmin_nm),
+                                                          C.c_float(dmax_nm), C.c_float(angle_deg), int(for_fp),
+                                                          int(phase_flip), C.c_float(wiener),
+                                                          (C.c_float * 4)(*beta))
+    return buf[..., 0] + 1j * buf[..., 1]
+
+
+def wbp_weight(spec, psi_deg, method, dtype=np.float64):
+    N = spec.shape[0]
+    assert spec.shape == (N, N // 2 + 1)
+    buf = _interleaved(spec, dtype)
+    getattr(load(), f"oracle_wbp_weight_{_suffix(dtype)}")(_p(buf), N, N // 2 + 1, C.c_float(psi_deg), int(method))
+    return buf[..., 0] + 1j * buf[..., 1]
+
+
+def defocus_distances(full, g, W, H):
+    a, b = C.c_float(), C.c_float()
+    load().oracle_defocus_distances(ref(full), ref(g), W, H, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def defocus_min_max(ray, tilt_deg, dmin_nm, dmax_nm, slab: CtfSlabParams):
+    a, b = C.c_float(), C.c_float()
+    load().oracle_defocus_min_max(C.c_float(ray), C.c_float(tilt_deg), C.c_float(dmin_nm), C.c_float(dmax_nm),
+                                  ref(slab), C.byref(a), C.byref(b))
+    return a.value, b.value
