@@ -45,3 +45,18 @@ def test_host_entry_points_validate_arguments():
     assert lib.artia_projection_index_list(_abi.farr([0.0]), 0, (C.c_int * 1)()) != 0
     assert lib.artia_bp(None, None, None, None, None, 1, 1, 1, 1.0, 1, 0.0, 1.0, 0, None) != 0
     assert lib.artia_version().decode().startswith("artia")
+
+
+def test_cpp_functor_adapter_compiles(tmp_path):
+    """include/artia_kernels.hpp (the Kernels.h-shaped C++ adapter) compiles against artia.h."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    assert gxx, "g++ is required"
+    src = tmp_path / "adapter.cpp"
+    src.write_text('#include "artia_kernels.hpp"\n'
+                   "int main() { artia::Context* c = nullptr; (void)c; return 0; }\n")
+    res = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", str(ROOT / "include"),
+                          str(src)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
