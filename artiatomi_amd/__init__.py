@@ -2,11 +2,13 @@
 backprojection, 3D-CTF slab correction, WBP weighting) as gfx950 HIP kernels behind the C ABI
 of include/artia.h.  This package is the Python view of that library; the compute path is the
 HIP library (libartia.so) and nothing else."""
-from ._abi import CtfParams, GlobalParams, SartParams, TiltDefocus, TiltGeom, TiltParams, VolGeom, load  # noqa: F401
+from ._abi import (CtfParams, GlobalParams, SartParams, Subvol, SubvolTuning, TiltDefocus, TiltGeom,  # noqa: F401
+                   TiltParams, VolGeom, load)
 from .core import (  # noqa: F401
     Acquisition,
     ArtiaContext,
     ArtiaError,
+    SubvolPlan,
     ctf_params,
     defocus_distances,
     defocus_minmax,
@@ -14,13 +16,16 @@ from .core import (  # noqa: F401
     projection_index_list,
     real_length,
     sart_params,
+    subvol_wbp_plain,
     subvolume_geometry,
+    subvolume_tilt_geometry,
     tilt_geometry,
     volume_geometry,
 )
 
 __all__ = [
-    "Acquisition", "ArtiaContext", "ArtiaError", "CtfParams", "GlobalParams", "TiltDefocus",
+    "Acquisition", "ArtiaContext", "ArtiaError", "CtfParams", "Subvol", "SubvolPlan", "SubvolTuning",
+    "subvol_wbp_plain", "subvolume_tilt_geometry", "GlobalParams", "TiltDefocus",
     "ctf_params", "defocus_distances", "defocus_minmax", "SartParams", "TiltGeom", "TiltParams",
     "VolGeom", "hitpoints", "load", "projection_index_list", "real_length", "sart_params",
     "subvolume_geometry", "tilt_geometry", "volume_geometry",

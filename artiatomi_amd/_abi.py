@@ -84,6 +84,19 @@ class TiltDefocus(C.Structure):
                 ("astig_angle_deg", C.c_float), ("tilt_deg", C.c_float)]
 
 
+class Subvol(C.Structure):
+    """artia_subvol: sub-volume placement and its device volume."""
+
+    _fields_ = [("geom", VolGeom), ("vol", C.c_void_p)]
+
+
+class SubvolTuning(C.Structure):
+    """artia_subvol_tuning of the batched sub-volume WBP."""
+
+    _fields_ = [("autotune", C.c_int), ("tile_policy", C.c_int), ("quads_per_thread", C.c_int),
+                ("blocks_per_cu", C.c_int)]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _F = C.c_float
@@ -122,6 +135,14 @@ SIGNATURES = {
     "artia_ctf_slab_filter": (_I, [_P, _P, _I, _I, _SZ, _P, _SZ, _I, _PI, _PF, _PF, C.POINTER(CtfParams), _F, _F,
                                    _F, _I, _PF]),
     "artia_wbp_filter": (_I, [_P, _P, _I, _I, _SZ, _P, _SZ, _F, _I, _PF]),
+    "artia_subvol_wbp_plain": (_I, [_P, C.POINTER(Subvol), _I, C.POINTER(TiltGeom), _P, _I, _P, _I, _I, _SZ, _I,
+                                    C.POINTER(CtfParams), _PF, C.POINTER(TiltDefocus), _PF]),
+    "artia_subvol_plan_create": (_I, [_P, C.POINTER(Subvol), _I, C.POINTER(TiltGeom), _P, _I, _P, _I, _I, _SZ, _I,
+                                      C.POINTER(CtfParams), _PF, C.POINTER(TiltDefocus), C.POINTER(SubvolTuning),
+                                      C.POINTER(_P)]),
+    "artia_subvol_plan_execute": (_I, [_P, _PF]),
+    "artia_subvol_plan_tuning": (_I, [_P, C.POINTER(SubvolTuning)]),
+    "artia_subvol_plan_destroy": (_I, [_P]),
     "artia_profile_enable": (_I, [_P, _I]),
     "artia_profile_read": (_I, [_P, _I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "artia_sart_run": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _I, _PI, _P, _I, _I, _P,

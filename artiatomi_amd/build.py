@@ -21,6 +21,7 @@ SOURCES = [
     "bp.hip",
     "image.hip",
     "spectral.hip",
+    "subvol.hip",
     "sart.hip",
 ]
 

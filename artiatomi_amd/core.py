@@ -19,7 +19,8 @@ from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
 from . import _abi
-from ._abi import CtfParams, GlobalParams, SartParams, TiltDefocus, TiltGeom, TiltParams, VolGeom
+from ._abi import (CtfParams, GlobalParams, SartParams, Subvol, SubvolTuning, TiltDefocus, TiltGeom, TiltParams,
+                   VolGeom)
 
 
 class ArtiaError(RuntimeError):
@@ -68,6 +69,18 @@ def subvolume_geometry(n: int, voxel_size_sub: float, voxel_size_full: Sequence[
                                     _abi.farr(position, 3), _abi.farr(shift, 3), C.byref(out)),
            what="artia_geom_subvolume")
     return out
+
+
+def subvolume_tilt_geometry(tilt_deg: float, width: int, height: int, psi_deg: float = 0.0, mag: float = 1.0,
+                            shift: Sequence[float] = (0.0, 0.0), extra_shift: Sequence[float] = (0.0, 0.0),
+                            glob: Optional[GlobalParams] = None) -> TiltGeom:
+    """Tilt geometry seen by one sub-volume: the marker shift plus the sub-volume's extra shift,
+    added in float as Projection::GetPosition does (Projection.cpp:183-184)."""
+    import numpy as np
+
+    sx = float(np.float32(shift[0]) + np.float32(extra_shift[0]))
+    sy = float(np.float32(shift[1]) + np.float32(extra_shift[1]))
+    return tilt_geometry(tilt_deg, width, height, psi_deg=psi_deg, mag=mag, shift=(sx, sy), glob=glob)
 
 
 def hitpoints(full: VolGeom, tg: TiltGeom) -> list:
@@ -347,3 +360,86 @@ class ArtiaContext:
                                          C.byref(ms) if timed else None)
         _check(st, self.h, "artia_sart_ctf_run")
         return (ms.value if timed else None), bool(empty.value)
+
+
+# ------------------------------------------------------------------ sub-volume WBP
+
+def _subvol_arrays(subs, geoms, active, ntilts):
+    """subs: [(VolGeom, tensor)], geoms: nsub lists of ntilts TiltGeom, active: nsub x ntilts or None."""
+    import numpy as np
+
+    nsub = len(subs)
+    sv = (Subvol * nsub)(*[Subvol(g, C.c_void_p(_ptr(v, "sub-volume", g.nx * g.ny * g.nz))) for g, v in subs])
+    flat = [g for row in geoms for g in row]
+    if len(flat) != nsub * ntilts:
+        raise ArtiaError(f"expected {nsub} x {ntilts} tilt geometries, got {len(flat)}")
+    tg = (TiltGeom * len(flat))(*flat)
+    act = None
+    if active is not None:
+        a = np.ascontiguousarray(np.asarray(active, dtype=np.uint8).reshape(nsub, ntilts))
+        act = (C.c_ubyte * a.size).from_buffer_copy(a.tobytes())
+    return sv, tg, act
+
+
+def _ctf_arrays(ctf, slab_ranges, defocus, ntilts):
+    if ctf is None:
+        return None, None, None
+    if slab_ranges is None or defocus is None or len(slab_ranges) != ntilts or len(defocus) != ntilts:
+        raise ArtiaError("CTF sub-volume WBP needs one slab range and one TiltDefocus per tilt")
+    sr = (C.c_float * (2 * ntilts))(*[v for r in slab_ranges for v in r])
+    df = (TiltDefocus * ntilts)(*defocus)
+    return C.byref(ctf), sr, df
+
+
+def subvol_wbp_plain(ctx: "ArtiaContext", subs, geoms, images, oversampling=1, active=None, ctf=None,
+                     slab_ranges=None, defocus=None, timed=False):
+    """The reference's sub-volume schedule: one BP launch per active (sub-volume, tilt) [and slab]."""
+    T, H, W = images.shape
+    sv, tg, act = _subvol_arrays(subs, geoms, active, T)
+    cp, sr, df = _ctf_arrays(ctf, slab_ranges, defocus, T)
+    ms = C.c_float() if timed else None
+    st = ctx.lib.artia_subvol_wbp_plain(ctx.h, sv, len(subs), tg, act, T, _ptr(images, "images", T * H * W), W, H, W,
+                                        int(oversampling), cp, sr, df, C.byref(ms) if timed else None)
+    _check(st, ctx.h, "artia_subvol_wbp_plain")
+    return ms.value if timed else None
+
+
+class SubvolPlan:
+    """Batched sub-volume WBP (one persistent launch, graph-replayed); see artia_subvol_plan_create."""
+
+    def __init__(self, ctx: "ArtiaContext", subs, geoms, images, oversampling=1, active=None, ctf=None,
+                 slab_ranges=None, defocus=None, tuning: Optional[SubvolTuning] = None):
+        T, H, W = images.shape
+        self.ctx = ctx
+        self._keep = (subs, images)   # the plan holds raw pointers into these tensors
+        sv, tg, act = _subvol_arrays(subs, geoms, active, T)
+        cp, sr, df = _ctf_arrays(ctf, slab_ranges, defocus, T)
+        h = C.c_void_p()
+        st = ctx.lib.artia_subvol_plan_create(ctx.h, sv, len(subs), tg, act, T, _ptr(images, "images", T * H * W), W,
+                                              H, W, int(oversampling), cp, sr, df,
+                                              C.byref(tuning) if tuning is not None else None, C.byref(h))
+        _check(st, ctx.h, "artia_subvol_plan_create")
+        self.h = h
+
+    def execute(self, timed=False):
+        ms = C.c_float() if timed else None
+        _check(self.ctx.lib.artia_subvol_plan_execute(self.h, C.byref(ms) if timed else None), self.ctx.h,
+               "artia_subvol_plan_execute")
+        return ms.value if timed else None
+
+    @property
+    def tuning(self) -> SubvolTuning:
+        t = SubvolTuning()
+        _check(self.ctx.lib.artia_subvol_plan_tuning(self.h, C.byref(t)), self.ctx.h, "artia_subvol_plan_tuning")
+        return t
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.artia_subvol_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

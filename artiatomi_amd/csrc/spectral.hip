@@ -279,7 +279,7 @@ artia_status ctf_slab_filter(artia_ctx* ctx, float* img, int W, int H, size_t pi
     const int N = W > H ? W : H;
     artia_status st = ctx->fft.ensure(ctx, N);
     if (st != ARTIA_OK) return st;
-    launch_crop_border(ctx, W, H, pitch, img, cut, dim, corners);
+    if (corners) launch_crop_border(ctx, W, H, pitch, img, cut, dim, corners);
     launch_make_square(ctx, W, H, pitch, img, N, ctx->fft.square, (N - W) / 2, (N - H) / 2, 0, 1);
     if (hipfftSetStream(ctx->fft.r2c, ctx->stream) != HIPFFT_SUCCESS ||
         hipfftSetStream(ctx->fft.c2r, ctx->stream) != HIPFFT_SUCCESS)
@@ -290,9 +290,9 @@ artia_status ctf_slab_filter(artia_ctx* ctx, float* img, int W, int H, size_t pi
     if (st != ARTIA_OK) return st;
     if (hipfftExecC2R(ctx->fft.c2r, (hipfftComplex*)ctx->fft.spec, ctx->fft.square) != HIPFFT_SUCCESS)
         return fail(ctx, ARTIA_ERR_FFT, "hipfftExecC2R failed");
-    const CropParams cp = make_crop_params(cut, dim, corners);
+    const CropParams cp = corners ? make_crop_params(cut, dim, corners) : CropParams{};
     hipLaunchKernelGGL(finish_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, ctx->fft.square, N, (N - W) / 2,
-                       (N - H) / 2, W, H, out, out_pitch, (float)(N * N), accumulate, 1, cp);
+                       (N - H) / 2, W, H, out, out_pitch, (float)(N * N), accumulate, corners ? 1 : 0, cp);
     return check_hip(ctx, hipGetLastError(), "slab filter");
 }
 
@@ -356,7 +356,7 @@ artia_status artia_ctf_slab_filter(artia_ctx* ctx, float* img, int width, int he
                                    const float dim_length[2], const artia_ctf_params* p, float defocus_min_nm,
                                    float defocus_max_nm, float angle_deg, int for_fp, float* ms) {
     if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
-    if (!img || !out || !corners || !cut_length || !dim_length || !valid_ctf(p) || width <= 0 || height <= 0 ||
+    if (!img || !out || (corners && (!cut_length || !dim_length)) || !valid_ctf(p) || width <= 0 || height <= 0 ||
         pitch < (size_t)width || out_pitch < (size_t)width)
         return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_ctf_slab_filter: invalid arguments");
     Timing t(ctx, ms);

@@ -1,0 +1,727 @@
+// Sub-volume WBP (EmSARTSubVolumes, EmSartSubVolumes.cpp:611-707 with
+// Reconstructor::BackProjection{NoCTF,CTF}(vol, subVolumes, ...), Reconstructor.cpp:894-1034).
+//
+// Every (sub-volume s, tilt t) pair is one backprojection with lambda 1 of tilt t's (WBP-filtered)
+// projection into sub-volume s, seen through tilt t's geometry shifted by the pair's extra shift.
+// Without CTF the gate is [0, 1e13) and the image is POINT sampled; with CTF each tilt's projection
+// is filtered once per defocus slab (no border crop) and every quad takes the slab image whose gate
+// holds its first macro-cell corner, LINEAR sampled.
+//
+// plain:   the reference's schedule, one backprojection launch per active (s, t) and slab.
+// batched: one persistent launch over a work queue of (sub-volume, voxel tile) items, sorted by
+//          sub-volume size (largest first) and popped through a device counter.  A thread keeps its
+//          voxels in registers across all tilts and writes them once.  The per-tilt update is the
+//          plain form's arithmetic, added in the same order, so the two forms agree bit for bit.
+//          Without CTF the counter reset and the persistent kernel are captured in one hipGraph;
+//          with CTF each tilt's slab images are filtered first and one persistent launch follows.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "artia_internal.hpp"
+#include "bp_common.hpp"
+
+namespace artia {
+namespace {
+
+struct SubvolDev {
+    float* vol;
+    int nx, ny, nz;
+};
+
+// Per (s, t) pair.  mode 0: inactive; 1: LDS-tile eligible (affine footprint, POINT, no
+// magnification anisotropy); 2: global-memory path.
+struct PairDev {
+    BPExact e;
+    TiltG g;
+    VolG v;
+    int mode;
+};
+
+// One block's work: sub-volume s, tile origin (quad x, y, z) and tile shape (quads x rows x slices
+// per quad layer); a thread handles kQPT quads stacked along z.
+struct Item {
+    int s, qx0, y0, z0;
+    int qx, ty, tz, pad;
+};
+
+// CTF slab gate of one tilt: bounds[k] = start of slab k (the reference's float loop values),
+// slab k covers [bounds[k], bounds[k+1]) and the last one [bounds[K-1], last_end).
+struct SlabArgs {
+    const float* bounds;
+    int count;
+    float last_end;
+};
+
+__global__ void reset_counter_kernel(unsigned* counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
+}
+
+__device__ __forceinline__ int find_slab(const SlabArgs& sl, double t) {
+    if (sl.count <= 0 || t < (double)sl.bounds[0]) return -1;
+    int lo = 0, hi = sl.count - 1;   // largest k with bounds[k] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((double)sl.bounds[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    if (lo == sl.count - 1 && !(t < (double)sl.last_end)) return -1;
+    return lo;
+}
+
+// Update (lambda = 1) of one quad for one tilt; returns false when the quad is gated out.
+template <int kOS, bool kTile, bool kPar>
+__device__ __forceinline__ void quad_update(const PairDev& p, const float* __restrict__ tile, int tw, int tx0, int ty0,
+                                            const float* __restrict__ img, int W, int H, size_t pitch, int filter,
+                                            int os, unsigned x, unsigned y, unsigned z, const SlabArgs& sl,
+                                            size_t slabStride, float out[4]) {
+    const BPExact& e = p.e;
+    int px0, px1, py0, py1;
+    const float* src = img;
+    if (sl.count > 0) {
+        // first macro-cell corner distance, as the gate of the per-slab launches computes it
+        double t;
+        if (e.affine) {
+            t = fabs(fma(e.gn[0], (double)x, fma(e.gn[1], (double)y, fma(e.gn[2], (double)z, e.g0))));
+        } else {
+            const double n[3] = {p.g.n[0], p.g.n[1], p.g.n[2]};
+            const double d[3] = {p.g.d[0], p.g.d[1], p.g.d[2]};
+            const double c[3] = {(double)p.v.bbMin[0] + (double)x * (double)p.v.vs[0],
+                                 (double)p.v.bbMin[1] + (double)y * (double)p.v.vs[1],
+                                 (double)p.v.bbMin[2] + (double)z * (double)p.v.vs[2]};
+            const double nd = ((-n[0]) * d[0] - n[1] * d[1]) - n[2] * d[2];
+            t = fabs(((n[0] * c[0] + n[1] * c[1]) + n[2] * c[2]) + nd);
+        }
+        const int k = find_slab(sl, t);
+        if (k < 0) return;
+        src = img + (size_t)k * slabStride;
+    }
+    if (e.affine) {
+        if (!footprint_affine(e, x, y, z, W, H, -1.0, INFINITY, px0, px1, py0, py1)) return;
+    } else {
+        if (!footprint_literal(p.v, p.g, x, y, z, W, H, -1.0, INFINITY, px0, px1, py0, py1)) return;
+    }
+    const int O = kOS > 0 ? kOS : os;
+    const int nI = (px1 - px0) * O, nJ = (py1 - py0) * O;
+    if (nI <= 0 || nJ <= 0) return;
+    const float osf = (float)O, delta = 1.0f / osf;
+    const float du0 = (float)(e.pu0 + (double)x * e.pui + (double)y * e.puj + (double)z * e.puk - (double)px0);
+    const float dv0 = (float)(e.pv0 + (double)x * e.pvi + (double)y * e.pvj + (double)z * e.pvk - (double)py0);
+    const float dui = (float)e.pui, dvi = (float)e.pvi;
+    const int tileOff = kTile ? (py0 - ty0) * tw + (px0 - tx0) : 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float du = __builtin_fmaf((float)k, dui, du0);
+        const float dv = __builtin_fmaf((float)k, dvi, dv0);
+        const int i0 = max(0, (int)ceilf((du + e.uMin) * osf - 0.5f));
+        const int i1 = min(nI - 1, (int)floorf((du + e.uMax) * osf - 0.5f));
+        const int j0 = max(0, (int)ceilf((dv + e.vMin) * osf - 0.5f));
+        const int j1 = min(nJ - 1, (int)floorf((dv + e.vMax) * osf - 0.5f));
+        float num = 0.0f, den = 0.0f;
+        if (kTile)
+            voxel_sums_tile<(kOS > 0 ? kOS : 0), kPar>(e, tile, tw, tileOff, O, delta, du, dv, i0, i1, j0, j1, num,
+                                                       den);
+        else
+            voxel_sums_global(e, p.g, src, W, H, pitch, filter, px0, py0, O, delta, du, dv, i0, i1, j0, j1, num, den);
+        den = fmaxf(0.0f, den);
+        if (den != 0.0f) out[k] += (1.0f * num) / den;
+    }
+}
+
+// Persistent backprojection over the work queue.  Tilts [t0, t1) of every item; image of tilt t at
+// images + (t - t0) * imgStride (no CTF), or slab k of the single tilt at images + k * imgStride.
+template <int kOS, int kQPT>
+__global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__ items, int nitems,
+                                                        unsigned* __restrict__ counter,
+                                                        const SubvolDev* __restrict__ subs,
+                                                        const PairDev* __restrict__ pairs, int ntilts, int t0, int t1,
+                                                        const float* __restrict__ images, size_t imgStride, int W,
+                                                        int H, size_t pitch, int os, int filter, SlabArgs slabs) {
+    __shared__ float tile[kTileFloats];
+    __shared__ int cur;
+    const int tid = (int)threadIdx.x;
+    for (;;) {
+        if (tid == 0) cur = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int it = cur;
+        __syncthreads();
+        if (it >= nitems) break;
+        const Item item = items[it];
+        const SubvolDev sv = subs[item.s];
+        const int qx = tid % item.qx, ty = (tid / item.qx) % item.ty, tz = tid / (item.qx * item.ty);
+        const bool lane = tz < item.tz;   // tile shapes may leave threads idle
+        const int xq = item.qx0 + qx, y = item.y0 + ty;
+        const unsigned x = (unsigned)xq * 4u;
+        float acc[kQPT][4];
+        bool in[kQPT];
+#pragma unroll
+        for (int q = 0; q < kQPT; q++) {
+            const int z = item.z0 + tz + q * item.tz;
+            in[q] = lane && xq < sv.nx / 4 && y < sv.ny && z < sv.nz;
+            if (in[q]) {
+                const float4 vb = *reinterpret_cast<const float4*>(
+                    sv.vol + ((size_t)z * (size_t)sv.ny + (size_t)y) * (size_t)sv.nx + x);
+                acc[q][0] = vb.x;
+                acc[q][1] = vb.y;
+                acc[q][2] = vb.z;
+                acc[q][3] = vb.w;
+            }
+        }
+        const int bX = item.qx * 4, bY = item.ty, bZ = item.tz * kQPT;
+        for (int t = t0; t < t1; t++) {
+            const PairDev& p = pairs[(size_t)item.s * (size_t)ntilts + (size_t)t];
+            if (p.mode == 0) continue;
+            // no CTF: tilt t's image; CTF (one tilt per launch): the tilt's slab stack
+            const float* img = images + (size_t)(t - t0) * imgStride;
+            bool useTile = false;
+            int tx0 = 0, ty0 = 0, tw = 0, th = 0;
+            if (p.mode == 1) {
+                const BPExact& e = p.e;
+                const double ox = (double)(item.qx0 * 4), oy = (double)item.y0, oz = (double)item.z0;
+                double lo[2], hi[2];
+                const double ext[3] = {(double)(bX - 4), (double)(bY - 1), (double)(bZ - 1)};
+                for (int r = 0; r < 2; r++) {
+                    const double base = e.G[r][0] * ox + e.G[r][1] * oy + e.G[r][2] * oz;
+                    lo[r] = base + e.lo[r];
+                    hi[r] = base + e.hi[r];
+                    for (int k = 0; k < 3; k++) {
+                        lo[r] += fmin(0.0, e.G[r][k] * ext[k]);
+                        hi[r] += fmax(0.0, e.G[r][k] * ext[k]);
+                    }
+                }
+                tx0 = max(0, (int)floor(lo[0]) - 1);
+                ty0 = max(0, (int)floor(lo[1]) - 1);
+                const int tx1 = min(W - 1, (int)ceil(hi[0]) + 1);
+                const int ty1 = min(H - 1, (int)ceil(hi[1]) + 1);
+                tw = max(0, tx1 - tx0 + 1);
+                th = max(0, ty1 - ty0 + 1);
+                useTile = tw * th <= kTileFloats;
+            }
+            if (useTile) {
+                for (int idx = tid; idx < tw * th; idx += 256) {
+                    const int r = idx / tw, c = idx - r * tw;
+                    tile[idx] = img[(size_t)(ty0 + r) * pitch + (size_t)(tx0 + c)];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < kQPT; q++) {
+                    if (!in[q]) continue;
+                    const unsigned z = (unsigned)(item.z0 + tz + q * item.tz);
+                    if (p.e.parallel)
+                        quad_update<kOS, true, true>(p, tile, tw, tx0, ty0, img, W, H, pitch, filter, os, x,
+                                                     (unsigned)y, z, slabs, imgStride, acc[q]);
+                    else
+                        quad_update<kOS, true, false>(p, tile, tw, tx0, ty0, img, W, H, pitch, filter, os, x,
+                                                      (unsigned)y, z, slabs, imgStride, acc[q]);
+                }
+                __syncthreads();
+            } else {
+#pragma unroll
+                for (int q = 0; q < kQPT; q++) {
+                    if (!in[q]) continue;
+                    const unsigned z = (unsigned)(item.z0 + tz + q * item.tz);
+                    quad_update<kOS, false, true>(p, tile, 0, 0, 0, img, W, H, pitch, filter, os, x, (unsigned)y, z,
+                                                  slabs, imgStride, acc[q]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kQPT; q++) {
+            if (!in[q]) continue;
+            const int z = item.z0 + tz + q * item.tz;
+            *reinterpret_cast<float4*>(sv.vol + ((size_t)z * (size_t)sv.ny + (size_t)y) * (size_t)sv.nx + x) =
+                make_float4(acc[q][0], acc[q][1], acc[q][2], acc[q][3]);
+        }
+    }
+}
+
+// Largest power of two <= cap dividing n (>= 1).
+int pow2_divisor(int n, int cap) {
+    int q = 1;
+    while (q * 2 <= cap && n % (q * 2) == 0) q *= 2;
+    return q;
+}
+
+// Tile shape of a sub-volume for a policy: quads along x, rows along y, slices along z per quad
+// layer (quads * rows * slices == 256 threads).
+void tile_shape(int nq, int policy, int& qx, int& ty, int& tz) {
+    if (policy == 0) {
+        qx = pow2_divisor(nq, 16);
+        ty = 4;
+    } else {
+        qx = std::max(1, pow2_divisor(nq, 16) / 2);
+        ty = 8;
+    }
+    tz = std::max(1, 256 / (qx * ty));
+}
+
+void build_items(const std::vector<SubvolDev>& subs, const std::vector<int>& order, int policy, int qpt,
+                 std::vector<Item>& items) {
+    items.clear();
+    for (int s : order) {
+        const SubvolDev& sv = subs[(size_t)s];
+        int qx, ty, tz;
+        tile_shape(sv.nx / 4, policy, qx, ty, tz);
+        const int zext = tz * qpt;
+        for (int z0 = 0; z0 < sv.nz; z0 += zext)
+            for (int y0 = 0; y0 < sv.ny; y0 += ty)
+                for (int q0 = 0; q0 < sv.nx / 4; q0 += qx) items.push_back(Item{s, q0, y0, z0, qx, ty, tz, 0});
+    }
+}
+
+void launch_subvol(artia_ctx* ctx, hipStream_t stream, int qpt, int grid, const Item* items, int nitems,
+                   unsigned* counter, const SubvolDev* subs, const PairDev* pairs, int ntilts, int t0, int t1,
+                   const float* images, size_t imgStride, int W, int H, size_t pitch, int os, int filter,
+                   SlabArgs slabs) {
+    (void)ctx;
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
+#define ARTIA_SUBVOL_LAUNCH(OS, QPT)                                                                                 \
+    hipLaunchKernelGGL((subvol_bp_kernel<OS, QPT>), dim3((unsigned)grid), dim3(256), 0, stream, items, nitems,      \
+                       counter, subs, pairs, ntilts, t0, t1, images, imgStride, W, H, pitch, os, filter, slabs)
+    const int o = (os == 1 || os == 2 || os == 4) ? os : 0;
+    if (qpt == 2) {
+        if (o == 1) ARTIA_SUBVOL_LAUNCH(1, 2);
+        else if (o == 2) ARTIA_SUBVOL_LAUNCH(2, 2);
+        else if (o == 4) ARTIA_SUBVOL_LAUNCH(4, 2);
+        else ARTIA_SUBVOL_LAUNCH(0, 2);
+    } else {
+        if (o == 1) ARTIA_SUBVOL_LAUNCH(1, 1);
+        else if (o == 2) ARTIA_SUBVOL_LAUNCH(2, 1);
+        else if (o == 4) ARTIA_SUBVOL_LAUNCH(4, 1);
+        else ARTIA_SUBVOL_LAUNCH(0, 1);
+    }
+#undef ARTIA_SUBVOL_LAUNCH
+}
+
+// Slab starts of one tilt: the reference's `for (float ray = t_in; ray < t_out; ray += step)`.
+void slab_starts(float t_in, float t_out, float step, std::vector<float>& out) {
+    out.clear();
+    for (float ray = t_in; ray < t_out; ray += step) out.push_back(ray);
+}
+
+struct Inputs {
+    const artia_subvol* subs;
+    int nsub;
+    const artia_tilt_geom* geoms;
+    const unsigned char* active;
+    int ntilts;
+    const float* images;
+    int W, H;
+    size_t pitch;
+    int os;
+    const artia_ctf_params* ctf;
+    const float* slab_range;
+    const artia_tilt_defocus* defocus;
+};
+
+artia_status validate(artia_ctx* ctx, const char* fn, const Inputs& in) {
+    if (!in.subs || !in.geoms || !in.images || in.nsub <= 0 || in.ntilts <= 0 || in.W <= 0 || in.H <= 0 ||
+        in.pitch < (size_t)in.W || in.os <= 0 || in.os > 64)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: invalid arguments", fn);
+    if (in.ctf && (!in.slab_range || !in.defocus || !(in.ctf->slice_thickness_nm > 0.0f) ||
+                   !(in.ctf->pixelsize_nm > 0.0f) || !(in.ctf->voltage_kV > 0.0f) || !(in.ctf->wiener >= 0.0f)))
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: incomplete CTF arguments", fn);
+    for (int s = 0; s < in.nsub; s++) {
+        const artia_subvol& sv = in.subs[s];
+        if (!sv.vol || sv.geom.nx <= 0 || sv.geom.ny <= 0 || sv.geom.nz <= 0 || sv.geom.nx % 4 != 0 ||
+            (reinterpret_cast<uintptr_t>(sv.vol) & 15) != 0)
+            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT,
+                        "%s: sub-volume %d needs a 16-byte aligned volume with x dimension a multiple of 4", fn, s);
+        for (int t = 0; t < in.ntilts; t++) {
+            const size_t i = (size_t)s * (size_t)in.ntilts + (size_t)t;
+            if ((!in.active || in.active[i]) && !valid_geometry(&sv.geom, &in.geoms[i]))
+                return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: invalid geometry for sub-volume %d, tilt %d", fn, s,
+                            t);
+        }
+    }
+    if (in.ctf) {
+        const float step = in.ctf->slice_thickness_nm / in.ctf->pixelsize_nm;
+        for (int t = 0; t < in.ntilts; t++) {
+            const float a = in.slab_range[2 * t], b = in.slab_range[2 * t + 1];
+            if (!std::isfinite(a) || !std::isfinite(b) || !(step > 0.0f) || (double)(b - a) / step > 65536.0 ||
+                (b > a && a + step == a))
+                return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: slab range of tilt %d holds too many slabs", fn, t);
+        }
+    }
+    return ARTIA_OK;
+}
+
+}  // namespace
+}  // namespace artia
+
+using namespace artia;
+
+struct artia_subvol_plan {
+    artia_ctx* ctx = nullptr;
+    artia::Inputs in{};
+    std::vector<artia_tilt_defocus> defocus;
+    std::vector<float> slab_range;
+    artia_ctf_params ctfp{};
+    bool ctf = false;
+    std::vector<SubvolDev> subs;
+    std::vector<int> order;   // sub-volumes by size, largest first
+    SubvolDev* d_subs = nullptr;
+    PairDev* d_pairs = nullptr;
+    Item* d_items = nullptr;
+    unsigned* d_counter = nullptr;
+    float* d_bounds = nullptr;   // CTF: slab starts of every tilt, concatenated
+    float* d_stack = nullptr;    // CTF: the current tilt's filtered slab images
+    std::vector<int> bound_off, bound_count;
+    std::vector<float> last_end;
+    int max_slabs = 0;
+    int nitems = 0;
+    artia_subvol_tuning tuning{};
+    int grid = 0;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~artia_subvol_plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+        for (void* p : {(void*)d_subs, (void*)d_pairs, (void*)d_items, (void*)d_counter, (void*)d_bounds,
+                        (void*)d_stack})
+            if (p) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+artia_status upload(artia_ctx* ctx, void** dst, const void* src, size_t bytes, const char* what) {
+    if (*dst) (void)hipFree(*dst);
+    *dst = nullptr;
+    artia_status st = check_hip(ctx, hipMalloc(dst, bytes > 0 ? bytes : 16), what);
+    if (st != ARTIA_OK) return st;
+    if (bytes > 0) st = check_hip(ctx, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);
+    return st;
+}
+
+artia_status set_items(artia_subvol_plan* p, int policy, int qpt) {
+    std::vector<Item> items;
+    build_items(p->subs, p->order, policy, qpt, items);
+    p->nitems = (int)items.size();
+    return upload(p->ctx, (void**)&p->d_items, items.data(), items.size() * sizeof(Item), "sub-volume queue");
+}
+
+// Enqueues one full backprojection of the plan on `stream` (the graph body for the no-CTF case).
+artia_status enqueue(artia_subvol_plan* p, hipStream_t stream) {
+    const Inputs& in = p->in;
+    if (!p->ctf) {
+        launch_subvol(p->ctx, stream, p->tuning.quads_per_thread, p->grid, p->d_items, p->nitems, p->d_counter,
+                      p->d_subs, p->d_pairs, in.ntilts, 0, in.ntilts, in.images, (size_t)in.H * in.pitch, in.W, in.H,
+                      in.pitch, in.os, ARTIA_FILTER_POINT, SlabArgs{nullptr, 0, 0.0f});
+        return check_hip(p->ctx, hipGetLastError(), "sub-volume backprojection launch");
+    }
+    artia_ctx* ctx = p->ctx;
+    hipStream_t saved = ctx->stream;
+    ctx->stream = stream;
+    const size_t slabStride = (size_t)in.H * in.pitch;
+    for (int t = 0; t < in.ntilts; t++) {
+        const float* bounds = nullptr;
+        int count = p->bound_count[(size_t)t];
+        if (count == 0) continue;
+        std::vector<float> starts;
+        slab_starts(p->slab_range[2 * (size_t)t], p->slab_range[2 * (size_t)t + 1],
+                    p->ctfp.slice_thickness_nm / p->ctfp.pixelsize_nm, starts);
+        for (int k = 0; k < count; k++) {
+            float dmin, dmax;
+            artia_geom_defocus_minmax(starts[(size_t)k], &p->defocus[(size_t)t], &p->ctfp, &dmin, &dmax);
+            artia_status st = ctf_slab_filter(ctx, const_cast<float*>(in.images) + (size_t)t * slabStride, in.W, in.H,
+                                              in.pitch, p->d_stack + (size_t)k * slabStride, in.pitch, 0, nullptr,
+                                              nullptr, nullptr, p->ctfp, dmin, dmax,
+                                              p->defocus[(size_t)t].astig_angle_deg, 0);
+            if (st != ARTIA_OK) {
+                ctx->stream = saved;
+                return st;
+            }
+        }
+        bounds = p->d_bounds + p->bound_off[(size_t)t];
+        launch_subvol(ctx, stream, p->tuning.quads_per_thread, p->grid, p->d_items, p->nitems, p->d_counter,
+                      p->d_subs, p->d_pairs, in.ntilts, t, t + 1, p->d_stack, slabStride, in.W, in.H, in.pitch, in.os,
+                      ARTIA_FILTER_LINEAR, SlabArgs{bounds, count, p->last_end[(size_t)t]});
+    }
+    ctx->stream = saved;
+    return check_hip(ctx, hipGetLastError(), "sub-volume CTF backprojection launch");
+}
+
+int cu_count(artia_ctx* ctx) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess || prop.multiProcessorCount <= 0) return 256;
+    return prop.multiProcessorCount;
+}
+
+// Times the candidate tunings on a sample of the queue (up to 8 sub-volumes per size, their
+// contents saved and restored) and keeps the fastest.
+artia_status autotune(artia_subvol_plan* p) {
+    artia_ctx* ctx = p->ctx;
+    std::vector<int> sample;
+    {
+        std::vector<std::pair<long long, int>> seen;
+        for (int s : p->order) {
+            const SubvolDev& sv = p->subs[(size_t)s];
+            const long long key = (long long)sv.nx * 1000000LL + (long long)sv.ny * 1000 + sv.nz;
+            int c = 0;
+            for (auto& kv : seen)
+                if (kv.first == key) c = ++kv.second;
+            if (c == 0) seen.push_back({key, 1});
+            if (c <= 8) sample.push_back(s);
+        }
+    }
+    size_t backupFloats = 0;
+    for (int s : sample) backupFloats += (size_t)p->subs[(size_t)s].nx * p->subs[(size_t)s].ny * p->subs[(size_t)s].nz;
+    float* backup = nullptr;
+    artia_status st = check_hip(ctx, hipMalloc(&backup, backupFloats * sizeof(float) + 16), "autotune backup");
+    if (st != ARTIA_OK) return st;
+    auto copy_all = [&](bool save) {
+        size_t off = 0;
+        for (int s : sample) {
+            const SubvolDev& sv = p->subs[(size_t)s];
+            const size_t n = (size_t)sv.nx * sv.ny * sv.nz;
+            if (save)
+                (void)hipMemcpyAsync(backup + off, sv.vol, n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream);
+            else
+                (void)hipMemcpyAsync(sv.vol, backup + off, n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream);
+            off += n;
+        }
+    };
+    copy_all(true);
+    const std::vector<int> fullOrder = p->order;
+    p->order = sample;
+    const int cus = cu_count(ctx);
+    const int policies[2] = {0, 1}, qpts[2] = {1, 2}, bpcs[3] = {2, 4, 8};
+    float best = INFINITY;
+    artia_subvol_tuning bestT = p->tuning;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    for (int pol : policies)
+        for (int q : qpts)
+            for (int b : bpcs) {
+                p->tuning = artia_subvol_tuning{0, pol, q, b};
+                p->grid = cus * b;
+                if ((st = set_items(p, pol, q)) != ARTIA_OK) break;
+                float ms = 0.0f;
+                for (int rep = 0; rep < 2 && st == ARTIA_OK; rep++) {
+                    (void)hipEventRecord(e0, ctx->stream);
+                    st = enqueue(p, ctx->stream);
+                    (void)hipEventRecord(e1, ctx->stream);
+                    copy_all(false);
+                    if (st == ARTIA_OK) st = check_hip(ctx, hipEventSynchronize(e1), "autotune run");
+                    if (st == ARTIA_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+                }
+                if (st != ARTIA_OK) break;
+                if (ms < best) {
+                    best = ms;
+                    bestT = p->tuning;
+                }
+            }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(backup);
+    p->order = fullOrder;
+    if (st != ARTIA_OK) return st;
+    p->tuning = bestT;
+    p->tuning.autotune = 1;
+    return ARTIA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+artia_status artia_subvol_wbp_plain(artia_ctx* ctx, const artia_subvol* subs, int nsub, const artia_tilt_geom* geoms,
+                                    const unsigned char* active, int ntilts, const float* images, int width,
+                                    int height, size_t pitch, int oversampling, const artia_ctf_params* ctf,
+                                    const float* slab_range, const artia_tilt_defocus* defocus, float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    const Inputs in{subs, nsub, geoms, active, ntilts, images, width, height, pitch, oversampling, ctf, slab_range,
+                    defocus};
+    artia_status st = validate(ctx, "artia_subvol_wbp_plain", in);
+    if (st != ARTIA_OK) return st;
+    const size_t npix = (size_t)height * pitch;
+    if (ctf && (st = ensure_workspace(ctx, npix)) != ARTIA_OK) return st;
+    Timing tm(ctx, ms);
+    for (int t = 0; t < ntilts; t++) {
+        const float* img = images + (size_t)t * npix;
+        if (!ctf) {
+            for (int s = 0; s < nsub; s++) {
+                const size_t i = (size_t)s * (size_t)ntilts + (size_t)t;
+                if (active && !active[i]) continue;
+                launch_bp(ctx, to_device(subs[s].geom), to_device(geoms[i]), subs[s].vol, img, width, height, pitch,
+                          1.0f, oversampling, 0.0f, 9999999999999.0f, ARTIA_FILTER_POINT);
+            }
+            continue;
+        }
+        // Reconstructor.cpp:993-1031: per slab, filter the projection, then every sub-volume
+        const float step = ctf->slice_thickness_nm / ctf->pixelsize_nm;
+        for (float ray = slab_range[2 * t]; ray < slab_range[2 * t + 1]; ray += step) {
+            float dmin, dmax;
+            artia_geom_defocus_minmax(ray, &defocus[t], ctf, &dmin, &dmax);
+            st = ctf_slab_filter(ctx, const_cast<float*>(img), width, height, pitch, ctx->ws, pitch, 0, nullptr,
+                                 nullptr, nullptr, *ctf, dmin, dmax, defocus[t].astig_angle_deg, 0);
+            if (st != ARTIA_OK) return st;
+            for (int s = 0; s < nsub; s++) {
+                const size_t i = (size_t)s * (size_t)ntilts + (size_t)t;
+                if (active && !active[i]) continue;
+                launch_bp(ctx, to_device(subs[s].geom), to_device(geoms[i]), subs[s].vol, ctx->ws, width, height,
+                          pitch, 1.0f, oversampling, ray, ray + step, ARTIA_FILTER_LINEAR);
+            }
+        }
+    }
+    st = check_hip(ctx, hipGetLastError(), "artia_subvol_wbp_plain launch");
+    if (st != ARTIA_OK) return st;
+    return tm.finish();
+}
+
+artia_status artia_subvol_plan_create(artia_ctx* ctx, const artia_subvol* subs, int nsub,
+                                      const artia_tilt_geom* geoms, const unsigned char* active, int ntilts,
+                                      const float* images, int width, int height, size_t pitch, int oversampling,
+                                      const artia_ctf_params* ctf, const float* slab_range,
+                                      const artia_tilt_defocus* defocus, const artia_subvol_tuning* tuning,
+                                      artia_subvol_plan** out) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!out) return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_subvol_plan_create: out is NULL");
+    *out = nullptr;
+    if (ctx->numerics != ARTIA_NUMERICS_EXACT)
+        return fail(ctx, ARTIA_ERR_UNSUPPORTED,
+                    "artia_subvol_plan_create: the batched form evaluates exact numerics only; use "
+                    "artia_subvol_wbp_plain for reference numerics");
+    const Inputs in{subs, nsub, geoms, active, ntilts, images, width, height, pitch, oversampling, ctf, slab_range,
+                    defocus};
+    artia_status st = validate(ctx, "artia_subvol_plan_create", in);
+    if (st != ARTIA_OK) return st;
+    if (tuning && (tuning->quads_per_thread < 1 || tuning->quads_per_thread > 2 || tuning->tile_policy < 0 ||
+                   tuning->tile_policy > 1 || tuning->blocks_per_cu < 1 || tuning->blocks_per_cu > 16))
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_subvol_plan_create: bad tuning");
+    artia_subvol_plan* p = new artia_subvol_plan();
+    p->ctx = ctx;
+    p->in = in;
+    p->ctf = ctf != nullptr;
+    if (ctf) {
+        p->ctfp = *ctf;
+        p->defocus.assign(defocus, defocus + ntilts);
+        p->slab_range.assign(slab_range, slab_range + 2 * (size_t)ntilts);
+        p->in.ctf = &p->ctfp;
+        p->in.slab_range = p->slab_range.data();
+        p->in.defocus = p->defocus.data();
+    }
+    p->subs.resize((size_t)nsub);
+    for (int s = 0; s < nsub; s++)
+        p->subs[(size_t)s] = SubvolDev{subs[s].vol, subs[s].geom.nx, subs[s].geom.ny, subs[s].geom.nz};
+    p->order.resize((size_t)nsub);
+    for (int s = 0; s < nsub; s++) p->order[(size_t)s] = s;
+    std::stable_sort(p->order.begin(), p->order.end(), [&](int a, int b) {
+        const SubvolDev &A = p->subs[(size_t)a], &B = p->subs[(size_t)b];
+        return (long long)A.nx * A.ny * A.nz > (long long)B.nx * B.ny * B.nz;
+    });
+    // per-pair constants (extra shifts are already folded into geoms)
+    std::vector<PairDev> pairs((size_t)nsub * (size_t)ntilts);
+    for (int s = 0; s < nsub; s++) {
+        const VolG v = to_device(subs[s].geom);
+        for (int t = 0; t < ntilts; t++) {
+            const size_t i = (size_t)s * (size_t)ntilts + (size_t)t;
+            PairDev& pd = pairs[i];
+            std::memset(&pd, 0, sizeof(PairDev));
+            pd.v = v;
+            if (active && !active[i]) continue;
+            pd.g = to_device(geoms[i]);
+            pd.e = prepare_exact(v, pd.g);
+            pd.mode = (!ctf && pd.e.affine && pd.e.identityAniso) ? 1 : 2;
+        }
+    }
+    if ((st = upload(ctx, (void**)&p->d_pairs, pairs.data(), pairs.size() * sizeof(PairDev), "pair table")) !=
+            ARTIA_OK ||
+        (st = upload(ctx, (void**)&p->d_subs, p->subs.data(), p->subs.size() * sizeof(SubvolDev), "sub-volumes")) !=
+            ARTIA_OK ||
+        (st = upload(ctx, (void**)&p->d_counter, nullptr, 0, "queue counter")) != ARTIA_OK) {
+        delete p;
+        return st;
+    }
+    if (ctf) {
+        std::vector<float> all, starts;
+        const float step = ctf->slice_thickness_nm / ctf->pixelsize_nm;
+        for (int t = 0; t < ntilts; t++) {
+            slab_starts(slab_range[2 * t], slab_range[2 * t + 1], step, starts);
+            p->bound_off.push_back((int)all.size());
+            p->bound_count.push_back((int)starts.size());
+            p->last_end.push_back(starts.empty() ? 0.0f : starts.back() + step);
+            all.insert(all.end(), starts.begin(), starts.end());
+            p->max_slabs = std::max(p->max_slabs, (int)starts.size());
+        }
+        if ((st = upload(ctx, (void**)&p->d_bounds, all.data(), all.size() * sizeof(float), "slab bounds")) !=
+                ARTIA_OK ||
+            (st = check_hip(ctx, hipMalloc(&p->d_stack, (size_t)std::max(1, p->max_slabs) * height * pitch *
+                                                            sizeof(float)),
+                            "slab image stack")) != ARTIA_OK) {
+            delete p;
+            return st;
+        }
+    }
+    const int cus = cu_count(ctx);
+    if (tuning) {
+        p->tuning = *tuning;
+        p->tuning.autotune = 0;
+    } else {
+        p->tuning = artia_subvol_tuning{0, 0, 1, 4};
+        if ((st = autotune(p)) != ARTIA_OK) {
+            delete p;
+            return st;
+        }
+    }
+    p->grid = cus * p->tuning.blocks_per_cu;
+    if ((st = set_items(p, p->tuning.tile_policy, p->tuning.quads_per_thread)) != ARTIA_OK) {
+        delete p;
+        return st;
+    }
+    if (!p->ctf) {
+        // capture the counter reset and the persistent kernel once; execute replays the graph
+        if ((st = check_hip(ctx, hipStreamCreate(&p->cap), "capture stream")) != ARTIA_OK ||
+            (st = check_hip(ctx, hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal), "begin capture")) !=
+                ARTIA_OK) {
+            delete p;
+            return st;
+        }
+        st = enqueue(p, p->cap);
+        const artia_status st2 = check_hip(ctx, hipStreamEndCapture(p->cap, &p->graph), "end capture");
+        if (st == ARTIA_OK) st = st2;
+        if (st == ARTIA_OK)
+            st = check_hip(ctx, hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0), "graph instantiate");
+        if (st != ARTIA_OK) {
+            delete p;
+            return st;
+        }
+    }
+    *out = p;
+    return ARTIA_OK;
+}
+
+artia_status artia_subvol_plan_execute(artia_subvol_plan* p, float* ms) {
+    if (!p) return ARTIA_ERR_INVALID_ARGUMENT;
+    artia_ctx* ctx = p->ctx;
+    Timing tm(ctx, ms);
+    artia_status st;
+    if (p->exec)
+        st = check_hip(ctx, hipGraphLaunch(p->exec, ctx->stream), "graph launch");
+    else
+        st = enqueue(p, ctx->stream);
+    if (st != ARTIA_OK) return st;
+    return tm.finish();
+}
+
+artia_status artia_subvol_plan_tuning(const artia_subvol_plan* p, artia_subvol_tuning* out) {
+    if (!p || !out) return ARTIA_ERR_INVALID_ARGUMENT;
+    *out = p->tuning;
+    return ARTIA_OK;
+}
+
+artia_status artia_subvol_plan_destroy(artia_subvol_plan* p) {
+    if (!p) return ARTIA_ERR_INVALID_ARGUMENT;
+    (void)hipStreamSynchronize(p->ctx->stream);
+    delete p;
+    return ARTIA_OK;
+}
+
+}  // extern "C"

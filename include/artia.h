@@ -217,8 +217,9 @@ artia_status artia_wbp_weight(artia_ctx* ctx, void* spectrum, int n, size_t pitc
                               int method, float* ms);
 /* One CTF slab filter (Reconstructor.cpp:557-583 forward, :952-965 backward), n = max(W, H):
  * cropBorder(img, corners) in place -> makeSquare(zero fill) -> R2C -> ctf -> C2R ->
- * out = cropBorder(square / n^2) (accumulate = 0) or out += it (accumulate = 1).  The FFT
- * plans and n x n buffers are cached in the context. */
+ * out = cropBorder(square / n^2) (accumulate = 0) or out += it (accumulate = 1).  With
+ * corners == NULL neither crop is applied (the sub-volume path, Reconstructor.cpp:1006-1012).
+ * The FFT plans and n x n buffers are cached in the context. */
 artia_status artia_ctf_slab_filter(artia_ctx* ctx, float* img, int width, int height, size_t pitch,
                                    float* out, size_t out_pitch, int accumulate, const int corners[8],
                                    const float cut_length[2], const float dim_length[2],
@@ -252,6 +253,56 @@ artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const 
                                 const float* projections, int width, int height, float* vol,
                                 const artia_sart_params* sp, const artia_ctf_params* ctf, int iterations,
                                 int* volume_is_empty, float* ms);
+
+/* ---------------------------------------------------------------- sub-volume WBP
+ * EmSARTSubVolumes (EmSartSubVolumes.cpp:611-707): every motive-list sub-volume is reconstructed
+ * by backprojecting each tilt's WBP-filtered projection (artia_wbp_filter) with lambda 1 through
+ * the tilt's geometry shifted by the pair's extra shift (Projection.cpp:183-184).
+ *   geoms[s * ntilts + t]   geometry of tilt t for sub-volume s (marker shift + extra shift);
+ *   active[s * ntilts + t]  nonzero to backproject the pair (EmSARTSubVolumes skips pairs with
+ *                           |extra shift| >= MaxShift - 0.5); NULL = all active;
+ *   images                  device, ntilts images of height rows at `pitch` floats, in the order
+ *                           the tilts are processed (the reference's index list order);
+ *   ctf (optional)          with slab_range[2 t] (artia_geom_defocus_distances of the full
+ *                           volume) and defocus[t]: each tilt is filtered per defocus slab without
+ *                           border crop and backprojected with LINEAR sampling inside the slab gate
+ *                           (Reconstructor.cpp:975-1031); NULL = POINT sampling, gate [0, 1e13). */
+typedef struct {
+    artia_vol_geom geom; /* artia_geom_subvolume; geom.nx % 4 == 0 */
+    float* vol;          /* device, dense [z][y][x] fp32, 16-byte aligned; accumulated into */
+} artia_subvol;
+
+/* The reference's schedule (Reconstructor.cpp:894-916, 975-1031): one backprojection launch per
+ * active (sub-volume, tilt) pair and slab. */
+artia_status artia_subvol_wbp_plain(artia_ctx* ctx, const artia_subvol* subs, int nsub,
+                                    const artia_tilt_geom* geoms, const unsigned char* active, int ntilts,
+                                    const float* images, int width, int height, size_t pitch,
+                                    int oversampling, const artia_ctf_params* ctf, const float* slab_range,
+                                    const artia_tilt_defocus* defocus, float* ms);
+
+/* Batched form of the same computation.  No reference counterpart: one persistent launch over a
+ * device work queue of (sub-volume, voxel tile) items sorted by size, each thread accumulating all
+ * tilts in registers; captured once in a hipGraph (without CTF) and replayed by execute.  Results
+ * equal artia_subvol_wbp_plain's bit for bit in ARTIA_NUMERICS_EXACT (the only numerics it
+ * supports).  The plan keeps the pointers it was given; they must stay valid until destroy.
+ * tuning NULL: the plan times the candidate tile shapes, quads per thread and persistent block
+ * counts on a sample of the queue (sub-volume contents are restored) and keeps the fastest. */
+typedef struct {
+    int autotune;         /* set by the plan when its tuning came from the autotuner */
+    int tile_policy;      /* 0: widest x tile (up to 16 quads x 4 rows), 1: 8-row tiles */
+    int quads_per_thread; /* 1 or 2 (stacked along z) */
+    int blocks_per_cu;    /* persistent blocks per compute unit, 1..16 */
+} artia_subvol_tuning;
+typedef struct artia_subvol_plan artia_subvol_plan;
+artia_status artia_subvol_plan_create(artia_ctx* ctx, const artia_subvol* subs, int nsub,
+                                      const artia_tilt_geom* geoms, const unsigned char* active, int ntilts,
+                                      const float* images, int width, int height, size_t pitch,
+                                      int oversampling, const artia_ctf_params* ctf, const float* slab_range,
+                                      const artia_tilt_defocus* defocus, const artia_subvol_tuning* tuning,
+                                      artia_subvol_plan** out);
+artia_status artia_subvol_plan_execute(artia_subvol_plan* plan, float* ms);
+artia_status artia_subvol_plan_tuning(const artia_subvol_plan* plan, artia_subvol_tuning* out);
+artia_status artia_subvol_plan_destroy(artia_subvol_plan* plan);
 
 /* ---------------------------------------------------------------- profiling
  * With profiling enabled, artia_sart_run brackets each kernel launch with HIP events on the

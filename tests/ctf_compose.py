@@ -13,17 +13,19 @@ import oracle_lib as O
 def slab_filter(img, corners, consts, dmin, dmax, angle, for_fp, cut=(0, 0), dim=(0, 0), phase_flip=False,
                 wiener=0.1, beta=(200.0, 0.0, 0.008, 0.0), dtype=np.float64):
     """cropBorder(img) -> makeSquare(zero) -> R2C -> ctf -> C2R / N^2 -> crop to W x H ->
-    cropBorder.  Returns (cropped img, filtered image)."""
+    cropBorder (no crops when corners is None).  Returns (cropped img, filtered image)."""
     H, W = img.shape
     N = max(W, H)
     bx, by = (N - W) // 2, (N - H) // 2
-    a = O.crop_border(img, corners, cut, dim, dtype=dtype)
+    a = O.crop_border(img, corners, cut, dim, dtype=dtype) if corners is not None else np.asarray(img, dtype)
     sq = O.make_square(a, N, bx, by, mirror_y=False, fill_zero=True, dtype=dtype)
     X = np.fft.rfft2(sq)
     X = O.ctf_apply(X, consts, dmin, dmax, angle, for_fp=for_fp, phase_flip=phase_flip, wiener=wiener,
                     beta=beta, dtype=dtype)
     r = np.fft.irfft2(X, s=(N, N)).astype(dtype)
     out = np.ascontiguousarray(r[by:by + H, bx:bx + W])
+    if corners is None:   # the sub-volume path filters without border crops
+        return a, out
     return a, O.crop_border(out, corners, cut, dim, dtype=dtype)
 
 
@@ -101,3 +103,38 @@ def sart_ctf(vg, full, geoms, hitpoints, real_lengths, slab_ranges, defocus, ind
                     vol = O.back_project(vg, g, vol, f, lam, sp.oversampling, ray,
                                          float(np.float32(ray) + np.float32(step)), filt=1, dtype=dtype)
     return vol, empty
+
+
+def subvol_wbp(subs, geoms, images, os_, active=None, ctf=None, slab_ranges=None, defocus=None, dtype=np.float64):
+    """EmSARTSubVolumes' loop (EmSartSubVolumes.cpp:611-707): per tilt, every active sub-volume gets
+    a lambda-1 backprojection of the tilt's image; with CTF, per defocus slab the image is filtered
+    without crops (Reconstructor.cpp:975-1031) and backprojected with LINEAR sampling in the gate.
+    subs: [(vol_geom, volume array)]; returns the updated volumes."""
+    T, H, W = images.shape
+    vols = [np.ascontiguousarray(v, dtype=dtype).copy() for _, v in subs]
+    if ctf is not None:
+        N = max(W, H)
+        consts = O.ctf_consts(ctf.cs_mm, ctf.voltage_kV, ctf.pixelsize_nm, N, ctf.amp_contrast)
+        slab = O.CtfSlabParams(ctf.slice_thickness_nm, ctf.pixelsize_nm, ctf.volume_shift_z, ctf.ignore_z_shift,
+                               ctf.switch_direction)
+        step = float(np.float32(ctf.slice_thickness_nm) / np.float32(ctf.pixelsize_nm))
+    for t in range(T):
+        if ctf is None:
+            for s, (vg, _) in enumerate(subs):
+                if active is not None and not active[s][t]:
+                    continue
+                vols[s] = O.back_project(vg, geoms[s][t], vols[s], images[t], 1.0, os_, 0.0, 9999999999999.0,
+                                         filt=0, dtype=dtype)
+            continue
+        dmin0, dmax0, ang, tilt = defocus[t]
+        for ray in slab_rays(*slab_ranges[t], step):
+            dmin, dmax = O.defocus_min_max(ray, tilt, dmin0, dmax0, slab)
+            _, f = slab_filter(images[t].astype(dtype), None, consts, dmin, dmax, ang, False,
+                               phase_flip=bool(ctf.phase_flip_only), wiener=ctf.wiener, beta=tuple(ctf.beta_fac),
+                               dtype=dtype)
+            for s, (vg, _) in enumerate(subs):
+                if active is not None and not active[s][t]:
+                    continue
+                vols[s] = O.back_project(vg, geoms[s][t], vols[s], f, 1.0, os_, ray,
+                                         float(np.float32(ray) + np.float32(step)), filt=1, dtype=dtype)
+    return vols
