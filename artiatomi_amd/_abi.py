@@ -143,6 +143,8 @@ SIGNATURES = {
     "artia_subvol_plan_execute": (_I, [_P, _PF]),
     "artia_subvol_plan_tuning": (_I, [_P, C.POINTER(SubvolTuning)]),
     "artia_subvol_plan_destroy": (_I, [_P]),
+    "artia_fp_adjoint": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _SZ, _P, _PI, _F, _PF]),
+    "artia_fp_shift_grad": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _P, _I, _I, _SZ, _PI, _P, _PF]),
     "artia_profile_enable": (_I, [_P, _I]),
     "artia_profile_read": (_I, [_P, _I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "artia_sart_run": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _I, _PI, _P, _I, _I, _P,

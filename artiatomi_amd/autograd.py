@@ -1,0 +1,115 @@
+"""Differentiable forward projection for refinement (volumes and per-tilt 2D shifts).
+
+`project(vol, shifts, setup)` returns the projections P(vol, shifts) of one volume (a whole
+tomogram or a motive-list sub-volume) through a tilt series, each tilt shifted by its row of
+`shifts` (pixels, added to the tilt's marker shift as Projection::GetPosition adds extra shifts,
+Projection.cpp:183-195).  The backward pass runs the library's hand-written kernels over the C
+ABI: the exact transpose of the forward projector for the volume gradient (artia_fp_adjoint) and
+the shift derivative (artia_fp_shift_grad), both recomputing the ray samples instead of storing
+them.  The forward projector is the exact-numerics one of artia_fp_march; the HIP library is
+required (there is no other implementation).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import torch
+
+from . import core
+from ._abi import GlobalParams, VolGeom
+
+
+@dataclass
+class ProjectionSetup:
+    """Fixed part of the projection: volume placement, tilt series and detector size."""
+
+    ctx: "core.ArtiaContext"
+    vg: VolGeom
+    width: int
+    height: int
+    tilts_deg: Sequence[float]
+    psis_deg: Optional[Sequence[float]] = None
+    base_shifts: Optional[Sequence[Sequence[float]]] = None
+    glob: Optional[GlobalParams] = None
+    roi: Optional[Sequence[int]] = None          # {x0, y0, x1, y1}; None: the whole image
+    _dist: Optional[torch.Tensor] = field(default=None, repr=False)
+
+    def geometries(self, shifts: torch.Tensor) -> list:
+        s = shifts.detach().to("cpu", torch.float64).numpy()
+        T = len(self.tilts_deg)
+        if s.shape != (T, 2):
+            raise core.ArtiaError(f"shifts must have shape ({T}, 2), got {tuple(s.shape)}")
+        psis = self.psis_deg if self.psis_deg is not None else [0.0] * T
+        base = self.base_shifts if self.base_shifts is not None else [(0.0, 0.0)] * T
+        return [core.subvolume_tilt_geometry(float(t), self.width, self.height, psi_deg=float(p),
+                                             shift=(float(b[0]), float(b[1])), extra_shift=(float(e[0]), float(e[1])),
+                                             glob=self.glob)
+                for t, p, b, e in zip(self.tilts_deg, psis, base, s)]
+
+    def scratch(self) -> torch.Tensor:
+        if self._dist is None:
+            self._dist = torch.empty((self.height, self.width), device="cuda")
+        return self._dist
+
+
+class _Project(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vol: torch.Tensor, shifts: torch.Tensor, setup: ProjectionSetup):
+        if setup.ctx.numerics != "exact":
+            raise core.ArtiaError("the differentiable projector needs an ARTIA_NUMERICS_EXACT context")
+        vol = vol.contiguous()
+        geoms = setup.geometries(shifts)
+        T = len(geoms)
+        out = torch.zeros((T, setup.height, setup.width), device=vol.device, dtype=torch.float32)
+        dist = setup.scratch()
+        for t, g in enumerate(geoms):
+            setup.ctx.fp_march(setup.vg, g, vol, out[t], dist, roi=setup.roi)
+        ctx.save_for_backward(vol, shifts)
+        ctx.geoms = geoms
+        ctx.setup = setup
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        vol, shifts = ctx.saved_tensors
+        setup: ProjectionSetup = ctx.setup
+        g = grad_out.contiguous().to(torch.float32)
+        gvol = torch.zeros_like(vol) if ctx.needs_input_grad[0] else None
+        gshift = None
+        if ctx.needs_input_grad[1]:
+            gshift = torch.zeros((len(ctx.geoms), 2), device=vol.device, dtype=torch.float64)
+        for t, geo in enumerate(ctx.geoms):
+            if gvol is not None:
+                setup.ctx.fp_adjoint(setup.vg, geo, g[t], gvol, roi=setup.roi)
+            if gshift is not None:
+                setup.ctx.fp_shift_grad(setup.vg, geo, vol, g[t], gshift[t], roi=setup.roi)
+        if gshift is not None:
+            gshift = gshift.to(shifts.dtype)
+        return gvol, gshift, None
+
+
+def project(vol: torch.Tensor, shifts: torch.Tensor, setup: ProjectionSetup) -> torch.Tensor:
+    """Projections (T, H, W) of `vol` ([z][y][x] float32, CUDA) with per-tilt shifts (T, 2)."""
+    return _Project.apply(vol, shifts, setup)
+
+
+def refine_shifts(vol: torch.Tensor, target: torch.Tensor, setup: ProjectionSetup, steps: int = 50,
+                  lr: float = 0.2, init: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Gradient-based replacement of EmSARTRefine's cross-correlation shift search: minimises
+    ||P(vol, s) - target||^2 over the per-tilt shifts s with Adam, starting from `init` (zeros)."""
+    T = len(setup.tilts_deg)
+    s = (init.clone() if init is not None else torch.zeros((T, 2), dtype=torch.float64, device="cuda"))
+    s.requires_grad_(True)
+    opt = torch.optim.Adam([s], lr=lr)
+    sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=0.94)
+    for _ in range(steps):
+        opt.zero_grad()
+        loss = ((project(vol, s, setup) - target) ** 2).sum()
+        loss.backward()
+        opt.step()
+        sched.step()
+    return s.detach()
+
+
+__all__ = ["ProjectionSetup", "project", "refine_shifts"]

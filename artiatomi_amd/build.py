@@ -22,6 +22,7 @@ SOURCES = [
     "image.hip",
     "spectral.hip",
     "subvol.hip",
+    "adjoint.hip",
     "sart.hip",
 ]
 

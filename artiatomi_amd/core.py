@@ -314,6 +314,23 @@ class ArtiaContext:
         return self._call("artia_wbp_filter", _ptr(img, "img"), W, H, W, _ptr(out, "out", W * H), W,
                           float(psi_deg), _abi.WBP_METHODS[method], timed=timed)
 
+    def fp_adjoint(self, vg: VolGeom, tg: TiltGeom, resid, vol_grad, scale=1.0, roi=None, timed=False):
+        """vol_grad += scale * P^T resid (exact transpose of the exact-numerics forward projector)."""
+        H, W = resid.shape
+        return self._call("artia_fp_adjoint", C.byref(vg), C.byref(tg), _ptr(resid, "resid"), W, H, W,
+                          _ptr(vol_grad, "vol_grad", vg.nx * vg.ny * vg.nz), self._roi(roi), float(scale),
+                          timed=timed)
+
+    def fp_shift_grad(self, vg: VolGeom, tg: TiltGeom, vol, grad_proj, out, roi=None, timed=False):
+        """out (2 float64 on the device) += sum grad_proj * dP/d(shift_x, shift_y)."""
+        import torch
+
+        H, W = grad_proj.shape
+        if not (isinstance(out, torch.Tensor) and out.is_cuda and out.dtype == torch.float64 and out.numel() >= 2):
+            raise ArtiaError("out must be a float64 CUDA tensor of 2 elements")
+        return self._call("artia_fp_shift_grad", C.byref(vg), C.byref(tg), _ptr(vol, "vol", vg.nx * vg.ny * vg.nz),
+                          _ptr(grad_proj, "grad_proj"), W, H, W, self._roi(roi), out.data_ptr(), timed=timed)
+
     def profile(self, enable: bool = True) -> None:
         """Enables (and resets) per-kernel-class event timing of driver launches."""
         _check(self.lib.artia_profile_enable(self.h, int(enable)), self.h, "artia_profile_enable")

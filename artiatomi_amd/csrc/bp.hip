@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) bp_ref_kernel(VolG v, TiltG g, float* __r
 // 4 slices in z.  kOS >= 0 selects the LDS-tile path (POINT sampling, no magnification
 // anisotropy, affine footprint): the block first stages the projection pixels its quads can
 // reach.  kOS < 0 is the global-memory path.
-template <int kOS, bool kPar>
+template <int kOS, int kPM>
 __global__ void __launch_bounds__(256) bp_exact_kernel(VolG v, TiltG g, BPExact e, float* __restrict__ vol,
                                                        const float* __restrict__ img, int W, int H, size_t pitch,
                                                        float lambda, int os, double distMin, double distMax,
@@ -200,8 +200,8 @@ __global__ void __launch_bounds__(256) bp_exact_kernel(VolG v, TiltG g, BPExact 
         const int j1 = min(nJ - 1, (int)floorf((dv + e.vMax) * osf - 0.5f));
         float num = 0.0f, den = 0.0f;
         if (kTile)
-            voxel_sums_tile<(kOS > 0 ? kOS : 0), kPar>(e, tile, tw, tileOff, O, delta, du, dv, i0, i1, j0, j1, num,
-                                                       den);
+            voxel_sums_tile<(kOS > 0 ? kOS : 0), kPM>(e, tile, tw, tileOff, O, delta, du, dv, i0, i1, j0, j1, num,
+                                                      den);
         else
             voxel_sums_global(e, g, img, W, H, pitch, filter, px0, py0, O, delta, du, dv, i0, i1, j0, j1, num, den);
         den = fmaxf(0.0f, den);
@@ -217,11 +217,14 @@ template <int kOS>
 void launch_tile(artia_ctx* ctx, dim3 grid, const VolG& v, const TiltG& g, const BPExact& e, float* vol,
                  const float* img, int W, int H, size_t pitch, float lambda, int os, double dmin, double dmax,
                  int filter) {
-    if (e.parallel)
-        hipLaunchKernelGGL((bp_exact_kernel<kOS, true>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H,
-                           pitch, lambda, os, dmin, dmax, filter);
+    if (e.parallel == 0)
+        hipLaunchKernelGGL((bp_exact_kernel<kOS, 0>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H, pitch,
+                           lambda, os, dmin, dmax, filter);
+    else if (e.parallel == 2)
+        hipLaunchKernelGGL((bp_exact_kernel<kOS, 2>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H, pitch,
+                           lambda, os, dmin, dmax, filter);
     else
-        hipLaunchKernelGGL((bp_exact_kernel<kOS, false>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H,
+        hipLaunchKernelGGL((bp_exact_kernel<kOS, -1>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H,
                            pitch, lambda, os, dmin, dmax, filter);
 }
 
@@ -242,7 +245,7 @@ void launch_bp(artia_ctx* ctx, const VolG& v, const TiltG& g, float* vol, const 
                       (long long)(e.tileW + 2) * (long long)(e.tileH + 2) <= kTileFloats;
     const double dmin = distMin, dmax = distMax;
     if (!tile)
-        hipLaunchKernelGGL((bp_exact_kernel<-1, true>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H,
+        hipLaunchKernelGGL((bp_exact_kernel<-1, -1>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H,
                            pitch, lambda, os, dmin, dmax, filter);
     else if (os == 1)
         launch_tile<1>(ctx, grid, v, g, e, vol, img, W, H, pitch, lambda, os, dmin, dmax, filter);

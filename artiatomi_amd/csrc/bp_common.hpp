@@ -221,44 +221,65 @@ __device__ __forceinline__ bool footprint_affine(const BPExact& e, unsigned x, u
 
 constexpr int kTileFloats = 4096;   // 16 KiB of LDS for the block's projection tile
 
-// Chord-weighted sums of one voxel over its sub-ray window [i0, i1] x [j0, j1]: entry/exit of
-// the three slabs are linear in the sub-ray offset (a, b); a chord <= 0 adds exactly zero, so the
-// loop is branch free and the per-voxel accumulation order (py outer, px inner) is kept.
-// kOS > 0 fixes the oversampling at compile time (the pixel of lattice index i is i / kOS);
-// kPar adds the inside test for axes parallel to the rays.
-template <int kOS, bool kPar>
+// Chord of the sub-ray at offset (a, b) through one voxel: entry/exit of the three slabs are linear
+// in the offset; axes parallel to the rays (n_c == 0) contribute an inside test instead of a slab.
+// kPM is the parallel-axis mask when known at compile time: 0 (none) or 2 (y only, every tilt about
+// the y axis); -1 reads the mask at run time.  With n_y == 0 the y slab is (-inf, +inf) and drops
+// out of the max/min exactly, so the kPM = 2 form gives the generic form's value bit for bit.
+template <int kPM>
+__device__ __forceinline__ float chord_at(const BPExact& e, float a, float b, float bV0, float bV1, float bV2) {
+    if (kPM == 2) {
+        const float e0 = __builtin_fmaf(a, e.U[0], bV0);
+        const float e2 = __builtin_fmaf(a, e.U[2], bV2);
+        const float tin = fmaxf(e0, e2);
+        const float tout = fminf(e0 + e.Sabs[0], e2 + e.Sabs[2]);
+        const float chord = fmaxf(tout - tin, 0.0f);
+        const float wc = __builtin_fmaf(a, e.uc[1], b * e.vc[1]);
+        return (wc > 0.0f && wc < e.sc[1]) ? chord : 0.0f;
+    }
+    const float e0 = __builtin_fmaf(a, e.U[0], bV0);
+    const float e1 = __builtin_fmaf(a, e.U[1], bV1);
+    const float e2 = __builtin_fmaf(a, e.U[2], bV2);
+    const float tin = fmaxf(fmaxf(e0, e1), e2);
+    const float tout = fminf(fminf(e0 + e.Sabs[0], e1 + e.Sabs[1]), e2 + e.Sabs[2]);
+    float chord = fmaxf(tout - tin, 0.0f);
+    if (kPM != 0) {
+        for (int c = 0; c < 3; c++) {
+            if (!((e.parallel >> c) & 1)) continue;
+            const float wc = __builtin_fmaf(a, e.uc[c], b * e.vc[c]);
+            chord = (wc > 0.0f && wc < e.sc[c]) ? chord : 0.0f;
+        }
+    }
+    return chord;
+}
+
+// Sub-ray offsets of a window row: a_i = (i - i0) * delta + a0 with a0 = (i0 + 0.5) * delta - du,
+// one rounding per lattice point, so every path below sees the same offsets.
+__device__ __forceinline__ float lattice_a(int i, int i0, float delta, float a0) {
+    return __builtin_fmaf((float)(i - i0), delta, a0);
+}
+
+// Chord-weighted sums of one voxel over its sub-ray window [i0, i1] x [j0, j1] from the LDS tile.
+// A chord <= 0 adds exactly zero, so the loop is branch free; the per-voxel accumulation order
+// (py outer, px inner) is the reference's.  kOS > 0 fixes the oversampling at compile time (the
+// pixel of lattice index i is i / kOS).
+template <int kOS, int kPM>
 __device__ __forceinline__ void voxel_sums_tile(const BPExact& e, const float* __restrict__ tile, int tw, int tileOff,
                                                 int os, float delta, float du, float dv, int i0, int i1, int j0,
                                                 int j1, float& num, float& den) {
     const int O = kOS > 0 ? kOS : os;
+    const float a0 = __builtin_fmaf((float)i0 + 0.5f, delta, -du);
     for (int j = j0; j <= j1; j++) {
         const float b = __builtin_fmaf((float)j + 0.5f, delta, -dv);
         const float bV0 = __builtin_fmaf(b, e.V[0], e.Smin[0]);
         const float bV1 = __builtin_fmaf(b, e.V[1], e.Smin[1]);
         const float bV2 = __builtin_fmaf(b, e.V[2], e.Smin[2]);
-        float bw[3];
-        if (kPar)
-            for (int c = 0; c < 3; c++) bw[c] = b * e.vc[c];
         const float* row = tile + tileOff + (j / O) * tw;
-        float a = __builtin_fmaf((float)i0 + 0.5f, delta, -du);
         for (int i = i0; i <= i1; i++) {
-            const float e0 = __builtin_fmaf(a, e.U[0], bV0);
-            const float e1 = __builtin_fmaf(a, e.U[1], bV1);
-            const float e2 = __builtin_fmaf(a, e.U[2], bV2);
-            const float tin = fmaxf(fmaxf(e0, e1), e2);
-            const float tout = fminf(fminf(e0 + e.Sabs[0], e1 + e.Sabs[1]), e2 + e.Sabs[2]);
-            float chord = fmaxf(tout - tin, 0.0f);
-            if (kPar) {
-                for (int c = 0; c < 3; c++) {
-                    if (!((e.parallel >> c) & 1)) continue;
-                    const float wc = __builtin_fmaf(a, e.uc[c], bw[c]);
-                    chord = (wc > 0.0f && wc < e.sc[c]) ? chord : 0.0f;
-                }
-            }
+            const float chord = chord_at<kPM>(e, lattice_a(i, i0, delta, a0), b, bV0, bV1, bV2);
             const float val = row[i / O];
             num = __builtin_fmaf(val, chord, num);
             den += chord;
-            a += delta;
         }
     }
 }
@@ -274,21 +295,10 @@ __device__ __forceinline__ void voxel_sums_global(const BPExact& e, const TiltG&
         const float bV0 = __builtin_fmaf(b, e.V[0], e.Smin[0]);
         const float bV1 = __builtin_fmaf(b, e.V[1], e.Smin[1]);
         const float bV2 = __builtin_fmaf(b, e.V[2], e.Smin[2]);
-        // the sub-ray offset advances by delta exactly as in voxel_sums_tile, so both paths give the
-        // same chords bit for bit
-        float a = __builtin_fmaf((float)i0 + 0.5f, delta, -du);
-        for (int i = i0; i <= i1; i++, a += delta) {
-            const float e0 = __builtin_fmaf(a, e.U[0], bV0);
-            const float e1 = __builtin_fmaf(a, e.U[1], bV1);
-            const float e2 = __builtin_fmaf(a, e.U[2], bV2);
-            const float tin = fmaxf(fmaxf(e0, e1), e2);
-            const float tout = fminf(fminf(e0 + e.Sabs[0], e1 + e.Sabs[1]), e2 + e.Sabs[2]);
-            float chord = tout - tin;
-            for (int c = 0; c < 3; c++) {
-                if (!((e.parallel >> c) & 1)) continue;
-                const float wc = __builtin_fmaf(a, e.uc[c], b * e.vc[c]);
-                if (!(wc > 0.0f && wc < e.sc[c])) chord = 0.0f;
-            }
+        // the same offsets and chords as voxel_sums_tile, so both paths agree bit for bit
+        const float a0 = __builtin_fmaf((float)i0 + 0.5f, delta, -du);
+        for (int i = i0; i <= i1; i++) {
+            const float chord = chord_at<-1>(e, lattice_a(i, i0, delta, a0), b, bV0, bV1, bV2);
             if (chord > 0.0f) {
                 const float pxf = (float)px0 + ((float)i + 0.5f) * delta;
                 const float pyf = (float)py0 + ((float)j + 0.5f) * delta;

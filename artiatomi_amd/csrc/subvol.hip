@@ -71,7 +71,7 @@ __device__ __forceinline__ int find_slab(const SlabArgs& sl, double t) {
 }
 
 // Update (lambda = 1) of one quad for one tilt; returns false when the quad is gated out.
-template <int kOS, bool kTile, bool kPar>
+template <int kOS, bool kTile, int kPM>
 __device__ __forceinline__ void quad_update(const PairDev& p, const float* __restrict__ tile, int tw, int tx0, int ty0,
                                             const float* __restrict__ img, int W, int H, size_t pitch, int filter,
                                             int os, unsigned x, unsigned y, unsigned z, const SlabArgs& sl,
@@ -120,8 +120,8 @@ __device__ __forceinline__ void quad_update(const PairDev& p, const float* __res
         const int j1 = min(nJ - 1, (int)floorf((dv + e.vMax) * osf - 0.5f));
         float num = 0.0f, den = 0.0f;
         if (kTile)
-            voxel_sums_tile<(kOS > 0 ? kOS : 0), kPar>(e, tile, tw, tileOff, O, delta, du, dv, i0, i1, j0, j1, num,
-                                                       den);
+            voxel_sums_tile<(kOS > 0 ? kOS : 0), kPM>(e, tile, tw, tileOff, O, delta, du, dv, i0, i1, j0, j1, num,
+                                                      den);
         else
             voxel_sums_global(e, p.g, src, W, H, pitch, filter, px0, py0, O, delta, du, dv, i0, i1, j0, j1, num, den);
         den = fmaxf(0.0f, den);
@@ -208,12 +208,15 @@ __global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__
                 for (int q = 0; q < kQPT; q++) {
                     if (!in[q]) continue;
                     const unsigned z = (unsigned)(item.z0 + tz + q * item.tz);
-                    if (p.e.parallel)
-                        quad_update<kOS, true, true>(p, tile, tw, tx0, ty0, img, W, H, pitch, filter, os, x,
-                                                     (unsigned)y, z, slabs, imgStride, acc[q]);
+                    if (p.e.parallel == 0)
+                        quad_update<kOS, true, 0>(p, tile, tw, tx0, ty0, img, W, H, pitch, filter, os, x, (unsigned)y,
+                                                  z, slabs, imgStride, acc[q]);
+                    else if (p.e.parallel == 2)
+                        quad_update<kOS, true, 2>(p, tile, tw, tx0, ty0, img, W, H, pitch, filter, os, x, (unsigned)y,
+                                                  z, slabs, imgStride, acc[q]);
                     else
-                        quad_update<kOS, true, false>(p, tile, tw, tx0, ty0, img, W, H, pitch, filter, os, x,
-                                                      (unsigned)y, z, slabs, imgStride, acc[q]);
+                        quad_update<kOS, true, -1>(p, tile, tw, tx0, ty0, img, W, H, pitch, filter, os, x,
+                                                   (unsigned)y, z, slabs, imgStride, acc[q]);
                 }
                 __syncthreads();
             } else {
@@ -221,8 +224,8 @@ __global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__
                 for (int q = 0; q < kQPT; q++) {
                     if (!in[q]) continue;
                     const unsigned z = (unsigned)(item.z0 + tz + q * item.tz);
-                    quad_update<kOS, false, true>(p, tile, 0, 0, 0, img, W, H, pitch, filter, os, x, (unsigned)y, z,
-                                                  slabs, imgStride, acc[q]);
+                    quad_update<kOS, false, -1>(p, tile, 0, 0, 0, img, W, H, pitch, filter, os, x, (unsigned)y, z,
+                                                slabs, imgStride, acc[q]);
                 }
             }
         }

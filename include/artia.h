@@ -230,6 +230,24 @@ artia_status artia_ctf_slab_filter(artia_ctx* ctx, float* img, int width, int he
 artia_status artia_wbp_filter(artia_ctx* ctx, const float* img, int width, int height, size_t pitch,
                               float* out, size_t out_pitch, float psi_deg, int method, float* ms);
 
+/* ---------------------------------------------------------------- adjoint and gradients
+ * No reference counterpart (the reference's backprojector is SART's normalised update, not the
+ * transpose of its ray marcher); for gradient-based refinement of volumes and shifts.  P is the
+ * forward projector of artia_fp_march in ARTIA_NUMERICS_EXACT:
+ *   (P x)[pixel] = 0.25 * step * sum over the pixel's 4 sub-rays and samples of trilinear(x, q).
+ * The volume needs at least 2 voxels per axis.  roi = {x0, y0, x1, y1} or NULL for the image. */
+/* vol_grad += scale * P^T resid: the exact transpose of P (same sub-rays, samples and trilinear
+ * weights, recomputed), accumulated with device float atomics. */
+artia_status artia_fp_adjoint(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                              const float* resid, int width, int height, size_t pitch, float* vol_grad,
+                              const int* roi, float scale, float* ms);
+/* out[0..1] += sum_pixels grad_proj[pixel] * d(P vol)[pixel] / d(shift_x, shift_y), the tilt's 2D
+ * shift as in artia_tilt_params (the detector origin moves by -shift_x*u - shift_y*v,
+ * Projection.cpp:183-195).  out is a device array of 2 doubles. */
+artia_status artia_fp_shift_grad(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                                 const float* vol, const float* grad_proj, int width, int height, size_t pitch,
+                                 const int* roi, double* out, float* ms);
+
 /* ---------------------------------------------------------------- drivers
  * EmSART's ordered-subset SART loop (EmSart.cpp:579-675) without CTF, device resident:
  * for every subset, per tilt FP (volTraversalLength while the volume is empty) -> compare

@@ -362,6 +362,98 @@ void march(const VolGeom& vg, const TiltGeom& tg, const T* vol, int W, int H, si
 }
 
 template <class T>
+void march_adjoint(const VolGeom& vg, const TiltGeom& tg, const T* resid, int W, int H, size_t pitch, T* grad,
+                   Roi roi, T scale) {
+    const GeomT<T> G(vg, tg);
+    const T step = G.vs[0] * T(0.15);
+    const int nx = vg.nx, ny = vg.ny, nz = vg.nz;
+    const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
+    const int x0 = std::max(roi.x0, 0), y0 = std::max(roi.y0, 0);
+    const int x1 = std::min(roi.x1, W), y1 = std::min(roi.y1, H);
+    // serial: scatter targets overlap between pixels
+    for (int y = y0; y < y1; y++) {
+        for (int x = x0; x < x1; x++) {
+            const T r = resid[(size_t)y * pitch + (size_t)x] * scale * (G.vs[0] * T(0.15)) * T(0.25);
+            if (r == T(0)) continue;
+            for (T osx = T(0.25); osx < T(0.9); osx += T(0.5)) {
+                for (T osy = T(0.25); osy < T(0.9); osy += T(0.5)) {
+                    T c[3];
+                    sub_ray_origin(G, (T)x + osx, (T)y + osy, c);
+                    T t_in, t_out;
+                    slab_test(G, c, t_in, t_out);
+                    if (!((t_out - t_in) > T(0))) continue;
+                    T f[3] = {c[0] + t_in * G.n[0], c[1] + t_in * G.n[1], c[2] + t_in * G.n[2]};
+                    int guard = 0;
+                    while (t_in <= t_out && guard++ < kMaxSamplesPerRay) {
+                        T q[3] = {((f[0] - G.bbMin[0]) * G.rcp[0]) * G.dim[0],
+                                  ((f[1] - G.bbMin[1]) * G.rcp[1]) * G.dim[1],
+                                  ((f[2] - G.bbMin[2]) * G.rcp[2]) * G.dim[2] - G.zshift};
+                        // tex3d_linear's weights and clamped indices
+                        const int n3[3] = {nx, ny, nz};
+                        int lo[3], hi[3];
+                        T wlo[3], whi[3];
+                        for (int a = 0; a < 3; a++) {
+                            const T qc = fminT(fmaxT(q[a], T(-1)), T(n3[a] + 1)) - T(0.5);
+                            const T fl = std::floor(qc);
+                            const T w = qc - fl;
+                            lo[a] = clampi((int)fl, 0, n3[a] - 1);
+                            hi[a] = clampi((int)fl + 1, 0, n3[a] - 1);
+                            wlo[a] = T(1) - w;
+                            whi[a] = w;
+                        }
+                        for (int k = 0; k < 2; k++)
+                            for (int j = 0; j < 2; j++)
+                                for (int i = 0; i < 2; i++) {
+                                    const T w = (i ? whi[0] : wlo[0]) * (j ? whi[1] : wlo[1]) * (k ? whi[2] : wlo[2]);
+                                    grad[(size_t)(k ? hi[2] : lo[2]) * sz + (size_t)(j ? hi[1] : lo[1]) * sy +
+                                         (size_t)(i ? hi[0] : lo[0])] += r * w;
+                                }
+                        t_in += step;
+                        for (int i = 0; i < 3; i++) f[i] = c[i] + t_in * G.n[i];
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <class T>
+void march_shifted(const VolGeom& vg, const TiltGeom& tg, const T* vol, int W, int H, size_t pitch, T* proj,
+                   Roi roi, double sx, double sy) {
+    const GeomT<T> G(vg, tg);
+    const T step = G.vs[0] * T(0.15);
+    const int x0 = std::max(roi.x0, 0), y0 = std::max(roi.y0, 0);
+    const int x1 = std::min(roi.x1, W), y1 = std::min(roi.y1, H);
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = y0; y < y1; y++) {
+        for (int x = x0; x < x1; x++) {
+            T temp = 0;
+            for (T osx = T(0.25); osx < T(0.9); osx += T(0.5)) {
+                for (T osy = T(0.25); osy < T(0.9); osy += T(0.5)) {
+                    T c[3];
+                    sub_ray_origin(G, (T)x + osx, (T)y + osy, c);
+                    for (int i = 0; i < 3; i++) c[i] = c[i] - ((T)sx * G.u[i] + (T)sy * G.v[i]);
+                    T t_in, t_out;
+                    slab_test(G, c, t_in, t_out);
+                    if (!((t_out - t_in) > T(0))) continue;
+                    T f[3] = {c[0] + t_in * G.n[0], c[1] + t_in * G.n[1], c[2] + t_in * G.n[2]};
+                    int guard = 0;
+                    while (t_in <= t_out && guard++ < kMaxSamplesPerRay) {
+                        T qx = ((f[0] - G.bbMin[0]) * G.rcp[0]) * G.dim[0];
+                        T qy = ((f[1] - G.bbMin[1]) * G.rcp[1]) * G.dim[1];
+                        T qz = ((f[2] - G.bbMin[2]) * G.rcp[2]) * G.dim[2] - G.zshift;
+                        temp += (tex3d_linear(vol, vg.nx, vg.ny, vg.nz, qx, qy, qz) * G.vs[0]) * T(0.15);
+                        t_in += step;
+                        for (int i = 0; i < 3; i++) f[i] = c[i] + t_in * G.n[i];
+                    }
+                }
+            }
+            proj[(size_t)y * pitch + (size_t)x] += temp * T(0.25);
+        }
+    }
+}
+
+template <class T>
 void slicer(const VolGeom& vg, const TiltGeom& tg, const T* vol, int W, int H, size_t pitch,
             T* proj, T tminDef, T tmaxDef, Roi roi) {
     const GeomT<T> G(vg, tg);
@@ -878,6 +970,9 @@ void sart(const VolGeom& vg, const VolGeom& full, const std::vector<TiltGeom>& t
     template void crop_border<T>(int, int, size_t, T*, const float*, const float*, const int*);         \
     template void back_project<T>(const VolGeom&, const TiltGeom&, T*, const T*, int, int, size_t, float, int, float, float, int); \
     template void make_square<T>(int, int, size_t, const T*, int, T*, int, int, bool, bool);          \
+    template void march_adjoint<T>(const VolGeom&, const TiltGeom&, const T*, int, int, size_t, T*, Roi, T); \
+    template void march_shifted<T>(const VolGeom&, const TiltGeom&, const T*, int, int, size_t, T*, Roi, double, \
+                                   double);                                                            \
     template void sart<T>(const VolGeom&, const VolGeom&, const std::vector<TiltGeom>&, const std::vector<int>&, \
                           const T*, int, int, T*, const SartParams&, int, bool*);
 

@@ -27,6 +27,8 @@
 //   backProject   constant-residual invariant (every hit voxel gets exactly lambda*r),
 //                 impulse footprint, FP64 brute-force chord lengths on tiny volumes
 //   index list    glibc srand(0)/rand() of the running C library
+//   march adjoint dot-product identity <P x, y> = <x, P^T y> against march itself
+//   march shifted zero shift equals march; integer pixel shifts translate the projection
 #pragma once
 
 #include <algorithm>
@@ -193,6 +195,21 @@ void crop_border(int W, int H, size_t pitch, T* img, const float cut[2], const f
 template <class T>
 void back_project(const VolGeom& vg, const TiltGeom& tg, T* vol, const T* img, int W, int H,
                   size_t pitch, float lambda, int os, float distMin, float distMax, int filter);
+
+// Transpose of march's projection output (the forward projector used for gradients; no reference
+// counterpart): grad += scale * P^T resid, with P x = 0.25 * step * sum over sub-rays and samples of
+// tex3d_linear(x, q).  Scatters each sample's exact trilinear weights (those of tex3d_linear,
+// including its index clamping) along march's sub-rays and samples.
+template <class T>
+void march_adjoint(const VolGeom& vg, const TiltGeom& tg, const T* resid, int W, int H, size_t pitch, T* grad,
+                   Roi roi, T scale);
+
+// march's projection output with the tilt's 2D shift changed by (sx, sy) pixels, applied in T
+// precision to every sub-ray origin (c -= sx*u + sy*v, as Projection::GetPosition subtracts the
+// shifted pitch vectors, Projection.cpp:183-195).  For finite differences in shift.
+template <class T>
+void march_shifted(const VolGeom& vg, const TiltGeom& tg, const T* vol, int W, int H, size_t pitch, T* proj,
+                   Roi roi, double sx, double sy);
 
 // CopyToSquare.cu:37-92 (makeSquare)
 template <class T>

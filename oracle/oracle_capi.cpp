@@ -60,6 +60,14 @@ void oracle_glibc_rand(unsigned seed, int count, int* out) { glibc_rand_sequence
                                    int filter) {                                                                \
         back_project<T>(*vg, *tg, vol, img, W, H, pitch, lambda, os, distMin, distMax, filter);                 \
     }                                                                                                           \
+    void oracle_march_adjoint_##SUF(const VolGeom* vg, const TiltGeom* tg, const T* resid, int W, int H,      \
+                                    size_t pitch, T* grad, const int* roi, double scale) {                  \
+        march_adjoint<T>(*vg, *tg, resid, W, H, pitch, grad, Roi{roi[0], roi[1], roi[2], roi[3]}, (T)scale); \
+    }                                                                                                       \
+    void oracle_march_shifted_##SUF(const VolGeom* vg, const TiltGeom* tg, const T* vol, int W, int H,        \
+                                    size_t pitch, T* proj, const int* roi, double sx, double sy) {          \
+        march_shifted<T>(*vg, *tg, vol, W, H, pitch, proj, Roi{roi[0], roi[1], roi[2], roi[3]}, sx, sy);    \
+    }                                                                                                       \
     void oracle_make_square_##SUF(int W, int H, size_t pitch, const T* in, int N, T* out, int bx, int by,       \
                                   int mirrorY, int fillZero) {                                                  \
         make_square<T>(W, H, pitch, in, N, out, bx, by, mirrorY != 0, fillZero != 0);                          \

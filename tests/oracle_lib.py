@@ -58,6 +58,10 @@ def load() -> C.CDLL:
         getattr(lib, f"oracle_make_square_{suf}").argtypes = [C.c_int, C.c_int, C.c_size_t, P, C.c_int, P, C.c_int,
                                                                C.c_int, C.c_int, C.c_int]
         getattr(lib, f"oracle_sart_{suf}").argtypes = [P, P, P, C.c_int, P, P, C.c_int, C.c_int, P, P, C.c_int, P]
+        getattr(lib, f"oracle_march_adjoint_{suf}").argtypes = [P, P, P, C.c_int, C.c_int, C.c_size_t, P, P,
+                                                                 C.c_double]
+        getattr(lib, f"oracle_march_shifted_{suf}").argtypes = [P, P, P, C.c_int, C.c_int, C.c_size_t, P, P,
+                                                                 C.c_double, C.c_double]
     lib.oracle_real_length.restype = C.c_float
     lib.oracle_ctf_wavelength.restype = C.c_float
     lib.oracle_ctf_wavelength.argtypes = [C.c_float]
@@ -138,6 +142,27 @@ def march(vg, tg, vol, W, H, dtype=np.float32, roi=None, proj=None):
     r = (C.c_int * 4)(*(roi or (0, 0, W, H)))
     getattr(load(), f"oracle_march_{_suffix(dtype)}")(ref(vg), ref(tg), _p(vol), W, H, W, _p(proj), _p(dist), r)
     return proj, dist
+
+
+def march_adjoint(vg, tg, resid, dtype=np.float64, roi=None, scale=1.0):
+    """scale * P^T resid for march's projection output P."""
+    H, W = resid.shape
+    resid = np.ascontiguousarray(resid, dtype=dtype)
+    grad = np.zeros((vg.nz, vg.ny, vg.nx), dtype)
+    r = (C.c_int * 4)(*(roi or (0, 0, W, H)))
+    getattr(load(), f"oracle_march_adjoint_{_suffix(dtype)}")(ref(vg), ref(tg), _p(resid), W, H, W, _p(grad), r,
+                                                              float(scale))
+    return grad
+
+
+def march_shifted(vg, tg, vol, W, H, sx, sy, dtype=np.float64, roi=None):
+    """march's projection with the tilt shift changed by (sx, sy) pixels in T precision."""
+    vol = np.ascontiguousarray(vol, dtype=dtype)
+    proj = np.zeros((H, W), dtype)
+    r = (C.c_int * 4)(*(roi or (0, 0, W, H)))
+    getattr(load(), f"oracle_march_shifted_{_suffix(dtype)}")(ref(vg), ref(tg), _p(vol), W, H, W, _p(proj), r,
+                                                              float(sx), float(sy))
+    return proj
 
 
 def slicer(vg, tg, vol, W, H, tmin, tmax, dtype=np.float32, roi=None):
