@@ -95,21 +95,35 @@ def project(vol: torch.Tensor, shifts: torch.Tensor, setup: ProjectionSetup) -> 
 
 
 def refine_shifts(vol: torch.Tensor, target: torch.Tensor, setup: ProjectionSetup, steps: int = 50,
-                  lr: float = 0.2, init: Optional[torch.Tensor] = None) -> torch.Tensor:
+                  init: Optional[torch.Tensor] = None, first_step_px: float = 0.1,
+                  max_step_px: float = 2.0) -> torch.Tensor:
     """Gradient-based replacement of EmSARTRefine's cross-correlation shift search: minimises
-    ||P(vol, s) - target||^2 over the per-tilt shifts s with Adam, starting from `init` (zeros)."""
+    ||P(vol, s) - target||^2 over the per-tilt shifts s with `steps` gradient evaluations.  A tilt's
+    shift changes only its own projection, so the problem separates per tilt; each tilt takes
+    Barzilai-Borwein steps (first step `first_step_px` along its gradient, every step capped at
+    `max_step_px`)."""
     T = len(setup.tilts_deg)
-    s = (init.clone() if init is not None else torch.zeros((T, 2), dtype=torch.float64, device="cuda"))
-    s.requires_grad_(True)
-    opt = torch.optim.Adam([s], lr=lr)
-    sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=0.94)
-    for _ in range(steps):
-        opt.zero_grad()
-        loss = ((project(vol, s, setup) - target) ** 2).sum()
-        loss.backward()
-        opt.step()
-        sched.step()
-    return s.detach()
+    s = (init.clone() if init is not None else torch.zeros((T, 2), dtype=torch.float64, device="cuda")).detach()
+
+    def grad_at(x):
+        x = x.clone().requires_grad_(True)
+        ((project(vol, x, setup) - target) ** 2).sum().backward()
+        return x.grad.detach()
+
+    g = grad_at(s)
+    gn = g.norm(dim=1, keepdim=True)
+    alpha = torch.where(gn > 0, first_step_px / gn.clamp_min(1e-300), torch.zeros_like(gn))
+    for _ in range(steps - 1):
+        d = alpha * g
+        dn = d.norm(dim=1, keepdim=True)
+        d = torch.where(dn > max_step_px, d * (max_step_px / dn.clamp_min(1e-300)), d)
+        s_new = s - d
+        g_new = grad_at(s_new)
+        ds, dg = s_new - s, g_new - g
+        den = (ds * dg).sum(1, keepdim=True)
+        alpha = torch.where(den > 0, (ds * ds).sum(1, keepdim=True) / den.clamp_min(1e-300), alpha)
+        s, g = s_new, g_new
+    return s
 
 
 __all__ = ["ProjectionSetup", "project", "refine_shifts"]

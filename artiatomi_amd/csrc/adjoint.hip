@@ -3,8 +3,8 @@
 // is SART's normalised update, not the transpose of its ray marcher.  Here
 //   P x [pixel] = 0.25 * step * sum over the pixel's 4 sub-rays and their samples q_k of trilinear(x, q_k)
 // is the forward projector of fp.hip (march_exact_kernel), and
-//   fp_adjoint:    vol_grad += scale * P^T r, scattering each sample's trilinear weights along the
-//                  same sub-rays and samples (recomputed, never stored);
+//   fp_adjoint:    vol_grad += scale * P^T r, the transpose of the forward projector's per-cell
+//                  closed form along the same sub-rays and cells (recomputed, never stored);
 //   fp_shift_grad: out += sum_pixels g[pixel] * dP[pixel]/d(shift_x, shift_y), where the tilt's 2D
 //                  shift moves the detector origin by -shift_x*u - shift_y*v (Projection.cpp:183-195)
 //                  and with it every sample: dq_k/ds = scale * (-u + (dt_in/ds) n) per sub-ray.
@@ -39,17 +39,10 @@ __device__ __forceinline__ Cell cell_of(const VolG& v, float xb, float yb, float
     return q;
 }
 
-__device__ __forceinline__ void flush_cell(float* __restrict__ grad, size_t base, size_t sy, size_t sz, float r,
-                                           const float w[8]) {
-    const size_t off[8] = {0, 1, sy, sy + 1, sz, sz + 1, sz + sy, sz + sy + 1};
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const float add = r * w[i];
-        if (add != 0.0f) unsafeAtomicAdd(grad + base + off[i], add);
-    }
-}
-
-// One lane per sub-ray, the forward projector's mapping (8x8 pixels per 256-thread block).
+// One lane per sub-ray, the forward projector's mapping (8x8 pixels per 256-thread block).  The
+// lane walks the same cells as the forward projector's march_sum and scatters the transpose of its
+// per-cell closed form (cell_weights) with one set of eight atomics per cell.
+template <bool kBig>
 __global__ void __launch_bounds__(256) fp_adjoint_kernel(VolG v, TiltG g, const float* __restrict__ resid,
                                                          size_t pitch, Roi roi, float scale,
                                                          float* __restrict__ grad) {
@@ -59,37 +52,24 @@ __global__ void __launch_bounds__(256) fp_adjoint_kernel(VolG v, TiltG g, const 
     const int y = roi.y0 + (int)blockIdx.y * 8 + (wave >> 1) * 4 + (pix >> 2);
     if (x >= roi.x1 || y >= roi.y1) return;
     const double step = (double)v.vs[0] * 0.15;
-    const float r = resid[(size_t)y * pitch + (size_t)x] * scale * (float)(step * 0.25);
-    if (r == 0.0f) return;
+    // proj = (float)(0.25 * step * sum): the same factor per sample
+    const double r = (double)resid[(size_t)y * pitch + (size_t)x] * (double)scale * (step * 0.25);
+    if (r == 0.0) return;
     SubraySamples s;
     const double px = (double)x + ((sub & 2) ? 0.75 : 0.25);
     const double py = (double)y + ((sub & 1) ? 0.75 : 0.25);
     if (!subray_samples(v, g, px, py, step, s)) return;
-    const size_t sy = (size_t)v.nx, sz = (size_t)v.nx * (size_t)v.ny;
-    size_t cur = ~(size_t)0;
-    float w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < s.count; k++) {
-        const float fk = (float)k;
-        const Cell q = cell_of(v, __builtin_fmaf(fk, s.dq[0], s.q0[0]), __builtin_fmaf(fk, s.dq[1], s.q0[1]),
-                               __builtin_fmaf(fk, s.dq[2], s.q0[2]));
-        const size_t base = (size_t)q.kb * sz + (size_t)q.jb * sy + (size_t)q.ib;
-        if (base != cur) {
-            if (cur != ~(size_t)0) flush_cell(grad, cur, sy, sz, r, w);
-            cur = base;
+    for_each_cell(s.q0, s.dq, s.count, [&](const int cell[3], int k, int mi) {
+        size_t o[8];
+        cell_corners<kBig>(v, cell, o);
+        double W[8];
+        cell_weights(cell_moments(s.q0, s.dq, cell, k, mi), W);
 #pragma unroll
-            for (int i = 0; i < 8; i++) w[i] = 0.0f;
+        for (int i = 0; i < 8; i++) {
+            const float add = (float)(r * W[i]);
+            if (add != 0.0f) unsafeAtomicAdd(grad + o[i], add);
         }
-        const float a1 = 1.0f - q.a, b1 = 1.0f - q.b, c1 = 1.0f - q.c;
-        w[0] += a1 * b1 * c1;
-        w[1] += q.a * b1 * c1;
-        w[2] += a1 * q.b * c1;
-        w[3] += q.a * q.b * c1;
-        w[4] += a1 * b1 * q.c;
-        w[5] += q.a * b1 * q.c;
-        w[6] += a1 * q.b * q.c;
-        w[7] += q.a * q.b * q.c;
-    }
-    if (cur != ~(size_t)0) flush_cell(grad, cur, sy, sz, r, w);
+    });
 }
 
 // Gradient of the trilinear read with respect to the (shifted) voxel coordinates; zero along an
@@ -215,8 +195,15 @@ artia_status artia_fp_adjoint(artia_ctx* ctx, const artia_vol_geom* vg, const ar
     const Roi r = clip_roi(roi, width, height);
     Timing t(ctx, ms);
     if (r.x1 > r.x0 && r.y1 > r.y0)
-        hipLaunchKernelGGL(fp_adjoint_kernel, grid_of(r), dim3(256), 0, ctx->stream, to_device(*vg), to_device(*tg),
-                           resid, pitch, r, scale, vol_grad);
+    {
+        const bool big = (double)vg->nx * (double)vg->ny * (double)vg->nz >= 1073741824.0;
+        if (big)
+            hipLaunchKernelGGL(fp_adjoint_kernel<true>, grid_of(r), dim3(256), 0, ctx->stream, to_device(*vg),
+                               to_device(*tg), resid, pitch, r, scale, vol_grad);
+        else
+            hipLaunchKernelGGL(fp_adjoint_kernel<false>, grid_of(r), dim3(256), 0, ctx->stream, to_device(*vg),
+                               to_device(*tg), resid, pitch, r, scale, vol_grad);
+    }
     st = check_hip(ctx, hipGetLastError(), "artia_fp_adjoint launch");
     if (st != ARTIA_OK) return st;
     return t.finish();

@@ -9,9 +9,10 @@
 //   reference  the kernel bodies in the reference's float operation order, including the
 //              float accumulation t += 0.15*s at |t| ~ 1e4 that biases the step
 //              (ForwardProjectionRayMarcher_TL.cu:130-151);
-//   exact      the ray/box intersection in double, samples at t_in + k*0.15*s generated in
-//              float relative to the entry point, values summed in a double accumulator;
-//              this tracks the double-precision evaluation of the same algorithm.
+//   exact      the ray/box intersection in double, samples at t_in + k*0.15*s as float
+//              coordinates affine in k relative to the entry point, and their trilinear values
+//              summed cell by cell in closed form (march_sum); this tracks the double-precision
+//              evaluation of the same algorithm.
 #include "device_common.hpp"
 #include "fp_common.hpp"
 
@@ -105,30 +106,15 @@ __global__ void __launch_bounds__(256) vol_traversal_ref_kernel(VolG v, TiltG g,
 
 // ---------------------------------------------------------------- exact numerics
 
-// Sum of the trilinear samples q0 + k*dq, k = 0..count-1 (coordinates shifted by -0.5).  Two
-// samples are in flight per iteration; float partial sums are folded into double every 256.
+// Sum of the trilinear samples q0 + k*dq, k = 0..count-1 (coordinates shifted by -0.5, texel j
+// centred at j), cell by cell in closed form (cell_sum, fp_common.hpp).
 template <bool kBig>
 __device__ __forceinline__ double march_sum(const VolG& v, const float* __restrict__ vol, const float q0[3],
                                             const float dq[3], int count) {
     double total = 0.0;
-    for (int k0 = 0; k0 < count; k0 += 256) {
-        const int k1 = min(count, k0 + 256);
-        float s0 = 0.0f, s1 = 0.0f;
-        int k = k0;
-        for (; k + 1 < k1; k += 2) {
-            const float f0 = (float)k, f1 = (float)(k + 1);
-            s0 += tri_pairs<kBig>(vol, v.nx, v.ny, v.nz, __builtin_fmaf(f0, dq[0], q0[0]),
-                                  __builtin_fmaf(f0, dq[1], q0[1]), __builtin_fmaf(f0, dq[2], q0[2]));
-            s1 += tri_pairs<kBig>(vol, v.nx, v.ny, v.nz, __builtin_fmaf(f1, dq[0], q0[0]),
-                                  __builtin_fmaf(f1, dq[1], q0[1]), __builtin_fmaf(f1, dq[2], q0[2]));
-        }
-        if (k < k1) {
-            const float f0 = (float)k;
-            s0 += tri_pairs<kBig>(vol, v.nx, v.ny, v.nz, __builtin_fmaf(f0, dq[0], q0[0]),
-                                  __builtin_fmaf(f0, dq[1], q0[1]), __builtin_fmaf(f0, dq[2], q0[2]));
-        }
-        total += (double)s0 + (double)s1;
-    }
+    for_each_cell(q0, dq, count, [&](const int cell[3], int k, int mi) {
+        total += cell_sum<kBig>(v, vol, q0, dq, cell, k, mi);
+    });
     return total;
 }
 

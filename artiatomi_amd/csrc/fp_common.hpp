@@ -116,4 +116,134 @@ __device__ __forceinline__ bool subray_samples(const VolG& v, const TiltG& g, do
     return true;
 }
 
+// Cells of the texel lattice crossed by the samples q0 + k*dq, k = 0..count-1, in order: calls
+// f(cell, k, m) for every cell holding samples k..k+m-1.  A 3D DDA in double; a sample on a face is
+// given to either neighbour (the trilinear interpolant is continuous across faces).  Every step
+// either consumes samples or moves the walk forward by a positive distance, and the walk is
+// bounded by 4 * count + 64 steps.
+template <class F>
+__device__ __forceinline__ void for_each_cell(const float q0[3], const float dq[3], int count, F&& f) {
+    int cell[3], sgn[3];
+    double kb[3], st[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        cell[i] = (int)floorf(q0[i]);
+        sgn[i] = dq[i] > 0.0f ? 1 : (dq[i] < 0.0f ? -1 : 0);
+        if (sgn[i] == 0) {
+            kb[i] = INFINITY;
+            st[i] = INFINITY;
+        } else {
+            const double b = (double)(cell[i] + (sgn[i] > 0 ? 1 : 0));
+            kb[i] = (b - (double)q0[i]) / (double)dq[i];
+            st[i] = 1.0 / fabs((double)dq[i]);
+        }
+    }
+    int k = 0;
+    for (int guard = 0; k < count && guard < 4 * count + 64; guard++) {
+        const double kn = fmin(fmin(kb[0], kb[1]), kb[2]);
+        const int kend = kn < (double)count ? max((int)ceil(kn), 0) : count;
+        if (kend > k) {
+            f(cell, k, kend - k);
+            k = kend;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (kb[i] == kn) {
+                cell[i] += sgn[i];
+                kb[i] += st[i];
+            }
+        }
+    }
+}
+
+// The m samples of one cell: centre coordinates relative to the cell and the second moment of the
+// sample offsets about the centre, S2c = sum_j (j - (m-1)/2)^2 = m (m^2 - 1) / 12 (exact).  Centred
+// on the middle sample the odd moments vanish, so with the cell interpolant
+//   f = v000 + A a + B b + C c + AB ab + AC ac + BC bc + ABC abc
+// the samples sum to  m f(centre) + S2c (AB da db + AC da dc + BC db dc + ABC (a db dc + b da dc + c da db)).
+struct CellMoments {
+    float a, b, c, S2c, m;
+    float pab, pac, pbc, q;   // da db, da dc, db dc, a db dc + b da dc + c da db
+};
+
+__device__ __forceinline__ CellMoments cell_moments(const float q0[3], const float dq[3], const int cell[3], int k,
+                                                    int mi) {
+    CellMoments cm;
+    const float kc = (float)k + 0.5f * (float)(mi - 1);
+    cm.a = __builtin_fmaf(kc, dq[0], q0[0]) - (float)cell[0];
+    cm.b = __builtin_fmaf(kc, dq[1], q0[1]) - (float)cell[1];
+    cm.c = __builtin_fmaf(kc, dq[2], q0[2]) - (float)cell[2];
+    cm.m = (float)mi;
+    cm.S2c = (float)(mi * (mi * mi - 1)) / 12.0f;   // integer or half-integer: exact
+    cm.pab = dq[0] * dq[1];
+    cm.pac = dq[0] * dq[2];
+    cm.pbc = dq[1] * dq[2];
+    cm.q = cm.a * cm.pbc + cm.b * cm.pac + cm.c * cm.pab;
+    return cm;
+}
+
+// Corner offsets of a cell (clamped indices: clamp addressing), order 000 100 010 110 001 101 011 111.
+template <bool kBig>
+__device__ __forceinline__ void cell_corners(const VolG& v, const int cell[3], size_t off[8]) {
+    const int x0 = clampi(cell[0], 0, v.nx - 1), x1 = clampi(cell[0] + 1, 0, v.nx - 1);
+    const int y0 = clampi(cell[1], 0, v.ny - 1), y1 = clampi(cell[1] + 1, 0, v.ny - 1);
+    const int z0 = clampi(cell[2], 0, v.nz - 1), z1 = clampi(cell[2] + 1, 0, v.nz - 1);
+    if (kBig) {
+        const size_t sy = (size_t)v.nx, sz = (size_t)v.nx * (size_t)v.ny;
+        const size_t r00 = (size_t)z0 * sz + (size_t)y0 * sy, r10 = (size_t)z0 * sz + (size_t)y1 * sy;
+        const size_t r01 = (size_t)z1 * sz + (size_t)y0 * sy, r11 = (size_t)z1 * sz + (size_t)y1 * sy;
+        off[0] = r00 + x0; off[1] = r00 + x1; off[2] = r10 + x0; off[3] = r10 + x1;
+        off[4] = r01 + x0; off[5] = r01 + x1; off[6] = r11 + x0; off[7] = r11 + x1;
+    } else {
+        const uint32_t ny32 = (uint32_t)v.ny, nx32 = (uint32_t)v.nx;
+        const uint32_t r00 = __umul24(__umul24((uint32_t)z0, ny32) + (uint32_t)y0, nx32);
+        const uint32_t r10 = __umul24(__umul24((uint32_t)z0, ny32) + (uint32_t)y1, nx32);
+        const uint32_t r01 = __umul24(__umul24((uint32_t)z1, ny32) + (uint32_t)y0, nx32);
+        const uint32_t r11 = __umul24(__umul24((uint32_t)z1, ny32) + (uint32_t)y1, nx32);
+        off[0] = r00 + x0; off[1] = r00 + x1; off[2] = r10 + x0; off[3] = r10 + x1;
+        off[4] = r01 + x0; off[5] = r01 + x1; off[6] = r11 + x0; off[7] = r11 + x1;
+    }
+}
+
+// Sum of one cell's m samples: the centre value in double (its rounding is multiplied by m), the
+// small second-order correction in float.
+template <bool kBig>
+__device__ __forceinline__ double cell_sum(const VolG& v, const float* __restrict__ vol, const float q0[3],
+                                           const float dq[3], const int cell[3], int k, int mi) {
+    size_t o[8];
+    cell_corners<kBig>(v, cell, o);
+    const float v000 = vol[o[0]], v100 = vol[o[1]], v010 = vol[o[2]], v110 = vol[o[3]];
+    const float v001 = vol[o[4]], v101 = vol[o[5]], v011 = vol[o[6]], v111 = vol[o[7]];
+    const CellMoments cm = cell_moments(q0, dq, cell, k, mi);
+    const double ad = cm.a, bd = cm.b, cd = cm.c;
+    const double c00 = fma(ad, (double)v100 - (double)v000, (double)v000);
+    const double c10 = fma(ad, (double)v110 - (double)v010, (double)v010);
+    const double c01 = fma(ad, (double)v101 - (double)v001, (double)v001);
+    const double c11 = fma(ad, (double)v111 - (double)v011, (double)v011);
+    const double e0 = fma(bd, c10 - c00, c00), e1 = fma(bd, c11 - c01, c01);
+    const double f = fma(cd, e1 - e0, e0);
+    const float AB = (v110 - v100) - (v010 - v000), AC = (v101 - v100) - (v001 - v000);
+    const float BC = (v011 - v010) - (v001 - v000);
+    const float ABC = ((v111 - v110) - (v101 - v100)) - ((v011 - v010) - (v001 - v000));
+    const float corr = AB * cm.pab + AC * cm.pac + BC * cm.pbc + ABC * cm.q;
+    return fma((double)mi, f, (double)(cm.S2c * corr));
+}
+
+// The same sum as weights on the eight corners (cell_sum = sum_c v_c W_c): the transpose of
+// cell_sum, W_c = m w_c(centre) + S2c kappa_c with w_c the trilinear weights and kappa_c the
+// corner coefficients of the correction.
+__device__ __forceinline__ void cell_weights(const CellMoments& cm, double W[8]) {
+    const double a = cm.a, b = cm.b, c = cm.c, m = cm.m, s = cm.S2c;
+    const double a1 = 1.0 - a, b1 = 1.0 - b, c1 = 1.0 - c;
+    const double pab = cm.pab, pac = cm.pac, pbc = cm.pbc, q = cm.q;
+    W[0] = m * (a1 * b1 * c1) + s * (pab + pac + pbc - q);
+    W[1] = m * (a * b1 * c1) + s * (-pab - pac + q);
+    W[2] = m * (a1 * b * c1) + s * (-pab - pbc + q);
+    W[3] = m * (a * b * c1) + s * (pab - q);
+    W[4] = m * (a1 * b1 * c) + s * (-pac - pbc + q);
+    W[5] = m * (a * b1 * c) + s * (pac - q);
+    W[6] = m * (a1 * b * c) + s * (pbc - q);
+    W[7] = m * (a * b * c) + s * q;
+}
+
 }  // namespace artia
