@@ -8,10 +8,12 @@ from .core import (  # noqa: F401
     Acquisition,
     ArtiaContext,
     ArtiaError,
+    PairSet,
     SubvolPlan,
     ctf_params,
     defocus_distances,
     defocus_minmax,
+    footprint_window,
     hitpoints,
     projection_index_list,
     real_length,
@@ -24,9 +26,9 @@ from .core import (  # noqa: F401
 )
 
 __all__ = [
-    "Acquisition", "ArtiaContext", "ArtiaError", "CtfParams", "Subvol", "SubvolPlan", "SubvolTuning",
+    "Acquisition", "ArtiaContext", "ArtiaError", "CtfParams", "PairSet", "Subvol", "SubvolPlan", "SubvolTuning",
     "subvol_wbp_plain", "subvolume_tilt_geometry", "GlobalParams", "TiltDefocus",
-    "ctf_params", "defocus_distances", "defocus_minmax", "SartParams", "TiltGeom", "TiltParams",
+    "ctf_params", "defocus_distances", "defocus_minmax", "footprint_window", "SartParams", "TiltGeom", "TiltParams",
     "VolGeom", "hitpoints", "load", "projection_index_list", "real_length", "sart_params",
     "subvolume_geometry", "tilt_geometry", "volume_geometry",
 ]

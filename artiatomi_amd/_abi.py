@@ -97,6 +97,19 @@ class SubvolTuning(C.Structure):
                 ("blocks_per_cu", C.c_int)]
 
 
+class PairVolume(C.Structure):
+    """artia_pair_volume: a sub-volume's placement and its float offset in the volume buffer."""
+
+    _fields_ = [("geom", VolGeom), ("offset", C.c_longlong)]
+
+
+class Pair(C.Structure):
+    """artia_pair: sub-volume index, detector window and the window image's float offset."""
+
+    _fields_ = [("volume", C.c_int), ("x0", C.c_int), ("y0", C.c_int), ("w", C.c_int), ("h", C.c_int),
+                ("offset", C.c_longlong)]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _F = C.c_float
@@ -145,6 +158,14 @@ SIGNATURES = {
     "artia_subvol_plan_destroy": (_I, [_P]),
     "artia_fp_adjoint": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _SZ, _P, _PI, _F, _PF]),
     "artia_fp_shift_grad": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _P, _I, _I, _SZ, _PI, _P, _PF]),
+    "artia_pairs_create": (_I, [_P, C.POINTER(PairVolume), _I, C.POINTER(Pair), C.POINTER(TiltGeom), _I,
+                                C.POINTER(_P)]),
+    "artia_pairs_destroy": (_I, [_P]),
+    "artia_pairs_project": (_I, [_P, _P, _P, _P, _PF]),
+    "artia_pairs_loss_grad": (_I, [_P, _P, _P, _P, _P, _P, _P, _PF]),
+    "artia_pairs_shift_grad": (_I, [_P, _P, _P, _P, _P, _PF]),
+    "artia_pairs_adjoint": (_I, [_P, _P, _P, _F, _P, _PF]),
+    "artia_pairs_cc_search": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _PF]),
     "artia_profile_enable": (_I, [_P, _I]),
     "artia_profile_read": (_I, [_P, _I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "artia_sart_run": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _I, _PI, _P, _I, _I, _P,

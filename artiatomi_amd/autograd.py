@@ -94,22 +94,11 @@ def project(vol: torch.Tensor, shifts: torch.Tensor, setup: ProjectionSetup) -> 
     return _Project.apply(vol, shifts, setup)
 
 
-def refine_shifts(vol: torch.Tensor, target: torch.Tensor, setup: ProjectionSetup, steps: int = 50,
-                  init: Optional[torch.Tensor] = None, first_step_px: float = 0.1,
-                  max_step_px: float = 2.0) -> torch.Tensor:
-    """Gradient-based replacement of EmSARTRefine's cross-correlation shift search: minimises
-    ||P(vol, s) - target||^2 over the per-tilt shifts s with `steps` gradient evaluations.  A tilt's
-    shift changes only its own projection, so the problem separates per tilt; each tilt takes
-    Barzilai-Borwein steps (first step `first_step_px` along its gradient, every step capped at
-    `max_step_px`)."""
-    T = len(setup.tilts_deg)
-    s = (init.clone() if init is not None else torch.zeros((T, 2), dtype=torch.float64, device="cuda")).detach()
-
-    def grad_at(x):
-        x = x.clone().requires_grad_(True)
-        ((project(vol, x, setup) - target) ** 2).sum().backward()
-        return x.grad.detach()
-
+def _bb_descent(grad_at, s0: torch.Tensor, steps: int, first_step_px: float, max_step_px: float) -> torch.Tensor:
+    """Per-row Barzilai-Borwein descent on a separable objective: `steps` evaluations of
+    grad_at(s) -> (n, 2); the first step is `first_step_px` along each row's gradient, every step
+    is capped at `max_step_px`."""
+    s = s0.detach().clone()
     g = grad_at(s)
     gn = g.norm(dim=1, keepdim=True)
     alpha = torch.where(gn > 0, first_step_px / gn.clamp_min(1e-300), torch.zeros_like(gn))
@@ -126,4 +115,82 @@ def refine_shifts(vol: torch.Tensor, target: torch.Tensor, setup: ProjectionSetu
     return s
 
 
-__all__ = ["ProjectionSetup", "project", "refine_shifts"]
+def refine_shifts(vol: torch.Tensor, target: torch.Tensor, setup: ProjectionSetup, steps: int = 50,
+                  init: Optional[torch.Tensor] = None, first_step_px: float = 0.1,
+                  max_step_px: float = 2.0) -> torch.Tensor:
+    """Gradient-based replacement of EmSARTRefine's cross-correlation shift search: minimises
+    ||P(vol, s) - target||^2 over the per-tilt shifts s with `steps` gradient evaluations.  A tilt's
+    shift changes only its own projection, so the problem separates per tilt; each tilt takes
+    Barzilai-Borwein steps (first step `first_step_px` along its gradient, every step capped at
+    `max_step_px`)."""
+    T = len(setup.tilts_deg)
+    s0 = init.clone() if init is not None else torch.zeros((T, 2), dtype=torch.float64, device="cuda")
+
+    def grad_at(x):
+        x = x.clone().requires_grad_(True)
+        ((project(vol, x, setup) - target) ** 2).sum().backward()
+        return x.grad.detach()
+
+    return _bb_descent(grad_at, s0, steps, first_step_px, max_step_px)
+
+
+class _ProjectPairs(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, volumes: torch.Tensor, shifts: torch.Tensor, pairs: "core.PairSet"):
+        volumes = volumes.contiguous()
+        s = shifts.detach().to(torch.float64).contiguous()
+        out = pairs.project(volumes, s)
+        ctx.save_for_backward(volumes, s)
+        ctx.pairs = pairs
+        ctx.shift_dtype = shifts.dtype
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        volumes, s = ctx.saved_tensors
+        pairs: core.PairSet = ctx.pairs
+        g = grad_out.contiguous().to(torch.float32)
+        gvol = gshift = None
+        if ctx.needs_input_grad[0]:
+            gvol = torch.zeros_like(volumes)
+            pairs.adjoint(s, g, gvol)
+        if ctx.needs_input_grad[1]:
+            gshift = pairs.shift_grad(volumes, s, g).to(ctx.shift_dtype)
+        return gvol, gshift, None
+
+
+def project_pairs(volumes: torch.Tensor, shifts: torch.Tensor, pairs: "core.PairSet") -> torch.Tensor:
+    """Packed window images of every (sub-volume, tilt) pair of `pairs` (a core.PairSet): sub-volumes
+    in the flat float32 buffer `volumes`, per-pair 2D shifts (npairs, 2).  Differentiable in both."""
+    return _ProjectPairs.apply(volumes, shifts, pairs)
+
+
+def refine_pair_shifts(volumes: torch.Tensor, targets: torch.Tensor, pairs: "core.PairSet", steps: int = 50,
+                       init: Optional[torch.Tensor] = None, first_step_px: float = 0.1,
+                       max_step_px: float = 2.0) -> torch.Tensor:
+    """Per-(sub-volume, tilt) shifts minimising ||P_s - target||^2 of every pair, with `steps`
+    evaluations of the fused loss-and-gradient launch (artia_pairs_loss_grad) and per-pair
+    Barzilai-Borwein steps as refine_shifts."""
+    s0 = init.to(torch.float64) if init is not None else torch.zeros((pairs.npairs, 2), dtype=torch.float64,
+                                                                     device=volumes.device)
+    return _bb_descent(lambda x: pairs.loss_grad(volumes, x.contiguous(), targets)[1], s0, steps, first_step_px,
+                       max_step_px)
+
+
+def cc_search_shifts(volumes: torch.Tensor, targets: torch.Tensor, pairs: "core.PairSet", square: int,
+                     max_shift: int, init: Optional[torch.Tensor] = None):
+    """EmSARTRefine's search (GetDisplacement, Reconstructor.cpp:1746-1959) for every pair at once:
+    the models are the pairs projected at `init` (zero by default); a pair whose target matches its
+    model moved by d gets the shift init + d (integer d within the max_shift disk).  Pairs whose
+    search fails keep `init`.  Returns (shifts (npairs, 2) float64, peak CC (npairs,), failed mask)."""
+    s0 = init.to(torch.float64).contiguous() if init is not None else torch.zeros(
+        (pairs.npairs, 2), dtype=torch.float64, device=volumes.device)
+    models = pairs.project(volumes, s0)
+    peak, cc, _ = pairs.cc_search(models, targets, square, max_shift)
+    failed = peak[:, 0] == -1000
+    s = torch.where(failed[:, None], s0, s0 - peak.to(torch.float64))
+    return s, cc, failed
+
+
+__all__ = ["ProjectionSetup", "cc_search_shifts", "project", "project_pairs", "refine_pair_shifts",
+           "refine_shifts"]

@@ -23,6 +23,7 @@ SOURCES = [
     "spectral.hip",
     "subvol.hip",
     "adjoint.hip",
+    "refine.hip",
     "sart.hip",
 ]
 

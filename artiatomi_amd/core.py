@@ -83,6 +83,31 @@ def subvolume_tilt_geometry(tilt_deg: float, width: int, height: int, psi_deg: f
     return tilt_geometry(tilt_deg, width, height, psi_deg=psi_deg, mag=mag, shift=(sx, sy), glob=glob)
 
 
+def footprint_window(vg: VolGeom, tg: TiltGeom, width: int, height: int, margin: int = 0) -> tuple:
+    """Detector window (x0, y0, w, h) holding the projection of the volume's bounding box at tilt
+    geometry tg, widened by `margin` pixels and clipped to the width x height detector: the eight
+    box corners through the detector matrix (as BackProjectionSquareOS.cu:246-253 maps a point to
+    a pixel) and the inverse anisotropic magnification.  (0, 0, 0, 0) when nothing is left."""
+    import math
+
+    M = list(tg.det_matrix)
+    Ai = list(tg.mag_aniso_inv)
+    xs, ys = [], []
+    for c in range(8):
+        p = [vg.bbox_max[i] if (c >> i) & 1 else vg.bbox_min[i] for i in range(3)]
+        a = M[0] * p[0] + M[1] * p[1] + M[2] * p[2] + M[3]
+        b = M[4] * p[0] + M[5] * p[1] + M[6] * p[2] + M[7]
+        xs.append(Ai[0] * a + Ai[1] * b + Ai[2])
+        ys.append(Ai[3] * a + Ai[4] * b + Ai[5])
+    x0 = max(0, int(math.floor(min(xs))) - margin)
+    y0 = max(0, int(math.floor(min(ys))) - margin)
+    x1 = min(width, int(math.ceil(max(xs))) + 1 + margin)
+    y1 = min(height, int(math.ceil(max(ys))) + 1 + margin)
+    if x1 <= x0 or y1 <= y0:
+        return (0, 0, 0, 0)
+    return (x0, y0, x1 - x0, y1 - y0)
+
+
 def hitpoints(full: VolGeom, tg: TiltGeom) -> list:
     lib = _abi.load()
     out = (C.c_int * 8)()
@@ -453,6 +478,134 @@ class SubvolPlan:
     def close(self):
         if getattr(self, "h", None):
             self.ctx.lib.artia_subvol_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _tptr(t, name: str, dtype, numel: int) -> int:
+    """Device pointer of a contiguous CUDA tensor of `dtype` holding at least `numel` elements."""
+    import torch
+
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != dtype or not t.is_contiguous():
+        raise ArtiaError(f"{name} must be a contiguous {dtype} CUDA tensor")
+    if t.numel() < numel:
+        raise ArtiaError(f"{name} holds {t.numel()} elements, needs {numel}")
+    return t.data_ptr()
+
+
+class PairSet:
+    """All (sub-volume, tilt) pairs of a refinement step, each operation one launch over all of
+    them (artia_pairs_*).  Sub-volumes share one flat float32 buffer; window images are packed
+    in pair order into flat buffers of `n_pixels` floats; shifts are (npairs, 2) float64.
+
+    volumes: sequence of (VolGeom, float offset into the volume buffer); pairs: sequence of
+    (volume index, x0, y0, w, h) detector windows; geoms: one TiltGeom per pair."""
+
+    def __init__(self, ctx: "ArtiaContext", volumes, pairs, geoms):
+        import torch  # noqa: F401  (device buffers are torch tensors)
+
+        self.ctx = ctx
+        nv, n = len(volumes), len(pairs)
+        if len(geoms) != n:
+            raise ArtiaError(f"{n} pairs need {n} geometries, got {len(geoms)}")
+        vol_arr = (_abi.PairVolume * nv)()
+        self.vol_floats = 0
+        self.volumes = []
+        for k, (vg, off) in enumerate(volumes):
+            vol_arr[k].geom = vg
+            vol_arr[k].offset = int(off)
+            self.volumes.append((vg, int(off)))
+            self.vol_floats = max(self.vol_floats, int(off) + vg.nx * vg.ny * vg.nz)
+        pair_arr = (_abi.Pair * n)()
+        self.windows = []
+        off = 0
+        for p, (vi, x0, y0, w, h) in enumerate(pairs):
+            q = pair_arr[p]
+            q.volume, q.x0, q.y0, q.w, q.h, q.offset = int(vi), int(x0), int(y0), int(w), int(h), off
+            self.windows.append((int(vi), int(x0), int(y0), int(w), int(h), off))
+            off += int(w) * int(h)
+        self.n_pixels = off
+        self.npairs = n
+        geo_arr = (TiltGeom * n)(*geoms)
+        h = C.c_void_p()
+        _check(ctx.lib.artia_pairs_create(ctx.h, vol_arr, nv, pair_arr, geo_arr, n, C.byref(h)), ctx.h,
+               "artia_pairs_create")
+        self.h = h
+
+    def _shifts(self, shifts):
+        import torch
+
+        if shifts is None:
+            return None
+        return _tptr(shifts, "shifts", torch.float64, 2 * self.npairs)
+
+    def _run(self, name, *args, timed=False):
+        ms = C.c_float() if timed else None
+        _check(getattr(self.ctx.lib, name)(self.h, *args, C.byref(ms) if timed else None), self.ctx.h, name)
+        return ms.value if timed else None
+
+    def window(self, buf, p: int):
+        """(h, w) view of pair p's window in a packed image buffer."""
+        _, _, _, w, h, off = self.windows[p]
+        return buf[off:off + w * h].view(h, w)
+
+    def project(self, volumes, shifts=None, out=None, timed=False):
+        """Packed window images P_s volumes (a new buffer unless `out` is given)."""
+        import torch
+
+        if out is None:
+            out = torch.empty(self.n_pixels, device=volumes.device, dtype=torch.float32)
+        ms = self._run("artia_pairs_project", _ptr(volumes, "volumes", self.vol_floats), self._shifts(shifts),
+                       _ptr(out, "images", self.n_pixels), timed=timed)
+        return (out, ms) if timed else out
+
+    def loss_grad(self, volumes, shifts, targets, resid=None, timed=False):
+        """Per pair sum (P_s - targets)^2 and its shift gradient, (npairs,) and (npairs, 2) float64."""
+        import torch
+
+        loss = torch.empty(self.npairs, device=volumes.device, dtype=torch.float64)
+        grad = torch.empty((self.npairs, 2), device=volumes.device, dtype=torch.float64)
+        ms = self._run("artia_pairs_loss_grad", _ptr(volumes, "volumes", self.vol_floats), self._shifts(shifts),
+                       _ptr(targets, "targets", self.n_pixels), loss.data_ptr(), grad.data_ptr(),
+                       None if resid is None else _ptr(resid, "resid", self.n_pixels), timed=timed)
+        return (loss, grad, ms) if timed else (loss, grad)
+
+    def shift_grad(self, volumes, shifts, weights, timed=False):
+        """Per pair sum weights * dP_s/ds, (npairs, 2) float64."""
+        import torch
+
+        grad = torch.empty((self.npairs, 2), device=volumes.device, dtype=torch.float64)
+        ms = self._run("artia_pairs_shift_grad", _ptr(volumes, "volumes", self.vol_floats), self._shifts(shifts),
+                       _ptr(weights, "weights", self.n_pixels), grad.data_ptr(), timed=timed)
+        return (grad, ms) if timed else grad
+
+    def adjoint(self, shifts, images, vol_grads, scale=1.0, timed=False):
+        """vol_grads += scale * P_s^T images."""
+        return self._run("artia_pairs_adjoint", self._shifts(shifts), _ptr(images, "images", self.n_pixels),
+                         float(scale), _ptr(vol_grads, "vol_grads", self.vol_floats), timed=timed)
+
+    def cc_search(self, models, targets, square: int, max_shift: int, want_map=False, timed=False):
+        """GetDisplacement's search per pair: (peak (npairs, 2) int32, cc (npairs,) float32, map or None)."""
+        import torch
+
+        dev = models.device
+        peak = torch.empty((self.npairs, 2), device=dev, dtype=torch.int32)
+        cc = torch.empty(self.npairs, device=dev, dtype=torch.float32)
+        side = 2 * int(max_shift) + 1
+        ccmap = torch.empty((self.npairs, side, side), device=dev, dtype=torch.float32) if want_map else None
+        ms = self._run("artia_pairs_cc_search", _ptr(models, "models", self.n_pixels),
+                       _ptr(targets, "targets", self.n_pixels), int(square), int(max_shift), peak.data_ptr(),
+                       cc.data_ptr(), None if ccmap is None else ccmap.data_ptr(), timed=timed)
+        return (peak, cc, ccmap, ms) if timed else (peak, cc, ccmap)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.artia_pairs_destroy(self.h)
             self.h = None
 
     def __del__(self):

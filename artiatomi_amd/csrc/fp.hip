@@ -112,9 +112,27 @@ template <bool kBig>
 __device__ __forceinline__ double march_sum(const VolG& v, const float* __restrict__ vol, const float q0[3],
                                             const float dq[3], int count) {
     double total = 0.0;
-    for_each_cell(q0, dq, count, [&](const int cell[3], int k, int mi) {
-        total += cell_sum<kBig>(v, vol, q0, dq, cell, k, mi);
-    });
+    if (kBig) {
+        for_each_cell(q0, dq, count, [&](const int cell[3], int k, int mi) {
+            size_t o[8];
+            cell_corners<true>(v, cell, o);
+            float val[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) val[i] = vol[o[i]];
+            total += cell_sum_values(val, cell_moments(q0, dq, cell, k, mi), mi);
+        });
+    } else {
+        const __amdgpu_buffer_rsrc_t rs = volume_rsrc(vol, v);
+        for_each_cell(q0, dq, count, [&](const int cell[3], int k, int mi) {
+            uint32_t o[8];
+            cell_corners_bytes(v, cell, o);
+            float val[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++)   // the b32 load returns the word's bits
+                val[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)o[i], 0, 0));
+            total += cell_sum_values(val, cell_moments(q0, dq, cell, k, mi), mi);
+        });
+    }
     return total;
 }
 
@@ -241,7 +259,9 @@ void launch_march_impl(artia_ctx* ctx, const VolG& v, const TiltG& g, const floa
         return;
     }
     const dim3 grid((unsigned)((r.x1 - r.x0 + 7) / 8), (unsigned)((r.y1 - r.y0 + 7) / 8)), block(256);
-    const bool big = (uint64_t)v.nx * (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 30);
+    // 32-bit buffer offsets need fewer than 2^30 voxels and a row index z * ny + y below 2^24
+    const bool big = (uint64_t)v.nx * (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 30) ||
+                     (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 24) || v.nx >= (1 << 24);
     if (slicer) {
         if (big)
             hipLaunchKernelGGL((march_exact_kernel<true, true>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,

@@ -73,7 +73,9 @@ __device__ __forceinline__ float tri_pairs(const float* __restrict__ vol, int nx
 
 // Sample grid of one sub-ray of the forward projector: samples at t_in + k*step for
 // k = 0..count-1 (t <= t_out), as voxel coordinates shifted by -0.5 (texel j centred at j):
-// q_k = q0 + k*dq.  Returns false when the sub-ray misses the volume.
+// q_k = q0 + k*dq.  Returns false when the sub-ray misses the volume.  (sx, sy) is an extra 2D
+// detector shift in pixels, applied in double to the sub-ray origin (c -= sx*u + sy*v, as the
+// marker shifts enter the detector origin, Projection.cpp:183-195); zero leaves the sub-ray as is.
 struct SubraySamples {
     float q0[3], dq[3];
     int count;
@@ -81,9 +83,9 @@ struct SubraySamples {
 };
 
 __device__ __forceinline__ bool subray_samples(const VolG& v, const TiltG& g, double px, double py, double step,
-                                               SubraySamples& s) {
-    const double xa = ((double)g.A[0] * px + (double)g.A[1] * py) + (double)g.A[2];
-    const double ya = ((double)g.A[3] * px + (double)g.A[4] * py) + (double)g.A[5];
+                                               SubraySamples& s, double sx = 0.0, double sy = 0.0) {
+    const double xa = (((double)g.A[0] * px + (double)g.A[1] * py) + (double)g.A[2]) - sx;
+    const double ya = (((double)g.A[3] * px + (double)g.A[4] * py) + (double)g.A[5]) - sy;
     double c[3], lo[3], hi[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -117,31 +119,29 @@ __device__ __forceinline__ bool subray_samples(const VolG& v, const TiltG& g, do
 }
 
 // Cells of the texel lattice crossed by the samples q0 + k*dq, k = 0..count-1, in order: calls
-// f(cell, k, m) for every cell holding samples k..k+m-1.  A 3D DDA in double; a sample on a face is
-// given to either neighbour (the trilinear interpolant is continuous across faces).  Every step
-// either consumes samples or moves the walk forward by a positive distance, and the walk is
-// bounded by 4 * count + 64 steps.
+// f(cell, k, m) for every cell holding samples k..k+m-1.  The sample index of the next crossing on
+// axis i is recomputed from the cell index, (face - q0_i) / dq_i, as one fma (its float error, a
+// few 1e-4 samples for rays of thousands of samples, only moves a sample lying within that distance
+// of a face to the neighbouring cell, where the continuous trilinear interpolant has the same value).
+// Every step either consumes samples or advances an axis; the walk is bounded by 4 * count + 64
+// steps.
 template <class F>
 __device__ __forceinline__ void for_each_cell(const float q0[3], const float dq[3], int count, F&& f) {
     int cell[3], sgn[3];
-    double kb[3], st[3];
+    float inv[3], off[3], kb[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         cell[i] = (int)floorf(q0[i]);
         sgn[i] = dq[i] > 0.0f ? 1 : (dq[i] < 0.0f ? -1 : 0);
-        if (sgn[i] == 0) {
-            kb[i] = INFINITY;
-            st[i] = INFINITY;
-        } else {
-            const double b = (double)(cell[i] + (sgn[i] > 0 ? 1 : 0));
-            kb[i] = (b - (double)q0[i]) / (double)dq[i];
-            st[i] = 1.0 / fabs((double)dq[i]);
-        }
+        inv[i] = sgn[i] != 0 ? 1.0f / dq[i] : 0.0f;
+        off[i] = -q0[i] * inv[i];
+        kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]) : INFINITY;
     }
+    const float fcount = (float)count;
     int k = 0;
     for (int guard = 0; k < count && guard < 4 * count + 64; guard++) {
-        const double kn = fmin(fmin(kb[0], kb[1]), kb[2]);
-        const int kend = kn < (double)count ? max((int)ceil(kn), 0) : count;
+        const float kn = fminf(fminf(kb[0], kb[1]), kb[2]);
+        const int kend = kn < fcount ? max((int)ceilf(kn), 0) : count;
         if (kend > k) {
             f(cell, k, kend - k);
             k = kend;
@@ -150,7 +150,7 @@ __device__ __forceinline__ void for_each_cell(const float q0[3], const float dq[
         for (int i = 0; i < 3; i++) {
             if (kb[i] == kn) {
                 cell[i] += sgn[i];
-                kb[i] += st[i];
+                kb[i] = __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]);
             }
         }
     }
@@ -174,7 +174,7 @@ __device__ __forceinline__ CellMoments cell_moments(const float q0[3], const flo
     cm.b = __builtin_fmaf(kc, dq[1], q0[1]) - (float)cell[1];
     cm.c = __builtin_fmaf(kc, dq[2], q0[2]) - (float)cell[2];
     cm.m = (float)mi;
-    cm.S2c = (float)(mi * (mi * mi - 1)) / 12.0f;   // integer or half-integer: exact
+    cm.S2c = (float)((mi * (mi * mi - 1)) / 6) * 0.5f;   // (m-1) m (m+1) is a multiple of 6: exact
     cm.pab = dq[0] * dq[1];
     cm.pac = dq[0] * dq[2];
     cm.pbc = dq[1] * dq[2];
@@ -182,39 +182,56 @@ __device__ __forceinline__ CellMoments cell_moments(const float q0[3], const flo
     return cm;
 }
 
-// Corner offsets of a cell (clamped indices: clamp addressing), order 000 100 010 110 001 101 011 111.
+// Corner offsets of a cell (clamped indices: clamp addressing), order 000 100 010 110 001 101 011 111:
+// the 000 corner plus per-axis steps, each 0 where the index is clamped.
 template <bool kBig>
 __device__ __forceinline__ void cell_corners(const VolG& v, const int cell[3], size_t off[8]) {
-    const int x0 = clampi(cell[0], 0, v.nx - 1), x1 = clampi(cell[0] + 1, 0, v.nx - 1);
-    const int y0 = clampi(cell[1], 0, v.ny - 1), y1 = clampi(cell[1] + 1, 0, v.ny - 1);
-    const int z0 = clampi(cell[2], 0, v.nz - 1), z1 = clampi(cell[2] + 1, 0, v.nz - 1);
-    if (kBig) {
-        const size_t sy = (size_t)v.nx, sz = (size_t)v.nx * (size_t)v.ny;
-        const size_t r00 = (size_t)z0 * sz + (size_t)y0 * sy, r10 = (size_t)z0 * sz + (size_t)y1 * sy;
-        const size_t r01 = (size_t)z1 * sz + (size_t)y0 * sy, r11 = (size_t)z1 * sz + (size_t)y1 * sy;
-        off[0] = r00 + x0; off[1] = r00 + x1; off[2] = r10 + x0; off[3] = r10 + x1;
-        off[4] = r01 + x0; off[5] = r01 + x1; off[6] = r11 + x0; off[7] = r11 + x1;
-    } else {
-        const uint32_t ny32 = (uint32_t)v.ny, nx32 = (uint32_t)v.nx;
-        const uint32_t r00 = __umul24(__umul24((uint32_t)z0, ny32) + (uint32_t)y0, nx32);
-        const uint32_t r10 = __umul24(__umul24((uint32_t)z0, ny32) + (uint32_t)y1, nx32);
-        const uint32_t r01 = __umul24(__umul24((uint32_t)z1, ny32) + (uint32_t)y0, nx32);
-        const uint32_t r11 = __umul24(__umul24((uint32_t)z1, ny32) + (uint32_t)y1, nx32);
-        off[0] = r00 + x0; off[1] = r00 + x1; off[2] = r10 + x0; off[3] = r10 + x1;
-        off[4] = r01 + x0; off[5] = r01 + x1; off[6] = r11 + x0; off[7] = r11 + x1;
-    }
+    const int x0 = clampi(cell[0], 0, v.nx - 1), y0 = clampi(cell[1], 0, v.ny - 1), z0 = clampi(cell[2], 0, v.nz - 1);
+    const size_t sy = (size_t)v.nx, sz = (size_t)v.nx * (size_t)v.ny;
+    const size_t dx = (cell[0] + 1 > x0 && x0 + 1 < v.nx) ? 1 : 0;
+    const size_t dy = (cell[1] + 1 > y0 && y0 + 1 < v.ny) ? sy : 0;
+    const size_t dz = (cell[2] + 1 > z0 && z0 + 1 < v.nz) ? sz : 0;
+    off[0] = (size_t)z0 * sz + (size_t)y0 * sy + (size_t)x0;
+    off[1] = off[0] + dx;
+    off[2] = off[0] + dy;
+    off[3] = off[2] + dx;
+    off[4] = off[0] + dz;
+    off[5] = off[4] + dx;
+    off[6] = off[4] + dy;
+    off[7] = off[6] + dx;
+}
+
+// The same offsets in 32 bits, in bytes, for volumes below 2^30 voxels with ny * nz < 2^24 (buffer
+// loads; the row index z * ny + y then fits the 24-bit multiplier).
+__device__ __forceinline__ void cell_corners_bytes(const VolG& v, const int cell[3], uint32_t off[8]) {
+    const int x0 = clampi(cell[0], 0, v.nx - 1), y0 = clampi(cell[1], 0, v.ny - 1), z0 = clampi(cell[2], 0, v.nz - 1);
+    const uint32_t sy = (uint32_t)v.nx * 4u, sz = (uint32_t)v.nx * (uint32_t)v.ny * 4u;
+    const uint32_t dx = (cell[0] + 1 > x0 && x0 + 1 < v.nx) ? 4u : 0u;
+    const uint32_t dy = (cell[1] + 1 > y0 && y0 + 1 < v.ny) ? sy : 0u;
+    const uint32_t dz = (cell[2] + 1 > z0 && z0 + 1 < v.nz) ? sz : 0u;
+    const uint32_t row = __umul24((uint32_t)z0, (uint32_t)v.ny) + (uint32_t)y0;
+    off[0] = (__umul24(row, (uint32_t)v.nx) + (uint32_t)x0) * 4u;
+    off[1] = off[0] + dx;
+    off[2] = off[0] + dy;
+    off[3] = off[2] + dx;
+    off[4] = off[0] + dz;
+    off[5] = off[4] + dx;
+    off[6] = off[4] + dy;
+    off[7] = off[6] + dx;
+}
+
+// Buffer resource of a volume for 32-bit byte offsets (raw buffer, gfx9 descriptor word 3); loads
+// beyond the volume's bytes return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t volume_rsrc(const float* vol, const VolG& v) {
+    const uint32_t bytes = (uint32_t)v.nx * (uint32_t)v.ny * (uint32_t)v.nz * 4u;
+    return __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, (int)bytes, 0x00020000);
 }
 
 // Sum of one cell's m samples: the centre value in double (its rounding is multiplied by m), the
 // small second-order correction in float.
-template <bool kBig>
-__device__ __forceinline__ double cell_sum(const VolG& v, const float* __restrict__ vol, const float q0[3],
-                                           const float dq[3], const int cell[3], int k, int mi) {
-    size_t o[8];
-    cell_corners<kBig>(v, cell, o);
-    const float v000 = vol[o[0]], v100 = vol[o[1]], v010 = vol[o[2]], v110 = vol[o[3]];
-    const float v001 = vol[o[4]], v101 = vol[o[5]], v011 = vol[o[6]], v111 = vol[o[7]];
-    const CellMoments cm = cell_moments(q0, dq, cell, k, mi);
+__device__ __forceinline__ double cell_sum_values(const float val[8], const CellMoments& cm, int mi) {
+    const float v000 = val[0], v100 = val[1], v010 = val[2], v110 = val[3];
+    const float v001 = val[4], v101 = val[5], v011 = val[6], v111 = val[7];
     const double ad = cm.a, bd = cm.b, cd = cm.c;
     const double c00 = fma(ad, (double)v100 - (double)v000, (double)v000);
     const double c10 = fma(ad, (double)v110 - (double)v010, (double)v010);
@@ -244,6 +261,25 @@ __device__ __forceinline__ void cell_weights(const CellMoments& cm, double W[8])
     W[5] = m * (a * b1 * c) + s * (pac - q);
     W[6] = m * (a1 * b * c) + s * (pbc - q);
     W[7] = m * (a * b * c) + s * q;
+}
+
+// Sum over the cell's m samples of the interpolant's gradient (d/da, d/db, d/dc in texel units).
+// With samples a_j = a + (j - (m-1)/2) da about the centre (likewise b, c), d f/da = A + AB b + AC c
+// + ABC b c sums to m (d f/da)(centre) + S2c ABC db dc, and likewise for b and c.  Where a corner
+// index is clamped its step vanishes, and with it the derivative along that axis, as for the
+// clamp-addressed read.  Accumulated into G.
+__device__ __forceinline__ void cell_grad_sum(const float val[8], const CellMoments& cm, int mi, double G[3]) {
+    const float v000 = val[0], v100 = val[1], v010 = val[2], v110 = val[3];
+    const float v001 = val[4], v101 = val[5], v011 = val[6], v111 = val[7];
+    const float ABC = ((v111 - v110) - (v101 - v100)) - ((v011 - v010) - (v001 - v000));
+    // d f / d a at the centre: the x-differences interpolated in b, then c (likewise for b and c)
+    const float ga = lerpf(lerpf(v100 - v000, v110 - v010, cm.b), lerpf(v101 - v001, v111 - v011, cm.b), cm.c);
+    const float gb = lerpf(lerpf(v010 - v000, v110 - v100, cm.a), lerpf(v011 - v001, v111 - v101, cm.a), cm.c);
+    const float gc = lerpf(lerpf(v001 - v000, v101 - v100, cm.a), lerpf(v011 - v010, v111 - v110, cm.a), cm.b);
+    const float s = cm.S2c * ABC;
+    G[0] += (double)__builtin_fmaf(cm.m, ga, s * cm.pbc);
+    G[1] += (double)__builtin_fmaf(cm.m, gb, s * cm.pac);
+    G[2] += (double)__builtin_fmaf(cm.m, gc, s * cm.pab);
 }
 
 }  // namespace artia

@@ -248,6 +248,72 @@ artia_status artia_fp_shift_grad(artia_ctx* ctx, const artia_vol_geom* vg, const
                                  const float* vol, const float* grad_proj, int width, int height, size_t pitch,
                                  const int* roi, double* out, float* ms);
 
+/* ---------------------------------------------------------------- batched refinement (pair sets)
+ * EmSARTRefine projects each motive's sub-volume at each tilt (ForwardProjectionROI,
+ * EmSartRefine.cpp:809-858) and searches that tilt's 2D shift by cross-correlation
+ * (GetDisplacement, Reconstructor.cpp:1746-1959), one call sequence per (sub-volume, tilt).  A
+ * pair set keeps all (sub-volume, tilt) pairs of such a step on the device and runs each
+ * operation below over all of them in one launch (one workgroup per pair, the pairs of one
+ * sub-volume kept on one XCD).  No reference counterpart as an interface.
+ *
+ * Sub-volumes live in one device buffer, each a dense [z][y][x] fp32 array at its own float
+ * offset, with 2 .. 2^24 - 1 voxels per axis, below 2^30 voxels and ny * nz < 2^24.  Pair p
+ * projects sub-volume pairs[p].volume through geoms[p] (artia_geom_tilt of its tilt, with the
+ * sub-volume's own shifts added to the marker shift) into the detector window
+ * x0 <= x < x0 + w, y0 <= y < y0 + h; window images are packed row-major (w floats a row) at
+ * pairs[p].offset floats into image buffers, which must hold every window.  Each operation takes
+ * `shifts`: 2 doubles per pair on the device, or NULL for zero.  The refinement shift s moves the
+ * pair's detector origin by -s_x u - s_y v, applied in double to every sub-ray origin (as the
+ * marker shifts enter the origin, Projection.cpp:183-195), so that P_s[x] = P_0[x - s].  P is the
+ * exact projector of artia_fp_march restricted to the window,
+ *   (P vol)[pixel] = (float)(0.25 * step * sum over the pixel's 4 sub-rays and their samples),
+ * bit-identical to artia_fp_march's projection (added to a zero image) at s = 0. */
+typedef struct {
+    artia_vol_geom geom;  /* sub-volume placement */
+    long long offset;     /* float offset of its array in the volume buffer */
+} artia_pair_volume;
+typedef struct {
+    int volume;           /* index into the pair set's sub-volumes */
+    int x0, y0, w, h;     /* detector window */
+    long long offset;     /* float offset of the window image in image buffers */
+} artia_pair;
+typedef struct artia_pairs artia_pairs;
+artia_status artia_pairs_create(artia_ctx* ctx, const artia_pair_volume* volumes, int nvol, const artia_pair* pairs,
+                                const artia_tilt_geom* geoms, int npairs, artia_pairs** out);
+artia_status artia_pairs_destroy(artia_pairs* ps);
+/* images[window of p] = P_s(p) volumes */
+artia_status artia_pairs_project(artia_pairs* ps, const float* volumes, const double* shifts, float* images,
+                                 float* ms);
+/* Per pair, with r = P_s volumes - targets over its window: loss[p] = sum r^2 and
+ * grad[2p], grad[2p+1] = d loss / d s = 2 sum r dP_s/ds (double, overwritten).  resid (or NULL)
+ * receives r in float.  One pass: every cell a sub-ray crosses is read once for the value and the
+ * shift derivative, both summed in closed form over the cell's samples. */
+artia_status artia_pairs_loss_grad(artia_pairs* ps, const float* volumes, const double* shifts, const float* targets,
+                                   double* loss, double* grad, float* resid, float* ms);
+/* grad[2p], grad[2p+1] = sum over the window of weights * dP_s/ds (double, overwritten) */
+artia_status artia_pairs_shift_grad(artia_pairs* ps, const float* volumes, const double* shifts,
+                                    const float* weights, double* grad, float* ms);
+/* vol_grads += scale * P_s^T images: every pair's window scattered into its sub-volume's
+ * gradient (same layout as the volume buffer) by the transpose of P's per-cell closed form, with
+ * device float atomics. */
+artia_status artia_pairs_adjoint(artia_pairs* ps, const double* shifts, const float* images, float scale,
+                                 float* vol_grads, float* ms);
+/* GetDisplacement's shift search (Reconstructor.cpp:1746-1959) per pair over its window.  A is the
+ * pair's target window, B its model window; both count as zero elsewhere in the reference's
+ * N x N square (N = `square`, max(width, height)).  With the square's means and
+ * SumA = sum (A - mean A)^2 (likewise B), the normalised correlation
+ *   CC(r) = (sum_x A(x) B(x + r) - sum A sum B / N^2) / sqrt(SumA SumB)
+ * is searched over the integer shifts with sqrtf(rx^2 + ry^2) <= max_shift (maxShift,
+ * wbpWeighting.cu:265-294); the largest wins, ties to the first in the square's row-major order
+ * (shifts wrapped modulo N, nppiMaxIndx).  It equals the reference's FFT correlation when the
+ * window lies in the square with w, h <= N - 2 max_shift.  peak[2p], peak[2p+1] = (rx, ry): the
+ * target matches the model moved by -r, so the shift estimate is s = -r.  Both are -1000 (the
+ * reference's failure value) when the best CC is <= 0 or SumA SumB is 0.  cc_value[p] = the best
+ * CC; cc_map (or NULL) receives (2 max_shift + 1)^2 floats per pair, [ry + max_shift][rx +
+ * max_shift], 0 outside the disk.  0 <= max_shift <= 32. */
+artia_status artia_pairs_cc_search(artia_pairs* ps, const float* models, const float* targets, int square,
+                                   int max_shift, int* peak, float* cc_value, float* cc_map, float* ms);
+
 /* ---------------------------------------------------------------- drivers
  * EmSART's ordered-subset SART loop (EmSart.cpp:579-675) without CTF, device resident:
  * for every subset, per tilt FP (volTraversalLength while the volume is empty) -> compare
