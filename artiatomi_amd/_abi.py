@@ -162,7 +162,7 @@ SIGNATURES = {
                                 C.POINTER(_P)]),
     "artia_pairs_destroy": (_I, [_P]),
     "artia_pairs_project": (_I, [_P, _P, _P, _P, _PF]),
-    "artia_pairs_loss_grad": (_I, [_P, _P, _P, _P, _P, _P, _P, _PF]),
+    "artia_pairs_loss_grad": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _PF]),
     "artia_pairs_shift_grad": (_I, [_P, _P, _P, _P, _P, _PF]),
     "artia_pairs_adjoint": (_I, [_P, _P, _P, _F, _P, _PF]),
     "artia_pairs_cc_search": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _PF]),

@@ -94,15 +94,18 @@ def project(vol: torch.Tensor, shifts: torch.Tensor, setup: ProjectionSetup) -> 
     return _Project.apply(vol, shifts, setup)
 
 
-def _bb_descent(grad_at, s0: torch.Tensor, steps: int, first_step_px: float, max_step_px: float) -> torch.Tensor:
+def _bb_descent(grad_at, s0: torch.Tensor, steps: int, first_step_px: float, max_step_px: float,
+                callback=None) -> torch.Tensor:
     """Per-row Barzilai-Borwein descent on a separable objective: `steps` evaluations of
     grad_at(s) -> (n, 2); the first step is `first_step_px` along each row's gradient, every step
-    is capped at `max_step_px`."""
+    is capped at `max_step_px`.  callback(k, s) sees the iterate after evaluation k."""
     s = s0.detach().clone()
     g = grad_at(s)
+    if callback:
+        callback(1, s)
     gn = g.norm(dim=1, keepdim=True)
     alpha = torch.where(gn > 0, first_step_px / gn.clamp_min(1e-300), torch.zeros_like(gn))
-    for _ in range(steps - 1):
+    for k in range(steps - 1):
         d = alpha * g
         dn = d.norm(dim=1, keepdim=True)
         d = torch.where(dn > max_step_px, d * (max_step_px / dn.clamp_min(1e-300)), d)
@@ -112,6 +115,8 @@ def _bb_descent(grad_at, s0: torch.Tensor, steps: int, first_step_px: float, max
         den = (ds * dg).sum(1, keepdim=True)
         alpha = torch.where(den > 0, (ds * ds).sum(1, keepdim=True) / den.clamp_min(1e-300), alpha)
         s, g = s_new, g_new
+        if callback:
+            callback(k + 2, s)
     return s
 
 
@@ -165,16 +170,56 @@ def project_pairs(volumes: torch.Tensor, shifts: torch.Tensor, pairs: "core.Pair
     return _ProjectPairs.apply(volumes, shifts, pairs)
 
 
+def _lm_descent(eval_at, s0: torch.Tensor, steps: int, mu0: float = 1e-3, callback=None) -> torch.Tensor:
+    """Per-row Levenberg-Marquardt on a separable least-squares objective: `steps` evaluations of
+    eval_at(s) -> (loss (n,), grad (n, 2) = 2 J^T r, J^T J (n, 3) as xx, xy, yy).  A row's step
+    solves (J^T J + mu diag(J^T J)) d = -J^T r; it is kept when the loss drops (mu / 3), otherwise
+    discarded (mu * 4).  callback(k, s) sees the best iterate after evaluation k."""
+    s = s0.detach().clone()
+    loss, g, n = eval_at(s)
+    mu = torch.full_like(loss, mu0)
+    if callback:
+        callback(1, s)
+    for k in range(steps - 1):
+        a = n[:, 0] * (1.0 + mu)
+        b = n[:, 1]
+        c = n[:, 2] * (1.0 + mu)
+        det = a * c - b * b
+        ok = det > 0
+        det = torch.where(ok, det, torch.ones_like(det))
+        hx, hy = 0.5 * g[:, 0], 0.5 * g[:, 1]
+        d = torch.stack([(c * hx - b * hy) / det, (a * hy - b * hx) / det], dim=1)
+        d = torch.where(ok[:, None], d, torch.zeros_like(d))
+        s_new = s - d
+        l2, g2, n2 = eval_at(s_new)
+        acc = l2 < loss
+        s = torch.where(acc[:, None], s_new, s)
+        loss = torch.where(acc, l2, loss)
+        g = torch.where(acc[:, None], g2, g)
+        n = torch.where(acc[:, None], n2, n)
+        mu = torch.where(acc, mu / 3.0, mu * 4.0)
+        if callback:
+            callback(k + 2, s)
+    return s
+
+
 def refine_pair_shifts(volumes: torch.Tensor, targets: torch.Tensor, pairs: "core.PairSet", steps: int = 50,
-                       init: Optional[torch.Tensor] = None, first_step_px: float = 0.1,
-                       max_step_px: float = 2.0) -> torch.Tensor:
+                       init: Optional[torch.Tensor] = None, method: str = "gn", first_step_px: float = 0.1,
+                       max_step_px: float = 2.0, callback=None) -> torch.Tensor:
     """Per-(sub-volume, tilt) shifts minimising ||P_s - target||^2 of every pair, with `steps`
-    evaluations of the fused loss-and-gradient launch (artia_pairs_loss_grad) and per-pair
-    Barzilai-Borwein steps as refine_shifts."""
+    evaluations of the fused loss-and-gradient launch (artia_pairs_loss_grad).  method "gn":
+    per-pair Levenberg-Marquardt on the launch's gradient 2 J^T r and Gauss-Newton matrix J^T J
+    (J = dP_s/ds, first derivatives only); "bb": per-pair Barzilai-Borwein steps as
+    refine_shifts.  callback(k, s) sees the iterate after evaluation k."""
     s0 = init.to(torch.float64) if init is not None else torch.zeros((pairs.npairs, 2), dtype=torch.float64,
                                                                      device=volumes.device)
-    return _bb_descent(lambda x: pairs.loss_grad(volumes, x.contiguous(), targets)[1], s0, steps, first_step_px,
-                       max_step_px)
+    if method == "gn":
+        return _lm_descent(lambda x: pairs.loss_grad(volumes, x.contiguous(), targets, normal=True), s0, steps,
+                           callback=callback)
+    if method == "bb":
+        return _bb_descent(lambda x: pairs.loss_grad(volumes, x.contiguous(), targets)[1], s0, steps, first_step_px,
+                           max_step_px, callback=callback)
+    raise core.ArtiaError(f"unknown refinement method {method!r} (gn or bb)")
 
 
 def cc_search_shifts(volumes: torch.Tensor, targets: torch.Tensor, pairs: "core.PairSet", square: int,

@@ -564,16 +564,20 @@ class PairSet:
                        _ptr(out, "images", self.n_pixels), timed=timed)
         return (out, ms) if timed else out
 
-    def loss_grad(self, volumes, shifts, targets, resid=None, timed=False):
-        """Per pair sum (P_s - targets)^2 and its shift gradient, (npairs,) and (npairs, 2) float64."""
+    def loss_grad(self, volumes, shifts, targets, resid=None, normal=False, timed=False):
+        """Per pair sum (P_s - targets)^2 and its shift gradient, (npairs,) and (npairs, 2) float64;
+        with normal=True also J^T J as (npairs, 3) float64 (xx, xy, yy), J = dP_s/ds."""
         import torch
 
         loss = torch.empty(self.npairs, device=volumes.device, dtype=torch.float64)
         grad = torch.empty((self.npairs, 2), device=volumes.device, dtype=torch.float64)
+        jtj = torch.empty((self.npairs, 3), device=volumes.device, dtype=torch.float64) if normal else None
         ms = self._run("artia_pairs_loss_grad", _ptr(volumes, "volumes", self.vol_floats), self._shifts(shifts),
                        _ptr(targets, "targets", self.n_pixels), loss.data_ptr(), grad.data_ptr(),
+                       None if jtj is None else jtj.data_ptr(),
                        None if resid is None else _ptr(resid, "resid", self.n_pixels), timed=timed)
-        return (loss, grad, ms) if timed else (loss, grad)
+        res = (loss, grad, jtj) if normal else (loss, grad)
+        return res + (ms,) if timed else res
 
     def shift_grad(self, volumes, shifts, weights, timed=False):
         """Per pair sum weights * dP_s/ds, (npairs, 2) float64."""

@@ -42,6 +42,7 @@ struct PairArgs {
     float* out;              // images (kProject), residuals (kLossGrad, may be null)
     double* loss;
     double* grad;
+    double* normal;          // J^T J per pair (kLossGrad, may be null)
 };
 
 // Slot of workgroup b: XCD b % 8 takes slots [xcd * per_xcd, (xcd + 1) * per_xcd).
@@ -57,7 +58,8 @@ __device__ __forceinline__ void load_cell(const __amdgpu_buffer_rsrc_t rs, const
         val[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)o[i], 0, 0));
 }
 
-// Block sum of three doubles in a fixed order; the result is valid in thread 0.
+// Block sum of three doubles in a fixed order; the result is valid in thread 0.  Ends with a
+// barrier, so it can be called again right away.
 __device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
     __shared__ double part[3][4];
     for (int o = 32; o > 0; o >>= 1) {
@@ -77,6 +79,7 @@ __device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
         b = ((part[1][0] + part[1][1]) + part[1][2]) + part[1][3];
         c = ((part[2][0] + part[2][1]) + part[2][2]) + part[2][3];
     }
+    __syncthreads();
 }
 
 template <int kMode>
@@ -91,7 +94,7 @@ __global__ void __launch_bounds__(256) pair_march_kernel(PairArgs a) {
     const int sub = lane & 3, pix = lane >> 2;
     const int tx = (wave & 1) * 4 + (pix & 3), ty = (wave >> 1) * 4 + (pix >> 2);
     const int ntx = (w + 7) >> 3, ntiles = ntx * ((h + 7) >> 3);
-    double lsum = 0.0, gx = 0.0, gy = 0.0;
+    double lsum = 0.0, gx = 0.0, gy = 0.0, jxx = 0.0, jxy = 0.0, jyy = 0.0;
     for (int t = 0; t < ntiles; t++) {
         // The geometry is re-read (scalar loads) per tile: hoisted out of the loop, its double
         // conversions would stay live in vector registers and halve the occupancy.
@@ -157,6 +160,9 @@ __global__ void __launch_bounds__(256) pair_march_kernel(PairArgs a) {
                 lsum += r * r;
                 gx += 2.0 * r * dpx;
                 gy += 2.0 * r * dpy;
+                jxx += dpx * dpx;
+                jxy += dpx * dpy;
+                jyy += dpy * dpy;
             } else {
                 gx += (double)win * dpx;
                 gy += (double)win * dpy;
@@ -169,6 +175,14 @@ __global__ void __launch_bounds__(256) pair_march_kernel(PairArgs a) {
         if (kMode == kLossGrad) a.loss[p] = lsum;
         a.grad[2 * p] = gx;
         a.grad[2 * p + 1] = gy;
+    }
+    if (kMode == kLossGrad && a.normal) {   // uniform branch
+        block_sum3(jxx, jxy, jyy);
+        if (threadIdx.x == 0) {
+            a.normal[3 * p] = jxx;
+            a.normal[3 * p + 1] = jxy;
+            a.normal[3 * p + 2] = jyy;
+        }
     }
 }
 
@@ -252,7 +266,6 @@ __global__ void __launch_bounds__(256) pair_cc_kernel(const PairDev* __restrict_
         tot[1] = saa;
         tot[2] = sb;
     }
-    __syncthreads();   // block_sum3's partials are reused below
     double d = 0.0, e = 0.0;
     block_sum3(sbb, d, e);
     if (threadIdx.x == 0) tot[3] = sbb;
@@ -427,7 +440,7 @@ artia_status artia_pairs_project(artia_pairs* ps, const float* volumes, const do
 }
 
 artia_status artia_pairs_loss_grad(artia_pairs* ps, const float* volumes, const double* shifts, const float* targets,
-                                   double* loss, double* grad, float* resid, float* ms) {
+                                   double* loss, double* grad, double* normal, float* resid, float* ms) {
     if (!ps) return ARTIA_ERR_INVALID_ARGUMENT;
     if (!volumes || !targets || !loss || !grad)
         return pairs_fail(ps, ARTIA_ERR_INVALID_ARGUMENT, "artia_pairs_loss_grad: null pointer");
@@ -439,6 +452,7 @@ artia_status artia_pairs_loss_grad(artia_pairs* ps, const float* volumes, const 
     a.out = resid;
     a.loss = loss;
     a.grad = grad;
+    a.normal = normal;
     Timing t(ctx, ms);
     hipLaunchKernelGGL(pair_march_kernel<kLossGrad>, pair_grid(ps), dim3(256), 0, ctx->stream, a);
     artia_status st = check_hip(ctx, hipGetLastError(), "artia_pairs_loss_grad launch");

@@ -47,6 +47,17 @@ SUBVOL_WORKLOADS = {
 }
 
 
+# Gradient-refinement workload (EmSARTRefine's per-(sub-volume, tilt) shift search): 64^3
+# sub-volumes with zero margins in a 512 x 512 x 128 field, 41 tilts of 512 x 512, true shifts
+# U[-3, 3] px per (sub-volume, tilt); 50 fused loss-and-gradient evaluations against the
+# integer cross-correlation search over the maxShift disk.
+REFINE_WORKLOADS = {
+    "refine_500subvols_64_512x512x128_41tilts": dict(field=(512, 512, 128), W=512, H=512, n=64, cubes=500,
+                                                     step=3.0, true_shift=3.0, max_shift=5, margin=8, evals=50,
+                                                     seed=9),
+}
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -257,13 +268,224 @@ def bench_subvol(args) -> int:
     return 0
 
 
+def refine_phantoms(n: int, count: int, seed: int):
+    """count seeded n^3 phantoms of three Gaussian blobs each, < 1e-6 at the faces (zero margins),
+    as one flat float32 CUDA buffer."""
+    import torch
+
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    g = (torch.arange(n, device="cuda", dtype=torch.float32) + 0.5) / n
+    z, y, x = torch.meshgrid(g, g, g, indexing="ij")
+    out = torch.empty((count, n, n, n), device="cuda")
+    for c0 in range(0, count, 50):
+        c1 = min(count, c0 + 50)
+        cen = 0.38 + 0.24 * torch.rand((c1 - c0, 3, 3), generator=gen, device="cuda")
+        amp = 0.5 + 0.5 * torch.rand((c1 - c0, 3), generator=gen, device="cuda")
+        acc = torch.zeros((c1 - c0, n, n, n), device="cuda")
+        for b in range(3):
+            d2 = ((x[None] - cen[:, b, 0, None, None, None]) ** 2 + (y[None] - cen[:, b, 1, None, None, None]) ** 2
+                  + (z[None] - cen[:, b, 2, None, None, None]) ** 2)
+            acc += amp[:, b, None, None, None] * torch.exp(-d2 / 0.09 ** 2)
+        out[c0:c1] = acc
+    return out.reshape(-1)
+
+
+def plain_gradient_sample(ctx, ps, vols, subs, targets, shifts, geoms, W, H, npairs):
+    """One gradient evaluation as per-pair calls (artia_fp_march + residual + artia_fp_shift_grad
+    on the pair's window), the schedule without pair sets, timed over the first npairs pairs."""
+    import torch
+
+    proj = torch.zeros((H, W), device="cuda")
+    dist = torch.zeros((H, W), device="cuda")
+    gimg = torch.zeros((H, W), device="cuda")
+    out = torch.zeros((npairs, 2), device="cuda", dtype=torch.float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p in range(npairs):
+        k, x0, y0, w, h, off = ps.windows[p]
+        vg, voff = subs[k]
+        vol = vols[voff:voff + vg.nx * vg.ny * vg.nz]
+        roi = (x0, y0, x0 + w, y0 + h)
+        g = geoms[p]
+        proj[y0:y0 + h, x0:x0 + w].zero_()
+        ctx.fp_march(vg, g, vol, proj, dist, roi=roi)
+        gimg[y0:y0 + h, x0:x0 + w] = 2.0 * (proj[y0:y0 + h, x0:x0 + w] - targets[off:off + w * h].view(h, w))
+        ctx.fp_shift_grad(vg, g, vol, gimg, out[p], roi=roi)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def plain_cc_sample(ps, models, targets, square, max_shift, npairs):
+    """GetDisplacement's per-pair schedule (Reconstructor.cpp:1746-1959: square images, means,
+    R2C of both, conj multiply, C2R, maxShift mask, max index) with torch.fft on the GPU, timed over
+    the first npairs pairs: the cost of the reference's full-square search on this GPU."""
+    import torch
+
+    N = square
+    idx = torch.arange(N, device="cuda", dtype=torch.float32)
+    m = torch.where(idx > N // 2, N - idx, idx)
+    mask = torch.sqrt(m[None, :] ** 2 + m[:, None] ** 2) <= max_shift
+    peaks = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p in range(npairs):
+        _, x0, y0, w, h, off = ps.windows[p]
+        a = torch.zeros((N, N), device="cuda")
+        b = torch.zeros((N, N), device="cuda")
+        a[y0:y0 + h, x0:x0 + w] = targets[off:off + w * h].view(h, w)
+        b[y0:y0 + h, x0:x0 + w] = models[off:off + w * h].view(h, w)
+        a -= a.mean()
+        b -= b.mean()
+        den = N * N * torch.sqrt((a * a).sum() * (b * b).sum())
+        cc = torch.fft.irfft2(torch.conj(torch.fft.rfft2(a)) * torch.fft.rfft2(b), s=(N, N)) * (N * N) / den
+        cc = torch.where(mask, cc, torch.zeros_like(cc))
+        peaks.append(int(torch.argmax(cc)))
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, peaks
+
+
+def bench_refine(args) -> int:
+    import numpy as np
+    import torch
+
+    import artiatomi_amd as A
+    from artiatomi_amd import autograd as AG
+    from artiatomi_amd import phantom
+
+    w = REFINE_WORKLOADS[args.workload]
+    rng = np.random.default_rng(w["seed"])
+    FX, FY, FZ = w["field"]
+    W, H, n, K = w["W"], w["H"], w["n"], w["cubes"]
+    tilts = phantom.tilt_series(-60.0, 60.0, w["step"])
+    T = len(tilts)
+    full = A.volume_geometry(FX, FY, FZ)
+    ctx = A.ArtiaContext(0)
+    t0 = time.time()
+    vols = refine_phantoms(n, K, w["seed"])
+    base = [A.tilt_geometry(t, W, H) for t in tilts]
+    subs, pairs, geoms = [], [], []
+    for k in range(K):
+        pos = rng.uniform([n / 2, n / 2, n / 2], [FX - n / 2, FY - n / 2, FZ - n / 2]) + 1.0
+        vg = A.subvolume_geometry(n, 1.0, (1.0, 1.0, 1.0), full, pos.tolist())
+        subs.append((vg, k * n ** 3))
+        for g in base:
+            x0, y0, ww, hh = A.footprint_window(vg, g, W, H, w["margin"])
+            pairs.append((k, x0, y0, ww, hh))
+            geoms.append(g)
+    ps = A.PairSet(ctx, subs, pairs, geoms)
+    P = ps.npairs
+    s_true = torch.from_numpy(rng.uniform(-w["true_shift"], w["true_shift"], (P, 2))).cuda()
+    targets = ps.project(vols, s_true)
+    torch.cuda.synchronize()
+    log(f"workload {args.workload}: {K} sub-volumes of {n}^3, {T} tilts, {P} pairs, {ps.n_pixels / 1e6:.1f} M "
+        f"window pixels; inputs in {time.time() - t0:.1f} s")
+
+    # fused loss + shift gradient: kernel time (HIP events) and the 50-evaluation refinement
+    zero = torch.zeros((P, 2), device="cuda", dtype=torch.float64)
+    ps.loss_grad(vols, zero, targets)
+    ms = [ps.loss_grad(vols, zero, targets, timed=True)[2] for _ in range(max(1, args.steps))]
+    t_eval = sum(ms) / len(ms) / 1e3
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s_grad = AG.refine_pair_shifts(vols, targets, ps, steps=w["evals"])
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    e_grad = (s_grad - s_true).norm(dim=1)
+    log(f"gradient refinement: {w['evals']} evaluations in {t_grad:.2f} s ({t_eval * 1e3:.1f} ms per fused launch), "
+        f"rms error {float(e_grad.pow(2).mean().sqrt()):.2e} px, max {float(e_grad.max()):.2e} px")
+    # the error after each evaluation (an untimed second run that records its iterates)
+    trace = {}
+    AG.refine_pair_shifts(vols, targets, ps, steps=w["evals"],
+                          callback=lambda k, s: trace.__setitem__(k, float((s - s_true).norm(dim=1).max())))
+    to_1e3 = next((k for k in sorted(trace) if trace[k] <= 1e-3), None)
+    log(f"max error by evaluation: {[f'{trace[k]:.1e}' for k in sorted(trace)[:12]]}; <= 1e-3 px after {to_1e3}")
+
+    # integer cross-correlation search: one projection + one search launch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s_cc, cc, failed = AG.cc_search_shifts(vols, targets, ps, W, w["max_shift"])
+    torch.cuda.synchronize()
+    t_cc = time.perf_counter() - t0
+    e_cc = (s_cc - s_true).norm(dim=1)
+    log(f"CC search: {t_cc:.3f} s, rms error {float(e_cc.pow(2).mean().sqrt()):.3f} px, "
+        f"max {float(e_cc.max()):.3f} px, failures {int(failed.sum())}")
+
+    # per-pair schedules on a sample, extrapolated to all pairs
+    sample = min(P, 10 * T)
+    t_plain_grad = plain_gradient_sample(ctx, ps, vols, subs, targets, zero, geoms, W, H, sample) * P / sample
+    models = ps.project(vols)
+    t_plain_cc_s, _ = plain_cc_sample(ps, models, targets, W, w["max_shift"], sample)
+    t_plain_cc = t_plain_cc_s * P / sample
+    log(f"per-pair calls (extrapolated from {sample} pairs): gradient evaluation {t_plain_grad:.2f} s, "
+        f"full-square FFT search {t_plain_cc:.2f} s")
+
+    # dot-product test of the whole-volume projector and its adjoint at this size
+    vg_full = A.volume_geometry(FX, FY, FZ)
+    gen = torch.Generator(device="cuda").manual_seed(w["seed"])
+    xv = torch.randn((FZ, FY, FX), generator=gen, device="cuda")
+    lhs = rhs = 0.0
+    proj = torch.empty((H, W), device="cuda")
+    dist = torch.empty((H, W), device="cuda")
+    for g in base:
+        yv = torch.randn((H, W), generator=gen, device="cuda")
+        proj.zero_()
+        ctx.fp_march(vg_full, g, xv, proj, dist)
+        pty = torch.zeros_like(xv)
+        ctx.fp_adjoint(vg_full, g, yv, pty)
+        lhs += float((proj.double() * yv.double()).sum())
+        rhs += float((xv.double() * pty.double()).sum())
+    dot_rel = abs(lhs - rhs) / abs(lhs)
+    log(f"dot test {FX}x{FY}x{FZ}, {T} tilts: rel {dot_rel:.2e}")
+
+    vox_tilts = K * n ** 3 * T
+    bytes_alg = 4.0 * K * n ** 3 + 4.0 * ps.n_pixels + 24.0 * P
+    achieved = bytes_alg / t_eval / 1e9
+    out = {
+        "metric": "Gvoxel-updates/s per fused loss + shift-gradient evaluation",
+        "value": vox_tilts / t_eval / 1e9,
+        "unit": "Gvoxel-updates/s",
+        "n_gpus": 1,
+        "steps": len(ms),
+        "warmup": 1,
+        "ms_per_step": t_eval * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded zero-margin blob sub-volumes, targets projected at seeded true shifts, no noise)",
+        "config": {"workload": args.workload, "field": list(w["field"]), "projection": [W, H], "tilts": T,
+                   "subvolumes": K, "subvolume_edge": n, "pairs": P, "true_shift_px": w["true_shift"],
+                   "max_shift": w["max_shift"], "numerics": "exact", "parallelism": "single GPU"},
+        "roofline": {"kernel": "pair_march_kernel<loss+grad>", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "gradient_refinement": {"method": "Levenberg-Marquardt on 2 J^T r and J^T J of the fused launch",
+                                "evaluations": w["evals"], "seconds": t_grad,
+                                "rms_error_px": float(e_grad.pow(2).mean().sqrt()),
+                                "max_error_px": float(e_grad.max()),
+                                "max_error_by_evaluation": [trace[k] for k in sorted(trace)],
+                                "evaluations_to_1e-3px": to_1e3,
+                                "seconds_to_1e-3px": None if to_1e3 is None else to_1e3 * t_grad / w["evals"]},
+        "cc_search": {"seconds": t_cc, "rms_error_px": float(e_cc.pow(2).mean().sqrt()),
+                      "max_error_px": float(e_cc.max()), "failures": int(failed.sum())},
+        "per_pair_calls_extrapolated": {"sample_pairs": sample, "gradient_evaluation_s": t_plain_grad,
+                                        "fft_cc_search_s": t_plain_cc,
+                                        "speedup_gradient_evaluation": t_plain_grad / t_eval,
+                                        "speedup_cc_search": t_plain_cc / t_cc},
+        "dot_test_full_volume_rel": dot_rel,
+    }
+    print(json.dumps(out), flush=True)
+    ps.close()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sart_1024x1024x256_61tilts",
-                    choices=sorted(WORKLOADS) + sorted(SUBVOL_WORKLOADS))
+                    choices=sorted(WORKLOADS) + sorted(SUBVOL_WORKLOADS) + sorted(REFINE_WORKLOADS))
     ap.add_argument("--numerics", default="exact", choices=["exact", "reference"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -291,6 +513,8 @@ def main() -> int:
 
     if args.workload in SUBVOL_WORKLOADS:
         return bench_subvol(args)
+    if args.workload in REFINE_WORKLOADS:
+        return bench_refine(args)
 
     import torch
 

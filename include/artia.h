@@ -284,12 +284,13 @@ artia_status artia_pairs_destroy(artia_pairs* ps);
 /* images[window of p] = P_s(p) volumes */
 artia_status artia_pairs_project(artia_pairs* ps, const float* volumes, const double* shifts, float* images,
                                  float* ms);
-/* Per pair, with r = P_s volumes - targets over its window: loss[p] = sum r^2 and
- * grad[2p], grad[2p+1] = d loss / d s = 2 sum r dP_s/ds (double, overwritten).  resid (or NULL)
- * receives r in float.  One pass: every cell a sub-ray crosses is read once for the value and the
- * shift derivative, both summed in closed form over the cell's samples. */
+/* Per pair, with r = P_s volumes - targets over its window and J = dP_s/ds (w h x 2):
+ * loss[p] = sum r^2, grad[2p..2p+1] = d loss / d s = 2 J^T r, and (normal non-NULL)
+ * normal[3p..3p+2] = the Gauss-Newton matrix J^T J as (xx, xy, yy); all double, overwritten.
+ * resid (or NULL) receives r in float.  One pass: every cell a sub-ray crosses is read once for
+ * the value and the shift derivative, both summed in closed form over the cell's samples. */
 artia_status artia_pairs_loss_grad(artia_pairs* ps, const float* volumes, const double* shifts, const float* targets,
-                                   double* loss, double* grad, float* resid, float* ms);
+                                   double* loss, double* grad, double* normal, float* resid, float* ms);
 /* grad[2p], grad[2p+1] = sum over the window of weights * dP_s/ds (double, overwritten) */
 artia_status artia_pairs_shift_grad(artia_pairs* ps, const float* volumes, const double* shifts,
                                     const float* weights, double* grad, float* ms);

@@ -8,8 +8,9 @@
     FP64 transpose per sub-volume (<= 1e-5);
   - the cross-correlation search returns the oracle's (oracle/cc_search.py, GetDisplacement)
     peaks exactly, with its correlation values and maps to 1e-5;
-  - 50 fused gradient evaluations recover per-pair shifts from U[-3, 3] px, where the integer
-    search is off by its quantisation.
+  - the Gauss-Newton matrix J^T J of the same launch matches the oracle's FP64 Jacobian;
+  - 12 fused evaluations (Levenberg-Marquardt) or 50 (Barzilai-Borwein) recover per-pair shifts
+    from U[-3, 3] px, where the integer search is off by its quantisation.
 """
 import sys
 from pathlib import Path
@@ -138,7 +139,9 @@ def test_loss_grad_vs_fd_and_per_tilt_kernel(ctx, setup):
     target = target + 0.01 * torch.from_numpy(rng.standard_normal(ps.n_pixels).astype(np.float32)).cuda()
     s = dev(s_true + rng.uniform(-0.8, 0.8, (ps.npairs, 2)), torch.float64)
     resid = torch.empty_like(target)
-    loss, grad = ps.loss_grad(setup["dflat"], s, target, resid=resid)
+    loss, grad, jtj = ps.loss_grad(setup["dflat"], s, target, resid=resid, normal=True)
+    loss2, grad2 = ps.loss_grad(setup["dflat"], s, target)
+    assert torch.equal(loss, loss2) and torch.equal(grad, grad2)
     model = ps.project(setup["dflat"], s)
     assert torch.equal(resid, model - target)
     r64 = host(resid).astype(np.float64)
@@ -146,28 +149,36 @@ def test_loss_grad_vs_fd_and_per_tilt_kernel(ctx, setup):
     for p in range(ps.npairs):
         assert abs(host(loss)[p] - (r64[pr_slice(ps, p)] ** 2).sum()) <= 1e-6 * host(loss)[p]
     # FP64 central differences of the oracle's loss, a few pairs
-    sh, gh = host(s), host(grad)
+    sh, gh, nh = host(s), host(grad), host(jtj)
     hstep = 1e-6
-    worst = 0.0
+    worst = worst_n = 0.0
     for p in (0, 3, 7, 11, 14):
         k, g = setup["meta"][p]
         vg = setup["vols"][k][0]
         _, x0, y0, w, h = setup["pairs"][p]
         v64 = setup["data"][k].astype(np.float64)
         b = tgt[pr_slice(ps, p)].reshape(h, w)
-        fd = []
+        fd, jac = [], []
         for ex, ey in ((1, 0), (0, 1)):
-            ls = []
+            ls, ps_ = [], []
             for sg in (1, -1):
                 P = O.march_shifted(vg, g, v64, W, H, sh[p, 0] + sg * hstep * ex, sh[p, 1] + sg * hstep * ey,
                                     roi=(x0, y0, x0 + w, y0 + h))
-                ls.append(((window_of(P, setup["pairs"][p]) - b) ** 2).sum())
+                ps_.append(window_of(P, setup["pairs"][p]))
+                ls.append(((ps_[-1] - b) ** 2).sum())
             fd.append((ls[0] - ls[1]) / (2 * hstep))
+            jac.append(((ps_[0] - ps_[1]) / (2 * hstep)).ravel())
         fd = np.array(fd)
         e = np.abs(gh[p] - fd).max() / np.abs(fd).max()
-        print(f"pair {p}: loss grad {gh[p]}, FP64 FD {fd}, rel {e:.2e}", flush=True)
+        jx, jy = jac
+        nfd = np.array([(jx * jx).sum(), (jx * jy).sum(), (jy * jy).sum()])
+        en = np.abs(nh[p] - nfd).max() / np.abs(nfd).max()
+        print(f"pair {p}: loss grad {gh[p]}, FP64 FD {fd}, rel {e:.2e}; J^T J {nh[p]} vs FD {nfd}, rel {en:.2e}",
+              flush=True)
         worst = max(worst, e)
+        worst_n = max(worst_n, en)
     assert worst <= 1e-5
+    assert worst_n <= 1e-5
     # per-sample formula of the per-tilt kernel at zero shift, weights on the windows
     wts = torch.from_numpy(rng.standard_normal(ps.n_pixels).astype(np.float32)).cuda()
     gp = host(ps.shift_grad(setup["dflat"], None, wts))
@@ -246,10 +257,11 @@ def test_refinement_recovers_shifts(ctx, setup):
     rng = np.random.default_rng(12)
     s_true = rng.uniform(-3, 3, (ps.npairs, 2))
     targets = ps.project(setup["dflat"], dev(s_true, torch.float64))
-    s = AG.refine_pair_shifts(setup["dflat"], targets, ps, steps=50)
-    err = np.abs(host(s) - s_true)
-    print(f"gradient refinement: max error {err.max():.2e} px", flush=True)
-    assert err.max() <= 1e-2
+    for method, steps, tol in (("gn", 12, 1e-4), ("bb", 50, 1e-2)):
+        s = AG.refine_pair_shifts(setup["dflat"], targets, ps, steps=steps, method=method)
+        err = np.abs(host(s) - s_true)
+        print(f"gradient refinement ({method}, {steps} evaluations): max error {err.max():.2e} px", flush=True)
+        assert err.max() <= tol
     sc, cc, failed = AG.cc_search_shifts(setup["dflat"], targets, ps, W, 5)
     assert not bool(host(failed).any())
     ec = np.abs(host(sc) - s_true)
