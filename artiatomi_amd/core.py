@@ -245,6 +245,11 @@ class ArtiaContext:
         _check(self.lib.artia_set_numerics(self.h, mode), self.h, "artia_set_numerics")
         self.numerics = numerics
 
+    def set_fp_staging(self, enable: bool) -> None:
+        """LDS-staged exact forward projection where it applies (default) or direct reads; both
+        give bit-identical projections."""
+        _check(self.lib.artia_set_fp_staging(self.h, int(bool(enable))), self.h, "artia_set_fp_staging")
+
     def close(self) -> None:
         if getattr(self, "h", None):
             self.lib.artia_destroy(self.h)

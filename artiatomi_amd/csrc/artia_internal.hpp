@@ -30,6 +30,7 @@ struct artia_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int numerics = ARTIA_NUMERICS_EXACT;
+    bool march_lds = true;   // exact forward projection through LDS-staged slabs where it applies
     std::string last_error;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     // driver workspace (projection-sized scratch images), grown on demand

@@ -208,6 +208,12 @@ artia_status artia_set_numerics(artia_ctx* ctx, int numerics) {
     return ARTIA_OK;
 }
 
+artia_status artia_set_fp_staging(artia_ctx* ctx, int enable) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    ctx->march_lds = enable != 0;
+    return ARTIA_OK;
+}
+
 artia_status artia_sync(artia_ctx* ctx) {
     if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
     return check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");

@@ -13,6 +13,9 @@
 //              coordinates affine in k relative to the entry point, and their trilinear values
 //              summed cell by cell in closed form (march_sum); this tracks the double-precision
 //              evaluation of the same algorithm.
+#include <climits>
+#include <cmath>
+
 #include "device_common.hpp"
 #include "fp_common.hpp"
 
@@ -136,72 +139,65 @@ __device__ __forceinline__ double march_sum(const VolG& v, const float* __restri
     return total;
 }
 
-// One lane per sub-ray: a wave covers 4x4 pixels x 4 sub-rays, a 256-thread block 8x8 pixels,
-// so the rays of a wave stay within a few voxels of each other through the whole volume.
-template <bool kSlicer, bool kBig>
-__global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const float* __restrict__ vol, Roi roi,
-                                                          size_t pitch, float* __restrict__ proj,
-                                                          float* __restrict__ dist, float tminDef, float tmaxDef) {
-    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
-    const int sub = lane & 3, pix = lane >> 2;
-    const int x = roi.x0 + (int)blockIdx.x * 8 + (wave & 1) * 4 + (pix & 3);
-    const int y = roi.y0 + (int)blockIdx.y * 8 + (wave >> 1) * 4 + (pix >> 2);
-    const bool active = x < roi.x1 && y < roi.y1;
-    const double step = (double)v.vs[0] * 0.15;
-    double acc = 0.0, len = 0.0, hits = 0.0;
-    if (active) {
-        const double px = (double)x + ((sub & 2) ? 0.75 : 0.25);
-        const double py = (double)y + ((sub & 1) ? 0.75 : 0.25);
-        const RayD r = ray_setup_d(v, g, px, py);
-        if ((r.t_out - r.t_in) > 0.0) {
-            double t_first = r.t_in, t_last = r.t_out;
-            bool run = true;
-            if (kSlicer) {
-                const double t_in = fmax(r.t_in, (double)tminDef);
-                t_last = fmin(r.t_out, (double)tmaxDef);
-                run = t_in <= t_last;
-                if (run) {
-                    // first sample at the detector point (ForwardProjectionSlicer.cu:129-133)
-                    acc += (double)tex3d_linear(
-                        vol, v.nx, v.ny, v.nz,
-                        (float)((r.c[0] - (double)v.bbMin[0]) * ((double)v.rcp[0] * (double)v.dim[0])),
-                        (float)((r.c[1] - (double)v.bbMin[1]) * ((double)v.rcp[1] * (double)v.dim[1])),
-                        (float)((r.c[2] - (double)v.bbMin[2]) * ((double)v.rcp[2] * (double)v.dim[2]) -
-                                (double)v.zshift));
-                    t_first = t_in + step;
-                }
-            } else {
-                hits = 1.0;
-                len = r.t_out - r.t_in;
-            }
-            if (run && t_first <= t_last) {
-                const int count = (int)fmin(floor((t_last - t_first) / step) + 1.0, (double)kMaxSamplesPerRay);
-                // voxel coordinate = (p - bbMin) * rcp * dim (the float-rounded rcp the reference uploads),
-                // shifted by -0.5 so that texel j is centred at integer j
-                float q0[3], dq[3];
+// One sub-ray of the exact marcher: the slicer's first sample at the detector point (acc), the
+// chord length and hit count for dist, and the sample grid q0 + k*dq, k < count (count 0: nothing
+// to walk).
+struct MarchLane {
+    float q0[3], dq[3];
+    int count;
+    double acc, len, hits;
+};
+
+template <bool kSlicer>
+__device__ __forceinline__ MarchLane march_lane(const VolG& v, const TiltG& g, const float* __restrict__ vol,
+                                                bool active, int x, int y, int sub, double step, float tminDef,
+                                                float tmaxDef) {
+    MarchLane L;
+    L.count = 0;
+    L.acc = L.len = L.hits = 0.0;
+    if (!active) return L;
+    const double px = (double)x + ((sub & 2) ? 0.75 : 0.25);
+    const double py = (double)y + ((sub & 1) ? 0.75 : 0.25);
+    const RayD r = ray_setup_d(v, g, px, py);
+    if (!((r.t_out - r.t_in) > 0.0)) return L;
+    double t_first = r.t_in, t_last = r.t_out;
+    bool run = true;
+    if (kSlicer) {
+        const double t_in = fmax(r.t_in, (double)tminDef);
+        t_last = fmin(r.t_out, (double)tmaxDef);
+        run = t_in <= t_last;
+        if (run) {
+            // first sample at the detector point (ForwardProjectionSlicer.cu:129-133)
+            L.acc = (double)tex3d_linear(
+                vol, v.nx, v.ny, v.nz, (float)((r.c[0] - (double)v.bbMin[0]) * ((double)v.rcp[0] * (double)v.dim[0])),
+                (float)((r.c[1] - (double)v.bbMin[1]) * ((double)v.rcp[1] * (double)v.dim[1])),
+                (float)((r.c[2] - (double)v.bbMin[2]) * ((double)v.rcp[2] * (double)v.dim[2]) - (double)v.zshift));
+            t_first = t_in + step;
+        }
+    } else {
+        L.hits = 1.0;
+        L.len = r.t_out - r.t_in;
+    }
+    if (run && t_first <= t_last) {
+        L.count = (int)fmin(floor((t_last - t_first) / step) + 1.0, (double)kMaxSamplesPerRay);
+        // voxel coordinate = (p - bbMin) * rcp * dim (the float-rounded rcp the reference uploads),
+        // shifted by -0.5 so that texel j is centred at integer j
 #pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    const double scale = (double)v.rcp[i] * (double)v.dim[i];
-                    const double p = (r.c[i] + t_first * (double)g.n[i]) - (double)v.bbMin[i];
-                    q0[i] = (float)(p * scale - 0.5 - (i == 2 ? (double)v.zshift : 0.0));
-                    dq[i] = (float)(step * (double)g.n[i] * scale);
-                }
-                if (v.nx >= 2 && v.ny >= 2 && v.nz >= 2) {
-                    acc += march_sum<kBig>(v, vol, q0, dq, count);
-                } else {
-                    double tot = 0.0;
-                    for (int k = 0; k < count; k++) {
-                        const float fk = (float)k;
-                        tot += (double)tex3d_linear(vol, v.nx, v.ny, v.nz, __builtin_fmaf(fk, dq[0], q0[0]) + 0.5f,
-                                                    __builtin_fmaf(fk, dq[1], q0[1]) + 0.5f,
-                                                    __builtin_fmaf(fk, dq[2], q0[2]) + 0.5f);
-                    }
-                    acc += tot;
-                }
-            }
+        for (int i = 0; i < 3; i++) {
+            const double scale = (double)v.rcp[i] * (double)v.dim[i];
+            const double p = (r.c[i] + t_first * (double)g.n[i]) - (double)v.bbMin[i];
+            L.q0[i] = (float)(p * scale - 0.5 - (i == 2 ? (double)v.zshift : 0.0));
+            L.dq[i] = (float)(step * (double)g.n[i] * scale);
         }
     }
-    // combine the four sub-rays of each pixel (lanes 4p .. 4p+3)
+    return L;
+}
+
+// Sum of a pixel's four sub-rays (lanes 4p .. 4p+3) and the pixel's outputs.
+template <bool kSlicer>
+__device__ __forceinline__ void march_store(double acc, double len, double hits, bool active, int sub, int x, int y,
+                                            size_t pitch, double step, float* __restrict__ proj,
+                                            float* __restrict__ dist) {
     acc += __shfl_xor(acc, 1);
     acc += __shfl_xor(acc, 2);
     if (!kSlicer) {
@@ -215,6 +211,185 @@ __global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const
         proj[i] += (float)(acc * step * 0.25);
         if (!kSlicer) dist[i] = hits > 0.0 ? (float)(len / hits) : 0.0f;
     }
+}
+
+// One lane per sub-ray: a wave covers 4x4 pixels x 4 sub-rays, a 256-thread block 8x8 pixels,
+// so the rays of a wave stay within a few voxels of each other through the whole volume.
+template <bool kSlicer, bool kBig>
+__global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const float* __restrict__ vol, Roi roi,
+                                                          size_t pitch, float* __restrict__ proj,
+                                                          float* __restrict__ dist, float tminDef, float tmaxDef) {
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int sub = lane & 3, pix = lane >> 2;
+    const int x = roi.x0 + (int)blockIdx.x * 8 + (wave & 1) * 4 + (pix & 3);
+    const int y = roi.y0 + (int)blockIdx.y * 8 + (wave >> 1) * 4 + (pix >> 2);
+    const bool active = x < roi.x1 && y < roi.y1;
+    const double step = (double)v.vs[0] * 0.15;
+    const MarchLane L = march_lane<kSlicer>(v, g, vol, active, x, y, sub, step, tminDef, tmaxDef);
+    double acc = L.acc;
+    if (L.count > 0) {
+        if (v.nx >= 2 && v.ny >= 2 && v.nz >= 2) {
+            acc += march_sum<kBig>(v, vol, L.q0, L.dq, L.count);
+        } else {
+            double tot = 0.0;
+            for (int k = 0; k < L.count; k++) {
+                const float fk = (float)k;
+                tot += (double)tex3d_linear(vol, v.nx, v.ny, v.nz, __builtin_fmaf(fk, L.dq[0], L.q0[0]) + 0.5f,
+                                            __builtin_fmaf(fk, L.dq[1], L.q0[1]) + 0.5f,
+                                            __builtin_fmaf(fk, L.dq[2], L.q0[2]) + 0.5f);
+            }
+            acc += tot;
+        }
+    }
+    march_store<kSlicer>(acc, L.len, L.hits, active, sub, x, y, pitch, step, proj, dist);
+}
+
+// LDS-staged form of march_exact_kernel for volumes of at least 2 voxels per axis.  The block's
+// 256 sub-rays are parallel, so the cells they cross between two z planes lie in a box that grows
+// linearly with the slab: the block walks the volume in slabs of kSlabCells cell layers along the
+// rays' z direction, stages each slab's box of voxels in LDS with coalesced row loads, and reads
+// the eight corners of every cell from LDS instead of through the texture addresser.  Each lane
+// runs the same cell walk in the same order as march_exact_kernel (the walk is only paused at slab
+// boundaries), so the projection is bit-identical; a corner outside the staged box (not expected)
+// is read from global memory.
+constexpr int kSlabCells = 8;
+constexpr int kChunkFloats = 5120;   // 20 KB of LDS per block
+
+struct SlabPlan {
+    float bx, by;   // dq_x / dq_z and dq_y / dq_z (texel units; equal for all sub-rays)
+    int zdir;       // sign of dq_z (nonzero)
+};
+
+template <bool kBig>
+__device__ __forceinline__ void chunk_corners(const VolG& v, const float* __restrict__ vol,
+                                              const float* __restrict__ chunk, bool staged, int X0, int X1, int Y0,
+                                              int Y1, int Z0, int Z1, int CX, int CXY, const int cell[3],
+                                              float val[8]) {
+    const int x0 = clampi(cell[0], 0, v.nx - 1), y0 = clampi(cell[1], 0, v.ny - 1), z0 = clampi(cell[2], 0, v.nz - 1);
+    const int sx = (cell[0] + 1 > x0 && x0 + 1 < v.nx) ? 1 : 0;
+    const int sy = (cell[1] + 1 > y0 && y0 + 1 < v.ny) ? 1 : 0;
+    const int sz = (cell[2] + 1 > z0 && z0 + 1 < v.nz) ? 1 : 0;
+    if (staged && x0 >= X0 && x0 + sx <= X1 && y0 >= Y0 && y0 + sy <= Y1 && z0 >= Z0 && z0 + sz <= Z1) {
+        const int o = (z0 - Z0) * CXY + (y0 - Y0) * CX + (x0 - X0);
+        const int dy = sy ? CX : 0, dz = sz ? CXY : 0;
+        val[0] = chunk[o];
+        val[1] = chunk[o + sx];
+        val[2] = chunk[o + dy];
+        val[3] = chunk[o + dy + sx];
+        val[4] = chunk[o + dz];
+        val[5] = chunk[o + dz + sx];
+        val[6] = chunk[o + dz + dy];
+        val[7] = chunk[o + dz + dy + sx];
+    } else {
+        size_t o[8];
+        cell_corners<true>(v, cell, o);
+#pragma unroll
+        for (int i = 0; i < 8; i++) val[i] = vol[o[i]];
+    }
+}
+
+template <bool kSlicer, bool kBig>
+__global__ void __launch_bounds__(256) march_lds_kernel(VolG v, TiltG g, const float* __restrict__ vol, Roi roi,
+                                                        size_t pitch, float* __restrict__ proj,
+                                                        float* __restrict__ dist, float tminDef, float tmaxDef,
+                                                        SlabPlan sp) {
+    __shared__ float chunk[kChunkFloats];
+    __shared__ int zr[2];
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int sub = lane & 3, pix = lane >> 2;
+    const int tx0 = roi.x0 + (int)blockIdx.x * 8, ty0 = roi.y0 + (int)blockIdx.y * 8;
+    const int x = tx0 + (wave & 1) * 4 + (pix & 3);
+    const int y = ty0 + (wave >> 1) * 4 + (pix >> 2);
+    const bool active = x < roi.x1 && y < roi.y1;
+    const double step = (double)v.vs[0] * 0.15;
+    const MarchLane L = march_lane<kSlicer>(v, g, vol, active, x, y, sub, step, tminDef, tmaxDef);
+
+    // cell layers the block's sub-rays cross
+    if (threadIdx.x == 0) {
+        zr[0] = INT_MAX;
+        zr[1] = INT_MIN;
+    }
+    __syncthreads();
+    if (L.count > 0) {
+        const int za = (int)floorf(L.q0[2]);
+        const int zb = (int)floorf(__builtin_fmaf((float)(L.count - 1), L.dq[2], L.q0[2]));
+        atomicMin(&zr[0], min(za, zb) - 1);
+        atomicMax(&zr[1], max(za, zb) + 1);
+    }
+    __syncthreads();
+    const int zlo = max(zr[0], -1), zhi = min(zr[1], v.nz - 1);
+
+    // Every sub-ray of the tile lies on a line Q(z) = (a_x + bx z, a_y + by z, z) in texel
+    // coordinates; the tile's four corner sub-rays bound a_x and a_y.
+    float axl = INFINITY, axh = -INFINITY, ayl = INFINITY, ayh = -INFINITY;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const double px = (double)tx0 + ((c & 1) ? 7.75 : 0.25), py = (double)ty0 + ((c & 2) ? 7.75 : 0.25);
+        const double xa = ((double)g.A[0] * px + (double)g.A[1] * py) + (double)g.A[2];
+        const double ya = ((double)g.A[3] * px + (double)g.A[4] * py) + (double)g.A[5];
+        double q[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double ci = ((double)g.d[i] + xa * (double)g.u[i]) + ya * (double)g.v[i];
+            q[i] = (ci - (double)v.bbMin[i]) * ((double)v.rcp[i] * (double)v.dim[i]) - 0.5 -
+                   (i == 2 ? (double)v.zshift : 0.0);
+        }
+        const float ax = (float)(q[0] - q[2] * (double)sp.bx), ay = (float)(q[1] - q[2] * (double)sp.by);
+        axl = fminf(axl, ax);
+        axh = fmaxf(axh, ax);
+        ayl = fminf(ayl, ay);
+        ayh = fmaxf(ayh, ay);
+    }
+
+    CellWalker w;
+    w.init(L.q0, L.dq, L.count);
+    double total = 0.0;   // summed as march_sum does, then added to the slicer's first sample
+    const int nslab = zlo <= zhi ? (zhi - zlo) / kSlabCells + 1 : 0;
+    for (int j = 0; j < nslab; j++) {
+        // cell layers [zs, ze] of this slab, in the rays' z direction
+        const int zs = sp.zdir > 0 ? zlo + j * kSlabCells : zhi - (j + 1) * kSlabCells + 1;
+        const int ze = zs + kSlabCells - 1;
+        const float z0 = (float)zs, z1 = (float)(ze + 1);
+        const int X0 = max((int)floorf(axl + fminf(sp.bx * z0, sp.bx * z1)) - 1, 0);
+        const int X1 = min((int)floorf(axh + fmaxf(sp.bx * z0, sp.bx * z1)) + 2, v.nx - 1);
+        const int Y0 = max((int)floorf(ayl + fminf(sp.by * z0, sp.by * z1)) - 1, 0);
+        const int Y1 = min((int)floorf(ayh + fmaxf(sp.by * z0, sp.by * z1)) + 2, v.ny - 1);
+        const int Z0 = max(zs, 0), Z1 = min(ze + 1, v.nz - 1);
+        const int CX = X1 - X0 + 1, CY = Y1 - Y0 + 1, CZ = Z1 - Z0 + 1;
+        const bool staged = CX > 0 && CY > 0 && CZ > 0 && CX * CY * CZ <= kChunkFloats;
+        const int CXY = CX * CY;
+        __syncthreads();   // the previous slab's reads are done
+        if (staged) {
+            const int n = CXY * CZ;
+            for (int e = (int)threadIdx.x; e < n; e += 256) {
+                const int row = e / CX, xx = e - row * CX;
+                const int zz = row / CY, yy = row - zz * CY;
+                chunk[e] = vol[((size_t)(Z0 + zz) * (size_t)v.ny + (size_t)(Y0 + yy)) * (size_t)v.nx + (size_t)(X0 + xx)];
+            }
+        }
+        __syncthreads();
+        while (w.active() && (sp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
+            w.step([&](const int cell[3], int k, int mi) {
+                float val[8];
+                chunk_corners<kBig>(v, vol, chunk, staged, X0, X1, Y0, Y1, Z0, Z1, CX, CXY, cell, val);
+                total += cell_sum_values(val, cell_moments(L.q0, L.dq, cell, k, mi), mi);
+            });
+        }
+    }
+    // cells beyond the slabs (not expected): global reads
+    while (w.active()) {
+        w.step([&](const int cell[3], int k, int mi) {
+            size_t o[8];
+            cell_corners<true>(v, cell, o);
+            float val[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) val[i] = vol[o[i]];
+            total += cell_sum_values(val, cell_moments(L.q0, L.dq, cell, k, mi), mi);
+        });
+    }
+    double acc = L.acc;
+    if (L.count > 0) acc += total;
+    march_store<kSlicer>(acc, L.len, L.hits, active, sub, x, y, pitch, step, proj, dist);
 }
 
 __global__ void __launch_bounds__(256) vol_traversal_exact_kernel(VolG v, TiltG g, Roi roi, size_t pitch,
@@ -245,6 +420,31 @@ dim3 grid_for(const Roi& r) {
     return dim3((unsigned)((w + 63) / 64), (unsigned)((h + 3) / 4));
 }
 
+// Slab plan of the LDS-staged marcher, or false when it does not apply: volumes below 2 voxels on
+// an axis, rays parallel to the z layers, or a tile's slab box larger than the LDS chunk (steep
+// rays), where march_exact_kernel runs instead.  The box of an 8x8-pixel tile spans the tile's
+// extent in a_x = Q_x - bx Q_z (affine in the pixel) plus the rays' drift over a slab.
+bool slab_plan(const VolG& v, const TiltG& g, SlabPlan& sp) {
+    if (v.nx < 2 || v.ny < 2 || v.nz < 2) return false;
+    double sc[3], N[3], dpx[3], dpy[3];
+    for (int i = 0; i < 3; i++) {
+        sc[i] = (double)v.rcp[i] * (double)v.dim[i];
+        N[i] = (double)g.n[i] * sc[i];
+        dpx[i] = ((double)g.A[0] * g.u[i] + (double)g.A[3] * g.v[i]) * sc[i];
+        dpy[i] = ((double)g.A[1] * g.u[i] + (double)g.A[4] * g.v[i]) * sc[i];
+    }
+    if (!(std::fabs(N[2]) > 1e-3 * (std::fabs(N[0]) + std::fabs(N[1]) + std::fabs(N[2])))) return false;
+    const double bx = N[0] / N[2], by = N[1] / N[2];
+    const double wx = 7.5 * (std::fabs(dpx[0] - bx * dpx[2]) + std::fabs(dpy[0] - bx * dpy[2]));
+    const double wy = 7.5 * (std::fabs(dpx[1] - by * dpx[2]) + std::fabs(dpy[1] - by * dpy[2]));
+    const double cx = wx + std::fabs(bx) * (kSlabCells + 1) + 5.0, cy = wy + std::fabs(by) * (kSlabCells + 1) + 5.0;
+    if (cx * cy * (kSlabCells + 1) > (double)kChunkFloats) return false;
+    sp.bx = (float)bx;
+    sp.by = (float)by;
+    sp.zdir = N[2] > 0.0 ? 1 : -1;
+    return true;
+}
+
 void launch_march_impl(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vol, const Roi& r, size_t pitch,
                        float* proj, float* dist, bool slicer, float tmin, float tmax) {
     if (r.x1 <= r.x0 || r.y1 <= r.y0) return;
@@ -262,6 +462,25 @@ void launch_march_impl(artia_ctx* ctx, const VolG& v, const TiltG& g, const floa
     // 32-bit buffer offsets need fewer than 2^30 voxels and a row index z * ny + y below 2^24
     const bool big = (uint64_t)v.nx * (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 30) ||
                      (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 24) || v.nx >= (1 << 24);
+    SlabPlan sp;
+    if (ctx->march_lds && slab_plan(v, g, sp)) {
+        if (slicer) {
+            if (big)
+                hipLaunchKernelGGL((march_lds_kernel<true, true>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+                                   proj, dist, tmin, tmax, sp);
+            else
+                hipLaunchKernelGGL((march_lds_kernel<true, false>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+                                   proj, dist, tmin, tmax, sp);
+        } else {
+            if (big)
+                hipLaunchKernelGGL((march_lds_kernel<false, true>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+                                   proj, dist, tmin, tmax, sp);
+            else
+                hipLaunchKernelGGL((march_lds_kernel<false, false>), grid, block, 0, ctx->stream, v, g, vol, r,
+                                   pitch, proj, dist, tmin, tmax, sp);
+        }
+        return;
+    }
     if (slicer) {
         if (big)
             hipLaunchKernelGGL((march_exact_kernel<true, true>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,

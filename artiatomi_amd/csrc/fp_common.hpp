@@ -118,28 +118,38 @@ __device__ __forceinline__ bool subray_samples(const VolG& v, const TiltG& g, do
     return true;
 }
 
-// Cells of the texel lattice crossed by the samples q0 + k*dq, k = 0..count-1, in order: calls
-// f(cell, k, m) for every cell holding samples k..k+m-1.  The sample index of the next crossing on
-// axis i is recomputed from the cell index, (face - q0_i) / dq_i, as one fma (its float error, a
-// few 1e-4 samples for rays of thousands of samples, only moves a sample lying within that distance
-// of a face to the neighbouring cell, where the continuous trilinear interpolant has the same value).
-// Every step either consumes samples or advances an axis; the walk is bounded by 4 * count + 64
-// steps.
-template <class F>
-__device__ __forceinline__ void for_each_cell(const float q0[3], const float dq[3], int count, F&& f) {
+// Walk over the cells of the texel lattice crossed by the samples q0 + k*dq, k = 0..count-1, in
+// order.  The sample index of the next crossing on axis i is recomputed from the cell index,
+// (face - q0_i) / dq_i, as one fma (its float error, a few 1e-4 samples for rays of thousands of
+// samples, only moves a sample lying within that distance of a face to the neighbouring cell,
+// where the continuous trilinear interpolant has the same value).  Every step either consumes
+// samples or advances an axis; the walk is bounded by 4 * count + 64 steps.  init() sets it up,
+// step() handles one cell (calling f(cell, k, m) when it holds samples k..k+m-1) and advances to
+// the next, active() says whether samples remain, so the walk can be resumed (the LDS-staged
+// projector runs it slab by slab).
+struct CellWalker {
     int cell[3], sgn[3];
     float inv[3], off[3], kb[3];
+    float fcount;
+    int k, count, guard;
+
+    __device__ __forceinline__ void init(const float q0[3], const float dq[3], int n) {
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        cell[i] = (int)floorf(q0[i]);
-        sgn[i] = dq[i] > 0.0f ? 1 : (dq[i] < 0.0f ? -1 : 0);
-        inv[i] = sgn[i] != 0 ? 1.0f / dq[i] : 0.0f;
-        off[i] = -q0[i] * inv[i];
-        kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]) : INFINITY;
+        for (int i = 0; i < 3; i++) {
+            cell[i] = (int)floorf(q0[i]);
+            sgn[i] = dq[i] > 0.0f ? 1 : (dq[i] < 0.0f ? -1 : 0);
+            inv[i] = sgn[i] != 0 ? 1.0f / dq[i] : 0.0f;
+            off[i] = -q0[i] * inv[i];
+            kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]) : INFINITY;
+        }
+        count = n;
+        fcount = (float)n;
+        k = 0;
+        guard = 0;
     }
-    const float fcount = (float)count;
-    int k = 0;
-    for (int guard = 0; k < count && guard < 4 * count + 64; guard++) {
+    __device__ __forceinline__ bool active() const { return k < count && guard < 4 * count + 64; }
+    template <class F>
+    __device__ __forceinline__ void step(F&& f) {
         const float kn = fminf(fminf(kb[0], kb[1]), kb[2]);
         const int kend = kn < fcount ? max((int)ceilf(kn), 0) : count;
         if (kend > k) {
@@ -153,7 +163,16 @@ __device__ __forceinline__ void for_each_cell(const float q0[3], const float dq[
                 kb[i] = __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]);
             }
         }
+        guard++;
     }
+};
+
+// The whole walk: f(cell, k, m) for every cell holding samples k..k+m-1, in order.
+template <class F>
+__device__ __forceinline__ void for_each_cell(const float q0[3], const float dq[3], int count, F&& f) {
+    CellWalker w;
+    w.init(q0, dq, count);
+    while (w.active()) w.step(f);
 }
 
 // The m samples of one cell: centre coordinates relative to the cell and the second moment of the

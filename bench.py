@@ -488,6 +488,8 @@ def main() -> int:
                     choices=sorted(WORKLOADS) + sorted(SUBVOL_WORKLOADS) + sorted(REFINE_WORKLOADS))
     ap.add_argument("--numerics", default="exact", choices=["exact", "reference"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp-staging", action="store_true",
+                    help="forward-project with direct volume reads instead of LDS-staged slabs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only an elementwise copy of 1 GiB (three launches) to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -522,6 +524,7 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     ctx = A.ArtiaContext(0, numerics=args.numerics)
+    ctx.set_fp_staging(not args.no_fp_staging)
     vg, acq, projs, w = build_workload(args.workload, ctx)
     nx, ny, nz = w["dims"]
     V, T = nx * ny * nz, len(acq.geoms)
@@ -591,7 +594,7 @@ def main() -> int:
         "data": "synthetic (seeded ellipsoid + gold-bead phantom, projections from the forward projector + noise)",
         "config": {"workload": args.workload, "volume": list(w["dims"]), "projection": [W, H], "tilts": T,
                    "tilt_range_deg": [-60, 60], "oversampling": w["os"], "sirt_count": 1, "lambda": 0.3,
-                   "numerics": args.numerics, "parallelism": "single GPU"},
+                   "numerics": args.numerics, "fp_staging": not args.no_fp_staging, "parallelism": "single GPU"},
         "roofline": roofline,
         "kernels": per_kernel,
         "cpu_baseline": cpu,
