@@ -155,6 +155,8 @@ __device__ __forceinline__ MarchLane march_lane(const VolG& v, const TiltG& g, c
     MarchLane L;
     L.count = 0;
     L.acc = L.len = L.hits = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) L.q0[i] = L.dq[i] = 0.0f;
     if (!active) return L;
     const double px = (double)x + ((sub & 2) ? 0.75 : 0.25);
     const double py = (double)y + ((sub & 1) ? 0.75 : 0.25);
@@ -191,6 +193,16 @@ __device__ __forceinline__ MarchLane march_lane(const VolG& v, const TiltG& g, c
         }
     }
     return L;
+}
+
+// The pixel's mean chord over its four sub-rays (lanes 4p .. 4p+3) into dist.
+__device__ __forceinline__ void march_store_dist(double len, double hits, bool active, int sub, int x, int y,
+                                                 size_t pitch, float* __restrict__ dist) {
+    len += __shfl_xor(len, 1);
+    len += __shfl_xor(len, 2);
+    hits += __shfl_xor(hits, 1);
+    hits += __shfl_xor(hits, 2);
+    if (active && sub == 0) dist[(size_t)y * pitch + (size_t)x] = hits > 0.0 ? (float)(len / hits) : 0.0f;
 }
 
 // Sum of a pixel's four sub-rays (lanes 4p .. 4p+3) and the pixel's outputs.
@@ -260,26 +272,25 @@ struct SlabPlan {
     int zdir;       // sign of dq_z (nonzero)
 };
 
-template <bool kBig>
+// Corners of a cell from the staged box when it lies inside (the box is padded with the
+// clamp-addressed border voxels, indices -1 and n, so no per-cell clamping is needed), otherwise
+// from global memory with clamped indices.
 __device__ __forceinline__ void chunk_corners(const VolG& v, const float* __restrict__ vol,
-                                              const float* __restrict__ chunk, bool staged, int X0, int X1, int Y0,
-                                              int Y1, int Z0, int Z1, int CX, int CXY, const int cell[3],
-                                              float val[8]) {
-    const int x0 = clampi(cell[0], 0, v.nx - 1), y0 = clampi(cell[1], 0, v.ny - 1), z0 = clampi(cell[2], 0, v.nz - 1);
-    const int sx = (cell[0] + 1 > x0 && x0 + 1 < v.nx) ? 1 : 0;
-    const int sy = (cell[1] + 1 > y0 && y0 + 1 < v.ny) ? 1 : 0;
-    const int sz = (cell[2] + 1 > z0 && z0 + 1 < v.nz) ? 1 : 0;
-    if (staged && x0 >= X0 && x0 + sx <= X1 && y0 >= Y0 && y0 + sy <= Y1 && z0 >= Z0 && z0 + sz <= Z1) {
-        const int o = (z0 - Z0) * CXY + (y0 - Y0) * CX + (x0 - X0);
-        const int dy = sy ? CX : 0, dz = sz ? CXY : 0;
+                                              const float* __restrict__ chunk, bool staged, int X0, int Y0, int Z0,
+                                              int CX, int CY, int CZ, const int cell[3], float val[8]) {
+    const int cx = cell[0] - X0, cy = cell[1] - Y0, cz = cell[2] - Z0;
+    if (staged && (unsigned)cx < (unsigned)(CX - 1) && (unsigned)cy < (unsigned)(CY - 1) &&
+        (unsigned)cz < (unsigned)(CZ - 1)) {
+        const int CXY = CX * CY;
+        const int o = (cz * CY + cy) * CX + cx;
         val[0] = chunk[o];
-        val[1] = chunk[o + sx];
-        val[2] = chunk[o + dy];
-        val[3] = chunk[o + dy + sx];
-        val[4] = chunk[o + dz];
-        val[5] = chunk[o + dz + sx];
-        val[6] = chunk[o + dz + dy];
-        val[7] = chunk[o + dz + dy + sx];
+        val[1] = chunk[o + 1];
+        val[2] = chunk[o + CX];
+        val[3] = chunk[o + CX + 1];
+        val[4] = chunk[o + CXY];
+        val[5] = chunk[o + CXY + 1];
+        val[6] = chunk[o + CXY + CX];
+        val[7] = chunk[o + CXY + CX + 1];
     } else {
         size_t o[8];
         cell_corners<true>(v, cell, o);
@@ -288,8 +299,16 @@ __device__ __forceinline__ void chunk_corners(const VolG& v, const float* __rest
     }
 }
 
+// A value equal in every lane, moved to a scalar register.
+__device__ __forceinline__ float wave_uniform(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Six waves per SIMD (80 registers): the walk is VALU bound, and the occupancy hides the LDS
+// latency better than the few registers it spills.
 template <bool kSlicer, bool kBig>
-__global__ void __launch_bounds__(256) march_lds_kernel(VolG v, TiltG g, const float* __restrict__ vol, Roi roi,
+__global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, const float* __restrict__ vol, Roi roi,
                                                         size_t pitch, float* __restrict__ proj,
                                                         float* __restrict__ dist, float tminDef, float tmaxDef,
                                                         SlabPlan sp) {
@@ -303,6 +322,7 @@ __global__ void __launch_bounds__(256) march_lds_kernel(VolG v, TiltG g, const f
     const bool active = x < roi.x1 && y < roi.y1;
     const double step = (double)v.vs[0] * 0.15;
     const MarchLane L = march_lane<kSlicer>(v, g, vol, active, x, y, sub, step, tminDef, tmaxDef);
+    if (!kSlicer) march_store_dist(L.len, L.hits, active, sub, x, y, pitch, dist);   // before the walk
 
     // cell layers the block's sub-rays cross
     if (threadIdx.x == 0) {
@@ -340,39 +360,55 @@ __global__ void __launch_bounds__(256) march_lds_kernel(VolG v, TiltG g, const f
         ayl = fminf(ayl, ay);
         ayh = fmaxf(ayh, ay);
     }
+    axl = wave_uniform(axl);
+    axh = wave_uniform(axh);
+    ayl = wave_uniform(ayl);
+    ayh = wave_uniform(ayh);
 
+    // the sample step is the same for every sub-ray (march_lane's dq, evaluated in every lane)
+    int sgn_u[3];
+    float dq_u[3], inv_u[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        dq_u[i] = wave_uniform((float)(step * (double)g.n[i] * ((double)v.rcp[i] * (double)v.dim[i])));
+        sgn_u[i] = dq_u[i] > 0.0f ? 1 : (dq_u[i] < 0.0f ? -1 : 0);
+        inv_u[i] = wave_uniform(sgn_u[i] != 0 ? 1.0f / dq_u[i] : 0.0f);
+    }
     CellWalker w;
-    w.init(L.q0, L.dq, L.count);
+    w.init_dir(L.q0, sgn_u, inv_u, L.count);
     double total = 0.0;   // summed as march_sum does, then added to the slicer's first sample
     const int nslab = zlo <= zhi ? (zhi - zlo) / kSlabCells + 1 : 0;
     for (int j = 0; j < nslab; j++) {
         // cell layers [zs, ze] of this slab, in the rays' z direction
         const int zs = sp.zdir > 0 ? zlo + j * kSlabCells : zhi - (j + 1) * kSlabCells + 1;
         const int ze = zs + kSlabCells - 1;
+        // box of lattice indices the slab's cells touch, in index space padded by the clamp border
+        // (-1 .. n); the margins cover the float rounding of the walk
         const float z0 = (float)zs, z1 = (float)(ze + 1);
-        const int X0 = max((int)floorf(axl + fminf(sp.bx * z0, sp.bx * z1)) - 1, 0);
-        const int X1 = min((int)floorf(axh + fmaxf(sp.bx * z0, sp.bx * z1)) + 2, v.nx - 1);
-        const int Y0 = max((int)floorf(ayl + fminf(sp.by * z0, sp.by * z1)) - 1, 0);
-        const int Y1 = min((int)floorf(ayh + fmaxf(sp.by * z0, sp.by * z1)) + 2, v.ny - 1);
-        const int Z0 = max(zs, 0), Z1 = min(ze + 1, v.nz - 1);
-        const int CX = X1 - X0 + 1, CY = Y1 - Y0 + 1, CZ = Z1 - Z0 + 1;
-        const bool staged = CX > 0 && CY > 0 && CZ > 0 && CX * CY * CZ <= kChunkFloats;
-        const int CXY = CX * CY;
+        const int X0 = wave_uniform(max((int)floorf(axl + fminf(sp.bx * z0, sp.bx * z1)) - 1, -1));
+        const int X1 = min((int)floorf(axh + fmaxf(sp.bx * z0, sp.bx * z1)) + 2, v.nx);
+        const int Y0 = wave_uniform(max((int)floorf(ayl + fminf(sp.by * z0, sp.by * z1)) - 1, -1));
+        const int Y1 = min((int)floorf(ayh + fmaxf(sp.by * z0, sp.by * z1)) + 2, v.ny);
+        const int Z0 = max(zs, -1), Z1 = min(ze + 1, v.nz);
+        const int CX = wave_uniform(X1 - X0 + 1), CY = wave_uniform(Y1 - Y0 + 1), CZ = Z1 - Z0 + 1;
+        const bool staged = CX > 1 && CY > 1 && CZ > 1 && CX * CY * CZ <= kChunkFloats;
         __syncthreads();   // the previous slab's reads are done
         if (staged) {
-            const int n = CXY * CZ;
+            const int n = CX * CY * CZ;
             for (int e = (int)threadIdx.x; e < n; e += 256) {
                 const int row = e / CX, xx = e - row * CX;
                 const int zz = row / CY, yy = row - zz * CY;
-                chunk[e] = vol[((size_t)(Z0 + zz) * (size_t)v.ny + (size_t)(Y0 + yy)) * (size_t)v.nx + (size_t)(X0 + xx)];
+                const size_t gx = (size_t)clampi(X0 + xx, 0, v.nx - 1), gy = (size_t)clampi(Y0 + yy, 0, v.ny - 1);
+                const size_t gz = (size_t)clampi(Z0 + zz, 0, v.nz - 1);
+                chunk[e] = vol[(gz * (size_t)v.ny + gy) * (size_t)v.nx + gx];
             }
         }
         __syncthreads();
         while (w.active() && (sp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
             w.step([&](const int cell[3], int k, int mi) {
                 float val[8];
-                chunk_corners<kBig>(v, vol, chunk, staged, X0, X1, Y0, Y1, Z0, Z1, CX, CXY, cell, val);
-                total += cell_sum_values(val, cell_moments(L.q0, L.dq, cell, k, mi), mi);
+                chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
+                total += cell_sum_values(val, cell_moments(L.q0, dq_u, cell, k, mi), mi);
             });
         }
     }
@@ -384,12 +420,12 @@ __global__ void __launch_bounds__(256) march_lds_kernel(VolG v, TiltG g, const f
             float val[8];
 #pragma unroll
             for (int i = 0; i < 8; i++) val[i] = vol[o[i]];
-            total += cell_sum_values(val, cell_moments(L.q0, L.dq, cell, k, mi), mi);
+            total += cell_sum_values(val, cell_moments(L.q0, dq_u, cell, k, mi), mi);
         });
     }
     double acc = L.acc;
     if (L.count > 0) acc += total;
-    march_store<kSlicer>(acc, L.len, L.hits, active, sub, x, y, pitch, step, proj, dist);
+    march_store<true>(acc, 0.0, 0.0, active, sub, x, y, pitch, step, proj, nullptr);   // proj only
 }
 
 __global__ void __launch_bounds__(256) vol_traversal_exact_kernel(VolG v, TiltG g, Roi roi, size_t pitch,

@@ -147,6 +147,22 @@ struct CellWalker {
         k = 0;
         guard = 0;
     }
+    // The same walk with the direction part (sgn, inv) precomputed: when every lane walks parallel
+    // samples the caller passes them as wave-uniform values, which then stay in scalar registers.
+    __device__ __forceinline__ void init_dir(const float q0[3], const int sgn_u[3], const float inv_u[3], int n) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            cell[i] = (int)floorf(q0[i]);
+            sgn[i] = sgn_u[i];
+            inv[i] = inv_u[i];
+            off[i] = -q0[i] * inv[i];
+            kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]) : INFINITY;
+        }
+        count = n;
+        fcount = (float)n;
+        k = 0;
+        guard = 0;
+    }
     __device__ __forceinline__ bool active() const { return k < count && guard < 4 * count + 64; }
     template <class F>
     __device__ __forceinline__ void step(F&& f) {
@@ -246,23 +262,21 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t volume_rsrc(const float* vol, 
     return __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, (int)bytes, 0x00020000);
 }
 
-// Sum of one cell's m samples: the centre value in double (its rounding is multiplied by m), the
-// small second-order correction in float.
+// Sum of one cell's m samples, m f(centre) + S2c corr, rounded once to float (relative error
+// 6e-8 per cell); the ray adds the cells in double, so per-cell roundings do not accumulate into a
+// float running sum.
 __device__ __forceinline__ double cell_sum_values(const float val[8], const CellMoments& cm, int mi) {
     const float v000 = val[0], v100 = val[1], v010 = val[2], v110 = val[3];
     const float v001 = val[4], v101 = val[5], v011 = val[6], v111 = val[7];
-    const double ad = cm.a, bd = cm.b, cd = cm.c;
-    const double c00 = fma(ad, (double)v100 - (double)v000, (double)v000);
-    const double c10 = fma(ad, (double)v110 - (double)v010, (double)v010);
-    const double c01 = fma(ad, (double)v101 - (double)v001, (double)v001);
-    const double c11 = fma(ad, (double)v111 - (double)v011, (double)v011);
-    const double e0 = fma(bd, c10 - c00, c00), e1 = fma(bd, c11 - c01, c01);
-    const double f = fma(cd, e1 - e0, e0);
+    const float c00 = lerpf(v000, v100, cm.a), c10 = lerpf(v010, v110, cm.a);
+    const float c01 = lerpf(v001, v101, cm.a), c11 = lerpf(v011, v111, cm.a);
+    const float f = lerpf(lerpf(c00, c10, cm.b), lerpf(c01, c11, cm.b), cm.c);
     const float AB = (v110 - v100) - (v010 - v000), AC = (v101 - v100) - (v001 - v000);
     const float BC = (v011 - v010) - (v001 - v000);
     const float ABC = ((v111 - v110) - (v101 - v100)) - ((v011 - v010) - (v001 - v000));
     const float corr = AB * cm.pab + AC * cm.pac + BC * cm.pbc + ABC * cm.q;
-    return fma((double)mi, f, (double)(cm.S2c * corr));
+    (void)mi;
+    return (double)__builtin_fmaf(cm.m, f, cm.S2c * corr);
 }
 
 // The same sum as weights on the eight corners (cell_sum = sum_c v_c W_c): the transpose of
