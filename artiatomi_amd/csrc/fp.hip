@@ -270,32 +270,39 @@ constexpr int kChunkFloats = 5120;   // 20 KB of LDS per block
 struct SlabPlan {
     float bx, by;   // dq_x / dq_z and dq_y / dq_z (texel units; equal for all sub-rays)
     int zdir;       // sign of dq_z (nonzero)
+    int tiles_x, tiles_y, groups_x;   // 8x8-pixel tiles per row and column, tile groups per row
 };
+
+constexpr int kXcdGroup = 4;   // tiles per side of the groups dealt to one XCD
 
 // Corners of a cell from the staged box when it lies inside (the box is padded with the
 // clamp-addressed border voxels, indices -1 and n, so no per-cell clamping is needed), otherwise
-// from global memory with clamped indices.
+// from global memory with clamped indices.  The LDS reads are issued unconditionally (at offset 0
+// when outside, which stays inside the box) and overwritten on the rare global path: branching
+// between the two sources would let the compiler merge them into generic loads that wait on both
+// memory counters.
 __device__ __forceinline__ void chunk_corners(const VolG& v, const float* __restrict__ vol,
                                               const float* __restrict__ chunk, bool staged, int X0, int Y0, int Z0,
                                               int CX, int CY, int CZ, const int cell[3], float val[8]) {
     const int cx = cell[0] - X0, cy = cell[1] - Y0, cz = cell[2] - Z0;
-    if (staged && (unsigned)cx < (unsigned)(CX - 1) && (unsigned)cy < (unsigned)(CY - 1) &&
-        (unsigned)cz < (unsigned)(CZ - 1)) {
-        const int CXY = CX * CY;
-        const int o = (cz * CY + cy) * CX + cx;
-        val[0] = chunk[o];
-        val[1] = chunk[o + 1];
-        val[2] = chunk[o + CX];
-        val[3] = chunk[o + CX + 1];
-        val[4] = chunk[o + CXY];
-        val[5] = chunk[o + CXY + 1];
-        val[6] = chunk[o + CXY + CX];
-        val[7] = chunk[o + CXY + CX + 1];
-    } else {
-        size_t o[8];
-        cell_corners<true>(v, cell, o);
+    const bool inside = staged && (unsigned)cx < (unsigned)(CX - 1) && (unsigned)cy < (unsigned)(CY - 1) &&
+                        (unsigned)cz < (unsigned)(CZ - 1);
+    // outside: every read at chunk[0] or chunk[1]
+    const int o = inside ? (cz * CY + cy) * CX + cx : 0;
+    const int dy = inside ? CX : 0, dz = inside ? CX * CY : 0;
+    val[0] = chunk[o];
+    val[1] = chunk[o + 1];
+    val[2] = chunk[o + dy];
+    val[3] = chunk[o + dy + 1];
+    val[4] = chunk[o + dz];
+    val[5] = chunk[o + dz + 1];
+    val[6] = chunk[o + dz + dy];
+    val[7] = chunk[o + dz + dy + 1];
+    if (!inside) {
+        size_t g[8];
+        cell_corners<true>(v, cell, g);
 #pragma unroll
-        for (int i = 0; i < 8; i++) val[i] = vol[o[i]];
+        for (int i = 0; i < 8; i++) val[i] = vol[g[i]];
     }
 }
 
@@ -316,7 +323,15 @@ __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, cons
     __shared__ int zr[2];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     const int sub = lane & 3, pix = lane >> 2;
-    const int tx0 = roi.x0 + (int)blockIdx.x * 8, ty0 = roi.y0 + (int)blockIdx.y * 8;
+    // Workgroup b runs on XCD b mod 8.  Tiles come in groups of kXcdGroup x kXcdGroup, dealt to the
+    // XCDs in turn: a group's overlapping ray tubes are staged through one L2, and every XCD gets
+    // groups from all over the detector (rows outside the volume's shadow are cheap).
+    const int xcd = (int)(blockIdx.x & 7u), slot = (int)(blockIdx.x >> 3);
+    const int grp = (slot / (kXcdGroup * kXcdGroup)) * 8 + xcd, mem = slot % (kXcdGroup * kXcdGroup);
+    const int gtx = (grp % sp.groups_x) * kXcdGroup + mem % kXcdGroup;
+    const int gty = (grp / sp.groups_x) * kXcdGroup + mem / kXcdGroup;
+    if (gtx >= sp.tiles_x || gty >= sp.tiles_y) return;   // whole workgroup
+    const int tx0 = roi.x0 + gtx * 8, ty0 = roi.y0 + gty * 8;
     const int x = tx0 + (wave & 1) * 4 + (pix & 3);
     const int y = ty0 + (wave >> 1) * 4 + (pix >> 2);
     const bool active = x < roi.x1 && y < roi.y1;
@@ -500,19 +515,24 @@ void launch_march_impl(artia_ctx* ctx, const VolG& v, const TiltG& g, const floa
                      (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 24) || v.nx >= (1 << 24);
     SlabPlan sp;
     if (ctx->march_lds && slab_plan(v, g, sp)) {
+        sp.tiles_x = (int)grid.x;
+        sp.tiles_y = (int)grid.y;
+        sp.groups_x = (sp.tiles_x + kXcdGroup - 1) / kXcdGroup;
+        const int groups = sp.groups_x * ((sp.tiles_y + kXcdGroup - 1) / kXcdGroup);
+        const dim3 grid1((unsigned)(((groups + 7) / 8) * 8 * kXcdGroup * kXcdGroup));
         if (slicer) {
             if (big)
-                hipLaunchKernelGGL((march_lds_kernel<true, true>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+                hipLaunchKernelGGL((march_lds_kernel<true, true>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch,
                                    proj, dist, tmin, tmax, sp);
             else
-                hipLaunchKernelGGL((march_lds_kernel<true, false>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+                hipLaunchKernelGGL((march_lds_kernel<true, false>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch,
                                    proj, dist, tmin, tmax, sp);
         } else {
             if (big)
-                hipLaunchKernelGGL((march_lds_kernel<false, true>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+                hipLaunchKernelGGL((march_lds_kernel<false, true>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch,
                                    proj, dist, tmin, tmax, sp);
             else
-                hipLaunchKernelGGL((march_lds_kernel<false, false>), grid, block, 0, ctx->stream, v, g, vol, r,
+                hipLaunchKernelGGL((march_lds_kernel<false, false>), grid1, block, 0, ctx->stream, v, g, vol, r,
                                    pitch, proj, dist, tmin, tmax, sp);
         }
         return;
