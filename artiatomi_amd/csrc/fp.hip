@@ -419,12 +419,48 @@ __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, cons
             }
         }
         __syncthreads();
-        while (w.active() && (sp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
-            w.step([&](const int cell[3], int k, int mi) {
-                float val[8];
-                chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
-                total += cell_sum_values(val, cell_moments(L.q0, dq_u, cell, k, mi), mi);
-            });
+        // A lane whose cells in this slab provably lie in the box (its samples here run from the
+        // current cell to the slab's exit plane; one cell of margin covers the walk's rounding)
+        // walks with an LDS offset updated per step and no per-cell test.
+        bool fast = false;
+        if (staged && w.active()) {
+            const float zp = sp.zdir > 0 ? (float)(ze + 1) : (float)zs;
+            const float ke = fminf(fmaxf((zp - L.q0[2]) * inv_u[2], (float)w.k), (float)(w.count - 1));
+            const int xb = (int)floorf(__builtin_fmaf(ke, dq_u[0], L.q0[0]));
+            const int yb = (int)floorf(__builtin_fmaf(ke, dq_u[1], L.q0[1]));
+            const int zl = sp.zdir > 0 ? w.cell[2] : max(zs, -1);
+            const int zh = sp.zdir > 0 ? min(ze + 1, v.nz) : w.cell[2] + 1;
+            fast = min(w.cell[0], xb) - 1 >= X0 && max(w.cell[0], xb) + 2 <= X1 && min(w.cell[1], yb) - 1 >= Y0 &&
+                   max(w.cell[1], yb) + 2 <= Y1 && zl >= Z0 && zh <= Z1;
+        }
+        if (fast) {
+            const int CXY = CX * CY;
+            const int st0 = sgn_u[0], st1 = sgn_u[1] * CX, st2 = sgn_u[2] * CXY;
+            int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
+            while (w.active() && (sp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
+                w.step(
+                    [&](const int cell[3], int k, int mi) {
+                        float val[8];
+                        val[0] = chunk[o];
+                        val[1] = chunk[o + 1];
+                        val[2] = chunk[o + CX];
+                        val[3] = chunk[o + CX + 1];
+                        val[4] = chunk[o + CXY];
+                        val[5] = chunk[o + CXY + 1];
+                        val[6] = chunk[o + CXY + CX];
+                        val[7] = chunk[o + CXY + CX + 1];
+                        total += cell_sum_values(val, cell_moments(L.q0, dq_u, cell, k, mi), mi);
+                    },
+                    [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
+            }
+        } else {
+            while (w.active() && (sp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
+                w.step([&](const int cell[3], int k, int mi) {
+                    float val[8];
+                    chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
+                    total += cell_sum_values(val, cell_moments(L.q0, dq_u, cell, k, mi), mi);
+                });
+            }
         }
     }
     // cells beyond the slabs (not expected): global reads

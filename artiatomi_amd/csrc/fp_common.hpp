@@ -164,8 +164,9 @@ struct CellWalker {
         guard = 0;
     }
     __device__ __forceinline__ bool active() const { return k < count && guard < 4 * count + 64; }
-    template <class F>
-    __device__ __forceinline__ void step(F&& f) {
+    // One step; moved(i) is called for every axis i the walk advances along.
+    template <class F, class M>
+    __device__ __forceinline__ void step(F&& f, M&& moved) {
         const float kn = fminf(fminf(kb[0], kb[1]), kb[2]);
         const int kend = kn < fcount ? max((int)ceilf(kn), 0) : count;
         if (kend > k) {
@@ -177,9 +178,14 @@ struct CellWalker {
             if (kb[i] == kn) {
                 cell[i] += sgn[i];
                 kb[i] = __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]);
+                moved(i);
             }
         }
         guard++;
+    }
+    template <class F>
+    __device__ __forceinline__ void step(F&& f) {
+        step(f, [](int) {});
     }
 };
 
@@ -209,7 +215,7 @@ __device__ __forceinline__ CellMoments cell_moments(const float q0[3], const flo
     cm.b = __builtin_fmaf(kc, dq[1], q0[1]) - (float)cell[1];
     cm.c = __builtin_fmaf(kc, dq[2], q0[2]) - (float)cell[2];
     cm.m = (float)mi;
-    cm.S2c = (float)((mi * (mi * mi - 1)) / 6) * 0.5f;   // (m-1) m (m+1) is a multiple of 6: exact
+    cm.S2c = cm.m * __builtin_fmaf(cm.m, cm.m, -1.0f) * (1.0f / 12.0f);   // m, m^2 - 1 and their product exact
     cm.pab = dq[0] * dq[1];
     cm.pac = dq[0] * dq[2];
     cm.pbc = dq[1] * dq[2];
