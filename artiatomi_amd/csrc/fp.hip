@@ -18,6 +18,7 @@
 
 #include "device_common.hpp"
 #include "fp_common.hpp"
+#include "tube_walk.hpp"
 
 namespace artia {
 namespace {
@@ -256,69 +257,16 @@ __global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const
     march_store<kSlicer>(acc, L.len, L.hits, active, sub, x, y, pitch, step, proj, dist);
 }
 
-// LDS-staged form of march_exact_kernel for volumes of at least 2 voxels per axis.  The block's
-// 256 sub-rays are parallel, so the cells they cross between two z planes lie in a box that grows
-// linearly with the slab: the block walks the volume in slabs of kSlabCells cell layers along the
-// rays' z direction, stages each slab's box of voxels in LDS with coalesced row loads, and reads
-// the eight corners of every cell from LDS instead of through the texture addresser.  Each lane
-// runs the same cell walk in the same order as march_exact_kernel (the walk is only paused at slab
-// boundaries), so the projection is bit-identical; a corner outside the staged box (not expected)
-// is read from global memory.
-constexpr int kSlabCells = 8;
-constexpr int kChunkFloats = 5120;   // 20 KB of LDS per block
-
-struct SlabPlan {
-    float bx, by;   // dq_x / dq_z and dq_y / dq_z (texel units; equal for all sub-rays)
-    int zdir;       // sign of dq_z (nonzero)
-    int tiles_x, tiles_y, groups_x;   // 8x8-pixel tiles per row and column, tile groups per row
-};
-
-constexpr int kXcdGroup = 4;   // tiles per side of the groups dealt to one XCD
-
-// Corners of a cell from the staged box when it lies inside (the box is padded with the
-// clamp-addressed border voxels, indices -1 and n, so no per-cell clamping is needed), otherwise
-// from global memory with clamped indices.  The LDS reads are issued unconditionally (at offset 0
-// when outside, which stays inside the box) and overwritten on the rare global path: branching
-// between the two sources would let the compiler merge them into generic loads that wait on both
-// memory counters.
-__device__ __forceinline__ void chunk_corners(const VolG& v, const float* __restrict__ vol,
-                                              const float* __restrict__ chunk, bool staged, int X0, int Y0, int Z0,
-                                              int CX, int CY, int CZ, const int cell[3], float val[8]) {
-    const int cx = cell[0] - X0, cy = cell[1] - Y0, cz = cell[2] - Z0;
-    const bool inside = staged && (unsigned)cx < (unsigned)(CX - 1) && (unsigned)cy < (unsigned)(CY - 1) &&
-                        (unsigned)cz < (unsigned)(CZ - 1);
-    // outside: every read at chunk[0] or chunk[1]
-    const int o = inside ? (cz * CY + cy) * CX + cx : 0;
-    const int dy = inside ? CX : 0, dz = inside ? CX * CY : 0;
-    val[0] = chunk[o];
-    val[1] = chunk[o + 1];
-    val[2] = chunk[o + dy];
-    val[3] = chunk[o + dy + 1];
-    val[4] = chunk[o + dz];
-    val[5] = chunk[o + dz + 1];
-    val[6] = chunk[o + dz + dy];
-    val[7] = chunk[o + dz + dy + 1];
-    if (!inside) {
-        size_t g[8];
-        cell_corners<true>(v, cell, g);
-#pragma unroll
-        for (int i = 0; i < 8; i++) val[i] = vol[g[i]];
-    }
-}
-
-// A value equal in every lane, moved to a scalar register.
-__device__ __forceinline__ float wave_uniform(float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
-}
-__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
-
-// Six waves per SIMD (80 registers): the walk is VALU bound, and the occupancy hides the LDS
-// latency better than the few registers it spills.
-template <bool kSlicer, bool kBig>
+// LDS-staged form of march_exact_kernel (tube_walk.hpp) for volumes of at least 2 voxels per
+// axis.  Each lane runs the same cell walk in the same order as march_exact_kernel (the walk is only
+// paused at slab boundaries), so the projection is bit-identical.  Six waves per SIMD (80
+// registers): the walk is VALU bound, and the occupancy hides the LDS latency better than the few
+// registers it spills.
+template <bool kSlicer>
 __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, const float* __restrict__ vol, Roi roi,
                                                         size_t pitch, float* __restrict__ proj,
                                                         float* __restrict__ dist, float tminDef, float tmaxDef,
-                                                        SlabPlan sp) {
+                                                        TubePlan tp) {
     __shared__ float chunk[kChunkFloats];
     __shared__ int zr[2];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
@@ -328,9 +276,9 @@ __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, cons
     // groups from all over the detector (rows outside the volume's shadow are cheap).
     const int xcd = (int)(blockIdx.x & 7u), slot = (int)(blockIdx.x >> 3);
     const int grp = (slot / (kXcdGroup * kXcdGroup)) * 8 + xcd, mem = slot % (kXcdGroup * kXcdGroup);
-    const int gtx = (grp % sp.groups_x) * kXcdGroup + mem % kXcdGroup;
-    const int gty = (grp / sp.groups_x) * kXcdGroup + mem / kXcdGroup;
-    if (gtx >= sp.tiles_x || gty >= sp.tiles_y) return;   // whole workgroup
+    const int gtx = (grp % tp.groups_x) * kXcdGroup + mem % kXcdGroup;
+    const int gty = (grp / tp.groups_x) * kXcdGroup + mem / kXcdGroup;
+    if (gtx >= tp.tiles_x || gty >= tp.tiles_y) return;   // whole workgroup
     const int tx0 = roi.x0 + gtx * 8, ty0 = roi.y0 + gty * 8;
     const int x = tx0 + (wave & 1) * 4 + (pix & 3);
     const int y = ty0 + (wave >> 1) * 4 + (pix >> 2);
@@ -338,142 +286,13 @@ __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, cons
     const double step = (double)v.vs[0] * 0.15;
     const MarchLane L = march_lane<kSlicer>(v, g, vol, active, x, y, sub, step, tminDef, tmaxDef);
     if (!kSlicer) march_store_dist(L.len, L.hits, active, sub, x, y, pitch, dist);   // before the walk
-
-    // cell layers the block's sub-rays cross
-    if (threadIdx.x == 0) {
-        zr[0] = INT_MAX;
-        zr[1] = INT_MIN;
-    }
-    __syncthreads();
-    if (L.count > 0) {
-        const int za = (int)floorf(L.q0[2]);
-        const int zb = (int)floorf(__builtin_fmaf((float)(L.count - 1), L.dq[2], L.q0[2]));
-        atomicMin(&zr[0], min(za, zb) - 1);
-        atomicMax(&zr[1], max(za, zb) + 1);
-    }
-    __syncthreads();
-    const int zlo = max(zr[0], -1), zhi = min(zr[1], v.nz - 1);
-
-    // Every sub-ray of the tile lies on a line Q(z) = (a_x + bx z, a_y + by z, z) in texel
-    // coordinates; the tile's four corner sub-rays bound a_x and a_y.
-    float axl = INFINITY, axh = -INFINITY, ayl = INFINITY, ayh = -INFINITY;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const double px = (double)tx0 + ((c & 1) ? 7.75 : 0.25), py = (double)ty0 + ((c & 2) ? 7.75 : 0.25);
-        const double xa = ((double)g.A[0] * px + (double)g.A[1] * py) + (double)g.A[2];
-        const double ya = ((double)g.A[3] * px + (double)g.A[4] * py) + (double)g.A[5];
-        double q[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const double ci = ((double)g.d[i] + xa * (double)g.u[i]) + ya * (double)g.v[i];
-            q[i] = (ci - (double)v.bbMin[i]) * ((double)v.rcp[i] * (double)v.dim[i]) - 0.5 -
-                   (i == 2 ? (double)v.zshift : 0.0);
-        }
-        const float ax = (float)(q[0] - q[2] * (double)sp.bx), ay = (float)(q[1] - q[2] * (double)sp.by);
-        axl = fminf(axl, ax);
-        axh = fmaxf(axh, ax);
-        ayl = fminf(ayl, ay);
-        ayh = fmaxf(ayh, ay);
-    }
-    axl = wave_uniform(axl);
-    axh = wave_uniform(axh);
-    ayl = wave_uniform(ayl);
-    ayh = wave_uniform(ayh);
-
-    // the sample step is the same for every sub-ray (march_lane's dq, evaluated in every lane)
-    int sgn_u[3];
-    float dq_u[3], inv_u[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        dq_u[i] = wave_uniform((float)(step * (double)g.n[i] * ((double)v.rcp[i] * (double)v.dim[i])));
-        sgn_u[i] = dq_u[i] > 0.0f ? 1 : (dq_u[i] < 0.0f ? -1 : 0);
-        inv_u[i] = wave_uniform(sgn_u[i] != 0 ? 1.0f / dq_u[i] : 0.0f);
-    }
-    CellWalker w;
-    w.init_dir(L.q0, sgn_u, inv_u, L.count);
+    const TubeDir td = tube_dir(v, g, step);
+    const TubeBox tb = tube_box(v, g, tp, (double)tx0 + 0.25, (double)tx0 + 7.75, (double)ty0 + 0.25,
+                                (double)ty0 + 7.75, 0.0, 0.0);
     double total = 0.0;   // summed as march_sum does, then added to the slicer's first sample
-    const int nslab = zlo <= zhi ? (zhi - zlo) / kSlabCells + 1 : 0;
-    for (int j = 0; j < nslab; j++) {
-        // cell layers [zs, ze] of this slab, in the rays' z direction
-        const int zs = sp.zdir > 0 ? zlo + j * kSlabCells : zhi - (j + 1) * kSlabCells + 1;
-        const int ze = zs + kSlabCells - 1;
-        // box of lattice indices the slab's cells touch, in index space padded by the clamp border
-        // (-1 .. n); the margins cover the float rounding of the walk
-        const float z0 = (float)zs, z1 = (float)(ze + 1);
-        const int X0 = wave_uniform(max((int)floorf(axl + fminf(sp.bx * z0, sp.bx * z1)) - 1, -1));
-        const int X1 = min((int)floorf(axh + fmaxf(sp.bx * z0, sp.bx * z1)) + 2, v.nx);
-        const int Y0 = wave_uniform(max((int)floorf(ayl + fminf(sp.by * z0, sp.by * z1)) - 1, -1));
-        const int Y1 = min((int)floorf(ayh + fmaxf(sp.by * z0, sp.by * z1)) + 2, v.ny);
-        const int Z0 = max(zs, -1), Z1 = min(ze + 1, v.nz);
-        const int CX = wave_uniform(X1 - X0 + 1), CY = wave_uniform(Y1 - Y0 + 1), CZ = Z1 - Z0 + 1;
-        const bool staged = CX > 1 && CY > 1 && CZ > 1 && CX * CY * CZ <= kChunkFloats;
-        __syncthreads();   // the previous slab's reads are done
-        if (staged) {
-            const int n = CX * CY * CZ;
-            for (int e = (int)threadIdx.x; e < n; e += 256) {
-                const int row = e / CX, xx = e - row * CX;
-                const int zz = row / CY, yy = row - zz * CY;
-                const size_t gx = (size_t)clampi(X0 + xx, 0, v.nx - 1), gy = (size_t)clampi(Y0 + yy, 0, v.ny - 1);
-                const size_t gz = (size_t)clampi(Z0 + zz, 0, v.nz - 1);
-                chunk[e] = vol[(gz * (size_t)v.ny + gy) * (size_t)v.nx + gx];
-            }
-        }
-        __syncthreads();
-        // A lane whose cells in this slab provably lie in the box (its samples here run from the
-        // current cell to the slab's exit plane; one cell of margin covers the walk's rounding)
-        // walks with an LDS offset updated per step and no per-cell test.
-        bool fast = false;
-        if (staged && w.active()) {
-            const float zp = sp.zdir > 0 ? (float)(ze + 1) : (float)zs;
-            const float ke = fminf(fmaxf((zp - L.q0[2]) * inv_u[2], (float)w.k), (float)(w.count - 1));
-            const int xb = (int)floorf(__builtin_fmaf(ke, dq_u[0], L.q0[0]));
-            const int yb = (int)floorf(__builtin_fmaf(ke, dq_u[1], L.q0[1]));
-            const int zl = sp.zdir > 0 ? w.cell[2] : max(zs, -1);
-            const int zh = sp.zdir > 0 ? min(ze + 1, v.nz) : w.cell[2] + 1;
-            fast = min(w.cell[0], xb) - 1 >= X0 && max(w.cell[0], xb) + 2 <= X1 && min(w.cell[1], yb) - 1 >= Y0 &&
-                   max(w.cell[1], yb) + 2 <= Y1 && zl >= Z0 && zh <= Z1;
-        }
-        if (fast) {
-            const int CXY = CX * CY;
-            const int st0 = sgn_u[0], st1 = sgn_u[1] * CX, st2 = sgn_u[2] * CXY;
-            int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
-            while (w.active() && (sp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
-                w.step(
-                    [&](const int cell[3], int k, int mi) {
-                        float val[8];
-                        val[0] = chunk[o];
-                        val[1] = chunk[o + 1];
-                        val[2] = chunk[o + CX];
-                        val[3] = chunk[o + CX + 1];
-                        val[4] = chunk[o + CXY];
-                        val[5] = chunk[o + CXY + 1];
-                        val[6] = chunk[o + CXY + CX];
-                        val[7] = chunk[o + CXY + CX + 1];
-                        total += cell_sum_values(val, cell_moments(L.q0, dq_u, cell, k, mi), mi);
-                    },
-                    [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
-            }
-        } else {
-            while (w.active() && (sp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
-                w.step([&](const int cell[3], int k, int mi) {
-                    float val[8];
-                    chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
-                    total += cell_sum_values(val, cell_moments(L.q0, dq_u, cell, k, mi), mi);
-                });
-            }
-        }
-    }
-    // cells beyond the slabs (not expected): global reads
-    while (w.active()) {
-        w.step([&](const int cell[3], int k, int mi) {
-            size_t o[8];
-            cell_corners<true>(v, cell, o);
-            float val[8];
-#pragma unroll
-            for (int i = 0; i < 8; i++) val[i] = vol[o[i]];
-            total += cell_sum_values(val, cell_moments(L.q0, dq_u, cell, k, mi), mi);
-        });
-    }
+    tube_walk(v, vol, chunk, zr, tp, tb, td, L.q0, L.count, [&](const float val[8], const int cell[3], int k, int mi) {
+        total += cell_sum_values(val, cell_moments(L.q0, td.dq, cell, k, mi), mi);
+    });
     double acc = L.acc;
     if (L.count > 0) acc += total;
     march_store<true>(acc, 0.0, 0.0, active, sub, x, y, pitch, step, proj, nullptr);   // proj only
@@ -507,31 +326,6 @@ dim3 grid_for(const Roi& r) {
     return dim3((unsigned)((w + 63) / 64), (unsigned)((h + 3) / 4));
 }
 
-// Slab plan of the LDS-staged marcher, or false when it does not apply: volumes below 2 voxels on
-// an axis, rays parallel to the z layers, or a tile's slab box larger than the LDS chunk (steep
-// rays), where march_exact_kernel runs instead.  The box of an 8x8-pixel tile spans the tile's
-// extent in a_x = Q_x - bx Q_z (affine in the pixel) plus the rays' drift over a slab.
-bool slab_plan(const VolG& v, const TiltG& g, SlabPlan& sp) {
-    if (v.nx < 2 || v.ny < 2 || v.nz < 2) return false;
-    double sc[3], N[3], dpx[3], dpy[3];
-    for (int i = 0; i < 3; i++) {
-        sc[i] = (double)v.rcp[i] * (double)v.dim[i];
-        N[i] = (double)g.n[i] * sc[i];
-        dpx[i] = ((double)g.A[0] * g.u[i] + (double)g.A[3] * g.v[i]) * sc[i];
-        dpy[i] = ((double)g.A[1] * g.u[i] + (double)g.A[4] * g.v[i]) * sc[i];
-    }
-    if (!(std::fabs(N[2]) > 1e-3 * (std::fabs(N[0]) + std::fabs(N[1]) + std::fabs(N[2])))) return false;
-    const double bx = N[0] / N[2], by = N[1] / N[2];
-    const double wx = 7.5 * (std::fabs(dpx[0] - bx * dpx[2]) + std::fabs(dpy[0] - bx * dpy[2]));
-    const double wy = 7.5 * (std::fabs(dpx[1] - by * dpx[2]) + std::fabs(dpy[1] - by * dpy[2]));
-    const double cx = wx + std::fabs(bx) * (kSlabCells + 1) + 5.0, cy = wy + std::fabs(by) * (kSlabCells + 1) + 5.0;
-    if (cx * cy * (kSlabCells + 1) > (double)kChunkFloats) return false;
-    sp.bx = (float)bx;
-    sp.by = (float)by;
-    sp.zdir = N[2] > 0.0 ? 1 : -1;
-    return true;
-}
-
 void launch_march_impl(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vol, const Roi& r, size_t pitch,
                        float* proj, float* dist, bool slicer, float tmin, float tmax) {
     if (r.x1 <= r.x0 || r.y1 <= r.y0) return;
@@ -549,28 +343,19 @@ void launch_march_impl(artia_ctx* ctx, const VolG& v, const TiltG& g, const floa
     // 32-bit buffer offsets need fewer than 2^30 voxels and a row index z * ny + y below 2^24
     const bool big = (uint64_t)v.nx * (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 30) ||
                      (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 24) || v.nx >= (1 << 24);
-    SlabPlan sp;
-    if (ctx->march_lds && slab_plan(v, g, sp)) {
-        sp.tiles_x = (int)grid.x;
-        sp.tiles_y = (int)grid.y;
-        sp.groups_x = (sp.tiles_x + kXcdGroup - 1) / kXcdGroup;
-        const int groups = sp.groups_x * ((sp.tiles_y + kXcdGroup - 1) / kXcdGroup);
+    TubePlan tp;
+    if (ctx->march_lds && tube_plan(v, g, tp)) {
+        tp.tiles_x = (int)grid.x;
+        tp.tiles_y = (int)grid.y;
+        tp.groups_x = (tp.tiles_x + kXcdGroup - 1) / kXcdGroup;
+        const int groups = tp.groups_x * ((tp.tiles_y + kXcdGroup - 1) / kXcdGroup);
         const dim3 grid1((unsigned)(((groups + 7) / 8) * 8 * kXcdGroup * kXcdGroup));
-        if (slicer) {
-            if (big)
-                hipLaunchKernelGGL((march_lds_kernel<true, true>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch,
-                                   proj, dist, tmin, tmax, sp);
-            else
-                hipLaunchKernelGGL((march_lds_kernel<true, false>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch,
-                                   proj, dist, tmin, tmax, sp);
-        } else {
-            if (big)
-                hipLaunchKernelGGL((march_lds_kernel<false, true>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch,
-                                   proj, dist, tmin, tmax, sp);
-            else
-                hipLaunchKernelGGL((march_lds_kernel<false, false>), grid1, block, 0, ctx->stream, v, g, vol, r,
-                                   pitch, proj, dist, tmin, tmax, sp);
-        }
+        if (slicer)
+            hipLaunchKernelGGL((march_lds_kernel<true>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch, proj,
+                               dist, tmin, tmax, tp);
+        else
+            hipLaunchKernelGGL((march_lds_kernel<false>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch, proj,
+                               dist, tmin, tmax, tp);
         return;
     }
     if (slicer) {

@@ -1,0 +1,252 @@
+// LDS-staged cell walk of a tile's parallel sub-rays (the exact forward projector and the
+// refinement pair kernels).
+//
+// The 256 sub-rays of an 8x8-pixel tile are parallel, so the lattice cells they cross between two
+// z planes lie in a box that grows linearly with the slab: the block walks the volume in slabs of
+// kSlabCells cell layers along the rays' z direction, stages each slab's box of voxels in LDS with
+// coalesced row loads, and reads the eight corners of every cell from LDS instead of through the
+// texture addresser.  The box is padded with the clamp-addressed border voxels (indices -1 and n),
+// so cells need no clamping.  A lane whose cells in a slab provably lie in the box walks with an
+// LDS offset updated per step; otherwise each cell is tested, and a cell outside the box (not
+// expected) is read from global memory.  Every lane runs the same CellWalker steps in the same
+// order as an unstaged walk, so sums are bit-identical to it.
+#pragma once
+
+#include <climits>
+#include <cmath>
+
+#include "fp_common.hpp"
+
+namespace artia {
+
+constexpr int kSlabCells = 8;        // cell layers per slab
+constexpr int kChunkFloats = 5120;   // LDS staging capacity per block (20 KB)
+constexpr int kXcdGroup = 4;         // tiles per side of the tile groups dealt to one XCD
+
+struct TubePlan {
+    float bx, by;   // dq_x / dq_z and dq_y / dq_z (texel units; equal for all sub-rays)
+    int zdir;       // sign of dq_z (nonzero)
+    int tiles_x, tiles_y, groups_x;   // launch layout of the forward projector (8x8-pixel tiles)
+};
+
+// The plan of one tilt geometry, or false when the staged walk does not apply: volumes below 2
+// voxels on an axis, rays parallel to the z layers, or a tile's slab box larger than the LDS chunk
+// (steep rays).  The box of an 8x8-pixel tile spans the tile's extent in a_x = Q_x - bx Q_z
+// (affine in the pixel) plus the rays' drift over a slab.
+inline bool tube_plan(const VolG& v, const TiltG& g, TubePlan& tp) {
+    if (v.nx < 2 || v.ny < 2 || v.nz < 2) return false;
+    double sc[3], N[3], dpx[3], dpy[3];
+    for (int i = 0; i < 3; i++) {
+        sc[i] = (double)v.rcp[i] * (double)v.dim[i];
+        N[i] = (double)g.n[i] * sc[i];
+        dpx[i] = ((double)g.A[0] * g.u[i] + (double)g.A[3] * g.v[i]) * sc[i];
+        dpy[i] = ((double)g.A[1] * g.u[i] + (double)g.A[4] * g.v[i]) * sc[i];
+    }
+    if (!(std::fabs(N[2]) > 1e-3 * (std::fabs(N[0]) + std::fabs(N[1]) + std::fabs(N[2])))) return false;
+    const double bx = N[0] / N[2], by = N[1] / N[2];
+    const double wx = 7.5 * (std::fabs(dpx[0] - bx * dpx[2]) + std::fabs(dpy[0] - bx * dpy[2]));
+    const double wy = 7.5 * (std::fabs(dpx[1] - by * dpx[2]) + std::fabs(dpy[1] - by * dpy[2]));
+    const double cx = wx + std::fabs(bx) * (kSlabCells + 1) + 5.0, cy = wy + std::fabs(by) * (kSlabCells + 1) + 5.0;
+    if (cx * cy * (kSlabCells + 1) > (double)kChunkFloats) return false;
+    tp.bx = (float)bx;
+    tp.by = (float)by;
+    tp.zdir = N[2] > 0.0 ? 1 : -1;
+    tp.tiles_x = tp.tiles_y = tp.groups_x = 0;
+    return true;
+}
+
+// A value equal in every lane, moved to a scalar register.
+__device__ __forceinline__ float wave_uniform(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Sample step shared by all sub-rays (subray_samples' dq) and the walk's direction part, in scalar
+// registers.
+struct TubeDir {
+    float dq[3], inv[3];
+    int sgn[3];
+};
+
+__device__ __forceinline__ TubeDir tube_dir(const VolG& v, const TiltG& g, double step) {
+    TubeDir d;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        d.dq[i] = wave_uniform((float)(step * (double)g.n[i] * ((double)v.rcp[i] * (double)v.dim[i])));
+        d.sgn[i] = d.dq[i] > 0.0f ? 1 : (d.dq[i] < 0.0f ? -1 : 0);
+        d.inv[i] = wave_uniform(d.sgn[i] != 0 ? 1.0f / d.dq[i] : 0.0f);
+    }
+    return d;
+}
+
+// Every sub-ray of the tile lies on a line Q(z) = (a_x + bx z, a_y + by z, z) in texel
+// coordinates; the tile's corner sub-rays (pixel coordinates px_lo..px_hi, py_lo..py_hi, detector
+// shift (sx, sy) as in subray_samples) bound a_x and a_y.
+struct TubeBox {
+    float axl, axh, ayl, ayh;
+};
+
+__device__ __forceinline__ TubeBox tube_box(const VolG& v, const TiltG& g, const TubePlan& tp, double px_lo,
+                                            double px_hi, double py_lo, double py_hi, double sx, double sy) {
+    float axl = INFINITY, axh = -INFINITY, ayl = INFINITY, ayh = -INFINITY;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const double px = (c & 1) ? px_hi : px_lo, py = (c & 2) ? py_hi : py_lo;
+        const double xa = (((double)g.A[0] * px + (double)g.A[1] * py) + (double)g.A[2]) - sx;
+        const double ya = (((double)g.A[3] * px + (double)g.A[4] * py) + (double)g.A[5]) - sy;
+        double q[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double ci = ((double)g.d[i] + xa * (double)g.u[i]) + ya * (double)g.v[i];
+            q[i] = (ci - (double)v.bbMin[i]) * ((double)v.rcp[i] * (double)v.dim[i]) - 0.5 -
+                   (i == 2 ? (double)v.zshift : 0.0);
+        }
+        const float ax = (float)(q[0] - q[2] * (double)tp.bx), ay = (float)(q[1] - q[2] * (double)tp.by);
+        axl = fminf(axl, ax);
+        axh = fmaxf(axh, ax);
+        ayl = fminf(ayl, ay);
+        ayh = fmaxf(ayh, ay);
+    }
+    return TubeBox{wave_uniform(axl), wave_uniform(axh), wave_uniform(ayl), wave_uniform(ayh)};
+}
+
+// Corners of a cell from the staged box when it lies inside, otherwise from global memory with
+// clamped indices.  The LDS reads are issued unconditionally (at offset 0 when outside, which stays
+// inside the box) and overwritten on the rare global path: branching between the two sources would
+// let the compiler merge them into generic loads that wait on both memory counters.
+__device__ __forceinline__ void chunk_corners(const VolG& v, const float* __restrict__ vol,
+                                              const float* __restrict__ chunk, bool staged, int X0, int Y0, int Z0,
+                                              int CX, int CY, int CZ, const int cell[3], float val[8]) {
+    const int cx = cell[0] - X0, cy = cell[1] - Y0, cz = cell[2] - Z0;
+    const bool inside = staged && (unsigned)cx < (unsigned)(CX - 1) && (unsigned)cy < (unsigned)(CY - 1) &&
+                        (unsigned)cz < (unsigned)(CZ - 1);
+    // outside: every read at chunk[0] or chunk[1]
+    const int o = inside ? (cz * CY + cy) * CX + cx : 0;
+    const int dy = inside ? CX : 0, dz = inside ? CX * CY : 0;
+    val[0] = chunk[o];
+    val[1] = chunk[o + 1];
+    val[2] = chunk[o + dy];
+    val[3] = chunk[o + dy + 1];
+    val[4] = chunk[o + dz];
+    val[5] = chunk[o + dz + 1];
+    val[6] = chunk[o + dz + dy];
+    val[7] = chunk[o + dz + dy + 1];
+    if (!inside) {
+        size_t g[8];
+        cell_corners<true>(v, cell, g);
+#pragma unroll
+        for (int i = 0; i < 8; i++) val[i] = vol[g[i]];
+    }
+}
+
+// Walks every lane's samples q0 + k*dq (k < count) cell by cell, calling
+// f(val[8], cell, k, m) for every cell holding samples k..k+m-1 with its eight corner values.  Must
+// be called by all 256 threads of the block (it synchronises); chunk holds kChunkFloats floats and
+// zr two ints of shared memory.
+template <class F>
+__device__ __forceinline__ void tube_walk(const VolG& v, const float* __restrict__ vol, float* __restrict__ chunk,
+                                          int* zr, const TubePlan& tp, const TubeBox& tb, const TubeDir& td,
+                                          const float q0[3], int count, F&& f) {
+    // cell layers the block's sub-rays cross
+    __syncthreads();   // a previous walk's readers are done with zr and chunk
+    if (threadIdx.x == 0) {
+        zr[0] = INT_MAX;
+        zr[1] = INT_MIN;
+    }
+    __syncthreads();
+    if (count > 0) {
+        const int za = (int)floorf(q0[2]);
+        const int zb = (int)floorf(__builtin_fmaf((float)(count - 1), td.dq[2], q0[2]));
+        atomicMin(&zr[0], min(za, zb) - 1);
+        atomicMax(&zr[1], max(za, zb) + 1);
+    }
+    __syncthreads();
+    const int zlo = max(zr[0], -1), zhi = min(zr[1], v.nz - 1);
+
+    CellWalker w;
+    w.init_dir(q0, td.sgn, td.inv, count);
+    const int nslab = zlo <= zhi ? (zhi - zlo) / kSlabCells + 1 : 0;
+    for (int j = 0; j < nslab; j++) {
+        // cell layers [zs, ze] of this slab, in the rays' z direction
+        const int zs = tp.zdir > 0 ? zlo + j * kSlabCells : zhi - (j + 1) * kSlabCells + 1;
+        const int ze = zs + kSlabCells - 1;
+        // box of lattice indices the slab's cells touch, in index space padded by the clamp border
+        // (-1 .. n); the margins cover the float rounding of the walk
+        const float z0 = (float)zs, z1 = (float)(ze + 1);
+        const int X0 = wave_uniform(max((int)floorf(tb.axl + fminf(tp.bx * z0, tp.bx * z1)) - 1, -1));
+        const int X1 = min((int)floorf(tb.axh + fmaxf(tp.bx * z0, tp.bx * z1)) + 2, v.nx);
+        const int Y0 = wave_uniform(max((int)floorf(tb.ayl + fminf(tp.by * z0, tp.by * z1)) - 1, -1));
+        const int Y1 = min((int)floorf(tb.ayh + fmaxf(tp.by * z0, tp.by * z1)) + 2, v.ny);
+        const int Z0 = max(zs, -1), Z1 = min(ze + 1, v.nz);
+        const int CX = wave_uniform(X1 - X0 + 1), CY = wave_uniform(Y1 - Y0 + 1), CZ = Z1 - Z0 + 1;
+        const bool staged = CX > 1 && CY > 1 && CZ > 1 && CX * CY * CZ <= kChunkFloats;
+        __syncthreads();   // the previous slab's reads are done
+        if (staged) {
+            const int n = CX * CY * CZ;
+            for (int e = (int)threadIdx.x; e < n; e += 256) {
+                const int row = e / CX, xx = e - row * CX;
+                const int zz = row / CY, yy = row - zz * CY;
+                const size_t gx = (size_t)clampi(X0 + xx, 0, v.nx - 1), gy = (size_t)clampi(Y0 + yy, 0, v.ny - 1);
+                const size_t gz = (size_t)clampi(Z0 + zz, 0, v.nz - 1);
+                chunk[e] = vol[(gz * (size_t)v.ny + gy) * (size_t)v.nx + gx];
+            }
+        }
+        __syncthreads();
+        // A lane whose cells in this slab provably lie in the box (its samples here run from the
+        // current cell to the slab's exit plane; one cell of margin covers the walk's rounding)
+        // walks with an LDS offset updated per step and no per-cell test.
+        bool fast = false;
+        if (staged && w.active()) {
+            const float zp = tp.zdir > 0 ? (float)(ze + 1) : (float)zs;
+            const float ke = fminf(fmaxf((zp - q0[2]) * td.inv[2], (float)w.k), (float)(w.count - 1));
+            const int xb = (int)floorf(__builtin_fmaf(ke, td.dq[0], q0[0]));
+            const int yb = (int)floorf(__builtin_fmaf(ke, td.dq[1], q0[1]));
+            const int zl = tp.zdir > 0 ? w.cell[2] : max(zs, -1);
+            const int zh = tp.zdir > 0 ? min(ze + 1, v.nz) : w.cell[2] + 1;
+            fast = min(w.cell[0], xb) - 1 >= X0 && max(w.cell[0], xb) + 2 <= X1 && min(w.cell[1], yb) - 1 >= Y0 &&
+                   max(w.cell[1], yb) + 2 <= Y1 && zl >= Z0 && zh <= Z1;
+        }
+        if (fast) {
+            const int CXY = CX * CY;
+            const int st0 = td.sgn[0], st1 = td.sgn[1] * CX, st2 = td.sgn[2] * CXY;
+            int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
+            while (w.active() && (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
+                w.step(
+                    [&](const int cell[3], int k, int mi) {
+                        float val[8];
+                        val[0] = chunk[o];
+                        val[1] = chunk[o + 1];
+                        val[2] = chunk[o + CX];
+                        val[3] = chunk[o + CX + 1];
+                        val[4] = chunk[o + CXY];
+                        val[5] = chunk[o + CXY + 1];
+                        val[6] = chunk[o + CXY + CX];
+                        val[7] = chunk[o + CXY + CX + 1];
+                        f(val, cell, k, mi);
+                    },
+                    [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
+            }
+        } else {
+            while (w.active() && (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
+                w.step([&](const int cell[3], int k, int mi) {
+                    float val[8];
+                    chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
+                    f(val, cell, k, mi);
+                });
+            }
+        }
+    }
+    // cells beyond the slabs (not expected): global reads
+    while (w.active()) {
+        w.step([&](const int cell[3], int k, int mi) {
+            size_t o[8];
+            cell_corners<true>(v, cell, o);
+            float val[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) val[i] = vol[o[i]];
+            f(val, cell, k, mi);
+        });
+    }
+}
+
+}  // namespace artia
