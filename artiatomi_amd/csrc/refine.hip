@@ -18,6 +18,7 @@
 
 #include "artia_internal.hpp"
 #include "fp_common.hpp"
+#include "tube_walk.hpp"
 
 namespace artia {
 namespace {
@@ -25,6 +26,8 @@ namespace {
 struct PairDev {
     TiltG g;
     VolG v;
+    TubePlan tp;            // LDS-staged walk of the window's tiles (staged != 0)
+    int staged;
     long long vol_offset;   // floats into the volume buffer
     long long img_offset;   // floats into image buffers
     int x0, y0, w, h;
@@ -84,6 +87,8 @@ __device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
 
 template <int kMode>
 __global__ void __launch_bounds__(256) pair_march_kernel(PairArgs a) {
+    __shared__ float chunk[kChunkFloats];
+    __shared__ int zr[2];
     const int slot = slot_of_block(a.per_xcd);
     if (slot >= a.npairs) return;   // whole workgroup
     const int p = a.order[slot];
@@ -113,16 +118,32 @@ __global__ void __launch_bounds__(256) pair_march_kernel(PairArgs a) {
         SubraySamples s;
         const double px = (double)(x0 + i) + ((sub & 2) ? 0.75 : 0.25);
         const double py = (double)(y0 + j) + ((sub & 1) ? 0.75 : 0.25);
-        if (active && (kMode != kShiftGrad || win != 0.0f) && subray_samples(v, g, px, py, step, s, sx, sy)) {
-            double G[3] = {0.0, 0.0, 0.0};
+        const bool ok = active && (kMode != kShiftGrad || win != 0.0f) && subray_samples(v, g, px, py, step, s, sx, sy);
+        if (!ok) {
+            s.count = 0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) s.q0[c] = s.dq[c] = 0.0f;
+        }
+        double G[3] = {0.0, 0.0, 0.0};
+        auto cell_fn = [&](const float val[8], const int cell[3], int k, int mi) {
+            const CellMoments cm = cell_moments(s.q0, s.dq, cell, k, mi);
+            if (kMode != kShiftGrad) acc += cell_sum_values(val, cm, mi);
+            if (kMode != kProject) cell_grad_sum(val, cm, mi, G);
+        };
+        if (P.staged) {   // uniform: the whole block walks the tile's tube through LDS
+            const TubeDir td = tube_dir(v, g, step);
+            const double pi0 = (double)(x0 + (t % ntx) * 8), pj0 = (double)(y0 + (t / ntx) * 8);
+            const TubeBox tb = tube_box(v, g, P.tp, pi0 + 0.25, pi0 + 7.75, pj0 + 0.25, pj0 + 7.75, sx, sy);
+            tube_walk(v, a.vols + P.vol_offset, chunk, zr, P.tp, tb, td, s.q0, s.count, cell_fn);
+        } else if (ok) {
             for_each_cell(s.q0, s.dq, s.count, [&](const int cell[3], int k, int mi) {
                 float val[8];
                 uint32_t o[8];
                 load_cell(rs, v, cell, val, o);
-                const CellMoments cm = cell_moments(s.q0, s.dq, cell, k, mi);
-                if (kMode != kShiftGrad) acc += cell_sum_values(val, cm, mi);
-                if (kMode != kProject) cell_grad_sum(val, cm, mi, G);
+                cell_fn(val, cell, k, mi);
             });
+        }
+        if (ok) {
             if (kMode != kProject) {
                 // entry-axis components by select (a dynamic index would put the geometry in scratch)
                 const int ax = s.entry_axis;
@@ -383,6 +404,7 @@ artia_status artia_pairs_create(artia_ctx* ctx, const artia_pair_volume* volumes
         PairDev& d = host[(size_t)p];
         d.g = to_device(geoms[p]);
         d.v = to_device(volumes[q.volume].geom);
+        d.staged = tube_plan(d.v, d.g, d.tp) ? 1 : 0;
         d.vol_offset = volumes[q.volume].offset;
         d.img_offset = q.offset;
         d.x0 = q.x0;
