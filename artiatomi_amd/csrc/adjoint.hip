@@ -10,6 +10,7 @@
 //                  and with it every sample: dq_k/ds = scale * (-u + (dt_in/ds) n) per sub-ray.
 #include "artia_internal.hpp"
 #include "fp_common.hpp"
+#include "tube_walk.hpp"
 
 namespace artia {
 namespace {
@@ -69,6 +70,48 @@ __global__ void __launch_bounds__(256) fp_adjoint_kernel(VolG v, TiltG g, const 
             const float add = (float)(r * W[i]);
             if (add != 0.0f) unsafeAtomicAdd(grad + o[i], add);
         }
+    });
+}
+
+// The same transpose with each tile's rays walked through LDS-staged slabs (tube_walk.hpp): the
+// weights are summed per slab in a box of shared memory and added to grad with one atomic per box
+// voxel, not eight per cell.  Tiles are dealt to the XCDs in groups, as in the forward projector.
+__global__ void __launch_bounds__(256) fp_adjoint_lds_kernel(VolG v, TiltG g, const float* __restrict__ resid,
+                                                             size_t pitch, Roi roi, float scale,
+                                                             float* __restrict__ grad, TubePlan tp) {
+    __shared__ float chunk[kChunkFloats];
+    __shared__ int zr[2];
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int sub = lane & 3, pix = lane >> 2;
+    const int xcd = (int)(blockIdx.x & 7u), slot = (int)(blockIdx.x >> 3);
+    const int grp = (slot / (kXcdGroup * kXcdGroup)) * 8 + xcd, mem = slot % (kXcdGroup * kXcdGroup);
+    const int gtx = (grp % tp.groups_x) * kXcdGroup + mem % kXcdGroup;
+    const int gty = (grp / tp.groups_x) * kXcdGroup + mem / kXcdGroup;
+    if (gtx >= tp.tiles_x || gty >= tp.tiles_y) return;   // whole workgroup
+    const int tx0 = roi.x0 + gtx * 8, ty0 = roi.y0 + gty * 8;
+    const int x = tx0 + (wave & 1) * 4 + (pix & 3);
+    const int y = ty0 + (wave >> 1) * 4 + (pix >> 2);
+    const double step = (double)v.vs[0] * 0.15;
+    double r = 0.0;
+    SubraySamples s;
+    s.count = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) s.q0[c] = s.dq[c] = 0.0f;
+    if (x < roi.x1 && y < roi.y1) {
+        // proj = (float)(0.25 * step * sum): the same factor per sample
+        r = (double)resid[(size_t)y * pitch + (size_t)x] * (double)scale * (step * 0.25);
+        const double px = (double)x + ((sub & 2) ? 0.75 : 0.25);
+        const double py = (double)y + ((sub & 1) ? 0.75 : 0.25);
+        if (r == 0.0 || !subray_samples(v, g, px, py, step, s)) s.count = 0;
+    }
+    const TubeDir td = tube_dir(v, g, step);
+    const TubeBox tb = tube_box(v, g, tp, (double)tx0 + 0.25, (double)tx0 + 7.75, (double)ty0 + 0.25,
+                                (double)ty0 + 7.75, 0.0, 0.0);
+    tube_scatter(v, grad, chunk, zr, tp, tb, td, s.q0, s.count, [&](float wgt[8], const int cell[3], int k, int mi) {
+        double W[8];
+        cell_weights(cell_moments(s.q0, td.dq, cell, k, mi), W);
+#pragma unroll
+        for (int i = 0; i < 8; i++) wgt[i] = (float)(r * W[i]);
     });
 }
 
@@ -194,7 +237,19 @@ artia_status artia_fp_adjoint(artia_ctx* ctx, const artia_vol_geom* vg, const ar
     if (!resid || !vol_grad) return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_fp_adjoint: null pointer");
     const Roi r = clip_roi(roi, width, height);
     Timing t(ctx, ms);
-    if (r.x1 > r.x0 && r.y1 > r.y0)
+    const VolG vd = to_device(*vg);
+    const TiltG gd = to_device(*tg);
+    TubePlan tp;
+    if (r.x1 > r.x0 && r.y1 > r.y0 && ctx->march_lds && tube_plan(vd, gd, tp)) {
+        const dim3 grid = grid_of(r);
+        tp.tiles_x = (int)grid.x;
+        tp.tiles_y = (int)grid.y;
+        tp.groups_x = (tp.tiles_x + kXcdGroup - 1) / kXcdGroup;
+        const int groups = tp.groups_x * ((tp.tiles_y + kXcdGroup - 1) / kXcdGroup);
+        const dim3 grid1((unsigned)(((groups + 7) / 8) * 8 * kXcdGroup * kXcdGroup));
+        hipLaunchKernelGGL(fp_adjoint_lds_kernel, grid1, dim3(256), 0, ctx->stream, vd, gd, resid, pitch, r, scale,
+                           vol_grad, tp);
+    } else if (r.x1 > r.x0 && r.y1 > r.y0)
     {
         const bool big = (double)vg->nx * (double)vg->ny * (double)vg->nz >= 1073741824.0;
         if (big)

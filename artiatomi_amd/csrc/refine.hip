@@ -212,6 +212,8 @@ __global__ void __launch_bounds__(256, 5) pair_march_kernel(PairArgs a) {
 // vol_grads += scale * P_s^T in: the transpose of the per-cell closed form, one set of eight
 // atomics per cell a sub-ray crosses (as fp_adjoint_kernel).
 __global__ void __launch_bounds__(256) pair_adjoint_kernel(PairArgs a, float scale, float* __restrict__ grads) {
+    __shared__ float chunk[kChunkFloats];
+    __shared__ int zr[2];
     const int slot = slot_of_block(a.per_xcd);
     if (slot >= a.npairs) return;
     const int p = a.order[slot];
@@ -230,25 +232,40 @@ __global__ void __launch_bounds__(256) pair_adjoint_kernel(PairArgs a, float sca
         const double sx = a.shifts ? a.shifts[2 * p] : 0.0, sy = a.shifts ? a.shifts[2 * p + 1] : 0.0;
         const double step = (double)v.vs[0] * 0.15;
         const int i = (t % ntx) * 8 + tx, j = (t / ntx) * 8 + ty;
-        if (i >= w || j >= h) continue;
-        // proj = (float)(0.25 * step * sum): the same factor per sample
-        const double r = (double)a.in[io + (long long)j * w + i] * (double)scale * (step * 0.25);
-        if (r == 0.0) continue;
+        double r = 0.0;
         SubraySamples s;
-        const double px = (double)(x0 + i) + ((sub & 2) ? 0.75 : 0.25);
-        const double py = (double)(y0 + j) + ((sub & 1) ? 0.75 : 0.25);
-        if (!subray_samples(v, g, px, py, step, s, sx, sy)) continue;
-        for_each_cell(s.q0, s.dq, s.count, [&](const int cell[3], int k, int mi) {
-            uint32_t o[8];
-            cell_corners_bytes(v, cell, o);
+        s.count = 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) s.q0[c] = s.dq[c] = 0.0f;
+        if (i < w && j < h) {
+            // proj = (float)(0.25 * step * sum): the same factor per sample
+            r = (double)a.in[io + (long long)j * w + i] * (double)scale * (step * 0.25);
+            const double px = (double)(x0 + i) + ((sub & 2) ? 0.75 : 0.25);
+            const double py = (double)(y0 + j) + ((sub & 1) ? 0.75 : 0.25);
+            if (r == 0.0 || !subray_samples(v, g, px, py, step, s, sx, sy)) s.count = 0;
+        }
+        auto weights = [&](float wgt[8], const int cell[3], int k, int mi) {
             double W[8];
             cell_weights(cell_moments(s.q0, s.dq, cell, k, mi), W);
 #pragma unroll
-            for (int c = 0; c < 8; c++) {
-                const float add = (float)(r * W[c]);
-                if (add != 0.0f) unsafeAtomicAdd(gv + (o[c] >> 2), add);
-            }
-        });
+            for (int c = 0; c < 8; c++) wgt[c] = (float)(r * W[c]);
+        };
+        if (P.staged) {   // uniform: per-slab sums in LDS, one atomic per box voxel
+            const TubeDir td = tube_dir(v, g, step);
+            const double pi0 = (double)(x0 + (t % ntx) * 8), pj0 = (double)(y0 + (t / ntx) * 8);
+            const TubeBox tb = tube_box(v, g, P.tp, pi0 + 0.25, pi0 + 7.75, pj0 + 0.25, pj0 + 7.75, sx, sy);
+            tube_scatter(v, gv, chunk, zr, P.tp, tb, td, s.q0, s.count, weights);
+        } else if (s.count > 0) {
+            for_each_cell(s.q0, s.dq, s.count, [&](const int cell[3], int k, int mi) {
+                uint32_t o[8];
+                cell_corners_bytes(v, cell, o);
+                float wgt[8];
+                weights(wgt, cell, k, mi);
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+                    if (wgt[c] != 0.0f) unsafeAtomicAdd(gv + (o[c] >> 2), wgt[c]);
+            });
+        }
     }
 }
 

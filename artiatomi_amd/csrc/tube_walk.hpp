@@ -139,14 +139,27 @@ __device__ __forceinline__ void chunk_corners(const VolG& v, const float* __rest
     }
 }
 
-// Walks every lane's samples q0 + k*dq (k < count) cell by cell, calling
-// f(val[8], cell, k, m) for every cell holding samples k..k+m-1 with its eight corner values.  Must
-// be called by all 256 threads of the block (it synchronises); chunk holds kChunkFloats floats and
+// Global index of a padded box entry (clamp-addressed: -1 and n map to the edge voxels).
+__device__ __forceinline__ size_t box_voxel(const VolG& v, int X0, int Y0, int Z0, int CX, int CY, int e) {
+    const int row = e / CX, xx = e - row * CX;
+    const int zz = row / CY, yy = row - zz * CY;
+    const size_t gx = (size_t)clampi(X0 + xx, 0, v.nx - 1), gy = (size_t)clampi(Y0 + yy, 0, v.ny - 1);
+    const size_t gz = (size_t)clampi(Z0 + zz, 0, v.nz - 1);
+    return (gz * (size_t)v.ny + gy) * (size_t)v.nx + gx;
+}
+
+// Walks every lane's samples q0 + k*dq (k < count) cell by cell, slab by slab through LDS.  Must be
+// called by all 256 threads of the block (it synchronises); chunk holds kChunkFloats floats and
 // zr two ints of shared memory.
-template <class F>
-__device__ __forceinline__ void tube_walk(const VolG& v, const float* __restrict__ vol, float* __restrict__ chunk,
-                                          int* zr, const TubePlan& tp, const TubeBox& tb, const TubeDir& td,
-                                          const float q0[3], int count, F&& f) {
+//   gather (kScatter false): f(const float val[8], cell, k, m) for every cell holding samples
+//     k..k+m-1, with the cell's eight corner values read from vol (staged in LDS);
+//   scatter (kScatter true): f(float wgt[8], cell, k, m) returns eight corner weights, which are
+//     added to the slab's box in LDS (shared-memory atomics) and then to grad with one global
+//     atomic per box voxel, instead of eight global atomics per cell.
+template <bool kScatter, class F>
+__device__ __forceinline__ void tube_slabs(const VolG& v, const float* __restrict__ vol, float* __restrict__ grad,
+                                           float* __restrict__ chunk, int* zr, const TubePlan& tp, const TubeBox& tb,
+                                           const TubeDir& td, const float q0[3], int count, F&& f) {
     // cell layers the block's sub-rays cross
     __syncthreads();   // a previous walk's readers are done with zr and chunk
     if (threadIdx.x == 0) {
@@ -163,6 +176,14 @@ __device__ __forceinline__ void tube_walk(const VolG& v, const float* __restrict
     __syncthreads();
     const int zlo = max(zr[0], -1), zhi = min(zr[1], v.nz - 1);
 
+    // corner weights into the box (LDS) or, outside it, straight to grad
+    auto scatter_global = [&](const int cell[3], const float wgt[8]) {
+        size_t o[8];
+        cell_corners<true>(v, cell, o);
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            if (wgt[i] != 0.0f) unsafeAtomicAdd(grad + o[i], wgt[i]);
+    };
     CellWalker w;
     w.init_dir(q0, td.sgn, td.inv, count);
     const int nslab = zlo <= zhi ? (zhi - zlo) / kSlabCells + 1 : 0;
@@ -180,15 +201,14 @@ __device__ __forceinline__ void tube_walk(const VolG& v, const float* __restrict
         const int Z0 = max(zs, -1), Z1 = min(ze + 1, v.nz);
         const int CX = wave_uniform(X1 - X0 + 1), CY = wave_uniform(Y1 - Y0 + 1), CZ = Z1 - Z0 + 1;
         const bool staged = CX > 1 && CY > 1 && CZ > 1 && CX * CY * CZ <= kChunkFloats;
-        __syncthreads();   // the previous slab's reads are done
+        const int n = CX * CY * CZ;
+        __syncthreads();   // the previous slab's reads (or flush) are done
         if (staged) {
-            const int n = CX * CY * CZ;
             for (int e = (int)threadIdx.x; e < n; e += 256) {
-                const int row = e / CX, xx = e - row * CX;
-                const int zz = row / CY, yy = row - zz * CY;
-                const size_t gx = (size_t)clampi(X0 + xx, 0, v.nx - 1), gy = (size_t)clampi(Y0 + yy, 0, v.ny - 1);
-                const size_t gz = (size_t)clampi(Z0 + zz, 0, v.nz - 1);
-                chunk[e] = vol[(gz * (size_t)v.ny + gy) * (size_t)v.nx + gx];
+                if (kScatter)
+                    chunk[e] = 0.0f;
+                else
+                    chunk[e] = vol[box_voxel(v, X0, Y0, Z0, CX, CY, e)];
             }
         }
         __syncthreads();
@@ -209,20 +229,22 @@ __device__ __forceinline__ void tube_walk(const VolG& v, const float* __restrict
         if (fast) {
             const int CXY = CX * CY;
             const int st0 = td.sgn[0], st1 = td.sgn[1] * CX, st2 = td.sgn[2] * CXY;
+            const int off[8] = {0, 1, CX, CX + 1, CXY, CXY + 1, CXY + CX, CXY + CX + 1};
             int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
             while (w.active() && (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
                 w.step(
                     [&](const int cell[3], int k, int mi) {
                         float val[8];
-                        val[0] = chunk[o];
-                        val[1] = chunk[o + 1];
-                        val[2] = chunk[o + CX];
-                        val[3] = chunk[o + CX + 1];
-                        val[4] = chunk[o + CXY];
-                        val[5] = chunk[o + CXY + 1];
-                        val[6] = chunk[o + CXY + CX];
-                        val[7] = chunk[o + CXY + CX + 1];
-                        f(val, cell, k, mi);
+                        if (kScatter) {
+                            f(val, cell, k, mi);
+#pragma unroll
+                            for (int i = 0; i < 8; i++)
+                                if (val[i] != 0.0f) atomicAdd(chunk + o + off[i], val[i]);
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < 8; i++) val[i] = chunk[o + off[i]];
+                            f(val, cell, k, mi);
+                        }
                     },
                     [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
             }
@@ -230,23 +252,66 @@ __device__ __forceinline__ void tube_walk(const VolG& v, const float* __restrict
             while (w.active() && (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
                 w.step([&](const int cell[3], int k, int mi) {
                     float val[8];
-                    chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
-                    f(val, cell, k, mi);
+                    if (kScatter) {
+                        f(val, cell, k, mi);
+                        const int cx = cell[0] - X0, cy = cell[1] - Y0, cz = cell[2] - Z0;
+                        if (staged && (unsigned)cx < (unsigned)(CX - 1) && (unsigned)cy < (unsigned)(CY - 1) &&
+                            (unsigned)cz < (unsigned)(CZ - 1)) {
+                            const int o = (cz * CY + cy) * CX + cx, CXY = CX * CY;
+                            const int off[8] = {0, 1, CX, CX + 1, CXY, CXY + 1, CXY + CX, CXY + CX + 1};
+#pragma unroll
+                            for (int i = 0; i < 8; i++)
+                                if (val[i] != 0.0f) atomicAdd(chunk + o + off[i], val[i]);
+                        } else {
+                            scatter_global(cell, val);
+                        }
+                    } else {
+                        chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
+                        f(val, cell, k, mi);
+                    }
                 });
             }
         }
+        if (kScatter && staged) {
+            __syncthreads();   // the box is complete
+            for (int e = (int)threadIdx.x; e < n; e += 256) {
+                const float a = chunk[e];
+                if (a != 0.0f) unsafeAtomicAdd(grad + box_voxel(v, X0, Y0, Z0, CX, CY, e), a);
+            }
+        }
     }
-    // cells beyond the slabs (not expected): global reads
+    // cells beyond the slabs (not expected): global memory
     while (w.active()) {
         w.step([&](const int cell[3], int k, int mi) {
-            size_t o[8];
-            cell_corners<true>(v, cell, o);
             float val[8];
+            if (kScatter) {
+                f(val, cell, k, mi);
+                scatter_global(cell, val);
+            } else {
+                size_t o[8];
+                cell_corners<true>(v, cell, o);
 #pragma unroll
-            for (int i = 0; i < 8; i++) val[i] = vol[o[i]];
-            f(val, cell, k, mi);
+                for (int i = 0; i < 8; i++) val[i] = vol[o[i]];
+                f(val, cell, k, mi);
+            }
         });
     }
+}
+
+// The gather walk (see tube_slabs).
+template <class F>
+__device__ __forceinline__ void tube_walk(const VolG& v, const float* __restrict__ vol, float* __restrict__ chunk,
+                                          int* zr, const TubePlan& tp, const TubeBox& tb, const TubeDir& td,
+                                          const float q0[3], int count, F&& f) {
+    tube_slabs<false>(v, vol, nullptr, chunk, zr, tp, tb, td, q0, count, f);
+}
+
+// The scatter walk (see tube_slabs): grad += the weights f returns.
+template <class F>
+__device__ __forceinline__ void tube_scatter(const VolG& v, float* __restrict__ grad, float* __restrict__ chunk,
+                                             int* zr, const TubePlan& tp, const TubeBox& tb, const TubeDir& td,
+                                             const float q0[3], int count, F&& f) {
+    tube_slabs<true>(v, nullptr, grad, chunk, zr, tp, tb, td, q0, count, f);
 }
 
 }  // namespace artia
