@@ -423,7 +423,7 @@ artia_status artia_pairs_create(artia_ctx* ctx, const artia_pair_volume* volumes
         PairDev& d = host[(size_t)p];
         d.g = to_device(geoms[p]);
         d.v = to_device(volumes[q.volume].geom);
-        d.staged = tube_plan(d.v, d.g, d.tp) ? 1 : 0;
+        d.staged = ctx->march_lds && tube_plan(d.v, d.g, d.tp) ? 1 : 0;
         d.vol_offset = volumes[q.volume].offset;
         d.img_offset = q.offset;
         d.x0 = q.x0;

@@ -436,6 +436,28 @@ def bench_refine(args) -> int:
         rhs += float((xv.double() * pty.double()).sum())
     dot_rel = abs(lhs - rhs) / abs(lhs)
     log(f"dot test {FX}x{FY}x{FZ}, {T} tilts: rel {dot_rel:.2e}")
+    # adjoint and forward launch times on the whole volume, LDS-staged and direct
+    yv = torch.randn((H, W), generator=gen, device="cuda")
+    pty = torch.zeros_like(xv)
+    adj = {}
+    for staged in (True, False):
+        ctx.set_fp_staging(staged)
+        ctx.fp_adjoint(vg_full, base[0], yv, pty)
+        ta = [ctx.fp_adjoint(vg_full, g, yv, pty, timed=True) for g in base]
+        tf = [ctx.fp_march(vg_full, g, xv, proj, dist, timed=True) for g in base]
+        adj["staged" if staged else "direct"] = {"adjoint_ms": sum(ta) / len(ta), "forward_ms": sum(tf) / len(tf)}
+    ctx.set_fp_staging(True)
+    ms_pair_adj = ps.adjoint(zero, targets, torch.zeros_like(vols), timed=True)
+    ctx.set_fp_staging(False)
+    ps_direct = A.PairSet(ctx, subs, pairs, geoms)
+    ctx.set_fp_staging(True)
+    ms_pair_adj_direct = ps_direct.adjoint(zero, targets, torch.zeros_like(vols), timed=True)
+    ms_pair_lg_direct = ps_direct.loss_grad(vols, zero, targets, normal=True, timed=True)[3]
+    ps_direct.close()
+    log(f"whole-volume adjoint per tilt: {adj['staged']['adjoint_ms']:.2f} ms staged, "
+        f"{adj['direct']['adjoint_ms']:.2f} ms direct; forward {adj['staged']['forward_ms']:.2f} / "
+        f"{adj['direct']['forward_ms']:.2f} ms; pair-set adjoint launch {ms_pair_adj:.1f} ms staged, "
+        f"{ms_pair_adj_direct:.1f} ms direct; pair-set loss+gradient launch direct {ms_pair_lg_direct:.1f} ms")
 
     vox_tilts = K * n ** 3 * T
     bytes_alg = 4.0 * K * n ** 3 + 4.0 * ps.n_pixels + 24.0 * P
@@ -472,6 +494,9 @@ def bench_refine(args) -> int:
                                         "speedup_gradient_evaluation": t_plain_grad / t_eval,
                                         "speedup_cc_search": t_plain_cc / t_cc},
         "dot_test_full_volume_rel": dot_rel,
+        "whole_volume_per_tilt_ms": adj,
+        "pair_adjoint_launch_ms": {"staged": ms_pair_adj, "direct": ms_pair_adj_direct},
+        "pair_loss_grad_launch_direct_ms": ms_pair_lg_direct,
     }
     print(json.dumps(out), flush=True)
     ps.close()

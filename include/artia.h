@@ -140,8 +140,9 @@ artia_status artia_set_numerics(artia_ctx* ctx, int numerics);
 /* Exact forward projection (march and slicer) and its adjoint: enable != 0 (the default) walks
  * each 8x8-pixel tile's rays through LDS-staged slabs of the volume where the geometry allows it,
  * 0 always reads (or, for the adjoint, atomically adds to) the volume directly.  Projections are
- * bit-identical either way; adjoint sums differ only in the order of their float atomics.  No
- * reference counterpart (a tuning choice). */
+ * bit-identical either way; adjoint sums differ only in the order of their float atomics.  Pair
+ * sets take the setting in force when they are created.  No reference counterpart (a tuning
+ * choice). */
 artia_status artia_set_fp_staging(artia_ctx* ctx, int enable);
 artia_status artia_sync(artia_ctx* ctx);
 const char* artia_last_error(artia_ctx* ctx);
