@@ -96,6 +96,11 @@ void launch_crop_border(artia_ctx* ctx, int W, int H, size_t pitch, float* img, 
                         const float dim[2], const int p[8]);
 void launch_bp(artia_ctx* ctx, const VolG& vg, const TiltG& tg, float* vol, const float* img, int W, int H,
                size_t pitch, float lambda, int os, float distMin, float distMax, int filter);
+// exact-numerics BP of one tilt's CTF slabs in one launch: slab k's image at stack + k * stride, slab k
+// covering distances [bounds[k], bounds[k+1]) (the last [bounds[n-1], last_end)); bounds on the device
+void launch_bp_stack(artia_ctx* ctx, const VolG& vg, const TiltG& tg, float* vol, const float* stack, size_t stride,
+                     int W, int H, size_t pitch, float lambda, int os, int filter, const float* bounds, int nslabs,
+                     float last_end);
 void launch_fill(artia_ctx* ctx, float* p, size_t n, float value);
 void launch_slicer(artia_ctx* ctx, const VolG& vg, const TiltG& tg, const float* vol, int W, int H, size_t pitch,
                    float* proj, float tmin, float tmax);

@@ -213,6 +213,57 @@ __global__ void __launch_bounds__(256) bp_exact_kernel(VolG v, TiltG g, BPExact 
     if (any) *reinterpret_cast<float4*>(vp) = make_float4(out[0], out[1], out[2], out[3]);
 }
 
+// The CTF backprojection of one tilt in one launch: every voxel takes its slab's filtered image
+// from the stack (slab k at stack + k * stride) instead of one gated launch per slab
+// (Reconstructor.cpp:920-969).  A voxel lies in exactly one slab, so it receives the same single
+// update, in the same arithmetic, as in the per-slab launches.
+__global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact e, float* __restrict__ vol,
+                                                       const float* __restrict__ stack, size_t stride, int W, int H,
+                                                       size_t pitch, float lambda, int os, int filter, SlabArgs sl) {
+    const int tid = (int)threadIdx.x;
+    const int xq = (int)blockIdx.x * 16 + (tid & 15);
+    const int y = (int)blockIdx.y * 4 + ((tid >> 4) & 3);
+    const int z = (int)blockIdx.z * 4 + (tid >> 6);
+    if (!(xq < v.nx / 4 && y < v.ny && z < v.nz)) return;
+    const unsigned x = (unsigned)xq * 4u;
+    const int slab = find_slab(sl, gate_distance(e, v, g, x, (unsigned)y, (unsigned)z));
+    if (slab < 0) return;
+    const float* __restrict__ img = stack + (size_t)slab * stride;
+    int px0, px1, py0, py1;
+    if (e.affine) {
+        if (!footprint_affine(e, x, (unsigned)y, (unsigned)z, W, H, -1.0, INFINITY, px0, px1, py0, py1)) return;
+    } else {
+        if (!footprint_literal(v, g, x, (unsigned)y, (unsigned)z, W, H, -1.0, INFINITY, px0, px1, py0, py1)) return;
+    }
+    const int nI = (px1 - px0) * os, nJ = (py1 - py0) * os;
+    if (nI <= 0 || nJ <= 0) return;
+    float* vp = vol + ((size_t)z * (size_t)v.ny + (size_t)y) * (size_t)v.nx + x;
+    const float4 vb = *reinterpret_cast<const float4*>(vp);
+    float out[4] = {vb.x, vb.y, vb.z, vb.w};
+    const float osf = (float)os, delta = 1.0f / osf;
+    const float du0 = (float)(e.pu0 + (double)x * e.pui + (double)y * e.puj + (double)z * e.puk - (double)px0);
+    const float dv0 = (float)(e.pv0 + (double)x * e.pvi + (double)y * e.pvj + (double)z * e.pvk - (double)py0);
+    const float dui = (float)e.pui, dvi = (float)e.pvi;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float du = __builtin_fmaf((float)k, dui, du0);
+        const float dv = __builtin_fmaf((float)k, dvi, dv0);
+        const int i0 = max(0, (int)ceilf((du + e.uMin) * osf - 0.5f));
+        const int i1 = min(nI - 1, (int)floorf((du + e.uMax) * osf - 0.5f));
+        const int j0 = max(0, (int)ceilf((dv + e.vMin) * osf - 0.5f));
+        const int j1 = min(nJ - 1, (int)floorf((dv + e.vMax) * osf - 0.5f));
+        float num = 0.0f, den = 0.0f;
+        voxel_sums_global(e, g, img, W, H, pitch, filter, px0, py0, os, delta, du, dv, i0, i1, j0, j1, num, den);
+        den = fmaxf(0.0f, den);
+        if (den != 0.0f) {
+            out[k] += (lambda * num) / den;
+            any = true;
+        }
+    }
+    if (any) *reinterpret_cast<float4*>(vp) = make_float4(out[0], out[1], out[2], out[3]);
+}
+
 template <int kOS>
 void launch_tile(artia_ctx* ctx, dim3 grid, const VolG& v, const TiltG& g, const BPExact& e, float* vol,
                  const float* img, int W, int H, size_t pitch, float lambda, int os, double dmin, double dmax,
@@ -229,6 +280,16 @@ void launch_tile(artia_ctx* ctx, dim3 grid, const VolG& v, const TiltG& g, const
 }
 
 }  // namespace
+
+void launch_bp_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, float* vol, const float* stack, size_t stride,
+                     int W, int H, size_t pitch, float lambda, int os, int filter, const float* bounds, int nslabs,
+                     float last_end) {
+    const BPExact e = prepare_exact(v, g);
+    const int nq = v.nx / 4;
+    const dim3 grid((unsigned)((nq + 15) / 16), (unsigned)((v.ny + 3) / 4), (unsigned)((v.nz + 3) / 4));
+    hipLaunchKernelGGL(bp_stack_kernel, grid, dim3(256), 0, ctx->stream, v, g, e, vol, stack, stride, W, H, pitch,
+                       lambda, os, filter, SlabArgs{bounds, nslabs, last_end});
+}
 
 void launch_bp(artia_ctx* ctx, const VolG& v, const TiltG& g, float* vol, const float* img, int W, int H, size_t pitch,
                float lambda, int os, float distMin, float distMax, int filter) {

@@ -221,6 +221,42 @@ __device__ __forceinline__ bool footprint_affine(const BPExact& e, unsigned x, u
 
 constexpr int kTileFloats = 4096;   // 16 KiB of LDS for the block's projection tile
 
+// CTF slab gate of one tilt: bounds[k] = start of slab k (the reference's float loop values,
+// Reconstructor.cpp:920-969), slab k covers [bounds[k], bounds[k+1]) and the last one
+// [bounds[K-1], last_end).
+struct SlabArgs {
+    const float* bounds;
+    int count;
+    float last_end;
+};
+
+// The slab holding distance t (the gate's distance of a voxel's first macro-cell corner), or -1.
+__device__ __forceinline__ int find_slab(const SlabArgs& sl, double t) {
+    if (sl.count <= 0 || t < (double)sl.bounds[0]) return -1;
+    int lo = 0, hi = sl.count - 1;   // largest k with bounds[k] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((double)sl.bounds[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    if (lo == sl.count - 1 && !(t < (double)sl.last_end)) return -1;
+    return lo;
+}
+
+// The gate's distance of voxel (x, y, z): |n . c - n . d| at its first macro-cell corner, as
+// footprint_affine / footprint_literal compute it.
+__device__ __forceinline__ double gate_distance(const BPExact& e, const VolG& v, const TiltG& g, unsigned x, unsigned y,
+                                                unsigned z) {
+    if (e.affine) return fabs(fma(e.gn[0], (double)x, fma(e.gn[1], (double)y, fma(e.gn[2], (double)z, e.g0))));
+    const double n[3] = {g.n[0], g.n[1], g.n[2]};
+    const double d[3] = {g.d[0], g.d[1], g.d[2]};
+    const double c[3] = {(double)v.bbMin[0] + (double)x * (double)v.vs[0],
+                         (double)v.bbMin[1] + (double)y * (double)v.vs[1],
+                         (double)v.bbMin[2] + (double)z * (double)v.vs[2]};
+    const double nd = ((-n[0]) * d[0] - n[1] * d[1]) - n[2] * d[2];
+    return fabs(((n[0] * c[0] + n[1] * c[1]) + n[2] * c[2]) + nd);
+}
+
 // Chord of the sub-ray at offset (a, b) through one voxel: entry/exit of the three slabs are linear
 // in the offset; axes parallel to the rays (n_c == 0) contribute an inside test instead of a slab.
 // kPM is the parallel-axis mask when known at compile time: 0 (none) or 2 (y only, every tilt about

@@ -4,6 +4,9 @@
 // Reconstructor.cpp:1395).  All launches go to the context stream back to back.
 #include <cmath>
 
+#include <algorithm>
+#include <vector>
+
 #include "artia_internal.hpp"
 
 using namespace artia;
@@ -125,10 +128,39 @@ artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const 
     }
     const int S = sp->sirt_count;
     const size_t npix = (size_t)width * (size_t)height;
-    artia_status st = ensure_workspace(ctx, npix * (size_t)(1 + S));
+    // the backprojection slabs of every tilt, as the reference's float loop steps through them
+    std::vector<float> bounds;
+    std::vector<size_t> first((size_t)ntilts);
+    std::vector<int> nslab((size_t)ntilts);
+    std::vector<float> last_end((size_t)ntilts);
+    int max_slabs = 0;
+    for (int i = 0; i < ntilts; i++) {
+        first[(size_t)i] = bounds.size();
+        float ray = slab_range[2 * i];
+        for (; ray < slab_range[2 * i + 1]; ray += step) bounds.push_back(ray);
+        nslab[(size_t)i] = (int)(bounds.size() - first[(size_t)i]);
+        last_end[(size_t)i] = ray;
+        max_slabs = std::max(max_slabs, nslab[(size_t)i]);
+    }
+    // one launch per tilt over a stack of its filtered slab images, while the stack fits 32 GiB
+    const bool stacked = ctx->numerics == ARTIA_NUMERICS_EXACT && max_slabs > 0 &&
+                         (double)max_slabs * (double)npix * 4.0 <= 34359738368.0;
+    const size_t stack_floats = stacked ? (size_t)max_slabs * npix : 0;
+    artia_status st = ensure_workspace(ctx, npix * (size_t)(1 + S) + stack_floats + (stacked ? bounds.size() : 0));
     if (st != ARTIA_OK) return st;
     float* dist = ctx->ws;
     float* residual = ctx->ws + npix;
+    float* stack = ctx->ws + npix * (size_t)(1 + S);
+    float* d_bounds = stack + stack_floats;
+    if (stacked) {
+        // earlier work on the stream may still read the workspace
+        st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        if (st == ARTIA_OK)
+            st = check_hip(ctx, hipMemcpy(d_bounds, bounds.data(), bounds.size() * sizeof(float),
+                                          hipMemcpyHostToDevice),
+                           "hipMemcpy(slab bounds)");
+        if (st != ARTIA_OK) return st;
+    }
     const VolG v = to_device(*vg);
     const size_t pitch = (size_t)width;
     const float lam = sp->lambda / (float)S;
@@ -183,6 +215,27 @@ artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const 
                 const TiltG g = to_device(tilts[idx]);
                 float* res = residual + (size_t)i * npix;
                 const int* hp = hitpoints + 8 * (size_t)idx;
+                if (stacked) {
+                    // every slab's filtered image into the stack, then one launch
+                    const size_t f = first[(size_t)idx];
+                    for (int k = 0; k < nslab[(size_t)idx]; k++) {
+                        float dmin, dmax;
+                        artia_geom_defocus_minmax(bounds[f + (size_t)k], &defocus[idx], ctf, &dmin, &dmax);
+                        int pr = prof_start(ctx, ARTIA_PROF_FFT);
+                        st = ctf_slab_filter(ctx, res, width, height, pitch, stack + (size_t)k * npix, pitch, 0, hp,
+                                             sp->cut_length, sp->dim_length, *ctf, dmin, dmax,
+                                             defocus[idx].astig_angle_deg, 0);
+                        prof_stop(ctx, pr);
+                        if (st != ARTIA_OK) return st;
+                    }
+                    if (nslab[(size_t)idx] > 0) {
+                        const int pr = prof_start(ctx, ARTIA_PROF_BP);
+                        launch_bp_stack(ctx, v, g, vol, stack, npix, width, height, pitch, lam, sp->oversampling,
+                                        ARTIA_FILTER_LINEAR, d_bounds + f, nslab[(size_t)idx], last_end[(size_t)idx]);
+                        prof_stop(ctx, pr);
+                    }
+                    continue;
+                }
                 for (float ray = slab_range[2 * idx]; ray < slab_range[2 * idx + 1]; ray += step) {
                     float dmin, dmax;
                     artia_geom_defocus_minmax(ray, &defocus[idx], ctf, &dmin, &dmax);

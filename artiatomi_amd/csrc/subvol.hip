@@ -46,28 +46,8 @@ struct Item {
     int qx, ty, tz, pad;
 };
 
-// CTF slab gate of one tilt: bounds[k] = start of slab k (the reference's float loop values),
-// slab k covers [bounds[k], bounds[k+1]) and the last one [bounds[K-1], last_end).
-struct SlabArgs {
-    const float* bounds;
-    int count;
-    float last_end;
-};
-
 __global__ void reset_counter_kernel(unsigned* counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
-}
-
-__device__ __forceinline__ int find_slab(const SlabArgs& sl, double t) {
-    if (sl.count <= 0 || t < (double)sl.bounds[0]) return -1;
-    int lo = 0, hi = sl.count - 1;   // largest k with bounds[k] <= t
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if ((double)sl.bounds[mid] <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    if (lo == sl.count - 1 && !(t < (double)sl.last_end)) return -1;
-    return lo;
 }
 
 // Update (lambda = 1) of one quad for one tilt; returns false when the quad is gated out.
@@ -81,19 +61,7 @@ __device__ __forceinline__ void quad_update(const PairDev& p, const float* __res
     const float* src = img;
     if (sl.count > 0) {
         // first macro-cell corner distance, as the gate of the per-slab launches computes it
-        double t;
-        if (e.affine) {
-            t = fabs(fma(e.gn[0], (double)x, fma(e.gn[1], (double)y, fma(e.gn[2], (double)z, e.g0))));
-        } else {
-            const double n[3] = {p.g.n[0], p.g.n[1], p.g.n[2]};
-            const double d[3] = {p.g.d[0], p.g.d[1], p.g.d[2]};
-            const double c[3] = {(double)p.v.bbMin[0] + (double)x * (double)p.v.vs[0],
-                                 (double)p.v.bbMin[1] + (double)y * (double)p.v.vs[1],
-                                 (double)p.v.bbMin[2] + (double)z * (double)p.v.vs[2]};
-            const double nd = ((-n[0]) * d[0] - n[1] * d[1]) - n[2] * d[2];
-            t = fabs(((n[0] * c[0] + n[1] * c[1]) + n[2] * c[2]) + nd);
-        }
-        const int k = find_slab(sl, t);
+        const int k = find_slab(sl, gate_distance(e, p.v, p.g, x, y, z));
         if (k < 0) return;
         src = img + (size_t)k * slabStride;
     }

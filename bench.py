@@ -36,6 +36,18 @@ WORKLOADS = {
 }
 
 
+# SART with 3D-CTF slab correction: 2048 x 2048 x 512 ellipsoid phantom, 61 tilts of 2048^2 at
+# 0.25 nm/px, 300 kV, Cs 2.7 mm, amplitude contrast 0.07, defocus U[3, 5] um per tilt with 0.2 um
+# astigmatism at a random angle, slabs of 16 z-slices (4 nm), oversampling 2.  Projections come from
+# the forward projector with Poisson noise at 60 e/A^2 over the series (the timing does not depend
+# on the data).
+CTF_WORKLOADS = {
+    "sart_ctf_2048x2048x512_61tilts": dict(dims=(2048, 2048, 512), W=2048, H=2048, step=2.0, n_ell=400, os=2,
+                                           seed=3, pixelsize_nm=0.25, slab_nm=4.0, amp_contrast=0.07,
+                                           dose_e_per_A2=60.0),
+}
+
+
 # Sub-volume WBP workloads (EmSARTSubVolumes): cubes of 32/64/96/128 voxels in ratio 4:3:2:1 with
 # centres uniform in the field, per-(cube, tilt) extra shifts U[-4, 4] px, seeded random WBP-filtered
 # projections; the batched plan against the per-call plain schedule on the same inputs.
@@ -170,6 +182,95 @@ def cpu_baseline(vg, acq, vol_dev, projs_dev, w):
                    f"({t_bp * zs / nz:.2f} s), compare+cropBorder full image ({t_cmp:.3f} s); extrapolated "
                    f"x{H // rows} rows, x{nz // zs} slices, x{len(acq.geoms)} tilts -> {t_iter:.0f} s/iteration"),
     }
+
+
+def bench_ctf(args) -> int:
+    import numpy as np
+    import torch
+
+    import artiatomi_amd as A
+    from artiatomi_amd import phantom
+
+    w = CTF_WORKLOADS[args.workload]
+    nx, ny, nz = w["dims"]
+    W, H = w["W"], w["H"]
+    tilts = phantom.tilt_series(-60.0, 60.0, w["step"])
+    T = len(tilts)
+    rng = np.random.default_rng(w["seed"])
+    ctx = A.ArtiaContext(0)
+    vg = A.volume_geometry(nx, ny, nz)
+    acq = A.Acquisition.build(vg, W, H, tilts)
+    t0 = time.time()
+    shapes = phantom.make_shapes(w["dims"], w["n_ell"], w["seed"])
+    truth = phantom.rasterize(w["dims"], shapes, device="cuda")
+    projs = torch.zeros((T, H, W), device="cuda")
+    dist = torch.zeros((H, W), device="cuda")
+    for i, g in enumerate(acq.geoms):
+        ctx.fp_march(vg, g, truth, projs[i], dist)
+    del truth
+    # Poisson counts at the dose per pixel and tilt, attenuated by the projection, back to -log
+    dose = w["dose_e_per_A2"] / T * (10.0 * w["pixelsize_nm"]) ** 2
+    gen = torch.Generator(device="cuda").manual_seed(w["seed"])
+    scale = 1.0 / max(float(projs.mean()), 1e-30)
+    counts = torch.poisson(dose * torch.exp(-projs * scale), generator=gen)
+    projs = -torch.log(counts.clamp_min(0.5) / dose) / scale
+    del counts
+    torch.cuda.synchronize()
+    dz = rng.uniform(3000.0, 5000.0, T)
+    ang = rng.uniform(-90.0, 90.0, T)
+    defocus = [A.TiltDefocus(float(d), float(d) + 200.0, float(a), float(t)) for d, a, t in zip(dz, ang, tilts)]
+    ctf = A.ctf_params(pixelsize_nm=w["pixelsize_nm"], slice_thickness_nm=w["slab_nm"],
+                       amp_contrast=w["amp_contrast"])
+    sp = A.sart_params(lambda_=0.3, sirt_count=1, oversampling=w["os"])
+    log(f"workload {args.workload}: phantom + {T} noisy projections in {time.time() - t0:.1f} s")
+    vol = torch.zeros((nz, ny, nx), device="cuda")
+    empty = True
+    for i in range(args.warmup):
+        _, empty = ctx.sart_ctf_run(vg, acq, defocus, projs, vol, sp, ctf, iterations=1, volume_is_empty=empty)
+        torch.cuda.synchronize()
+        log(f"warmup iteration {i + 1}/{args.warmup} done")
+    ctx.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, empty = ctx.sart_ctf_run(vg, acq, defocus, projs, vol, sp, ctf, iterations=1, volume_is_empty=empty)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    log(f"timed {args.steps} iterations in {elapsed:.3f} s; per-kernel totals {prof}")
+    V = nx * ny * nz
+    per_kernel = {name: {"total_ms_per_iteration": ms / args.steps, "launches_per_iteration": n / args.steps,
+                         "share": ms / (elapsed * 1e3)} for name, (ms, n) in prof.items() if n}
+    # per iteration, every tilt reads the volume once across its slab passes (FP) and reads and
+    # writes every voxel once (BP)
+    alg = {"fp": (4.0 * V + 8.0 * W * H) * T, "bp": (8.0 * V + 4.0 * W * H) * T}
+    dom = max(("fp", "bp"), key=lambda k: prof[k][0])
+    achieved = alg[dom] / (prof[dom][0] / args.steps / 1e3) / 1e9
+    out = {
+        "metric": "Gvoxel-updates/s per CTF-SART iteration",
+        "value": V * T * args.steps / elapsed / 1e9,
+        "unit": "Gvoxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantom, forward-projected, Poisson noise at the stated dose)",
+        "config": {"workload": args.workload, "volume": list(w["dims"]), "projection": [W, H], "tilts": T,
+                   "pixelsize_nm": w["pixelsize_nm"], "ctf_slab_nm": w["slab_nm"], "oversampling": w["os"],
+                   "amp_contrast": w["amp_contrast"], "numerics": "exact", "parallelism": "single GPU"},
+        "roofline": {"kernel": "march (slicer passes)" if dom == "fp" else "backProjection (slab passes)",
+                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernels": per_kernel,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
 
 
 def bench_subvol(args) -> int:
@@ -510,7 +611,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sart_1024x1024x256_61tilts",
-                    choices=sorted(WORKLOADS) + sorted(SUBVOL_WORKLOADS) + sorted(REFINE_WORKLOADS))
+                    choices=sorted(WORKLOADS) + sorted(CTF_WORKLOADS) + sorted(SUBVOL_WORKLOADS) +
+                    sorted(REFINE_WORKLOADS))
     ap.add_argument("--numerics", default="exact", choices=["exact", "reference"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp-staging", action="store_true",
@@ -542,6 +644,8 @@ def main() -> int:
         return bench_subvol(args)
     if args.workload in REFINE_WORKLOADS:
         return bench_refine(args)
+    if args.workload in CTF_WORKLOADS:
+        return bench_ctf(args)
 
     import torch
 
