@@ -24,6 +24,21 @@ struct FftPlans {
     artia_status ensure(artia_ctx* ctx, int n);
     void release();
 };
+
+// Batched plans for stacks of N x N slab images: one R2C and one C2R plan per power-of-two batch
+// 1, 2, 4, .., cap sharing one work area, with `cap` squares and half spectra.  cap is the largest
+// power of two up to kFftBatchMax whose squares and spectra fit in 2 GiB.
+constexpr int kFftBatchMax = 32;
+constexpr int kFftBatchPlans = 6;   // log2(kFftBatchMax) + 1
+struct FftBatch {
+    int N = 0, cap = 0;
+    hipfftHandle r2c[kFftBatchPlans] = {}, c2r[kFftBatchPlans] = {};
+    float* square = nullptr;
+    float2* spec = nullptr;
+    void* work = nullptr;
+    artia_status ensure(artia_ctx* ctx, int n);
+    void release();
+};
 }  // namespace artia
 
 struct artia_ctx {
@@ -43,6 +58,7 @@ struct artia_ctx {
     double prof_ms[ARTIA_PROF_COUNT] = {};
     long long prof_launches[ARTIA_PROF_COUNT] = {};
     artia::FftPlans fft;
+    artia::FftBatch fftb;
 };
 
 namespace artia {
@@ -116,6 +132,16 @@ artia_status run_wbp_weight(artia_ctx* ctx, float2* spec, int N, size_t pitch, f
 artia_status ctf_slab_filter(artia_ctx* ctx, float* img, int W, int H, size_t pitch, float* out, size_t out_pitch,
                              int accumulate, const int corners[8], const float cut[2], const float dim[2],
                              const artia_ctf_params& p, float dmin_nm, float dmax_nm, float angle_deg, int forFP);
+// The slab filter over a stack of n slabs, batched (exact numerics), bit-identical to n calls of
+// ctf_slab_filter in slab order:
+//   forFP = 1: slab s reads src + s * src_stride (cropped once); out (one image) += every slab's
+//              cropped result in slab order
+//   forFP = 0: every slab reads src (src_stride 0), cropped s + 1 times as the repeated in-place
+//              crop leaves it; slab s's result is stored at out + s * out_stride; src is not changed
+artia_status ctf_filter_slabs(artia_ctx* ctx, const float* src, size_t src_stride, int n, int W, int H, size_t pitch,
+                              float* out, size_t out_stride, const int corners[8], const float cut[2],
+                              const float dim[2], const artia_ctf_params& p, const float* dmin_nm,
+                              const float* dmax_nm, float angle_deg, int forFP);
 // makeSquare (reflect) -> R2C -> wbpWeighting -> C2R -> out = square / N^2
 artia_status wbp_filter(artia_ctx* ctx, const float* img, int W, int H, size_t pitch, float* out, size_t out_pitch,
                         float psi_deg, int method);

@@ -181,6 +181,7 @@ artia_status artia_destroy(artia_ctx* ctx) {
     else (void)hipDeviceSynchronize();
     if (ctx->ws) (void)hipFree(ctx->ws);
     ctx->fft.release();
+    ctx->fftb.release();
     for (hipEvent_t e : ctx->prof_events) (void)hipEventDestroy(e);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);

@@ -161,6 +161,13 @@ artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const 
                            "hipMemcpy(slab bounds)");
         if (st != ARTIA_OK) return st;
     }
+    // the defocus range of every slab (Reconstructor.cpp:522, 942)
+    std::vector<float> sdmin(bounds.size()), sdmax(bounds.size());
+    for (int i = 0; i < ntilts; i++)
+        for (int k = 0; k < nslab[(size_t)i]; k++) {
+            const size_t j = first[(size_t)i] + (size_t)k;
+            artia_geom_defocus_minmax(bounds[j], &defocus[i], ctf, &sdmin[j], &sdmax[j]);
+        }
     const VolG v = to_device(*vg);
     const size_t pitch = (size_t)width;
     const float lam = sp->lambda / (float)S;
@@ -180,7 +187,28 @@ artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const 
                 launch_fill(ctx, proj, npix, 0.0f);
                 launch_fill(ctx, dist, npix, 0.0f);
                 prof_stop(ctx, pr);
-                if (!empty) {
+                if (!empty && stacked) {
+                    // every slab's slicer image into the stack, then the batched filter sums them in
+                    // slab order
+                    const size_t f = first[(size_t)idx];
+                    const int ns = nslab[(size_t)idx];
+                    pr = prof_start(ctx, ARTIA_PROF_FILL);
+                    launch_fill(ctx, stack, (size_t)ns * npix, 0.0f);
+                    prof_stop(ctx, pr);
+                    pr = prof_start(ctx, ARTIA_PROF_FP);
+                    for (int k = 0; k < ns; k++)
+                        launch_slicer(ctx, v, g, vol, width, height, pitch, stack + (size_t)k * npix, bounds[f + k],
+                                      bounds[f + k] + step);
+                    prof_stop(ctx, pr);
+                    pr = prof_start(ctx, ARTIA_PROF_FFT);
+                    st = ctf_filter_slabs(ctx, stack, npix, ns, width, height, pitch, proj, 0, frame, sp->cut_length,
+                                          sp->dim_length, *ctf, &sdmin[f], &sdmax[f], defocus[idx].astig_angle_deg, 1);
+                    prof_stop(ctx, pr);
+                    if (st != ARTIA_OK) return st;
+                    pr = prof_start(ctx, ARTIA_PROF_FILL);
+                    launch_fill(ctx, dist, npix, 0.0f);
+                    prof_stop(ctx, pr);
+                } else if (!empty) {
                     for (float ray = slab_range[2 * idx]; ray < slab_range[2 * idx + 1]; ray += step) {
                         float dmin, dmax;
                         artia_geom_defocus_minmax(ray, &defocus[idx], ctf, &dmin, &dmax);
@@ -218,18 +246,14 @@ artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const 
                 if (stacked) {
                     // every slab's filtered image into the stack, then one launch
                     const size_t f = first[(size_t)idx];
-                    for (int k = 0; k < nslab[(size_t)idx]; k++) {
-                        float dmin, dmax;
-                        artia_geom_defocus_minmax(bounds[f + (size_t)k], &defocus[idx], ctf, &dmin, &dmax);
-                        int pr = prof_start(ctx, ARTIA_PROF_FFT);
-                        st = ctf_slab_filter(ctx, res, width, height, pitch, stack + (size_t)k * npix, pitch, 0, hp,
-                                             sp->cut_length, sp->dim_length, *ctf, dmin, dmax,
-                                             defocus[idx].astig_angle_deg, 0);
-                        prof_stop(ctx, pr);
-                        if (st != ARTIA_OK) return st;
-                    }
+                    int pr = prof_start(ctx, ARTIA_PROF_FFT);
+                    st = ctf_filter_slabs(ctx, res, 0, nslab[(size_t)idx], width, height, pitch, stack, npix, hp,
+                                          sp->cut_length, sp->dim_length, *ctf, &sdmin[f], &sdmax[f],
+                                          defocus[idx].astig_angle_deg, 0);
+                    prof_stop(ctx, pr);
+                    if (st != ARTIA_OK) return st;
                     if (nslab[(size_t)idx] > 0) {
-                        const int pr = prof_start(ctx, ARTIA_PROF_BP);
+                        pr = prof_start(ctx, ARTIA_PROF_BP);
                         launch_bp_stack(ctx, v, g, vol, stack, npix, width, height, pitch, lam, sp->oversampling,
                                         ARTIA_FILTER_LINEAR, d_bounds + f, nslab[(size_t)idx], last_end[(size_t)idx]);
                         prof_stop(ctx, pr);

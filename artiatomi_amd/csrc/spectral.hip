@@ -6,6 +6,7 @@
 //   WBP filter  : makeSquare (reflect) -> R2C -> wbpWeighting -> C2R -> / N^2 with crop
 // The FFTs are hipFFT single-precision plans cached per size in the context; the spectrum is the
 // (N/2+1) x N half spectrum, rows of N/2+1 complex values.
+#include <algorithm>
 #include <cmath>
 
 #include "device_common.hpp"
@@ -71,10 +72,7 @@ __global__ void __launch_bounds__(256) ctf_ref_kernel(float2* __restrict__ spec,
 
 // Exact numerics: the CTF factor in double (the phase reaches hundreds of radians at Nyquist, where
 // a float phase carries ~1e-5 absolute error), applied to the float spectrum.
-__global__ void __launch_bounds__(256) ctf_exact_kernel(float2* __restrict__ spec, size_t pitch, CtfK k) {
-    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
-    if (x >= k.N / 2 + 1 || y >= k.N) return;
+__device__ __forceinline__ void ctf_exact_at(float2* __restrict__ spec, size_t pitch, int x, int y, const CtfK& k) {
     double xpos = (double)x, ypos = (double)y;
     if (ypos > (double)k.N * 0.5) ypos = ((double)k.N - ypos) * -1.0;
     const double alpha = (xpos == 0) ? (double)(kPiF * 0.5f) : atan2(ypos, xpos);
@@ -109,6 +107,30 @@ __global__ void __launch_bounds__(256) ctf_exact_kernel(float2* __restrict__ spe
     r.x = (float)((double)r.x * f);
     r.y = (float)((double)r.y * f);
     spec[(size_t)y * pitch + x] = r;
+}
+
+__global__ void __launch_bounds__(256) ctf_exact_kernel(float2* __restrict__ spec, size_t pitch, CtfK k) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= k.N / 2 + 1 || y >= k.N) return;
+    ctf_exact_at(spec, pitch, x, y, k);
+}
+
+// The same over a stack of half spectra (stride N * (N/2+1)), slab blockIdx.z with its own defocus.
+struct CtfStack {
+    CtfK k;
+    float def0[kFftBatchMax], def1[kFftBatchMax];
+};
+
+__global__ void __launch_bounds__(256) ctf_exact_stack_kernel(float2* __restrict__ spec, size_t pitch, CtfStack b) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= b.k.N / 2 + 1 || y >= b.k.N) return;
+    const int s = (int)blockIdx.z;
+    CtfK k = b.k;
+    k.def0 = b.def0[s];
+    k.def1 = b.def1[s];
+    ctf_exact_at(spec + (size_t)s * pitch * (size_t)b.k.N, pitch, x, y, k);
 }
 
 // wbpWeighting.cu:48-144 (sin/cos of the angle as IEEE functions rather than __sinf/__cosf)
@@ -176,17 +198,60 @@ __global__ void __launch_bounds__(256) wbp_weight_kernel(float2* __restrict__ sp
     spec[(size_t)y * pitch + x] = r;
 }
 
-// out (+)= crop(x, y) * (square[by + y][bx + x] / N^2)  (nppiDivC + cropBorder + nppiAdd)
+// out (+)= crop(x, y) * (square[by + y][bx + x] / N^2)  (nppiDivC + cropBorder + nppiAdd); over a
+// stack, square blockIdx.z into out + blockIdx.z * out_stride
 __global__ void __launch_bounds__(256) finish_kernel(const float* __restrict__ sq, int N, int bx, int by, int W, int H,
                                                      float* __restrict__ out, size_t pitch, float norm, int accumulate,
-                                                     int crop, CropParams cp) {
+                                                     int crop, CropParams cp, size_t out_stride) {
     const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
     if (x >= W || y >= H) return;
+    sq += (size_t)blockIdx.z * (size_t)N * (size_t)N;
+    out += (size_t)blockIdx.z * out_stride;
     float v = sq[(size_t)(by + y) * N + (bx + x)] / norm;
     if (crop) v = crop_weight(x, y, W, H, cp) * v;
     const size_t i = (size_t)y * pitch + x;
     out[i] = accumulate ? out[i] + v : v;
+}
+
+// out += crop(x, y) * (square_s[by + y][bx + x] / N^2) for s = 0 .. ns-1 in turn: the sum, rounded
+// after every slab, of ns accumulating finish_kernel calls
+__global__ void __launch_bounds__(256) finish_sum_kernel(const float* __restrict__ sq, int ns, int N, int bx, int by,
+                                                         int W, int H, float* __restrict__ out, size_t pitch,
+                                                         float norm, CropParams cp) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= W || y >= H) return;
+    const float w = crop_weight(x, y, W, H, cp);
+    const size_t i = (size_t)y * pitch + x;
+    const size_t j = (size_t)(by + y) * N + (bx + x), plane = (size_t)N * (size_t)N;
+    float acc = out[i];
+    for (int s = 0; s < ns; s++) {
+        const float v = sq[(size_t)s * plane + j] / norm;
+        acc = acc + w * v;
+    }
+    out[i] = acc;
+}
+
+// Zero-filled centred squares of a stack of slab images: square_s = makeSquare(cropBorder^r(src_s))
+// with r = rep0 + rep_step * s crops applied in turn (the repeated in-place crop of one image)
+__global__ void __launch_bounds__(256) slab_square_kernel(const float* __restrict__ src, size_t src_stride, int W,
+                                                          int H, size_t pitch, int N, int bx, int by,
+                                                          float* __restrict__ sq, CropParams cp, int rep0,
+                                                          int rep_step) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= N || y >= N) return;
+    const int s = (int)blockIdx.z;
+    const int xi = x - bx, yi = y - by;
+    float val = 0.0f;
+    if (xi >= 0 && xi < W && yi >= 0 && yi < H) {
+        val = src[(size_t)s * src_stride + (size_t)yi * pitch + xi];
+        const float w = crop_weight(xi, yi, W, H, cp);
+        const int reps = rep0 + rep_step * s;
+        for (int r = 0; r < reps; r++) val = w * val;
+    }
+    sq[(size_t)s * (size_t)N * (size_t)N + (size_t)y * N + x] = val;
 }
 
 dim3 grid2d(int w, int h) { return dim3((unsigned)((w + 63) / 64), (unsigned)((h + 3) / 4)); }
@@ -256,6 +321,63 @@ void FftPlans::release() {
     N = 0;
 }
 
+artia_status FftBatch::ensure(artia_ctx* ctx, int n) {
+    if (N == n && cap > 0) return ARTIA_OK;
+    release();
+    const double per_slab = (double)n * n * sizeof(float) + (double)(n / 2 + 1) * n * sizeof(float2);
+    int c = kFftBatchMax;
+    while (c > 1 && (double)c * per_slab > 2147483648.0) c /= 2;
+    size_t work_bytes = 0;
+    int dims[2] = {n, n};
+    for (int i = 0, b = 1; b <= c; i++, b *= 2) {
+        size_t w1 = 0, w2 = 0;
+        if (hipfftCreate(&r2c[i]) != HIPFFT_SUCCESS || hipfftCreate(&c2r[i]) != HIPFFT_SUCCESS ||
+            hipfftSetAutoAllocation(r2c[i], 0) != HIPFFT_SUCCESS ||
+            hipfftSetAutoAllocation(c2r[i], 0) != HIPFFT_SUCCESS ||
+            hipfftMakePlanMany(r2c[i], 2, dims, nullptr, 1, n * n, nullptr, 1, n * (n / 2 + 1), HIPFFT_R2C, b,
+                               &w1) != HIPFFT_SUCCESS ||
+            hipfftMakePlanMany(c2r[i], 2, dims, nullptr, 1, n * (n / 2 + 1), nullptr, 1, n * n, HIPFFT_C2R, b,
+                               &w2) != HIPFFT_SUCCESS) {
+            release();
+            return fail(ctx, ARTIA_ERR_FFT, "hipfftMakePlanMany(%d x %d, batch %d) failed", n, n, b);
+        }
+        work_bytes = std::max(work_bytes, std::max(w1, w2));
+    }
+    artia_status st = check_hip(ctx, hipMalloc(&square, (size_t)c * n * n * sizeof(float)), "hipMalloc(squares)");
+    if (st == ARTIA_OK)
+        st = check_hip(ctx, hipMalloc(&spec, (size_t)c * (n / 2 + 1) * n * sizeof(float2)), "hipMalloc(spectra)");
+    if (st == ARTIA_OK && work_bytes > 0)
+        st = check_hip(ctx, hipMalloc(&work, work_bytes), "hipMalloc(fft work area)");
+    if (st != ARTIA_OK) {
+        release();
+        return st;
+    }
+    for (int i = 0, b = 1; b <= c; i++, b *= 2) {
+        if (hipfftSetWorkArea(r2c[i], work) != HIPFFT_SUCCESS || hipfftSetWorkArea(c2r[i], work) != HIPFFT_SUCCESS) {
+            release();
+            return fail(ctx, ARTIA_ERR_FFT, "hipfftSetWorkArea failed");
+        }
+    }
+    N = n;
+    cap = c;
+    return ARTIA_OK;
+}
+
+void FftBatch::release() {
+    for (int i = 0; i < kFftBatchPlans; i++) {
+        if (r2c[i]) hipfftDestroy(r2c[i]);
+        if (c2r[i]) hipfftDestroy(c2r[i]);
+        r2c[i] = c2r[i] = 0;
+    }
+    if (square) (void)hipFree(square);
+    if (spec) (void)hipFree(spec);
+    if (work) (void)hipFree(work);
+    square = nullptr;
+    spec = nullptr;
+    work = nullptr;
+    N = cap = 0;
+}
+
 artia_status run_ctf(artia_ctx* ctx, float2* spec, int N, size_t pitch, const artia_ctf_params& p, float dmin_nm,
                      float dmax_nm, float angle_deg, int forFP) {
     const CtfK k = make_ctf(N, p, dmin_nm, dmax_nm, angle_deg, forFP);
@@ -292,8 +414,59 @@ artia_status ctf_slab_filter(artia_ctx* ctx, float* img, int W, int H, size_t pi
         return fail(ctx, ARTIA_ERR_FFT, "hipfftExecC2R failed");
     const CropParams cp = corners ? make_crop_params(cut, dim, corners) : CropParams{};
     hipLaunchKernelGGL(finish_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, ctx->fft.square, N, (N - W) / 2,
-                       (N - H) / 2, W, H, out, out_pitch, (float)(N * N), accumulate, corners ? 1 : 0, cp);
+                       (N - H) / 2, W, H, out, out_pitch, (float)(N * N), accumulate, corners ? 1 : 0, cp, (size_t)0);
     return check_hip(ctx, hipGetLastError(), "slab filter");
+}
+
+artia_status ctf_filter_slabs(artia_ctx* ctx, const float* src, size_t src_stride, int n, int W, int H, size_t pitch,
+                              float* out, size_t out_stride, const int corners[8], const float cut[2],
+                              const float dim[2], const artia_ctf_params& p, const float* dmin_nm,
+                              const float* dmax_nm, float angle_deg, int forFP) {
+    if (n <= 0) return ARTIA_OK;
+    const int N = W > H ? W : H;
+    FftBatch& fb = ctx->fftb;
+    artia_status st = fb.ensure(ctx, N);
+    if (st != ARTIA_OK) return st;
+    const CropParams cp = make_crop_params(cut, dim, corners);
+    const int bx = (N - W) / 2, by = (N - H) / 2;
+    const size_t half = (size_t)(N / 2 + 1);
+    for (int k0 = 0; k0 < n;) {
+        // the largest power-of-two batch that fits: at most log2(cap) + 1 chunks past the full ones
+        int b = fb.cap, pi = 0;
+        while (b > n - k0) b /= 2;
+        for (int t = b; t > 1; t /= 2) pi++;
+        hipLaunchKernelGGL(slab_square_kernel, dim3((unsigned)((N + 63) / 64), (unsigned)((N + 3) / 4), (unsigned)b),
+                           dim3(64, 4), 0, ctx->stream, src + (size_t)k0 * src_stride, src_stride, W, H, pitch, N, bx,
+                           by, fb.square, cp, forFP ? 1 : k0 + 1, forFP ? 0 : 1);
+        if (hipfftSetStream(fb.r2c[pi], ctx->stream) != HIPFFT_SUCCESS ||
+            hipfftSetStream(fb.c2r[pi], ctx->stream) != HIPFFT_SUCCESS)
+            return fail(ctx, ARTIA_ERR_FFT, "hipfftSetStream failed");
+        if (hipfftExecR2C(fb.r2c[pi], fb.square, (hipfftComplex*)fb.spec) != HIPFFT_SUCCESS)
+            return fail(ctx, ARTIA_ERR_FFT, "hipfftExecR2C (batch %d) failed", b);
+        CtfStack cs;
+        cs.k = make_ctf(N, p, dmin_nm[k0], dmax_nm[k0], angle_deg, forFP);
+        for (int s = 0; s < b; s++) {
+            const CtfK ks = make_ctf(N, p, dmin_nm[k0 + s], dmax_nm[k0 + s], angle_deg, forFP);
+            cs.def0[s] = ks.def0;
+            cs.def1[s] = ks.def1;
+        }
+        hipLaunchKernelGGL(ctf_exact_stack_kernel, dim3((unsigned)((N / 2 + 1 + 63) / 64), (unsigned)((N + 3) / 4),
+                                                        (unsigned)b),
+                           dim3(64, 4), 0, ctx->stream, fb.spec, half, cs);
+        if (hipfftExecC2R(fb.c2r[pi], (hipfftComplex*)fb.spec, fb.square) != HIPFFT_SUCCESS)
+            return fail(ctx, ARTIA_ERR_FFT, "hipfftExecC2R (batch %d) failed", b);
+        if (forFP)
+            hipLaunchKernelGGL(finish_sum_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, fb.square, b, N, bx, by, W,
+                               H, out, pitch, (float)(N * N), cp);
+        else
+            hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((W + 63) / 64), (unsigned)((H + 3) / 4), (unsigned)b),
+                               dim3(64, 4), 0, ctx->stream, fb.square, N, bx, by, W, H, out + (size_t)k0 * out_stride,
+                               pitch, (float)(N * N), 0, 1, cp, out_stride);
+        st = check_hip(ctx, hipGetLastError(), "stacked slab filter");
+        if (st != ARTIA_OK) return st;
+        k0 += b;
+    }
+    return ARTIA_OK;
 }
 
 artia_status wbp_filter(artia_ctx* ctx, const float* img, int W, int H, size_t pitch, float* out, size_t out_pitch,
@@ -313,7 +486,7 @@ artia_status wbp_filter(artia_ctx* ctx, const float* img, int W, int H, size_t p
         return fail(ctx, ARTIA_ERR_FFT, "hipfftExecC2R failed");
     const CropParams cp{};
     hipLaunchKernelGGL(finish_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, ctx->fft.square, N, (N - W) / 2,
-                       (N - H) / 2, W, H, out, out_pitch, (float)(N * N), 0, 0, cp);
+                       (N - H) / 2, W, H, out, out_pitch, (float)(N * N), 0, 0, cp, (size_t)0);
     return check_hip(ctx, hipGetLastError(), "wbp filter");
 }
 
