@@ -120,6 +120,13 @@ void launch_bp_stack(artia_ctx* ctx, const VolG& vg, const TiltG& tg, float* vol
 void launch_fill(artia_ctx* ctx, float* p, size_t n, float value);
 void launch_slicer(artia_ctx* ctx, const VolG& vg, const TiltG& tg, const float* vol, int W, int H, size_t pitch,
                    float* proj, float tmin, float tmax);
+// The slicer over slabs bounds[k] .. bounds[k] + step (k < nslab) into stack + k * stride; bounds
+// on the host and, as d_bounds, on the device.  With scratch of slicer_stack_scratch_bytes(W, H, n)
+// bytes the slabs go in passes of n (one volume pass each), otherwise one slicer launch per slab.
+size_t slicer_stack_scratch_bytes(int W, int H, int nslab);
+void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vol, int W, int H, size_t pitch,
+                         float* stack, size_t stride, const float* bounds, const float* d_bounds, int nslab,
+                         float step, void* scratch, size_t scratch_bytes);
 void launch_make_square(artia_ctx* ctx, int W, int H, size_t pitch, const float* in, int N, float* out, int bx, int by,
                         int mirrorY, int fillZero);
 

@@ -13,6 +13,7 @@
 //              coordinates affine in k relative to the entry point, and their trilinear values
 //              summed cell by cell in closed form (march_sum); this tracks the double-precision
 //              evaluation of the same algorithm.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 
@@ -149,6 +150,32 @@ struct MarchLane {
     double acc, len, hits;
 };
 
+// The slicer's first sample, at the detector point c of the ray (ForwardProjectionSlicer.cu:129-133)
+__device__ __forceinline__ float detector_sample(const VolG& v, const float* __restrict__ vol, const RayD& r) {
+    return tex3d_linear(vol, v.nx, v.ny, v.nz,
+                        (float)((r.c[0] - (double)v.bbMin[0]) * ((double)v.rcp[0] * (double)v.dim[0])),
+                        (float)((r.c[1] - (double)v.bbMin[1]) * ((double)v.rcp[1] * (double)v.dim[1])),
+                        (float)((r.c[2] - (double)v.bbMin[2]) * ((double)v.rcp[2] * (double)v.dim[2]) -
+                                (double)v.zshift));
+}
+
+// Number of samples t_first, t_first + step, .. up to t_last (t_first <= t_last)
+__device__ __forceinline__ int sample_count(double t_first, double t_last, double step) {
+    return (int)fmin(floor((t_last - t_first) / step) + 1.0, (double)kMaxSamplesPerRay);
+}
+
+// Texel coordinates of the sample at t_first: (p - bbMin) * rcp * dim (the float-rounded rcp the
+// reference uploads), shifted by -0.5 so that texel j is centred at integer j
+__device__ __forceinline__ void sample_start(const VolG& v, const TiltG& g, const RayD& r, double t_first,
+                                             float q0[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double scale = (double)v.rcp[i] * (double)v.dim[i];
+        const double p = (r.c[i] + t_first * (double)g.n[i]) - (double)v.bbMin[i];
+        q0[i] = (float)(p * scale - 0.5 - (i == 2 ? (double)v.zshift : 0.0));
+    }
+}
+
 template <bool kSlicer>
 __device__ __forceinline__ MarchLane march_lane(const VolG& v, const TiltG& g, const float* __restrict__ vol,
                                                 bool active, int x, int y, int sub, double step, float tminDef,
@@ -170,11 +197,7 @@ __device__ __forceinline__ MarchLane march_lane(const VolG& v, const TiltG& g, c
         t_last = fmin(r.t_out, (double)tmaxDef);
         run = t_in <= t_last;
         if (run) {
-            // first sample at the detector point (ForwardProjectionSlicer.cu:129-133)
-            L.acc = (double)tex3d_linear(
-                vol, v.nx, v.ny, v.nz, (float)((r.c[0] - (double)v.bbMin[0]) * ((double)v.rcp[0] * (double)v.dim[0])),
-                (float)((r.c[1] - (double)v.bbMin[1]) * ((double)v.rcp[1] * (double)v.dim[1])),
-                (float)((r.c[2] - (double)v.bbMin[2]) * ((double)v.rcp[2] * (double)v.dim[2]) - (double)v.zshift));
+            L.acc = (double)detector_sample(v, vol, r);
             t_first = t_in + step;
         }
     } else {
@@ -182,16 +205,10 @@ __device__ __forceinline__ MarchLane march_lane(const VolG& v, const TiltG& g, c
         L.len = r.t_out - r.t_in;
     }
     if (run && t_first <= t_last) {
-        L.count = (int)fmin(floor((t_last - t_first) / step) + 1.0, (double)kMaxSamplesPerRay);
-        // voxel coordinate = (p - bbMin) * rcp * dim (the float-rounded rcp the reference uploads),
-        // shifted by -0.5 so that texel j is centred at integer j
+        L.count = sample_count(t_first, t_last, step);
+        sample_start(v, g, r, t_first, L.q0);
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const double scale = (double)v.rcp[i] * (double)v.dim[i];
-            const double p = (r.c[i] + t_first * (double)g.n[i]) - (double)v.bbMin[i];
-            L.q0[i] = (float)(p * scale - 0.5 - (i == 2 ? (double)v.zshift : 0.0));
-            L.dq[i] = (float)(step * (double)g.n[i] * scale);
-        }
+        for (int i = 0; i < 3; i++) L.dq[i] = (float)(step * (double)g.n[i] * ((double)v.rcp[i] * (double)v.dim[i]));
     }
     return L;
 }
@@ -298,6 +315,150 @@ __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, cons
     march_store<true>(acc, 0.0, 0.0, active, sub, x, y, pitch, step, proj, nullptr);   // proj only
 }
 
+// Every CTF slab of a tilt in one pass: slab k (kbeg <= k < kend) covers bounds[k] .. bounds[k] +
+// step_f along the rays.  Each sub-ray walks the volume once, its slabs as consecutive segments
+// (tube_walk_segments), and leaves per slab the double sum march_lane / march_lds_kernel<true>
+// would hold before the sub-rays are added: part[(k - kbeg) * pstride + 4 * pixel + sub], for the
+// slabs range[4 * pixel + sub] = (klo, khi) it may cross (a superset found by bisection on the
+// sorted bounds, where bounds[k] + step_f == bounds[k + 1]).  slicer_combine_kernel adds the four
+// sub-rays as march_store does.
+struct SlicerSegs {
+    const VolG& v;
+    const TiltG& g;
+    RayD r;   // the sub-ray (ray_setup_d)
+    const float* __restrict__ bounds;
+    double* __restrict__ part;
+    size_t pstride;
+    double step, total;
+    float step_f, px, py, tex;   // px, py: the sub-ray's pixel coordinates (exact in float)
+    float q0[3];
+    unsigned slot;
+    int count;   // samples of the current slab; -1 when the ray misses it
+    int k, klo, khi, kbeg;
+
+    // finish slab k (its sum into part), then set up slab k + 1
+    __device__ __forceinline__ bool next() {
+        if (k >= klo) {
+            // march_lane: the detector sample when the ray meets the slab, plus the walk's sum
+            const double acc = count < 0 ? 0.0 : (count > 0 ? (double)tex + total : (double)tex);
+            part[(size_t)(k - kbeg) * pstride + slot] = acc;
+        }
+        if (++k > khi) return false;
+        const float tmin = bounds[k], tmax = tmin + step_f;
+        const double t_in = fmax(r.t_in, (double)tmin), t_last = fmin(r.t_out, (double)tmax);
+        count = t_in <= t_last ? 0 : -1;
+        total = 0.0;
+        const double t_first = t_in + step;
+        if (count == 0 && t_first <= t_last) {
+            count = sample_count(t_first, t_last, step);
+            sample_start(v, g, r, t_first, q0);
+        }
+        return true;
+    }
+};
+
+__global__ void __launch_bounds__(256, 6) slicer_segments_lds_kernel(VolG v, TiltG g, const float* __restrict__ vol,
+                                                                  int W, int H, const float* __restrict__ bounds,
+                                                                  int kbeg, int kend, float step_f,
+                                                                  double* __restrict__ part, size_t pstride,
+                                                                  int2* __restrict__ range, TubePlan tp) {
+    __shared__ float chunk[kChunkFloats];
+    __shared__ int zr[2];
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int sub = lane & 3, pix = lane >> 2;
+    const int xcd = (int)(blockIdx.x & 7u), slot = (int)(blockIdx.x >> 3);
+    const int grp = (slot / (kXcdGroup * kXcdGroup)) * 8 + xcd, mem = slot % (kXcdGroup * kXcdGroup);
+    const int gtx = (grp % tp.groups_x) * kXcdGroup + mem % kXcdGroup;
+    const int gty = (grp / tp.groups_x) * kXcdGroup + mem / kXcdGroup;
+    if (gtx >= tp.tiles_x || gty >= tp.tiles_y) return;   // whole workgroup
+    const int tx0 = gtx * 8, ty0 = gty * 8;
+    const int x = tx0 + (wave & 1) * 4 + (pix & 3);
+    const int y = ty0 + (wave >> 1) * 4 + (pix >> 2);
+    const bool active = x < W && y < H;
+    const double step = (double)v.vs[0] * 0.15;
+    SlicerSegs seg{v, g, RayD{}, bounds, part};
+    seg.pstride = pstride;
+    seg.step = step;
+    seg.total = 0.0;
+    seg.step_f = step_f;
+    seg.px = (float)x + ((sub & 2) ? 0.75f : 0.25f);
+    seg.py = (float)y + ((sub & 1) ? 0.75f : 0.25f);
+    seg.tex = 0.0f;
+    seg.q0[0] = seg.q0[1] = seg.q0[2] = 0.0f;
+    seg.slot = 4u * ((unsigned)y * (unsigned)W + (unsigned)x) + (unsigned)sub;
+    seg.count = 0;
+    seg.kbeg = kbeg;
+    seg.klo = 0;   // no slabs: next() writes nothing and returns false
+    seg.k = seg.khi = -1;
+    int za = 1, zb = 0;
+    if (active) {
+        seg.r = ray_setup_d(v, g, (double)seg.px, (double)seg.py);
+        const RayD& r = seg.r;
+        int klo = kend, khi = kbeg - 1;
+        if ((r.t_out - r.t_in) > 0.0) {
+            // a slab is crossed only if its end reaches t_in and its start t_out
+            int lo = kbeg, hi = kend;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((double)(bounds[mid] + step_f) >= r.t_in) hi = mid;
+                else lo = mid + 1;
+            }
+            klo = lo;
+            hi = kend;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((double)bounds[mid] > r.t_out) hi = mid;
+                else lo = mid + 1;
+            }
+            khi = lo - 1;
+        }
+        range[seg.slot] = make_int2(klo, khi);
+        if (klo <= khi) {
+            seg.tex = detector_sample(v, vol, r);
+            seg.klo = klo;
+            seg.k = klo - 1;
+            seg.khi = khi;
+            // cell layers of the ray between the slabs' ends (the walk adds a layer of margin)
+            const double scale = (double)v.rcp[2] * (double)v.dim[2];
+            const double ta = fmax(r.t_in, (double)bounds[klo]), tb = fmin(r.t_out, (double)(bounds[khi] + step_f));
+            const double qa = ((r.c[2] + ta * (double)g.n[2]) - (double)v.bbMin[2]) * scale - 0.5 - (double)v.zshift;
+            const double qb = ((r.c[2] + tb * (double)g.n[2]) - (double)v.bbMin[2]) * scale - 0.5 - (double)v.zshift;
+            za = (int)floor(fmin(qa, qb));
+            zb = (int)floor(fmax(qa, qb));
+        }
+    }
+    const TubeDir td = tube_dir(v, g, step);
+    const TubeBox tb = tube_box(v, g, tp, (double)tx0 + 0.25, (double)tx0 + 7.75, (double)ty0 + 0.25,
+                                (double)ty0 + 7.75, 0.0, 0.0);
+    tube_walk_segments(v, vol, chunk, zr, tp, tb, td, za, zb, seg,
+                       [&](const float val[8], const int cell[3], int kk, int mi) {
+                           seg.total += cell_sum_values(val, cell_moments(seg.q0, td.dq, cell, kk, mi), mi);
+                       });
+}
+
+// Slab images from the sub-ray sums: stack[k * stride + y * pitch + x] = 0 + the four sub-rays'
+// sums added as march_store does ((s0 + s1) + (s2 + s3)), times step / 4, for kbeg <= k < kend; a
+// sub-ray's slabs outside its range hold 0.
+__global__ void __launch_bounds__(256) slicer_combine_kernel(const double* __restrict__ part, size_t pstride,
+                                                             const int2* __restrict__ range, int W, int H,
+                                                             size_t pitch, float* __restrict__ stack, size_t stride,
+                                                             int kbeg, double step) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)blockIdx.y;
+    if (x >= W) return;
+    const int k = kbeg + (int)blockIdx.z;
+    const size_t p4 = 4 * ((size_t)y * (size_t)W + (size_t)x);
+    const double* pk = part + (size_t)blockIdx.z * pstride + p4;
+    double a[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int2 r = range[p4 + s];
+        a[s] = (k >= r.x && k <= r.y) ? pk[s] : 0.0;
+    }
+    const double acc = (a[0] + a[1]) + (a[2] + a[3]);
+    stack[(size_t)k * stride + (size_t)y * pitch + (size_t)x] = 0.0f + (float)(acc * step * 0.25);
+}
+
 __global__ void __launch_bounds__(256) vol_traversal_exact_kernel(VolG v, TiltG g, Roi roi, size_t pitch,
                                                                   float* __restrict__ dist) {
     const int x = roi.x0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -387,6 +548,49 @@ void launch_slicer(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* v
                    float* proj, float tmin, float tmax) {
     const int roi[4] = {0, 0, W, H};
     launch_march_impl(ctx, v, g, vol, make_roi(roi, W, H), pitch, proj, nullptr, true, tmin, tmax);
+}
+
+size_t slicer_stack_scratch_bytes(int W, int H, int nslab) {
+    const size_t npix = (size_t)W * (size_t)H;
+    return npix * 4 * sizeof(int2) + (size_t)nslab * npix * 4 * sizeof(double);
+}
+
+void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vol, int W, int H, size_t pitch,
+                         float* stack, size_t stride, const float* bounds, const float* d_bounds, int nslab,
+                         float step, void* scratch, size_t scratch_bytes) {
+    if (nslab <= 0) return;
+    const size_t npix = (size_t)W * (size_t)H;
+    const size_t range_bytes = npix * 4 * sizeof(int2), slab_bytes = npix * 4 * sizeof(double);
+    const int per_pass = scratch_bytes > range_bytes ? (int)std::min<size_t>((scratch_bytes - range_bytes) / slab_bytes,
+                                                                             (size_t)nslab)
+                                                     : 0;
+    TubePlan tp;
+    // sub-ray pixel coordinates exact in float, 32-bit sub-ray slots and one grid row per image row
+    const bool fits = W < (1 << 22) && H <= 65535 && npix * 4 < ((size_t)1 << 32);
+    if (ctx->numerics == ARTIA_NUMERICS_EXACT && ctx->march_lds && per_pass > 0 && fits && tube_plan(v, g, tp)) {
+        tp.tiles_x = (W + 7) / 8;
+        tp.tiles_y = (H + 7) / 8;
+        tp.groups_x = (tp.tiles_x + kXcdGroup - 1) / kXcdGroup;
+        const int groups = tp.groups_x * ((tp.tiles_y + kXcdGroup - 1) / kXcdGroup);
+        const dim3 grid1((unsigned)(((groups + 7) / 8) * 8 * kXcdGroup * kXcdGroup));
+        int2* range = static_cast<int2*>(scratch);
+        double* part = reinterpret_cast<double*>(static_cast<char*>(scratch) + range_bytes);
+        const double dstep = (double)v.vs[0] * 0.15;
+        for (int kb = 0; kb < nslab; kb += per_pass) {
+            const int ke = std::min(nslab, kb + per_pass);
+            hipLaunchKernelGGL(slicer_segments_lds_kernel, grid1, dim3(256), 0, ctx->stream, v, g, vol, W, H,
+                               d_bounds, kb, ke, step, part, npix * 4, range, tp);
+            hipLaunchKernelGGL(slicer_combine_kernel, dim3((unsigned)((W + 255) / 256), (unsigned)H, (unsigned)(ke - kb)),
+                               dim3(256), 0, ctx->stream, part, npix * 4, range, W, H, pitch, stack, stride, kb, dstep);
+        }
+        return;
+    }
+    launch_fill(ctx, stack, (size_t)nslab * stride, 0.0f);
+    const int roiA[4] = {0, 0, W, H};
+    const Roi r = make_roi(roiA, W, H);
+    for (int k = 0; k < nslab; k++)
+        launch_march_impl(ctx, v, g, vol, r, pitch, stack + (size_t)k * stride, nullptr, true, bounds[k],
+                          bounds[k] + step);
 }
 
 void launch_vol_traversal(artia_ctx* ctx, const VolG& v, const TiltG& g, int W, int H, size_t pitch, float* dist,

@@ -146,12 +146,22 @@ artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const 
     const bool stacked = ctx->numerics == ARTIA_NUMERICS_EXACT && max_slabs > 0 &&
                          (double)max_slabs * (double)npix * 4.0 <= 34359738368.0;
     const size_t stack_floats = stacked ? (size_t)max_slabs * npix : 0;
-    artia_status st = ensure_workspace(ctx, npix * (size_t)(1 + S) + stack_floats + (stacked ? bounds.size() : 0));
+    // the stacked slicer's sub-ray sums: every slab of a tilt in one volume pass while they fit
+    // 16 GiB, in passes of that many slabs otherwise
+    size_t slicer_bytes = 0;
+    if (stacked) {
+        int pass = max_slabs;
+        while (pass > 1 && slicer_stack_scratch_bytes(width, height, pass) > ((size_t)16 << 30)) pass = (pass + 1) / 2;
+        slicer_bytes = slicer_stack_scratch_bytes(width, height, pass);
+    }
+    const size_t head = (npix * (size_t)(1 + S) + stack_floats + (stacked ? bounds.size() : 0) + 63) & ~(size_t)63;
+    artia_status st = ensure_workspace(ctx, head + (slicer_bytes + sizeof(float) - 1) / sizeof(float));
     if (st != ARTIA_OK) return st;
     float* dist = ctx->ws;
     float* residual = ctx->ws + npix;
     float* stack = ctx->ws + npix * (size_t)(1 + S);
     float* d_bounds = stack + stack_floats;
+    void* slicer_scratch = ctx->ws + head;
     if (stacked) {
         // earlier work on the stream may still read the workspace
         st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
@@ -192,13 +202,9 @@ artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const 
                     // slab order
                     const size_t f = first[(size_t)idx];
                     const int ns = nslab[(size_t)idx];
-                    pr = prof_start(ctx, ARTIA_PROF_FILL);
-                    launch_fill(ctx, stack, (size_t)ns * npix, 0.0f);
-                    prof_stop(ctx, pr);
                     pr = prof_start(ctx, ARTIA_PROF_FP);
-                    for (int k = 0; k < ns; k++)
-                        launch_slicer(ctx, v, g, vol, width, height, pitch, stack + (size_t)k * npix, bounds[f + k],
-                                      bounds[f + k] + step);
+                    launch_slicer_stack(ctx, v, g, vol, width, height, pitch, stack, npix, &bounds[f], d_bounds + f, ns,
+                                        step, slicer_scratch, slicer_bytes);
                     prof_stop(ctx, pr);
                     pr = prof_start(ctx, ARTIA_PROF_FFT);
                     st = ctf_filter_slabs(ctx, stack, npix, ns, width, height, pitch, proj, 0, frame, sp->cut_length,

@@ -298,6 +298,103 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const float* __restric
     }
 }
 
+// The gather walk over a run of segments per lane: consecutive sample sets along the lane's ray,
+// each with its own start q0 and count (a slicer sample lattice per CTF slab).  seg.next() finishes
+// the lane's current segment, if any, and loads the next into seg.q0 and seg.count, returning false
+// when none is left; f(val, cell, k, m) sees every cell of the current segment.  The segments follow
+// each other along the rays, so one pass over the slabs of cell layers serves all of them; za..zb
+// are cell layers bounding every sample of the lane (za > zb: none).  Each segment is walked with
+// tube_slabs' CellWalker steps, so its cells, their order and their values are those of a
+// tube_walk of that segment alone.  Must be called by all 256 threads of the block.
+template <class Seg, class F>
+__device__ __forceinline__ void tube_walk_segments(const VolG& v, const float* __restrict__ vol,
+                                                   float* __restrict__ chunk, int* zr, const TubePlan& tp,
+                                                   const TubeBox& tb, const TubeDir& td, int za, int zb, Seg& seg,
+                                                   F&& f) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        zr[0] = INT_MAX;
+        zr[1] = INT_MIN;
+    }
+    __syncthreads();
+    if (za <= zb) {
+        atomicMin(&zr[0], za - 1);
+        atomicMax(&zr[1], zb + 1);
+    }
+    __syncthreads();
+    const int zlo = max(zr[0], -1), zhi = min(zr[1], v.nz - 1);
+
+    CellWalker w;
+    w.init_dir(seg.q0, td.sgn, td.inv, 0);   // no segment yet: the first pass below loads one
+    bool more = true;
+    const int nslab = zlo <= zhi ? (zhi - zlo) / kSlabCells + 1 : 0;
+    // slabs of cell layers, then (j == nslab) the cells beyond them (not expected) from global memory
+    for (int j = 0; j <= nslab; j++) {
+        const bool tail = j == nslab;
+        const int zs = tp.zdir > 0 ? zlo + j * kSlabCells : zhi - (j + 1) * kSlabCells + 1;
+        const int ze = zs + kSlabCells - 1;
+        const float z0 = (float)zs, z1 = (float)(ze + 1);
+        const int X0 = wave_uniform(max((int)floorf(tb.axl + fminf(tp.bx * z0, tp.bx * z1)) - 1, -1));
+        const int X1 = min((int)floorf(tb.axh + fmaxf(tp.bx * z0, tp.bx * z1)) + 2, v.nx);
+        const int Y0 = wave_uniform(max((int)floorf(tb.ayl + fminf(tp.by * z0, tp.by * z1)) - 1, -1));
+        const int Y1 = min((int)floorf(tb.ayh + fmaxf(tp.by * z0, tp.by * z1)) + 2, v.ny);
+        const int Z0 = max(zs, -1), Z1 = min(ze + 1, v.nz);
+        const int CX = wave_uniform(X1 - X0 + 1), CY = wave_uniform(Y1 - Y0 + 1), CZ = Z1 - Z0 + 1;
+        const bool staged = !tail && CX > 1 && CY > 1 && CZ > 1 && CX * CY * CZ <= kChunkFloats;
+        const int n = CX * CY * CZ;
+        if (!tail) {
+            __syncthreads();
+            if (staged)
+                for (int e = (int)threadIdx.x; e < n; e += 256) chunk[e] = vol[box_voxel(v, X0, Y0, Z0, CX, CY, e)];
+            __syncthreads();
+        }
+        for (;;) {
+            if (!w.active()) {
+                if (!more) break;
+                more = seg.next();
+                w.init_dir(seg.q0, td.sgn, td.inv, more ? seg.count : 0);
+                continue;
+            }
+            if (!tail && (tp.zdir > 0 ? w.cell[2] > ze : w.cell[2] < zs)) break;   // resumes in a later slab
+            bool fast = false;
+            if (staged) {
+                const float zp = tp.zdir > 0 ? (float)(ze + 1) : (float)zs;
+                const float ke = fminf(fmaxf((zp - seg.q0[2]) * td.inv[2], (float)w.k), (float)(w.count - 1));
+                const int xb = (int)floorf(__builtin_fmaf(ke, td.dq[0], seg.q0[0]));
+                const int yb = (int)floorf(__builtin_fmaf(ke, td.dq[1], seg.q0[1]));
+                const int zl = tp.zdir > 0 ? w.cell[2] : max(zs, -1);
+                const int zh = tp.zdir > 0 ? min(ze + 1, v.nz) : w.cell[2] + 1;
+                fast = min(w.cell[0], xb) - 1 >= X0 && max(w.cell[0], xb) + 2 <= X1 && min(w.cell[1], yb) - 1 >= Y0 &&
+                       max(w.cell[1], yb) + 2 <= Y1 && zl >= Z0 && zh <= Z1;
+            }
+            if (fast) {
+                const int CXY = CX * CY;
+                const int st0 = td.sgn[0], st1 = td.sgn[1] * CX, st2 = td.sgn[2] * CXY;
+                const int off[8] = {0, 1, CX, CX + 1, CXY, CXY + 1, CXY + CX, CXY + CX + 1};
+                int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
+                while (w.active() && (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
+                    w.step(
+                        [&](const int cell[3], int k, int mi) {
+                            float val[8];
+#pragma unroll
+                            for (int i = 0; i < 8; i++) val[i] = chunk[o + off[i]];
+                            f(val, cell, k, mi);
+                        },
+                        [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
+                }
+            } else {
+                while (w.active() && (tail || (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs))) {
+                    w.step([&](const int cell[3], int k, int mi) {
+                        float val[8];
+                        chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
+                        f(val, cell, k, mi);
+                    });
+                }
+            }
+        }
+    }
+}
+
 // The gather walk (see tube_slabs).
 template <class F>
 __device__ __forceinline__ void tube_walk(const VolG& v, const float* __restrict__ vol, float* __restrict__ chunk,
