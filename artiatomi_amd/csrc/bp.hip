@@ -15,6 +15,7 @@
 //   each voxel (u/v/n frame, double-precision offsets, float slab arithmetic), and only the
 //   lattice points inside the voxel's own projected box are visited; the accumulation order
 //   per voxel (py outer, px inner) is the reference's.
+#include <climits>
 #include <cmath>
 
 #include "bp_common.hpp"
@@ -216,17 +217,57 @@ __global__ void __launch_bounds__(256) bp_exact_kernel(VolG v, TiltG g, BPExact 
 // The CTF backprojection of one tilt in one launch: every voxel takes its slab's filtered image
 // from the stack (slab k at stack + k * stride) instead of one gated launch per slab
 // (Reconstructor.cpp:920-969).  A voxel lies in exactly one slab, so it receives the same single
-// update, in the same arithmetic, as in the per-slab launches.
+// update, in the same arithmetic, as in the per-slab launches.  For LINEAR sampling without
+// magnification anisotropy (affine footprint) the block stages its footprint of every slab its
+// quads fall in, when that fits the LDS tile, and fetches from it with tex2d's arithmetic.
 __global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact e, float* __restrict__ vol,
                                                        const float* __restrict__ stack, size_t stride, int W, int H,
                                                        size_t pitch, float lambda, int os, int filter, SlabArgs sl) {
+    __shared__ float tile[kTileFloats];
+    __shared__ int kr[2];
     const int tid = (int)threadIdx.x;
     const int xq = (int)blockIdx.x * 16 + (tid & 15);
     const int y = (int)blockIdx.y * 4 + ((tid >> 4) & 3);
     const int z = (int)blockIdx.z * 4 + (tid >> 6);
-    if (!(xq < v.nx / 4 && y < v.ny && z < v.nz)) return;
+    const bool inVol = xq < v.nx / 4 && y < v.ny && z < v.nz;
     const unsigned x = (unsigned)xq * 4u;
-    const int slab = find_slab(sl, gate_distance(e, v, g, x, (unsigned)y, (unsigned)z));
+    if (tid == 0) {
+        kr[0] = INT_MAX;
+        kr[1] = INT_MIN;
+    }
+    __syncthreads();
+    const int slab = inVol ? find_slab(sl, gate_distance(e, v, g, x, (unsigned)y, (unsigned)z)) : -1;
+    if (slab >= 0) {
+        atomicMin(&kr[0], slab);
+        atomicMax(&kr[1], slab);
+    }
+    __syncthreads();
+    const int k0 = kr[0], k1 = kr[1];
+    bool tiled = false;
+    int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1, tw = 0, plane = 0;
+    if (e.affine && e.identityAniso && filter == ARTIA_FILTER_LINEAR && k0 <= k1) {
+        // the block's footprint (affine, so corners suffice) with two pixels of margin: the
+        // bilinear fetches reach one pixel below a quad's footprint, the other covers rounding
+        const double bx = (double)blockIdx.x * 64.0, by = (double)blockIdx.y * 4.0, bz = (double)blockIdx.z * 4.0;
+        const double base0 = e.G[0][0] * bx + e.G[0][1] * by + e.G[0][2] * bz;
+        const double base1 = e.G[1][0] * bx + e.G[1][1] * by + e.G[1][2] * bz;
+        tx0 = max(0, (int)floor(base0 + e.tlo[0]) - 2);
+        ty0 = max(0, (int)floor(base1 + e.tlo[1]) - 2);
+        tx1 = min(W - 1, (int)ceil(base0 + e.thi[0]) + 2);
+        ty1 = min(H - 1, (int)ceil(base1 + e.thi[1]) + 2);
+        tw = max(0, tx1 - tx0 + 1);
+        plane = tw * max(0, ty1 - ty0 + 1);
+        tiled = plane > 0 && (long long)plane * (long long)(k1 - k0 + 1) <= (long long)kTileFloats;
+        if (tiled) {
+            const int n = plane * (k1 - k0 + 1);
+            for (int idx = tid; idx < n; idx += 256) {
+                const int s = idx / plane, rem = idx - s * plane;
+                const int r = rem / tw, c = rem - r * tw;
+                tile[idx] = stack[(size_t)(k0 + s) * stride + (size_t)(ty0 + r) * pitch + (size_t)(tx0 + c)];
+            }
+            __syncthreads();
+        }
+    }
     if (slab < 0) return;
     const float* __restrict__ img = stack + (size_t)slab * stride;
     int px0, px1, py0, py1;
@@ -237,6 +278,9 @@ __global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact 
     }
     const int nI = (px1 - px0) * os, nJ = (py1 - py0) * os;
     if (nI <= 0 || nJ <= 0) return;
+    // the quad's fetches read pixels max(p0 - 1, 0) .. p1 on each axis
+    const bool fromTile = tiled && max(px0 - 1, 0) >= tx0 && px1 <= tx1 && max(py0 - 1, 0) >= ty0 && py1 <= ty1;
+    const float* __restrict__ slabTile = tile + (fromTile ? (slab - k0) * plane : 0);
     float* vp = vol + ((size_t)z * (size_t)v.ny + (size_t)y) * (size_t)v.nx + x;
     const float4 vb = *reinterpret_cast<const float4*>(vp);
     float out[4] = {vb.x, vb.y, vb.z, vb.w};
@@ -254,7 +298,11 @@ __global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact 
         const int j0 = max(0, (int)ceilf((dv + e.vMin) * osf - 0.5f));
         const int j1 = min(nJ - 1, (int)floorf((dv + e.vMax) * osf - 0.5f));
         float num = 0.0f, den = 0.0f;
-        voxel_sums_global(e, g, img, W, H, pitch, filter, px0, py0, os, delta, du, dv, i0, i1, j0, j1, num, den);
+        if (fromTile)
+            voxel_sums_linear_tile(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0, j1, num,
+                                   den);
+        else
+            voxel_sums_global(e, g, img, W, H, pitch, filter, px0, py0, os, delta, du, dv, i0, i1, j0, j1, num, den);
         den = fmaxf(0.0f, den);
         if (den != 0.0f) {
             out[k] += (lambda * num) / den;

@@ -353,4 +353,47 @@ __device__ __forceinline__ void voxel_sums_global(const BPExact& e, const TiltG&
     }
 }
 
+// tex2d's LINEAR fetch at (x, y) from an LDS copy of image pixels [tx0, tx0 + tw) x [ty0, ..) that
+// holds every pixel the fetch reads: the same clamps, weights and lerps, so the same value.
+__device__ __forceinline__ float tile_linear(const float* __restrict__ t, int tw, int tx0, int ty0, int W, int H,
+                                             float x, float y) {
+    x = fminf(fmaxf(x, -1.0f), (float)(W + 1));
+    y = fminf(fmaxf(y, -1.0f), (float)(H + 1));
+    const float xb = x - 0.5f, yb = y - 0.5f;
+    const float fx = floorf(xb), fy = floorf(yb);
+    const float a = xb - fx, b = yb - fy;
+    int i0 = (int)fx, j0 = (int)fy;
+    const int i1 = clampi(i0 + 1, 0, W - 1) - tx0, j1 = clampi(j0 + 1, 0, H - 1) - ty0;
+    i0 = clampi(i0, 0, W - 1) - tx0;
+    j0 = clampi(j0, 0, H - 1) - ty0;
+    const float r0 = lerpf(t[j0 * tw + i0], t[j0 * tw + i1], a);
+    const float r1 = lerpf(t[j1 * tw + i0], t[j1 * tw + i1], a);
+    return lerpf(r0, r1, b);
+}
+
+// voxel_sums_global for LINEAR sampling without magnification anisotropy (the fetch point is the
+// sub-ray's pixel position), reading the image from an LDS tile (tile_linear).
+__device__ __forceinline__ void voxel_sums_linear_tile(const BPExact& e, const float* __restrict__ t, int tw, int tx0,
+                                                       int ty0, int W, int H, int px0, int py0, float delta, float du,
+                                                       float dv, int i0, int i1, int j0, int j1, float& num,
+                                                       float& den) {
+    for (int j = j0; j <= j1; j++) {
+        const float b = __builtin_fmaf((float)j + 0.5f, delta, -dv);
+        const float bV0 = __builtin_fmaf(b, e.V[0], e.Smin[0]);
+        const float bV1 = __builtin_fmaf(b, e.V[1], e.Smin[1]);
+        const float bV2 = __builtin_fmaf(b, e.V[2], e.Smin[2]);
+        const float a0 = __builtin_fmaf((float)i0 + 0.5f, delta, -du);
+        for (int i = i0; i <= i1; i++) {
+            const float chord = chord_at<-1>(e, lattice_a(i, i0, delta, a0), b, bV0, bV1, bV2);
+            if (chord > 0.0f) {
+                const float pxf = (float)px0 + ((float)i + 0.5f) * delta;
+                const float pyf = (float)py0 + ((float)j + 0.5f) * delta;
+                const float val = tile_linear(t, tw, tx0, ty0, W, H, pxf, pyf);
+                num = __builtin_fmaf(val, chord, num);
+                den += chord;
+            }
+        }
+    }
+}
+
 }  // namespace artia
