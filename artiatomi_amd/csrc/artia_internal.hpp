@@ -145,6 +145,7 @@ artia_status ctf_slab_filter(artia_ctx* ctx, float* img, int W, int H, size_t pi
 //              cropped result in slab order
 //   forFP = 0: every slab reads src (src_stride 0), cropped s + 1 times as the repeated in-place
 //              crop leaves it; slab s's result is stored at out + s * out_stride; src is not changed
+// corners == nullptr: no border crop (the sub-volume WBP's slab filter)
 artia_status ctf_filter_slabs(artia_ctx* ctx, const float* src, size_t src_stride, int n, int W, int H, size_t pitch,
                               float* out, size_t out_stride, const int corners[8], const float cut[2],
                               const float dim[2], const artia_ctf_params& p, const float* dmin_nm,

@@ -218,17 +218,18 @@ __global__ void __launch_bounds__(256) finish_kernel(const float* __restrict__ s
 // after every slab, of ns accumulating finish_kernel calls
 __global__ void __launch_bounds__(256) finish_sum_kernel(const float* __restrict__ sq, int ns, int N, int bx, int by,
                                                          int W, int H, float* __restrict__ out, size_t pitch,
-                                                         float norm, CropParams cp) {
+                                                         float norm, int crop, CropParams cp) {
     const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
     if (x >= W || y >= H) return;
-    const float w = crop_weight(x, y, W, H, cp);
+    const float w = crop ? crop_weight(x, y, W, H, cp) : 1.0f;
     const size_t i = (size_t)y * pitch + x;
     const size_t j = (size_t)(by + y) * N + (bx + x), plane = (size_t)N * (size_t)N;
     float acc = out[i];
     for (int s = 0; s < ns; s++) {
-        const float v = sq[(size_t)s * plane + j] / norm;
-        acc = acc + w * v;
+        float v = sq[(size_t)s * plane + j] / norm;
+        if (crop) v = w * v;
+        acc = acc + v;
     }
     out[i] = acc;
 }
@@ -247,9 +248,11 @@ __global__ void __launch_bounds__(256) slab_square_kernel(const float* __restric
     float val = 0.0f;
     if (xi >= 0 && xi < W && yi >= 0 && yi < H) {
         val = src[(size_t)s * src_stride + (size_t)yi * pitch + xi];
-        const float w = crop_weight(xi, yi, W, H, cp);
         const int reps = rep0 + rep_step * s;
-        for (int r = 0; r < reps; r++) val = w * val;
+        if (reps > 0) {
+            const float w = crop_weight(xi, yi, W, H, cp);
+            for (int r = 0; r < reps; r++) val = w * val;
+        }
     }
     sq[(size_t)s * (size_t)N * (size_t)N + (size_t)y * N + x] = val;
 }
@@ -427,7 +430,8 @@ artia_status ctf_filter_slabs(artia_ctx* ctx, const float* src, size_t src_strid
     FftBatch& fb = ctx->fftb;
     artia_status st = fb.ensure(ctx, N);
     if (st != ARTIA_OK) return st;
-    const CropParams cp = make_crop_params(cut, dim, corners);
+    const int crop = corners ? 1 : 0;
+    const CropParams cp = crop ? make_crop_params(cut, dim, corners) : CropParams{};
     const int bx = (N - W) / 2, by = (N - H) / 2;
     const size_t half = (size_t)(N / 2 + 1);
     for (int k0 = 0; k0 < n;) {
@@ -437,7 +441,7 @@ artia_status ctf_filter_slabs(artia_ctx* ctx, const float* src, size_t src_strid
         for (int t = b; t > 1; t /= 2) pi++;
         hipLaunchKernelGGL(slab_square_kernel, dim3((unsigned)((N + 63) / 64), (unsigned)((N + 3) / 4), (unsigned)b),
                            dim3(64, 4), 0, ctx->stream, src + (size_t)k0 * src_stride, src_stride, W, H, pitch, N, bx,
-                           by, fb.square, cp, forFP ? 1 : k0 + 1, forFP ? 0 : 1);
+                           by, fb.square, cp, !crop ? 0 : (forFP ? 1 : k0 + 1), crop && !forFP ? 1 : 0);
         if (hipfftSetStream(fb.r2c[pi], ctx->stream) != HIPFFT_SUCCESS ||
             hipfftSetStream(fb.c2r[pi], ctx->stream) != HIPFFT_SUCCESS)
             return fail(ctx, ARTIA_ERR_FFT, "hipfftSetStream failed");
@@ -457,11 +461,11 @@ artia_status ctf_filter_slabs(artia_ctx* ctx, const float* src, size_t src_strid
             return fail(ctx, ARTIA_ERR_FFT, "hipfftExecC2R (batch %d) failed", b);
         if (forFP)
             hipLaunchKernelGGL(finish_sum_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, fb.square, b, N, bx, by, W,
-                               H, out, pitch, (float)(N * N), cp);
+                               H, out, pitch, (float)(N * N), crop, cp);
         else
             hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((W + 63) / 64), (unsigned)((H + 3) / 4), (unsigned)b),
                                dim3(64, 4), 0, ctx->stream, fb.square, N, bx, by, W, H, out + (size_t)k0 * out_stride,
-                               pitch, (float)(N * N), 0, 1, cp, out_stride);
+                               pitch, (float)(N * N), 0, crop, cp, out_stride);
         st = check_hip(ctx, hipGetLastError(), "stacked slab filter");
         if (st != ARTIA_OK) return st;
         k0 += b;

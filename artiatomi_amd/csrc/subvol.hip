@@ -15,6 +15,7 @@
 //          Without CTF the counter reset and the persistent kernel are captured in one hipGraph;
 //          with CTF each tilt's slab images are filtered first and one persistent launch follows.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -31,7 +32,8 @@ struct SubvolDev {
 };
 
 // Per (s, t) pair.  mode 0: inactive; 1: LDS-tile eligible (affine footprint, POINT, no
-// magnification anisotropy); 2: global-memory path.
+// magnification anisotropy); 2: global-memory path; 3: CTF slab stack, LDS-tile eligible for
+// LINEAR fetches (affine footprint, no magnification anisotropy).
 struct PairDev {
     BPExact e;
     TiltG g;
@@ -97,6 +99,43 @@ __device__ __forceinline__ void quad_update(const PairDev& p, const float* __res
     }
 }
 
+// quad_update for a CTF pair whose quad lies in slab k: LINEAR fetches from the slab's LDS tile
+// (pixels [tx0, tx1] x [ty0, ty1]) when the quad's fetches stay inside it, else from global memory.
+__device__ __forceinline__ void quad_update_stack(const PairDev& p, const float* __restrict__ slabTile, bool tiled,
+                                                  int tw, int tx0, int ty0, int tx1, int ty1,
+                                                  const float* __restrict__ src, int W, int H, size_t pitch, int os,
+                                                  unsigned x, unsigned y, unsigned z, float out[4]) {
+    const BPExact& e = p.e;
+    int px0, px1, py0, py1;
+    if (!footprint_affine(e, x, y, z, W, H, -1.0, INFINITY, px0, px1, py0, py1)) return;
+    const int nI = (px1 - px0) * os, nJ = (py1 - py0) * os;
+    if (nI <= 0 || nJ <= 0) return;
+    // the quad's fetches read pixels max(p0 - 1, 0) .. p1 on each axis
+    const bool fromTile = tiled && max(px0 - 1, 0) >= tx0 && px1 <= tx1 && max(py0 - 1, 0) >= ty0 && py1 <= ty1;
+    const float osf = (float)os, delta = 1.0f / osf;
+    const float du0 = (float)(e.pu0 + (double)x * e.pui + (double)y * e.puj + (double)z * e.puk - (double)px0);
+    const float dv0 = (float)(e.pv0 + (double)x * e.pvi + (double)y * e.pvj + (double)z * e.pvk - (double)py0);
+    const float dui = (float)e.pui, dvi = (float)e.pvi;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float du = __builtin_fmaf((float)k, dui, du0);
+        const float dv = __builtin_fmaf((float)k, dvi, dv0);
+        const int i0 = max(0, (int)ceilf((du + e.uMin) * osf - 0.5f));
+        const int i1 = min(nI - 1, (int)floorf((du + e.uMax) * osf - 0.5f));
+        const int j0 = max(0, (int)ceilf((dv + e.vMin) * osf - 0.5f));
+        const int j1 = min(nJ - 1, (int)floorf((dv + e.vMax) * osf - 0.5f));
+        float num = 0.0f, den = 0.0f;
+        if (fromTile)
+            voxel_sums_linear_tile(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0, j1, num,
+                                   den);
+        else
+            voxel_sums_global(e, p.g, src, W, H, pitch, ARTIA_FILTER_LINEAR, px0, py0, os, delta, du, dv, i0, i1, j0,
+                              j1, num, den);
+        den = fmaxf(0.0f, den);
+        if (den != 0.0f) out[k] += (1.0f * num) / den;
+    }
+}
+
 // Persistent backprojection over the work queue.  Tilts [t0, t1) of every item; image of tilt t at
 // images + (t - t0) * imgStride (no CTF), or slab k of the single tilt at images + k * imgStride.
 template <int kOS, int kQPT>
@@ -108,6 +147,7 @@ __global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__
                                                         int H, size_t pitch, int os, int filter, SlabArgs slabs) {
     __shared__ float tile[kTileFloats];
     __shared__ int cur;
+    __shared__ int kr[2];
     const int tid = (int)threadIdx.x;
     for (;;) {
         if (tid == 0) cur = (int)atomicAdd(counter, 1u);
@@ -142,6 +182,63 @@ __global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__
             if (p.mode == 0) continue;
             // no CTF: tilt t's image; CTF (one tilt per launch): the tilt's slab stack
             const float* img = images + (size_t)(t - t0) * imgStride;
+            if (p.mode == 3) {
+                // slab of each quad, and the item's slab range
+                if (tid == 0) {
+                    kr[0] = INT_MAX;
+                    kr[1] = INT_MIN;
+                }
+                __syncthreads();
+                int ks[kQPT];
+#pragma unroll
+                for (int q = 0; q < kQPT; q++) {
+                    const unsigned z = (unsigned)(item.z0 + tz + q * item.tz);
+                    ks[q] = in[q] ? find_slab(slabs, gate_distance(p.e, p.v, p.g, x, (unsigned)y, z)) : -1;
+                    if (ks[q] >= 0) {
+                        atomicMin(&kr[0], ks[q]);
+                        atomicMax(&kr[1], ks[q]);
+                    }
+                }
+                __syncthreads();
+                const int k0 = kr[0], k1 = kr[1];
+                // the item's footprint with two pixels of margin (see bp_stack_kernel), per slab
+                const BPExact& e = p.e;
+                const double ox = (double)(item.qx0 * 4), oy = (double)item.y0, oz = (double)item.z0;
+                const double ext[3] = {(double)(bX - 4), (double)(bY - 1), (double)(bZ - 1)};
+                double lo[2], hi[2];
+                for (int r = 0; r < 2; r++) {
+                    const double base = e.G[r][0] * ox + e.G[r][1] * oy + e.G[r][2] * oz;
+                    lo[r] = base + e.lo[r];
+                    hi[r] = base + e.hi[r];
+                    for (int k = 0; k < 3; k++) {
+                        lo[r] += fmin(0.0, e.G[r][k] * ext[k]);
+                        hi[r] += fmax(0.0, e.G[r][k] * ext[k]);
+                    }
+                }
+                const int tx0 = max(0, (int)floor(lo[0]) - 2), ty0 = max(0, (int)floor(lo[1]) - 2);
+                const int tx1 = min(W - 1, (int)ceil(hi[0]) + 2), ty1 = min(H - 1, (int)ceil(hi[1]) + 2);
+                const int tw = max(0, tx1 - tx0 + 1), plane = tw * max(0, ty1 - ty0 + 1);
+                const bool tiled = k0 <= k1 && plane > 0 &&
+                                   (long long)plane * (long long)(k1 - k0 + 1) <= (long long)kTileFloats;
+                if (tiled) {
+                    const int n = plane * (k1 - k0 + 1);
+                    for (int idx = tid; idx < n; idx += 256) {
+                        const int sl = idx / plane, rem = idx - sl * plane;
+                        const int r = rem / tw, c = rem - r * tw;
+                        tile[idx] = img[(size_t)(k0 + sl) * imgStride + (size_t)(ty0 + r) * pitch + (size_t)(tx0 + c)];
+                    }
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int q = 0; q < kQPT; q++) {
+                    if (ks[q] < 0) continue;
+                    const unsigned z = (unsigned)(item.z0 + tz + q * item.tz);
+                    quad_update_stack(p, tile + (tiled ? (ks[q] - k0) * plane : 0), tiled, tw, tx0, ty0, tx1, ty1,
+                                      img + (size_t)ks[q] * imgStride, W, H, pitch, os, x, (unsigned)y, z, acc[q]);
+                }
+                __syncthreads();   // the tile and kr are reused by the next tilt or item
+                continue;
+            }
             bool useTile = false;
             int tx0 = 0, ty0 = 0, tw = 0, th = 0;
             if (p.mode == 1) {
@@ -396,17 +493,17 @@ artia_status enqueue(artia_subvol_plan* p, hipStream_t stream) {
         std::vector<float> starts;
         slab_starts(p->slab_range[2 * (size_t)t], p->slab_range[2 * (size_t)t + 1],
                     p->ctfp.slice_thickness_nm / p->ctfp.pixelsize_nm, starts);
-        for (int k = 0; k < count; k++) {
-            float dmin, dmax;
-            artia_geom_defocus_minmax(starts[(size_t)k], &p->defocus[(size_t)t], &p->ctfp, &dmin, &dmax);
-            artia_status st = ctf_slab_filter(ctx, const_cast<float*>(in.images) + (size_t)t * slabStride, in.W, in.H,
-                                              in.pitch, p->d_stack + (size_t)k * slabStride, in.pitch, 0, nullptr,
-                                              nullptr, nullptr, p->ctfp, dmin, dmax,
-                                              p->defocus[(size_t)t].astig_angle_deg, 0);
-            if (st != ARTIA_OK) {
-                ctx->stream = saved;
-                return st;
-            }
+        // every slab's filtered image into the stack (batched FFTs, no border crop)
+        std::vector<float> dmin((size_t)count), dmax((size_t)count);
+        for (int k = 0; k < count; k++)
+            artia_geom_defocus_minmax(starts[(size_t)k], &p->defocus[(size_t)t], &p->ctfp, &dmin[(size_t)k],
+                                      &dmax[(size_t)k]);
+        const artia_status st = ctf_filter_slabs(ctx, in.images + (size_t)t * slabStride, 0, count, in.W, in.H,
+                                                 in.pitch, p->d_stack, slabStride, nullptr, nullptr, nullptr, p->ctfp,
+                                                 dmin.data(), dmax.data(), p->defocus[(size_t)t].astig_angle_deg, 0);
+        if (st != ARTIA_OK) {
+            ctx->stream = saved;
+            return st;
         }
         bounds = p->d_bounds + p->bound_off[(size_t)t];
         launch_subvol(ctx, stream, p->tuning.quads_per_thread, p->grid, p->d_items, p->nitems, p->d_counter,
@@ -600,7 +697,7 @@ artia_status artia_subvol_plan_create(artia_ctx* ctx, const artia_subvol* subs, 
             if (active && !active[i]) continue;
             pd.g = to_device(geoms[i]);
             pd.e = prepare_exact(v, pd.g);
-            pd.mode = (!ctf && pd.e.affine && pd.e.identityAniso) ? 1 : 2;
+            pd.mode = (pd.e.affine && pd.e.identityAniso) ? (ctf ? 3 : 1) : 2;
         }
     }
     if ((st = upload(ctx, (void**)&p->d_pairs, pairs.data(), pairs.size() * sizeof(PairDev), "pair table")) !=
