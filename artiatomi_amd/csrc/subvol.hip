@@ -473,6 +473,33 @@ artia_status set_items(artia_subvol_plan* p, int policy, int qpt) {
     return upload(p->ctx, (void**)&p->d_items, items.data(), items.size() * sizeof(Item), "sub-volume queue");
 }
 
+// CTF: tilt t's slab images, filtered into the stack (batched FFTs, no border crop).
+artia_status filter_tilt(artia_subvol_plan* p, int t) {
+    const Inputs& in = p->in;
+    const int count = p->bound_count[(size_t)t];
+    std::vector<float> starts;
+    slab_starts(p->slab_range[2 * (size_t)t], p->slab_range[2 * (size_t)t + 1],
+                p->ctfp.slice_thickness_nm / p->ctfp.pixelsize_nm, starts);
+    std::vector<float> dmin((size_t)count), dmax((size_t)count);
+    for (int k = 0; k < count; k++)
+        artia_geom_defocus_minmax(starts[(size_t)k], &p->defocus[(size_t)t], &p->ctfp, &dmin[(size_t)k],
+                                  &dmax[(size_t)k]);
+    const size_t slabStride = (size_t)in.H * in.pitch;
+    return ctf_filter_slabs(p->ctx, in.images + (size_t)t * slabStride, 0, count, in.W, in.H, in.pitch, p->d_stack,
+                            slabStride, nullptr, nullptr, nullptr, p->ctfp, dmin.data(), dmax.data(),
+                            p->defocus[(size_t)t].astig_angle_deg, 0);
+}
+
+// CTF: the persistent backprojection of tilt t from its filtered stack.
+void launch_tilt(artia_subvol_plan* p, int t, hipStream_t stream) {
+    const Inputs& in = p->in;
+    const size_t slabStride = (size_t)in.H * in.pitch;
+    launch_subvol(p->ctx, stream, p->tuning.quads_per_thread, p->grid, p->d_items, p->nitems, p->d_counter,
+                  p->d_subs, p->d_pairs, in.ntilts, t, t + 1, p->d_stack, slabStride, in.W, in.H, in.pitch, in.os,
+                  ARTIA_FILTER_LINEAR,
+                  SlabArgs{p->d_bounds + p->bound_off[(size_t)t], p->bound_count[(size_t)t], p->last_end[(size_t)t]});
+}
+
 // Enqueues one full backprojection of the plan on `stream` (the graph body for the no-CTF case).
 artia_status enqueue(artia_subvol_plan* p, hipStream_t stream) {
     const Inputs& in = p->in;
@@ -485,30 +512,14 @@ artia_status enqueue(artia_subvol_plan* p, hipStream_t stream) {
     artia_ctx* ctx = p->ctx;
     hipStream_t saved = ctx->stream;
     ctx->stream = stream;
-    const size_t slabStride = (size_t)in.H * in.pitch;
     for (int t = 0; t < in.ntilts; t++) {
-        const float* bounds = nullptr;
-        int count = p->bound_count[(size_t)t];
-        if (count == 0) continue;
-        std::vector<float> starts;
-        slab_starts(p->slab_range[2 * (size_t)t], p->slab_range[2 * (size_t)t + 1],
-                    p->ctfp.slice_thickness_nm / p->ctfp.pixelsize_nm, starts);
-        // every slab's filtered image into the stack (batched FFTs, no border crop)
-        std::vector<float> dmin((size_t)count), dmax((size_t)count);
-        for (int k = 0; k < count; k++)
-            artia_geom_defocus_minmax(starts[(size_t)k], &p->defocus[(size_t)t], &p->ctfp, &dmin[(size_t)k],
-                                      &dmax[(size_t)k]);
-        const artia_status st = ctf_filter_slabs(ctx, in.images + (size_t)t * slabStride, 0, count, in.W, in.H,
-                                                 in.pitch, p->d_stack, slabStride, nullptr, nullptr, nullptr, p->ctfp,
-                                                 dmin.data(), dmax.data(), p->defocus[(size_t)t].astig_angle_deg, 0);
+        if (p->bound_count[(size_t)t] == 0) continue;
+        const artia_status st = filter_tilt(p, t);
         if (st != ARTIA_OK) {
             ctx->stream = saved;
             return st;
         }
-        bounds = p->d_bounds + p->bound_off[(size_t)t];
-        launch_subvol(ctx, stream, p->tuning.quads_per_thread, p->grid, p->d_items, p->nitems, p->d_counter,
-                      p->d_subs, p->d_pairs, in.ntilts, t, t + 1, p->d_stack, slabStride, in.W, in.H, in.pitch, in.os,
-                      ARTIA_FILTER_LINEAR, SlabArgs{bounds, count, p->last_end[(size_t)t]});
+        launch_tilt(p, t, stream);
     }
     ctx->stream = saved;
     return check_hip(ctx, hipGetLastError(), "sub-volume CTF backprojection launch");
@@ -564,27 +575,51 @@ artia_status autotune(artia_subvol_plan* p) {
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    for (int pol : policies)
-        for (int q : qpts)
-            for (int b : bpcs) {
-                p->tuning = artia_subvol_tuning{0, pol, q, b};
-                p->grid = cus * b;
-                if ((st = set_items(p, pol, q)) != ARTIA_OK) break;
-                float ms = 0.0f;
-                for (int rep = 0; rep < 2 && st == ARTIA_OK; rep++) {
-                    (void)hipEventRecord(e0, ctx->stream);
-                    st = enqueue(p, ctx->stream);
-                    (void)hipEventRecord(e1, ctx->stream);
-                    copy_all(false);
-                    if (st == ARTIA_OK) st = check_hip(ctx, hipEventSynchronize(e1), "autotune run");
-                    if (st == ARTIA_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+    // With CTF only the backprojection launches are timed, on up to four tilts spread over the
+    // series, each filtered once; without CTF the whole (single-launch) pass.
+    std::vector<int> tuneTilts;
+    if (p->ctf) {
+        std::vector<int> withSlabs;
+        for (int t = 0; t < p->in.ntilts; t++)
+            if (p->bound_count[(size_t)t] > 0) withSlabs.push_back(t);
+        const int m = (int)withSlabs.size(), want = std::min(4, m);
+        for (int i = 0; i < want; i++) tuneTilts.push_back(withSlabs[(size_t)((2 * i + 1) * m / (2 * want))]);
+    } else {
+        tuneTilts.push_back(-1);
+    }
+    float total[2][2][3] = {};
+    for (int tt : tuneTilts) {
+        if (tt >= 0 && (st = filter_tilt(p, tt)) != ARTIA_OK) break;
+        for (int a = 0; a < 2 && st == ARTIA_OK; a++)
+            for (int c = 0; c < 2 && st == ARTIA_OK; c++)
+                for (int d = 0; d < 3 && st == ARTIA_OK; d++) {
+                    p->tuning = artia_subvol_tuning{0, policies[a], qpts[c], bpcs[d]};
+                    p->grid = cus * bpcs[d];
+                    if ((st = set_items(p, policies[a], qpts[c])) != ARTIA_OK) break;
+                    float ms = 0.0f;
+                    for (int rep = 0; rep < 2 && st == ARTIA_OK; rep++) {
+                        (void)hipEventRecord(e0, ctx->stream);
+                        if (tt >= 0) {
+                            launch_tilt(p, tt, ctx->stream);
+                            st = check_hip(ctx, hipGetLastError(), "autotune launch");
+                        } else {
+                            st = enqueue(p, ctx->stream);
+                        }
+                        (void)hipEventRecord(e1, ctx->stream);
+                        copy_all(false);
+                        if (st == ARTIA_OK) st = check_hip(ctx, hipEventSynchronize(e1), "autotune run");
+                        if (st == ARTIA_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+                    }
+                    total[a][c][d] += ms;
                 }
-                if (st != ARTIA_OK) break;
-                if (ms < best) {
-                    best = ms;
-                    bestT = p->tuning;
+    }
+    for (int a = 0; a < 2 && st == ARTIA_OK; a++)
+        for (int c = 0; c < 2; c++)
+            for (int d = 0; d < 3; d++)
+                if (total[a][c][d] < best) {
+                    best = total[a][c][d];
+                    bestT = artia_subvol_tuning{0, policies[a], qpts[c], bpcs[d]};
                 }
-            }
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

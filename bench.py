@@ -56,6 +56,10 @@ SUBVOL_WORKLOADS = {
                                               os=2, seed=5, ctf=False),
     "subvol_wbp_ctf_200cubes_1024_61tilts": dict(field=(1024, 1024, 256), W=1024, H=1024, cubes=200, step=2.0,
                                                  os=2, seed=6, ctf=True),
+    # the full-size case with 3D CTF and exact-weighted projections; the plain schedule (a launch
+    # per sub-volume, tilt and slab) is timed on every 10th tilt and scaled by the slab count
+    "subvol_wbp_ctf_2000cubes_4096_61tilts": dict(field=(4096, 4096, 1024), W=4096, H=4096, cubes=2000, step=2.0,
+                                                  os=2, seed=7, ctf=True, weighting="exact", plain_every=10),
 }
 
 
@@ -302,6 +306,14 @@ def bench_subvol(args) -> int:
         geoms.append([A.subvolume_tilt_geometry(t, W, H, extra_shift=extra[i]) for i, t in enumerate(tilts)])
     gen = torch.Generator(device="cuda").manual_seed(w["seed"])
     images = torch.rand((T, H, W), generator=gen, device="cuda")
+    ctx = A.ArtiaContext(0)
+    if w.get("weighting"):
+        # WBP weighting of every projection (the same input to both schedules)
+        weighted = torch.empty_like(images)
+        for t in range(T):
+            ctx.wbp_filter(images[t], weighted[t], 0.0, method=w["weighting"])
+        images = weighted
+        del weighted
     kw = {}
     if w["ctf"]:
         ctf = A.ctf_params(pixelsize_nm=0.25, slice_thickness_nm=4.0, amp_contrast=0.07)
@@ -314,24 +326,70 @@ def bench_subvol(args) -> int:
     vox = sum(n ** 3 for n in sizes)
     log(f"workload {args.workload}: {len(sizes)} cubes ({vox / 1e6:.0f} M voxels), {T} tilts of {W}x{H}, "
         f"inputs in {time.time() - t0:.1f} s")
-    ctx = A.ArtiaContext(0)
+    every = w.get("plain_every", 1)
+    plain_info = {}
+    if every > 1:
+        # plain on a tilt sample, one call per tilt; a plan over the same tilts for the bitwise check
+        sample = list(range(0, T, every))
+
+        def pick(seq):
+            return [seq[t] for t in sample]
+
+        kws = dict(ctf=kw["ctf"], slab_ranges=pick(kw["slab_ranges"]), defocus=pick(kw["defocus"])) if kw else {}
+        img_s = images[sample].contiguous()
+        plan_s = A.SubvolPlan(ctx, subs_b, [pick(g) for g in geoms], img_s, w["os"], **kws)
+        plan_s.execute()
+        plan_s.close()
+        torch.cuda.synchronize()
+        t_sample = 0.0
+        for t in sample:
+            kwt = dict(ctf=kw["ctf"], slab_ranges=[kw["slab_ranges"][t]], defocus=[kw["defocus"][t]]) if kw else {}
+            t0 = time.perf_counter()
+            A.subvol_wbp_plain(ctx, subs_a, [[g[t]] for g in geoms], images[t:t + 1], w["os"], **kwt)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            t_sample += dt
+            log(f"plain tilt {t}: {dt:.3f} s")
+        equal = all(torch.equal(a, b) for (_, a), (_, b) in zip(subs_a, subs_b))
+        del img_s, subs_a
+
+        def nslab(t):
+            if not kw:
+                return 1
+            step = np.float32(kw["ctf"].slice_thickness_nm) / np.float32(kw["ctf"].pixelsize_nm)
+            a, b = (np.float32(x) for x in kw["slab_ranges"][t])
+            n = 0
+            while a < b:
+                n += 1
+                a = np.float32(a + step)
+            return n
+
+        counts_all = [nslab(t) for t in range(T)]
+        scale = sum(counts_all) / sum(counts_all[t] for t in sample)
+        t_plain = t_sample * scale
+        plain_info = {"extrapolated": True, "sampled_tilts": sample, "sampled_seconds": t_sample,
+                      "scale": scale, "scaled_by": "slab count" if kw else "tilt count"}
+        log(f"plain: {t_sample:.3f} s on {len(sample)} tilts, {t_plain:.1f} s extrapolated (x{scale:.2f})")
+        subs_b = [(g, torch.zeros_like(v)) for g, v in subs_b]
     t0 = time.time()
     plan = A.SubvolPlan(ctx, subs_b, geoms, images, w["os"], **kw)
     tu = plan.tuning
     log(f"plan + autotune in {time.time() - t0:.1f} s: policy {tu.tile_policy}, quads/thread "
         f"{tu.quads_per_thread}, blocks/CU {tu.blocks_per_cu}")
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    A.subvol_wbp_plain(ctx, subs_a, geoms, images, w["os"], **kw)
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
-    log(f"plain: {t_plain:.3f} s")
+    if every == 1:
+        t0 = time.perf_counter()
+        A.subvol_wbp_plain(ctx, subs_a, geoms, images, w["os"], **kw)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        log(f"plain: {t_plain:.3f} s")
     t0 = time.perf_counter()
     plan.execute()
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - t0
     log(f"batched: {t_batched:.3f} s")
-    equal = all(torch.equal(a, b) for (_, a), (_, b) in zip(subs_a, subs_b))
+    if every == 1:
+        equal = all(torch.equal(a, b) for (_, a), (_, b) in zip(subs_a, subs_b))
     # further replays accumulate on top; timed for the steady-state rate
     reps = max(1, args.steps)
     t0 = time.perf_counter()
@@ -356,8 +414,9 @@ def bench_subvol(args) -> int:
         "data": "synthetic (seeded cube placements and shifts, random projections)",
         "config": {"workload": args.workload, "field": list(w["field"]), "projection": [W, H], "tilts": T,
                    "cubes": len(sizes), "cube_sizes": n_classes, "cube_counts": counts, "oversampling": w["os"],
-                   "ctf": w["ctf"], "numerics": "exact", "parallelism": "single GPU"},
-        "plain": {"seconds": t_plain, "Gvoxel_updates_per_s": updates / t_plain / 1e9},
+                   "ctf": w["ctf"], "weighting": w.get("weighting"), "numerics": "exact",
+                   "parallelism": "single GPU"},
+        "plain": {"seconds": t_plain, "Gvoxel_updates_per_s": updates / t_plain / 1e9, **plain_info},
         "batched_first": {"seconds": t_batched},
         "speedup_vs_plain": t_plain / t_rep,
         "bitwise_equal_to_plain": bool(equal),
