@@ -74,6 +74,27 @@ REFINE_WORKLOADS = {
 }
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of `workload`
+    (FETCH_SIZE and WRITE_SIZE passes, calibrated against a copy kernel), or None when the summary
+    has no row for it.  The counters need their own profiler runs, so this is the last measurement,
+    not one taken in this process."""
+    path = Path(__file__).resolve().parent / "profiles" / f"{workload}.md"
+    try:
+        text = path.read_text()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == f"`{kernel}`":
+            try:
+                return {"bytes": float(cells[6]) * 1e9, "read_bytes": float(cells[4]) * 1e9,
+                        "written_bytes": float(cells[5]) * 1e9, "source": f"profiles/{workload}.md"}
+            except ValueError:
+                return None
+    return None
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -757,6 +778,10 @@ def main() -> int:
     achieved = dom_bytes / dom_avg / 1e9
     roofline = {"kernel": "march" if dom == "fp" else "backProjection", "bound": "hbm", "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    measured = measured_traffic(args.workload, "march_lds_kernel<false>" if dom == "fp" else "bp_exact_kernel<2, 2>")
+    if measured:
+        roofline["traffic"] = measured.pop("bytes")
+        roofline["traffic_detail"] = measured
 
     cpu = None
     if not args.no_cpu_baseline:
