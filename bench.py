@@ -778,7 +778,10 @@ def main() -> int:
     achieved = dom_bytes / dom_avg / 1e9
     roofline = {"kernel": "march" if dom == "fp" else "backProjection", "bound": "hbm", "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
-    measured = measured_traffic(args.workload, "march_lds_kernel<false>" if dom == "fp" else "bp_exact_kernel<2, 2>")
+    # the profile covers the default configuration's kernels only
+    default_kernels = args.numerics == "exact" and not args.no_fp_staging
+    measured = measured_traffic(args.workload, "march_lds_kernel<false>" if dom == "fp" else "bp_exact_kernel<2, 2>") \
+        if default_kernels else None
     if measured:
         roofline["traffic"] = measured.pop("bytes")
         roofline["traffic_detail"] = measured
