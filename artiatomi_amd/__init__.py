@@ -11,6 +11,7 @@ from .core import (  # noqa: F401
     PairSet,
     SubvolPlan,
     ctf_params,
+    prep_params,
     defocus_distances,
     defocus_minmax,
     footprint_window,

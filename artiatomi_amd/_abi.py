@@ -77,6 +77,22 @@ class CtfParams(C.Structure):
                 ("volume_shift_z", C.c_float), ("ignore_z_shift", C.c_int), ("switch_direction", C.c_int)]
 
 
+DTYPES = {"int16": 0, "uint16": 1, "int32": 2, "uint32": 3, "float32": 4}
+NORMALIZATIONS = {"mean": 0, "std": 1, "none": 2}
+
+
+class PrepParams(C.Structure):
+    """artia_prep_params: data type, FRM_DM4, CorrectBadPixels, BadPixelValue, SkipFilter,
+    fourFilterLP/LPS/HP/HPS, DoseWeighting, pixel size, WBP, WBPFilter, DownWeightTiltsForWBP,
+    ProjectionNormalization."""
+
+    _fields_ = [("dtype", C.c_int), ("mirror_y", C.c_int), ("correct_bad_pixels", C.c_int),
+                ("bad_pixel_value", C.c_float), ("skip_filter", C.c_int), ("lp", C.c_float), ("lps", C.c_float),
+                ("hp", C.c_float), ("hps", C.c_float), ("dose_weighting", C.c_int), ("pixelsize_nm", C.c_float),
+                ("wbp", C.c_int), ("wbp_method", C.c_int), ("down_weight_tilts", C.c_int),
+                ("normalization", C.c_int)]
+
+
 class TiltDefocus(C.Structure):
     """artia_tilt_defocus: CtfFile min/max defocus (nm), astigmatism angle (deg), tilt angle."""
 
@@ -149,6 +165,8 @@ SIGNATURES = {
     "artia_ctf_slab_filter": (_I, [_P, _P, _I, _I, _SZ, _P, _SZ, _I, _PI, _PF, _PF, C.POINTER(CtfParams), _F, _F,
                                    _F, _I, _PF]),
     "artia_wbp_filter": (_I, [_P, _P, _I, _I, _SZ, _P, _SZ, _F, _I, _PF]),
+    "artia_prepare_projections": (_I, [_P, C.POINTER(PrepParams), _P, _I, _I, _I, _SZ, _P, _P, _P, _P, _SZ, _P, _P,
+                                       _P, _PF]),
     "artia_subvol_wbp_plain": (_I, [_P, C.POINTER(Subvol), _I, C.POINTER(TiltGeom), _P, _I, _P, _I, _I, _SZ, _I,
                                     C.POINTER(CtfParams), _PF, C.POINTER(TiltDefocus), _PF]),
     "artia_subvol_plan_create": (_I, [_P, C.POINTER(Subvol), _I, C.POINTER(TiltGeom), _P, _I, _P, _I, _I, _SZ, _I,

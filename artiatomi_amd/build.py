@@ -25,6 +25,7 @@ SOURCES = [
     "adjoint.hip",
     "refine.hip",
     "sart.hip",
+    "prep.hip",
 ]
 
 # -ffp-contract=off: the reference-numerics kernels keep the reference's float operation order;

@@ -203,6 +203,21 @@ def sart_params(lambda_: float = 0.3, sirt_count: int = 1, oversampling: int = 1
                       _abi.farr(cut_length, 2), _abi.farr(dim_length, 2), proj_scale)
 
 
+def prep_params(dtype: str = "float32", mirror_y: bool = False, correct_bad_pixels: bool = True,
+                bad_pixel_value: float = 10.0, skip_filter: bool = False, lp: float = 800.0, lps: float = 200.0,
+                hp: float = 0.0, hps: float = 0.0, dose_weighting: bool = False, pixelsize_nm: float = 1.0,
+                wbp: bool = False, wbp_method: str = "ramp", down_weight_tilts: bool = False,
+                normalization: str = "mean") -> "_abi.PrepParams":
+    """Projection preparation settings; the defaults are EmSART's (fourFilterLP 800, LPS 200, HP 0,
+    HPS 0, BadPixelValue 10, CorrectBadPixels on, ProjectionNormalization mean, WBPFilter ramp)."""
+    if dtype not in _abi.DTYPES:
+        raise ArtiaError(f"unknown projection data type {dtype!r}")
+    return _abi.PrepParams(_abi.DTYPES[dtype], int(bool(mirror_y)), int(bool(correct_bad_pixels)), bad_pixel_value,
+                           int(bool(skip_filter)), lp, lps, hp, hps, int(bool(dose_weighting)), pixelsize_nm,
+                           int(bool(wbp)), _abi.WBP_METHODS[wbp_method], int(bool(down_weight_tilts)),
+                           _abi.NORMALIZATIONS[normalization])
+
+
 # ------------------------------------------------------------------ device operators
 
 def _cptr(t, name: str, n: int) -> int:
@@ -343,6 +358,40 @@ class ArtiaContext:
         H, W = img.shape
         return self._call("artia_wbp_filter", _ptr(img, "img"), W, H, W, _ptr(out, "out", W * H), W,
                           float(psi_deg), _abi.WBP_METHODS[method], timed=timed)
+
+    def prepare_projections(self, pp, images, out, tilt_deg=None, psi_deg=None, dose=None, timed=False):
+        """PrepareProjection over a stack: images (n, H, W) CUDA tensor of pp's data type, out (n, H, W)
+        float32.  Returns (mean, std, bad_pixels) per projection (numpy), and the milliseconds when
+        timed."""
+        import numpy as np
+        import torch
+
+        tdt = {0: torch.int16, 1: torch.uint16, 2: torch.int32, 3: torch.uint32, 4: torch.float32}[pp.dtype]
+        if (not isinstance(images, torch.Tensor) or not images.is_cuda or images.dtype != tdt
+                or not images.is_contiguous() or images.dim() != 3):
+            raise ArtiaError(f"images must be a contiguous (n, H, W) {tdt} CUDA tensor")
+        n, H, W = images.shape
+        _ptr(out, "out", n * H * W)
+
+        def arr(v, what):
+            if v is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(v, np.float32).reshape(-1))
+            if a.size != n:
+                raise ArtiaError(f"{what} needs {n} values")
+            return a
+
+        tl, ps, ds = arr(tilt_deg, "tilt_deg"), arr(psi_deg, "psi_deg"), arr(dose, "dose")
+        mean, std = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        bad = np.zeros(n, np.int32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+        ms = C.c_float()
+        st = self.lib.artia_prepare_projections(self.h, C.byref(pp), C.c_void_p(images.data_ptr()), n, W, H, W,
+                                                fp(tl), fp(ps), fp(ds), C.c_void_p(out.data_ptr()), W, fp(mean),
+                                                fp(std), bad.ctypes.data_as(C.c_void_p),
+                                                C.byref(ms) if timed else None)
+        _check(st, self.h, "artia_prepare_projections")
+        return (mean, std, bad, ms.value) if timed else (mean, std, bad)
 
     def fp_adjoint(self, vg: VolGeom, tg: TiltGeom, resid, vol_grad, scale=1.0, roi=None, timed=False):
         """vol_grad += scale * P^T resid (exact transpose of the exact-numerics forward projector)."""

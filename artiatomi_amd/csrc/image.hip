@@ -73,14 +73,7 @@ __global__ void __launch_bounds__(256) make_square_kernel(int W, int H, size_t p
             val = in[(size_t)yi * pitch + xi];
         }
     } else {
-        if (xi < 0) xi = -xi - 1;
-        if (xi >= W) xi = W - (xi - W) - 1;
-        if (yi < 0) yi = -yi - 1;
-        if (yi >= H) yi = H - (yi - H) - 1;
-        if (mirrorY) yi = H - yi - 1;
-        // reflection of a pad wider than the image would leave the image; clamp keeps it inside
-        xi = clampi(xi, 0, W - 1);
-        yi = clampi(yi, 0, H - 1);
+        mirror_source(x, y, W, H, bx, by, mirrorY, xi, yi);
         val = in[(size_t)yi * pitch + xi];
     }
     out[(size_t)y * N + x] = val;

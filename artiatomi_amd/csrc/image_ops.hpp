@@ -22,6 +22,22 @@ inline CropParams make_crop_params(const float cut[2], const float dim[2], const
     return c;
 }
 
+// CopyToSquare.cu:37-92 mirror padding: source pixel of square pixel (x, y) for a W x H image
+// centred at (bx, by), reflected at the edges, optionally flipped in y; a pad wider than the image
+// would reflect out of it, so the indices are clamped.
+__device__ __forceinline__ void mirror_source(int x, int y, int W, int H, int bx, int by, int mirrorY, int& xi,
+                                              int& yi) {
+    xi = x - bx;
+    yi = y - by;
+    if (xi < 0) xi = -xi - 1;
+    if (xi >= W) xi = W - (xi - W) - 1;
+    if (yi < 0) yi = -yi - 1;
+    if (yi >= H) yi = H - (yi - H) - 1;
+    if (mirrorY) yi = H - yi - 1;
+    xi = xi < 0 ? 0 : (xi > W - 1 ? W - 1 : xi);
+    yi = yi < 0 ? 0 : (yi > H - 1 ? H - 1 : yi);
+}
+
 __device__ __forceinline__ float border_weight(float w) { return 1.0f - expf(-((w * w) * 9.0f)); }
 
 __device__ __forceinline__ float dist2(float ax, float ay, float bx, float by) {

@@ -237,6 +237,53 @@ artia_status artia_ctf_slab_filter(artia_ctx* ctx, float* img, int width, int he
 artia_status artia_wbp_filter(artia_ctx* ctx, const float* img, int width, int height, size_t pitch,
                               float* out, size_t out_pitch, float psi_deg, int method, float* ms);
 
+/* ---------------------------------------------------------------- projection preparation
+ * PrepareProjection (Reconstructor.cpp:1158-1383, called per projection by EmSart.cpp:480-545)
+ * over a stack of n projections in one call.  Per projection:
+ *   1. convert to float (nppiConvert_*32f), square it with mirror padding (CopyToSquare, y flipped
+ *      first for DM4 stacks), mean of the square;
+ *   2. bad pixels: square values > bad_pixel_value * mean are set to the mean (when
+ *      correct_bad_pixels), and counted;
+ *   3. SART (wbp == 0): unless skip_filter, R2C -> band-pass (fourierFilter) [-> doseWeighting] ->
+ *      C2R; crop / n^2 (n = max(W, H)); then out = (img - m) / -s, with m and the standard
+ *      deviation of that image (of its central W/2 x H/2 for ARTIA_NORM_NONE) and s = std, mean or 1
+ *      for ARTIA_NORM_STD, MEAN, NONE;
+ *   4. WBP (wbp == 1): the normalisation of step 3 first, on the raw cropped image (NONE: m = 0,
+ *      s = 1; s divided by cos(tilt) with down_weight_tilts), mirror-squared again, R2C ->
+ *      [band-pass] [-> doseWeighting] -> wbpWeighting(psi, wbp_method) -> C2R; out = crop / n^2.
+ * images: device, n images of height rows of `pitch` elements of dtype, image i at
+ * i * height * pitch elements.  out: device, n images of height x out_pitch floats.  tilt_deg,
+ * psi_deg, dose (AccumulatedDose, used with dose_weighting; may be NULL otherwise): host arrays of
+ * n.  mean, std (the statistics the reference reports: of the filtered image for SART, of the raw
+ * image for WBP) and bad_pixels: host arrays of n, filled before return (the call synchronises).
+ * Standard deviations are population deviations (division by the pixel count). */
+typedef enum {
+    ARTIA_DT_SHORT = 0,
+    ARTIA_DT_USHORT = 1,
+    ARTIA_DT_INT = 2,
+    ARTIA_DT_UINT = 3,
+    ARTIA_DT_FLOAT = 4
+} artia_dtype;
+typedef enum { ARTIA_NORM_MEAN = 0, ARTIA_NORM_STD = 1, ARTIA_NORM_NONE = 2 } artia_norm;
+typedef struct {
+    int dtype;                 /* artia_dtype */
+    int mirror_y;              /* FRM_DM4 read mode */
+    int correct_bad_pixels;    /* CorrectBadPixels */
+    float bad_pixel_value;     /* BadPixelValue (times the square's mean) */
+    int skip_filter;           /* SkipFilter */
+    float lp, lps, hp, hps;    /* fourFilterLP, fourFilterLPS, fourFilterHP, fourFilterHPS */
+    int dose_weighting;        /* DoseWeighting */
+    float pixelsize_nm;        /* doseWeighting's pixel size (passed in Angstrom, x10) */
+    int wbp;                   /* WBP */
+    int wbp_method;            /* WBPFilter: as artia_wbp_weight */
+    int down_weight_tilts;     /* DownWeightTiltsForWBP */
+    int normalization;         /* artia_norm (ProjectionNormalization) */
+} artia_prep_params;
+artia_status artia_prepare_projections(artia_ctx* ctx, const artia_prep_params* pp, const void* images, int n,
+                                       int width, int height, size_t pitch, const float* tilt_deg,
+                                       const float* psi_deg, const float* dose, float* out, size_t out_pitch,
+                                       float* mean, float* std, int* bad_pixels, float* ms);
+
 /* ---------------------------------------------------------------- adjoint and gradients
  * No reference counterpart (the reference's backprojector is SART's normalised update, not the
  * transpose of its ray marcher); for gradient-based refinement of volumes and shifts.  P is the

@@ -871,6 +871,63 @@ void wbp_weight(T* spec, int N, size_t pitch, float psi_deg, int method) {
     }
 }
 
+template <class T>
+void fourier_filter(T* spec, int N, size_t pitch, float lp, float hp, float lps, float hps) {
+    // the kernel's arithmetic in T (float: the reference's float factor)
+    const T LP = (T)lp - (T)lps, HP = (T)hp + (T)hps, LPS = (T)lps, HPS = (T)hps;
+    const int nx = N / 2 + 1;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < N; y++) {
+        for (int x = 0; x < nx; x++) {
+            T mx = (T)x, my = (T)y;
+            if (my > (T)N * T(0.5)) my = ((T)N - my) * T(-1);
+            const T dist = std::sqrt(mx * mx + my * my);
+            T fil = 0;
+            if (LP > 0) {
+                if (dist <= LP) fil = 1;
+            } else {
+                if (dist <= (T)(N / 2 - 1)) fil = 1;
+            }
+            if (LPS > 0) {
+                const T fil2 = (-fil + T(1)) * std::exp(-((dist - LP) * (dist - LP) / (T(2) * LPS * LPS)));
+                if (fil2 > (T)0.001f) fil = fil2;
+            }
+            if (LPS > 0 && LP == 0 && HP == 0 && HPS == 0)
+                fil = std::exp(-((dist - LP) * (dist - LP) / (T(2) * LPS * LPS)));
+            if (HP > 0) {
+                T fil2 = 0;
+                if (dist >= HP) fil2 = 1;
+                fil *= fil2;
+                if (HPS > 0) {
+                    const T fil3 = (-fil2 + T(1)) * std::exp(-((dist - HP) * (dist - HP) / (T(2) * HPS * HPS)));
+                    if (fil3 > (T)0.001f) fil = fil3;
+                }
+            }
+            T* r = spec + 2 * ((size_t)y * pitch + (size_t)x);
+            r[0] *= fil;
+            r[1] *= fil;
+        }
+    }
+}
+
+template <class T>
+void dose_weighting(T* spec, int N, size_t pitch, float dose, float pixelsize_A) {
+    const int nx = N / 2 + 1;
+    const T scale = (T)(N / 2) / (T)pixelsize_A;   // pixelcount / 2 is an integer division
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < N; y++) {
+        for (int x = 0; x < nx; x++) {
+            T mx = (T)x, my = (T)y;
+            if (my > (T)N * T(0.5)) my = ((T)N - my) * T(-1);
+            const T dist = std::sqrt(mx * mx + my * my) / scale;
+            const T fil = std::exp(-(T)dose * dist);
+            T* r = spec + 2 * ((size_t)y * pitch + (size_t)x);
+            r[0] *= fil;
+            r[1] *= fil;
+        }
+    }
+}
+
 void defocus_distances(const VolGeom& full, const TiltGeom& g, int W, int H, float& t_in, float& t_out) {
     const float* n = g.n;
     const float* d = g.d;
@@ -979,7 +1036,9 @@ void sart(const VolGeom& vg, const VolGeom& full, const std::vector<TiltGeom>& t
 #define ARTIA_ORACLE_INSTANTIATE_SPEC(T)                                                                  \
     template void ctf_apply<T>(T*, int, size_t, const CtfConsts&, float, float, float, bool, bool, float,   \
                                const float*);                                                              \
-    template void wbp_weight<T>(T*, int, size_t, float, int);
+    template void wbp_weight<T>(T*, int, size_t, float, int);                                            \
+    template void fourier_filter<T>(T*, int, size_t, float, float, float, float);                         \
+    template void dose_weighting<T>(T*, int, size_t, float, float);
 
 ARTIA_ORACLE_INSTANTIATE_SPEC(float)
 ARTIA_ORACLE_INSTANTIATE_SPEC(double)

@@ -245,6 +245,16 @@ void ctf_apply(T* spec, int N, size_t pitch, const CtfConsts& k, float defocusMi
 template <class T>
 void wbp_weight(T* spec, int N, size_t pitch, float psi_deg, int method);
 
+// fourierFilter (wbpWeighting.cu:146-212): the band-pass of PrepareProjection on an R2C half
+// spectrum (lp, hp, lps, hps in spectrum pixels; lp -= lps and hp += hps first, as the kernel does).
+template <class T>
+void fourier_filter(T* spec, int N, size_t pitch, float lp, float hp, float lps, float hps);
+
+// doseWeighting (wbpWeighting.cu:214-240): exp(-dose * |k| / (N/2 / pixelsize)), pixelsize in
+// Angstrom (PrepareProjection passes the pixel size in nm times 10).
+template <class T>
+void dose_weighting(T* spec, int N, size_t pitch, float dose, float pixelsize_A);
+
 // Reconstructor::GetDefocusDistances (Reconstructor.cpp:54-173): slab-loop range along n.
 void defocus_distances(const VolGeom& full, const TiltGeom& g, int W, int H, float& t_in, float& t_out);
 // Reconstructor::GetDefocusMinMax (Reconstructor.cpp:178-201).

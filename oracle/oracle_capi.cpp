@@ -108,6 +108,19 @@ void oracle_wbp_weight_f64(double* spec, int N, size_t pitch, float psi, int met
     wbp_weight<double>(spec, N, pitch, psi, method);
 }
 
+void oracle_fourier_filter_f32(float* spec, int N, size_t pitch, float lp, float hp, float lps, float hps) {
+    fourier_filter<float>(spec, N, pitch, lp, hp, lps, hps);
+}
+void oracle_fourier_filter_f64(double* spec, int N, size_t pitch, float lp, float hp, float lps, float hps) {
+    fourier_filter<double>(spec, N, pitch, lp, hp, lps, hps);
+}
+void oracle_dose_weighting_f32(float* spec, int N, size_t pitch, float dose, float pixA) {
+    dose_weighting<float>(spec, N, pitch, dose, pixA);
+}
+void oracle_dose_weighting_f64(double* spec, int N, size_t pitch, float dose, float pixA) {
+    dose_weighting<double>(spec, N, pitch, dose, pixA);
+}
+
 ORACLE_KERNELS(f32, float)
 ORACLE_KERNELS(f64, double)
 

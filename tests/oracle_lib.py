@@ -72,6 +72,9 @@ def load() -> C.CDLL:
         getattr(lib, f"oracle_ctf_apply_{suf}").argtypes = [P, C.c_int, C.c_size_t, P, C.c_float, C.c_float,
                                                              C.c_float, C.c_int, C.c_int, C.c_float, P]
         getattr(lib, f"oracle_wbp_weight_{suf}").argtypes = [P, C.c_int, C.c_size_t, C.c_float, C.c_int]
+        getattr(lib, f"oracle_fourier_filter_{suf}").argtypes = [P, C.c_int, C.c_size_t, C.c_float, C.c_float,
+                                                                  C.c_float, C.c_float]
+        getattr(lib, f"oracle_dose_weighting_{suf}").argtypes = [P, C.c_int, C.c_size_t, C.c_float, C.c_float]
     lib.oracle_set_threads.argtypes = [C.c_int]
     lib.oracle_get_threads.restype = C.c_int
     _lib = lib
@@ -276,6 +279,24 @@ def wbp_weight(spec, psi_deg, method, dtype=np.float64):
     assert spec.shape == (N, N // 2 + 1)
     buf = _interleaved(spec, dtype)
     getattr(load(), f"oracle_wbp_weight_{_suffix(dtype)}")(_p(buf), N, N // 2 + 1, C.c_float(psi_deg), int(method))
+    return buf[..., 0] + 1j * buf[..., 1]
+
+
+def fourier_filter(spec, lp, hp, lps, hps, dtype=np.float64):
+    N = spec.shape[0]
+    assert spec.shape == (N, N // 2 + 1)
+    buf = _interleaved(spec, dtype)
+    getattr(load(), f"oracle_fourier_filter_{_suffix(dtype)}")(_p(buf), N, N // 2 + 1, C.c_float(lp), C.c_float(hp),
+                                                               C.c_float(lps), C.c_float(hps))
+    return buf[..., 0] + 1j * buf[..., 1]
+
+
+def dose_weighting(spec, dose, pixelsize_A, dtype=np.float64):
+    N = spec.shape[0]
+    assert spec.shape == (N, N // 2 + 1)
+    buf = _interleaved(spec, dtype)
+    getattr(load(), f"oracle_dose_weighting_{_suffix(dtype)}")(_p(buf), N, N // 2 + 1, C.c_float(dose),
+                                                               C.c_float(pixelsize_A))
     return buf[..., 0] + 1j * buf[..., 1]
 
 
