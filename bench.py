@@ -74,6 +74,91 @@ REFINE_WORKLOADS = {
 }
 
 
+# Projection preparation workloads (PrepareProjection for every projection of a stack): 16-bit
+# counts with a gradient, noise and hot pixels; EmSART's default band-pass (LP 800, LPS 200 spectrum
+# pixels), bad-pixel correction at 10x the mean, mean normalisation, dose weighting.
+PREP_WORKLOADS = {
+    "prep_61x4096x4096_uint16": dict(n=61, W=4096, H=4096, seed=12, pixelsize_nm=0.25),
+}
+
+
+def bench_prep(args) -> int:
+    import numpy as np
+    import torch
+
+    import artiatomi_amd as A
+
+    w = PREP_WORKLOADS[args.workload]
+    n, W, H = w["n"], w["W"], w["H"]
+    N = max(W, H)
+    gen = torch.Generator(device="cuda").manual_seed(w["seed"])
+    yy = torch.arange(H, device="cuda", dtype=torch.float32)[:, None]
+    xx = torch.arange(W, device="cuda", dtype=torch.float32)[None, :]
+    base = 800.0 + 0.05 * xx - 0.03 * yy
+    imgs = torch.empty((n, H, W), device="cuda", dtype=torch.uint16)
+    for i in range(n):
+        a = base + 25.0 * torch.randn((H, W), generator=gen, device="cuda")
+        hot = torch.randint(0, H * W, (50,), generator=gen, device="cuda")
+        a.view(-1)[hot] = 60000.0
+        imgs[i] = a.clamp(0, 65535).round().to(torch.int32).to(torch.uint16)
+    out = torch.empty((n, H, W), device="cuda")
+    doses = np.linspace(1.0, 60.0, n).astype(np.float32)
+    pp = A.prep_params(dtype="uint16", normalization="mean", dose_weighting=True, pixelsize_nm=w["pixelsize_nm"])
+    ctx = A.ArtiaContext(0)
+    for _ in range(args.warmup):
+        ctx.prepare_projections(pp, imgs, out, dose=doses)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mean, std, bad = ctx.prepare_projections(pp, imgs, out, dose=doses)
+    torch.cuda.synchronize()
+    elapsed = (time.perf_counter() - t0) / args.steps
+    # per projection: 2 B in and 4 B out per pixel, and the square through an R2C/C2R pair
+    # (24 N^2 in the one-pass-per-FFT model)
+    alg = n * (6.0 * W * H + 24.0 * N * N)
+    achieved = alg / elapsed / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle composition (float64 FFT, OpenMP kernels) on one projection, scaled to the stack
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as O
+        import prep_compose as P
+
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        O.set_threads(cores)
+        one = imgs[n // 2].cpu().numpy()
+        t1 = time.perf_counter()
+        P.prepare(one, normalization="mean", dose_weighting=True, pixelsize_nm=w["pixelsize_nm"],
+                  dose=float(doses[n // 2]))
+        t_one = time.perf_counter() - t1
+        cpu = {"value": 1.0 / t_one, "unit": "projections/s", "cores": O.load().oracle_get_threads(),
+               "kind": "port", "sample": f"one {W}x{H} projection through the oracle composition, scaled to the stack"}
+    out_line = {
+        "metric": "projections/s prepared (PrepareProjection)",
+        "value": n / elapsed,
+        "unit": "projections/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (16-bit counts with a gradient, Gaussian noise and 50 hot pixels per projection)",
+        "config": {"workload": args.workload, "projections": n, "projection": [W, H], "input": "uint16",
+                   "band_pass": [800, 200, 0, 0], "bad_pixel_value": 10, "normalization": "mean",
+                   "dose_weighting": True, "parallelism": "single GPU"},
+        "roofline": {"kernel": "preparation pipeline (per stack)", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "bad_pixels_mean": float(np.mean(bad)),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out_line), flush=True)
+    ctx.close()
+    return 0
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of `workload`
     (FETCH_SIZE and WRITE_SIZE passes, calibrated against a copy kernel), or None when the summary
@@ -692,7 +777,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sart_1024x1024x256_61tilts",
                     choices=sorted(WORKLOADS) + sorted(CTF_WORKLOADS) + sorted(SUBVOL_WORKLOADS) +
-                    sorted(REFINE_WORKLOADS))
+                    sorted(REFINE_WORKLOADS) + sorted(PREP_WORKLOADS))
     ap.add_argument("--numerics", default="exact", choices=["exact", "reference"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp-staging", action="store_true",
@@ -726,6 +811,8 @@ def main() -> int:
         return bench_refine(args)
     if args.workload in CTF_WORKLOADS:
         return bench_ctf(args)
+    if args.workload in PREP_WORKLOADS:
+        return bench_prep(args)
 
     import torch
 
