@@ -12,6 +12,8 @@ from .core import (  # noqa: F401
     SubvolPlan,
     ctf_params,
     prep_params,
+    marker_tilt_params,
+    ctf_file_defocus,
     defocus_distances,
     defocus_minmax,
     footprint_window,

@@ -151,6 +151,8 @@ SIGNATURES = {
     "artia_geom_hitpoints": (_I, [C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI]),
     "artia_geom_real_length": (_I, [C.POINTER(VolGeom), C.POINTER(TiltGeom), _PF]),
     "artia_projection_index_list": (_I, [_PF, _I, _PI]),
+    "artia_marker_tilt_params": (_I, [_PF, _I, _I, _I, _I, C.POINTER(TiltParams), _PI]),
+    "artia_ctf_file_defocus": (_I, [_PF, _I, _I, _I, _F, C.POINTER(TiltDefocus)]),
     "artia_geom_defocus_distances": (_I, [C.POINTER(VolGeom), C.POINTER(TiltGeom), _PF, _PF]),
     "artia_geom_defocus_minmax": (_I, [_F, C.POINTER(TiltDefocus), C.POINTER(CtfParams), _PF, _PF]),
     "artia_fp_march": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _SZ, _P, _P, _PI, _PF]),

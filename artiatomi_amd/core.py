@@ -203,6 +203,38 @@ def sart_params(lambda_: float = 0.3, sirt_count: int = 1, oversampling: int = 1
                       _abi.farr(cut_length, 2), _abi.farr(dim_length, 2), proj_scale)
 
 
+def marker_tilt_params(marker_data, proj: int, ref_marker: int = 0):
+    """MarkerFile values of projection proj (tilt, psi, magnification, shifts) at the reference
+    marker, and whether the projection is good (CheckIfProjIndexIsGood).  marker_data: the marker
+    file's float array, shape (markers, projections, items)."""
+    import numpy as np
+
+    a = np.ascontiguousarray(marker_data, np.float32)
+    if a.ndim != 3:
+        raise ArtiaError("marker data must be (markers, projections, items)")
+    tp, good = TiltParams(), C.c_int()
+    lib = _abi.load()
+    st = lib.artia_marker_tilt_params(a.ctypes.data_as(C.POINTER(C.c_float)), a.shape[2], a.shape[1], ref_marker,
+                                      proj, C.byref(tp), C.byref(good))
+    _check(st, what="artia_marker_tilt_params")
+    return tp, bool(good.value)
+
+
+def ctf_file_defocus(ctf_data, index: int, tilt_deg: float) -> TiltDefocus:
+    """CtfFile defocus of projection index: ctf_data is the CTF file's float array, shape (rows,
+    projections)."""
+    import numpy as np
+
+    a = np.ascontiguousarray(ctf_data, np.float32)
+    if a.ndim != 2:
+        raise ArtiaError("CTF data must be (rows, projections)")
+    d = TiltDefocus()
+    st = _abi.load().artia_ctf_file_defocus(a.ctypes.data_as(C.POINTER(C.c_float)), a.shape[1], a.shape[0], index,
+                                            float(tilt_deg), C.byref(d))
+    _check(st, what="artia_ctf_file_defocus")
+    return d
+
+
 def prep_params(dtype: str = "float32", mirror_y: bool = False, correct_bad_pixels: bool = True,
                 bad_pixel_value: float = 10.0, skip_filter: bool = False, lp: float = 800.0, lps: float = 200.0,
                 hp: float = 0.0, hps: float = 0.0, dose_weighting: bool = False, pixelsize_nm: float = 1.0,

@@ -279,6 +279,33 @@ artia_status artia_geom_defocus_minmax(float ray, const artia_tilt_defocus* d, c
 
 // Fisher-Yates with glibc's rand() after srand(0) (additive feedback generator, degree 31,
 // separation 3, 310 warm-up draws), then the smallest |tilt| moved to the front.
+artia_status artia_marker_tilt_params(const float* data, int dim_x, int dim_y, int ref_marker, int proj,
+                                      artia_tilt_params* out, int* good) {
+    if (!data || !out || !good || dim_x < 10 || dim_y <= 0 || ref_marker < 0 || proj < 0 || proj >= dim_y)
+        return ARTIA_ERR_INVALID_ARGUMENT;
+    const float* m = data + (size_t)ref_marker * (size_t)dim_x * (size_t)dim_y + (size_t)proj * (size_t)dim_x;
+    // MarkerFileItem_enum: 0 tilt, 1 x, 2 y, 3 deviation, 4 x shift, 5 y shift, 6/7 projected x/y,
+    // 8 magnification, 9 psi
+    out->tilt_deg = m[0];
+    out->shift_x = m[4];
+    out->shift_y = m[5];
+    out->mag = m[8];
+    out->psi_deg = m[9];
+    *good = (m[1] > -1.0f && m[2] > -1.0f) ? 1 : 0;
+    return ARTIA_OK;
+}
+
+artia_status artia_ctf_file_defocus(const float* data, int dim_x, int dim_y, int index, float tilt_deg,
+                                    artia_tilt_defocus* out) {
+    if (!data || !out || dim_x <= 0 || dim_y < 5 || index < 0 || index >= dim_x) return ARTIA_ERR_INVALID_ARGUMENT;
+    out->defocus_min_nm = data[1 * (size_t)dim_x + (size_t)index];
+    out->defocus_max_nm = data[2 * (size_t)dim_x + (size_t)index];
+    // fdata[4 * DimX + index] * 180.0f / M_PI: the product in float, the quotient in double
+    out->astig_angle_deg = (float)((double)(data[4 * (size_t)dim_x + (size_t)index] * 180.0f) / 3.14159265358979323846);
+    out->tilt_deg = tilt_deg;
+    return ARTIA_OK;
+}
+
 artia_status artia_projection_index_list(const float* tilt_deg, int count, int* out) {
     if (!tilt_deg || !out || count <= 0) return ARTIA_ERR_INVALID_ARGUMENT;
     std::vector<uint32_t> s(344 + (size_t)count);

@@ -168,6 +168,19 @@ artia_status artia_geom_hitpoints(const artia_vol_geom* full, const artia_tilt_g
 artia_status artia_geom_real_length(const artia_vol_geom* full, const artia_tilt_geom* g, float* out);
 /* Projection::CreateProjectionIndexList(PLT_RANDOM_START_ZERO_TILT) (Projection.cpp:561-596) */
 artia_status artia_projection_index_list(const float* tilt_deg, int count, int* out);
+/* MarkerFile::operator() (MarkerFile.cpp:1216-1221): the values of projection proj at marker
+ * ref_marker in a marker file's float data, data[marker * dim_x * dim_y + proj * dim_x + item]
+ * (dim_x items per projection, at least 10; dim_y projections), read as EmSART reads them: tilt
+ * MFI_TiltAngle, psi MFI_RotationPsi, magnification MFI_Magnifiaction, shifts MFI_X_Shift and
+ * MFI_Y_Shift (MarkerFile.h:34-46).  *good = CheckIfProjIndexIsGood (MarkerFile.cpp:143-146):
+ * both coordinates of the reference marker above -1. */
+artia_status artia_marker_tilt_params(const float* data, int dim_x, int dim_y, int ref_marker, int proj,
+                                      artia_tilt_params* out, int* good);
+/* CtfFile (CtfFile.cpp:37-56): projection index's defocus from a CTF file's float data
+ * (dim_x columns, dim_y >= 5 rows): minimum row 1, maximum row 2 (nm), astigmatism angle row 4 in
+ * radians, returned in degrees; tilt_deg is stored as given. */
+artia_status artia_ctf_file_defocus(const float* data, int dim_x, int dim_y, int index, float tilt_deg,
+                                    artia_tilt_defocus* out);
 
 /* Reconstructor::GetDefocusDistances (Reconstructor.cpp:54-173): the range [t_in, t_out) of
  * the CTF slab loop along the tilt normal, in pixels */
