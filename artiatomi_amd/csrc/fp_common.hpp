@@ -210,7 +210,7 @@ struct CellMoments {
 __device__ __forceinline__ CellMoments cell_moments(const float q0[3], const float dq[3], const int cell[3], int k,
                                                     int mi) {
     CellMoments cm;
-    const float kc = (float)k + 0.5f * (float)(mi - 1);
+    const float kc = (float)(2 * k + mi - 1) * 0.5f;   // k + (m - 1) / 2, exact
     cm.a = __builtin_fmaf(kc, dq[0], q0[0]) - (float)cell[0];
     cm.b = __builtin_fmaf(kc, dq[1], q0[1]) - (float)cell[1];
     cm.c = __builtin_fmaf(kc, dq[2], q0[2]) - (float)cell[2];
@@ -219,7 +219,7 @@ __device__ __forceinline__ CellMoments cell_moments(const float q0[3], const flo
     cm.pab = dq[0] * dq[1];
     cm.pac = dq[0] * dq[2];
     cm.pbc = dq[1] * dq[2];
-    cm.q = cm.a * cm.pbc + cm.b * cm.pac + cm.c * cm.pab;
+    cm.q = __builtin_fmaf(cm.a, cm.pbc, __builtin_fmaf(cm.b, cm.pac, cm.c * cm.pab));
     return cm;
 }
 
@@ -274,13 +274,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t volume_rsrc(const float* vol, 
 __device__ __forceinline__ double cell_sum_values(const float val[8], const CellMoments& cm, int mi) {
     const float v000 = val[0], v100 = val[1], v010 = val[2], v110 = val[3];
     const float v001 = val[4], v101 = val[5], v011 = val[6], v111 = val[7];
-    const float c00 = lerpf(v000, v100, cm.a), c10 = lerpf(v010, v110, cm.a);
-    const float c01 = lerpf(v001, v101, cm.a), c11 = lerpf(v011, v111, cm.a);
+    // x differences, shared by the x lerps and the mixed differences
+    const float x00 = v100 - v000, x10 = v110 - v010, x01 = v101 - v001, x11 = v111 - v011;
+    const float c00 = __builtin_fmaf(cm.a, x00, v000), c10 = __builtin_fmaf(cm.a, x10, v010);
+    const float c01 = __builtin_fmaf(cm.a, x01, v001), c11 = __builtin_fmaf(cm.a, x11, v011);
     const float f = lerpf(lerpf(c00, c10, cm.b), lerpf(c01, c11, cm.b), cm.c);
-    const float AB = (v110 - v100) - (v010 - v000), AC = (v101 - v100) - (v001 - v000);
-    const float BC = (v011 - v010) - (v001 - v000);
-    const float ABC = ((v111 - v110) - (v101 - v100)) - ((v011 - v010) - (v001 - v000));
-    const float corr = AB * cm.pab + AC * cm.pac + BC * cm.pbc + ABC * cm.q;
+    const float AB = x10 - x00, AC = x01 - x00, ABC = (x11 - x01) - AB;
+    const float BC = (v011 - v001) - (v010 - v000);
+    const float corr = __builtin_fmaf(ABC, cm.q, __builtin_fmaf(BC, cm.pbc, __builtin_fmaf(AC, cm.pac, AB * cm.pab)));
     (void)mi;
     return (double)__builtin_fmaf(cm.m, f, cm.S2c * corr);
 }
