@@ -141,6 +141,7 @@ SIGNATURES = {
     "artia_get_stream": (_I, [_P, C.POINTER(_P)]),
     "artia_set_numerics": (_I, [_P, _I]),
     "artia_set_fp_staging": (_I, [_P, _I]),
+    "artia_set_stack_limit": (_I, [_P, _SZ]),
     "artia_sync": (_I, [_P]),
     "artia_last_error": (C.c_char_p, [_P]),
     "artia_version": (C.c_char_p, []),

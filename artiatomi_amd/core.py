@@ -297,6 +297,10 @@ class ArtiaContext:
         give bit-identical projections."""
         _check(self.lib.artia_set_fp_staging(self.h, int(bool(enable))), self.h, "artia_set_fp_staging")
 
+    def set_stack_limit(self, nbytes: int) -> None:
+        """Bytes the CTF driver may use for its per-tilt slab stacks (0: the built-in caps)."""
+        _check(self.lib.artia_set_stack_limit(self.h, int(nbytes)), self.h, "artia_set_stack_limit")
+
     def close(self) -> None:
         if getattr(self, "h", None):
             self.lib.artia_destroy(self.h)

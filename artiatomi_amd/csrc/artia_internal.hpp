@@ -46,6 +46,7 @@ struct artia_ctx {
     hipStream_t stream = nullptr;
     int numerics = ARTIA_NUMERICS_EXACT;
     bool march_lds = true;   // exact forward projection through LDS-staged slabs where it applies
+    size_t stack_limit = 0;  // bytes the CTF slab stacks may take (0: the built-in caps)
     std::string last_error;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     // driver workspace (projection-sized scratch images), grown on demand

@@ -215,6 +215,12 @@ artia_status artia_set_fp_staging(artia_ctx* ctx, int enable) {
     return ARTIA_OK;
 }
 
+artia_status artia_set_stack_limit(artia_ctx* ctx, size_t bytes) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    ctx->stack_limit = bytes;
+    return ARTIA_OK;
+}
+
 artia_status artia_sync(artia_ctx* ctx) {
     if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
     return check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");

@@ -143,16 +143,21 @@ artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const 
         max_slabs = std::max(max_slabs, nslab[(size_t)i]);
     }
     // one launch per tilt over a stack of its filtered slab images, while the stack fits 32 GiB
+    // the slab stack (32 GiB at most, or the context's stack limit) and the stacked slicer's sub-ray
+    // sums: every slab of a tilt in one volume pass while they fit 16 GiB (or what the limit leaves),
+    // in passes of fewer slabs otherwise; without room for the stack, one launch per slab
+    const double stack_cap = ctx->stack_limit ? (double)ctx->stack_limit : 34359738368.0;
     const bool stacked = ctx->numerics == ARTIA_NUMERICS_EXACT && max_slabs > 0 &&
-                         (double)max_slabs * (double)npix * 4.0 <= 34359738368.0;
+                         (double)max_slabs * (double)npix * 4.0 <= stack_cap;
     const size_t stack_floats = stacked ? (size_t)max_slabs * npix : 0;
-    // the stacked slicer's sub-ray sums: every slab of a tilt in one volume pass while they fit
-    // 16 GiB, in passes of that many slabs otherwise
     size_t slicer_bytes = 0;
     if (stacked) {
+        const size_t cap = ctx->stack_limit ? ctx->stack_limit - stack_floats * sizeof(float) : ((size_t)16 << 30);
         int pass = max_slabs;
-        while (pass > 1 && slicer_stack_scratch_bytes(width, height, pass) > ((size_t)16 << 30)) pass = (pass + 1) / 2;
-        slicer_bytes = slicer_stack_scratch_bytes(width, height, pass);
+        while (pass > 1 && slicer_stack_scratch_bytes(width, height, pass) > cap) pass = (pass + 1) / 2;
+        // no room for one slab's sums: the slicer goes slab by slab
+        slicer_bytes = slicer_stack_scratch_bytes(width, height, pass) <= cap ? slicer_stack_scratch_bytes(width, height, pass)
+                                                                              : 0;
     }
     const size_t head = (npix * (size_t)(1 + S) + stack_floats + (stacked ? bounds.size() : 0) + 63) & ~(size_t)63;
     artia_status st = ensure_workspace(ctx, head + (slicer_bytes + sizeof(float) - 1) / sizeof(float));

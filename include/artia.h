@@ -144,6 +144,12 @@ artia_status artia_set_numerics(artia_ctx* ctx, int numerics);
  * sets take the setting in force when they are created.  No reference counterpart (a tuning
  * choice). */
 artia_status artia_set_fp_staging(artia_ctx* ctx, int enable);
+/* Bytes the CTF driver (artia_sart_ctf_run) may use for its per-tilt slab stacks: the filtered slab
+ * images of a tilt and the forward slabs' sub-ray sums.  0 (the default) allows 32 GiB of images and
+ * 16 GiB of sums.  Within a smaller limit the forward slabs go in several volume passes, or slab by
+ * slab, and without room for the images every slab is a separate launch, as in the reference's loop;
+ * the results are the same. */
+artia_status artia_set_stack_limit(artia_ctx* ctx, size_t bytes);
 artia_status artia_sync(artia_ctx* ctx);
 const char* artia_last_error(artia_ctx* ctx);
 const char* artia_version(void);

@@ -236,3 +236,40 @@ def test_sart_ctf_equals_per_slab_composition(gpu):
         assert torch.equal(vol, ref)
     finally:
         ctx.close()
+
+
+def test_sart_ctf_same_under_stack_limits(gpu):
+    """Within a smaller stack limit the driver takes its other schedules: the forward slabs in
+    several volume passes, slab by slab, and without room for the slab images one launch per slab.
+    Every schedule gives the same volume bit for bit."""
+    dims, vg, acq, projs, defocus, ctf = ctf_case(seed=7)
+    nx, ny, nz = dims
+    H, W = projs.shape[1:]
+    npix = H * W
+    sp = A.sart_params(lambda_=0.3, oversampling=2, cut_length=(1, 1), dim_length=(2, 2))
+    tdef = [A.TiltDefocus(*d) for d in defocus]
+    step = np.float32(ctf.slice_thickness_nm) / np.float32(ctf.pixelsize_nm)
+    max_slabs = max(len(CC.slab_rays(a, b, step)) for a, b in acq.slab_ranges)
+    stack = max_slabs * npix * 4
+    limits = {
+        "default": 0,
+        "three passes or more": stack + 32 * npix + 32 * npix * max(1, max_slabs // 3),
+        "slab by slab": stack + 64 * npix - 1,
+        "no stack": stack - 1,
+    }
+    ctx = A.ArtiaContext(0)
+    try:
+        d_proj = torch.from_numpy(projs).cuda()
+        vols = {}
+        for name, lim in limits.items():
+            ctx.set_stack_limit(lim)
+            vol = torch.zeros((nz, ny, nx), device="cuda")
+            ctx.sart_ctf_run(vg, acq, tdef, d_proj, vol, sp, ctf, iterations=2)
+            torch.cuda.synchronize()
+            vols[name] = vol
+            print(f"stack limit {name} ({lim} bytes, {max_slabs} slabs): max |vol| {float(vol.abs().max()):.4g}",
+                  flush=True)
+        for name in limits:
+            assert torch.equal(vols[name], vols["default"]), name
+    finally:
+        ctx.close()
