@@ -231,7 +231,9 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const float* __restric
             const int st0 = td.sgn[0], st1 = td.sgn[1] * CX, st2 = td.sgn[2] * CXY;
             const int off[8] = {0, 1, CX, CX + 1, CXY, CXY + 1, CXY + CX, CXY + CX + 1};
             int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
-            while (w.active() && (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
+            // a fast lane enters with its cell in [zs, ze] and moves monotonically in z, so staying
+            // in the slab is one range test
+            while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
                 w.step(
                     [&](const int cell[3], int k, int mi) {
                         float val[8];
@@ -372,7 +374,7 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const float* _
                 const int st0 = td.sgn[0], st1 = td.sgn[1] * CX, st2 = td.sgn[2] * CXY;
                 const int off[8] = {0, 1, CX, CX + 1, CXY, CXY + 1, CXY + CX, CXY + CX + 1};
                 int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
-                while (w.active() && (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
+                while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
                     w.step(
                         [&](const int cell[3], int k, int mi) {
                             float val[8];
