@@ -72,41 +72,69 @@ __global__ void __launch_bounds__(256) ctf_ref_kernel(float2* __restrict__ spec,
 
 // Exact numerics: the CTF factor in double (the phase reaches hundreds of radians at Nyquist, where
 // a float phase carries ~1e-5 absolute error), applied to the float spectrum.
-__device__ __forceinline__ void ctf_exact_at(float2* __restrict__ spec, size_t pitch, int x, int y, const CtfK& k) {
+// The exact CTF factor of one spectrum point in two parts: what does not depend on the defocus
+// (ctf_point), and the factor and its application for one defocus pair (ctf_apply_at); together the
+// double arithmetic of the kernel body, in its order.
+struct CtfPoint {
+    double c2b;      // cos(2 beta)
+    double len;      // spatial frequency
+    double lam4;     // Cs lam^3 len^4, the defocus-free term of the phase
+    double env;      // the envelope, clamped as the direction requires (used when damped)
+    bool damped;     // outside the untouched low-frequency disk, and not phase flipping
+};
+
+__device__ __forceinline__ CtfPoint ctf_point(int x, int y, const CtfK& k) {
+    CtfPoint p;
     double xpos = (double)x, ypos = (double)y;
     if (ypos > (double)k.N * 0.5) ypos = ((double)k.N - ypos) * -1.0;
     const double alpha = (xpos == 0) ? (double)(kPiF * 0.5f) : atan2(ypos, xpos);
     const double beta = alpha - (double)k.angle;
-    const double def0 = k.def0, def1 = k.def1;
-    const double defocus = def0 + (1 - cos(2 * beta)) * (def1 - def0);
+    p.c2b = cos(2 * beta);
     double len = sqrt(xpos * xpos + ypos * ypos);
     len *= (double)k.freq_step;
+    p.len = len;
     const double lam = k.lambda, Cs = k.cs_m;
-    const double m = (double)(kPiF / 2.0f) * (Cs * lam * lam * lam * len * len * len * len - 2 * defocus * lam * len * len);
-    const double n = (double)k.pc * sin(m) + (double)k.ac * cos(m);
-    float2 r = spec[(size_t)y * pitch + x];
+    p.lam4 = Cs * lam * lam * lam * len * len * len * len;
     const double radius = sqrt(xpos * xpos + ypos * ypos);
-    double f = 1.0;
-    if (!k.phaseFlip && radius > (double)k.beta[0]) {
+    p.damped = !k.phaseFlip && radius > (double)k.beta[0];
+    p.env = 0.0;
+    if (p.damped) {
         const double l = len / 100000000.0;
-        double e = exp((-(double)k.beta[1] * l - (double)k.beta[2] * l * l) - (double)k.beta[3] * l * l * l);
+        const double e = exp((-(double)k.beta[1] * l - (double)k.beta[2] * l * l) - (double)k.beta[3] * l * l * l);
+        p.env = k.forFP ? fmax(e, (double)0.01f) : fmax(e, (double)k.wiener);
+    }
+    return p;
+}
+
+__device__ __forceinline__ float2 ctf_apply_at(float2 r, const CtfPoint& p, const CtfK& k, float def0f, float def1f) {
+    const double def0 = def0f, def1 = def1f;
+    const double defocus = def0 + (1 - p.c2b) * (def1 - def0);
+    const double lam = k.lambda, len = p.len;
+    const double m = (double)(kPiF / 2.0f) * (p.lam4 - 2 * defocus * lam * len * len);
+    const double n = (double)k.pc * sin(m) + (double)k.ac * cos(m);
+    double f = 1.0;
+    if (p.damped) {
         if (k.forFP) {
-            e = fmax(e, (double)0.01f);
-            double val = n * e;
+            double val = n * p.env;
             if (fabs(val) < (double)0.0001f && val >= 0) val = (double)0.0001f;
             if (fabs(val) < (double)0.0001f && val < 0) val = -(double)0.0001f;
             f = -val;
         } else {
             const double W = k.wiener;
-            e = fmax(e, W);
-            const double val = n * e;
+            const double val = n * p.env;
             f = -val / (val * val + W);
         }
     }
     if (k.phaseFlip && n >= 0) f = -f;
     r.x = (float)((double)r.x * f);
     r.y = (float)((double)r.y * f);
-    spec[(size_t)y * pitch + x] = r;
+    return r;
+}
+
+__device__ __forceinline__ void ctf_exact_at(float2* __restrict__ spec, size_t pitch, int x, int y, const CtfK& k) {
+    const CtfPoint p = ctf_point(x, y, k);
+    float2* q = spec + (size_t)y * pitch + x;
+    *q = ctf_apply_at(*q, p, k, k.def0, k.def1);
 }
 
 __global__ void __launch_bounds__(256) ctf_exact_kernel(float2* __restrict__ spec, size_t pitch, CtfK k) {
@@ -122,15 +150,15 @@ struct CtfStack {
     float def0[kFftBatchMax], def1[kFftBatchMax];
 };
 
-__global__ void __launch_bounds__(256) ctf_exact_stack_kernel(float2* __restrict__ spec, size_t pitch, CtfStack b) {
+__global__ void __launch_bounds__(256) ctf_exact_stack_kernel(float2* __restrict__ spec, size_t pitch, CtfStack b,
+                                                              int ns) {
     const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
     if (x >= b.k.N / 2 + 1 || y >= b.k.N) return;
-    const int s = (int)blockIdx.z;
-    CtfK k = b.k;
-    k.def0 = b.def0[s];
-    k.def1 = b.def1[s];
-    ctf_exact_at(spec + (size_t)s * pitch * (size_t)b.k.N, pitch, x, y, k);
+    const CtfPoint p = ctf_point(x, y, b.k);   // once for the stack's ns spectra
+    float2* q = spec + (size_t)y * pitch + x;
+    const size_t plane = pitch * (size_t)b.k.N;
+    for (int s = 0; s < ns; s++) q[(size_t)s * plane] = ctf_apply_at(q[(size_t)s * plane], p, b.k, b.def0[s], b.def1[s]);
 }
 
 // wbpWeighting.cu:48-144 (sin/cos of the angle as IEEE functions rather than __sinf/__cosf)
@@ -234,27 +262,39 @@ __global__ void __launch_bounds__(256) finish_sum_kernel(const float* __restrict
     out[i] = acc;
 }
 
-// Zero-filled centred squares of a stack of slab images: square_s = makeSquare(cropBorder^r(src_s))
-// with r = rep0 + rep_step * s crops applied in turn (the repeated in-place crop of one image)
+// Zero-filled centred squares of a stack of ns slab images: square_s = makeSquare(cropBorder^r(src_s))
+// with r = rep0 + rep_step * s crops applied in turn (rep_step 1: the repeated in-place crop of one
+// image, src_stride 0, carried from slab to slab with one multiplication each)
 __global__ void __launch_bounds__(256) slab_square_kernel(const float* __restrict__ src, size_t src_stride, int W,
                                                           int H, size_t pitch, int N, int bx, int by,
                                                           float* __restrict__ sq, CropParams cp, int rep0,
-                                                          int rep_step) {
+                                                          int rep_step, int ns) {
     const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
     if (x >= N || y >= N) return;
-    const int s = (int)blockIdx.z;
+    const size_t plane = (size_t)N * (size_t)N;
+    float* o = sq + (size_t)y * N + x;
     const int xi = x - bx, yi = y - by;
-    float val = 0.0f;
-    if (xi >= 0 && xi < W && yi >= 0 && yi < H) {
-        val = src[(size_t)s * src_stride + (size_t)yi * pitch + xi];
-        const int reps = rep0 + rep_step * s;
-        if (reps > 0) {
-            const float w = crop_weight(xi, yi, W, H, cp);
-            for (int r = 0; r < reps; r++) val = w * val;
+    if (!(xi >= 0 && xi < W && yi >= 0 && yi < H)) {
+        for (int s = 0; s < ns; s++) o[(size_t)s * plane] = 0.0f;
+        return;
+    }
+    const float* in = src + (size_t)yi * pitch + xi;
+    const float w = (rep0 > 0 || rep_step > 0) ? crop_weight(xi, yi, W, H, cp) : 1.0f;
+    if (rep_step == 0) {
+        for (int s = 0; s < ns; s++) {
+            float val = in[(size_t)s * src_stride];
+            for (int r = 0; r < rep0; r++) val = w * val;
+            o[(size_t)s * plane] = val;
+        }
+    } else {
+        float val = in[0];
+        for (int r = 0; r < rep0; r++) val = w * val;
+        for (int s = 0; s < ns; s++) {
+            o[(size_t)s * plane] = val;
+            val = w * val;
         }
     }
-    sq[(size_t)s * (size_t)N * (size_t)N + (size_t)y * N + x] = val;
 }
 
 dim3 grid2d(int w, int h) { return dim3((unsigned)((w + 63) / 64), (unsigned)((h + 3) / 4)); }
@@ -439,9 +479,9 @@ artia_status ctf_filter_slabs(artia_ctx* ctx, const float* src, size_t src_strid
         int b = fb.cap, pi = 0;
         while (b > n - k0) b /= 2;
         for (int t = b; t > 1; t /= 2) pi++;
-        hipLaunchKernelGGL(slab_square_kernel, dim3((unsigned)((N + 63) / 64), (unsigned)((N + 3) / 4), (unsigned)b),
-                           dim3(64, 4), 0, ctx->stream, src + (size_t)k0 * src_stride, src_stride, W, H, pitch, N, bx,
-                           by, fb.square, cp, !crop ? 0 : (forFP ? 1 : k0 + 1), crop && !forFP ? 1 : 0);
+        hipLaunchKernelGGL(slab_square_kernel, dim3((unsigned)((N + 63) / 64), (unsigned)((N + 3) / 4)), dim3(64, 4), 0,
+                           ctx->stream, src + (size_t)k0 * src_stride, src_stride, W, H, pitch, N, bx, by, fb.square,
+                           cp, !crop ? 0 : (forFP ? 1 : k0 + 1), crop && !forFP ? 1 : 0, b);
         if (hipfftSetStream(fb.r2c[pi], ctx->stream) != HIPFFT_SUCCESS ||
             hipfftSetStream(fb.c2r[pi], ctx->stream) != HIPFFT_SUCCESS)
             return fail(ctx, ARTIA_ERR_FFT, "hipfftSetStream failed");
@@ -454,9 +494,8 @@ artia_status ctf_filter_slabs(artia_ctx* ctx, const float* src, size_t src_strid
             cs.def0[s] = ks.def0;
             cs.def1[s] = ks.def1;
         }
-        hipLaunchKernelGGL(ctf_exact_stack_kernel, dim3((unsigned)((N / 2 + 1 + 63) / 64), (unsigned)((N + 3) / 4),
-                                                        (unsigned)b),
-                           dim3(64, 4), 0, ctx->stream, fb.spec, half, cs);
+        hipLaunchKernelGGL(ctf_exact_stack_kernel, dim3((unsigned)((N / 2 + 1 + 63) / 64), (unsigned)((N + 3) / 4)),
+                           dim3(64, 4), 0, ctx->stream, fb.spec, half, cs, b);
         if (hipfftExecC2R(fb.c2r[pi], (hipfftComplex*)fb.spec, fb.square) != HIPFFT_SUCCESS)
             return fail(ctx, ARTIA_ERR_FFT, "hipfftExecC2R (batch %d) failed", b);
         if (forFP)
