@@ -351,7 +351,24 @@ struct SlicerSegs {
         const double t_first = t_in + step;
         if (count == 0 && t_first <= t_last) {
             count = sample_count(t_first, t_last, step);
-            sample_start(v, g, r, t_first, q0);
+            // sample_start with the geometry widened here: hoisted out of the walk, the widened
+            // doubles are held in vector registers and spilled (scalar units have no float math)
+            float gn[3], bb[3], rc[3], dm[3], zs = v.zshift;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                gn[i] = g.n[i];
+                bb[i] = v.bbMin[i];
+                rc[i] = v.rcp[i];
+                dm[i] = v.dim[i];
+                __asm__ volatile("" : "+s"(gn[i]), "+s"(bb[i]), "+s"(rc[i]), "+s"(dm[i]));
+            }
+            __asm__ volatile("" : "+s"(zs));
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const double scale = (double)rc[i] * (double)dm[i];
+                const double p = (r.c[i] + t_first * (double)gn[i]) - (double)bb[i];
+                q0[i] = (float)(p * scale - 0.5 - (i == 2 ? (double)zs : 0.0));
+            }
         }
         return true;
     }
