@@ -378,6 +378,10 @@ def bench_ctf(args) -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "kernels": per_kernel,
     }
+    measured = measured_traffic(args.workload, "slicer_segments_lds_kernel" if dom == "fp" else "bp_stack_kernel")
+    if measured:
+        out["roofline"]["traffic"] = measured.pop("bytes")
+        out["roofline"]["traffic_detail"] = measured
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
