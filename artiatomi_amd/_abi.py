@@ -163,6 +163,12 @@ SIGNATURES = {
     "artia_crop_border": (_I, [_P, _I, _I, _SZ, _P, _PF, _PF, _PI, _PF]),
     "artia_bp": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _P, _I, _I, _SZ, _F, _I, _F, _F, _I, _PF]),
     "artia_make_square": (_I, [_P, _I, _I, _SZ, _P, _I, _P, _I, _I, _I, _I, _PF]),
+    "artia_fp_march_f16": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _SZ, _P, _P, _PI, _PF]),
+    "artia_fp_slicer_f16": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _SZ, _P, _F, _F, _PI,
+                                 _PF]),
+    "artia_bp_f16": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _P, _I, _I, _SZ, _F, _I, _F, _F, _I,
+                          _PF]),
+    "artia_volume_f16_to_f32": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _SZ, _PF]),
     "artia_ctf_apply": (_I, [_P, _P, _I, _SZ, C.POINTER(CtfParams), _F, _F, _F, _I, _PF]),
     "artia_wbp_weight": (_I, [_P, _P, _I, _SZ, _F, _I, _PF]),
     "artia_ctf_slab_filter": (_I, [_P, _P, _I, _I, _SZ, _P, _SZ, _I, _PI, _PF, _PF, C.POINTER(CtfParams), _F, _F,
@@ -195,6 +201,11 @@ SIGNATURES = {
     "artia_sart_ctf_run": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _PF, C.POINTER(TiltDefocus),
                                 _I, _PI, _P, _I, _I, _P, C.POINTER(SartParams), C.POINTER(CtfParams), _I, _PI,
                                 _PF]),
+    "artia_sart_run_f16": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _I, _PI, _P, _I, _I, _P,
+                                C.POINTER(SartParams), _I, _PI, _PF]),
+    "artia_sart_ctf_run_f16": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _PF,
+                                    C.POINTER(TiltDefocus), _I, _PI, _P, _I, _I, _P, C.POINTER(SartParams),
+                                    C.POINTER(CtfParams), _I, _PI, _PF]),
 }
 
 _lib = None

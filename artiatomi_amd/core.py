@@ -272,8 +272,22 @@ def _ptr(t, name: str, numel: Optional[int] = None) -> int:
     return t.data_ptr()
 
 
+def _vol(t, vg: VolGeom, name: str = "vol"):
+    """A volume tensor's pointer and entry-point suffix: contiguous float32 (fp32 storage) or float16
+    (half storage, the reference's FP16Volume) CUDA tensor of nx * ny * nz voxels."""
+    import torch
+
+    if isinstance(t, torch.Tensor) and t.dtype == torch.float16:
+        if not t.is_cuda or not t.is_contiguous() or t.numel() < vg.nx * vg.ny * vg.nz:
+            raise ArtiaError(f"{name} must be a contiguous CUDA tensor of {vg.nx * vg.ny * vg.nz} voxels")
+        return t.data_ptr(), "_f16"
+    return _ptr(t, name, vg.nx * vg.ny * vg.nz), ""
+
+
 class ArtiaContext:
-    """One libartia context on one GPU, bound to torch's current stream of that device."""
+    """One libartia context on one GPU, bound to torch's current stream of that device.  Volume
+    arguments of the projectors, the backprojector and the SART drivers are float32 or float16
+    tensors; float16 selects the half-storage entry points."""
 
     def __init__(self, device: int = 0, numerics: str = "exact"):
         import torch
@@ -327,13 +341,15 @@ class ArtiaContext:
 
     def fp_march(self, vg: VolGeom, tg: TiltGeom, vol, proj, dist, roi=None, timed=False):
         H, W = proj.shape
-        return self._call("artia_fp_march", C.byref(vg), C.byref(tg), _ptr(vol, "vol", vg.nx * vg.ny * vg.nz), W, H,
-                          W, _ptr(proj, "proj"), _ptr(dist, "dist", W * H), self._roi(roi), timed=timed)
+        vp, sfx = _vol(vol, vg)
+        return self._call("artia_fp_march" + sfx, C.byref(vg), C.byref(tg), vp, W, H, W, _ptr(proj, "proj"),
+                          _ptr(dist, "dist", W * H), self._roi(roi), timed=timed)
 
     def fp_slicer(self, vg: VolGeom, tg: TiltGeom, vol, proj, tmin, tmax, roi=None, timed=False):
         H, W = proj.shape
-        return self._call("artia_fp_slicer", C.byref(vg), C.byref(tg), _ptr(vol, "vol", vg.nx * vg.ny * vg.nz), W, H,
-                          W, _ptr(proj, "proj"), float(tmin), float(tmax), self._roi(roi), timed=timed)
+        vp, sfx = _vol(vol, vg)
+        return self._call("artia_fp_slicer" + sfx, C.byref(vg), C.byref(tg), vp, W, H, W, _ptr(proj, "proj"),
+                          float(tmin), float(tmax), self._roi(roi), timed=timed)
 
     def vol_traversal(self, vg: VolGeom, tg: TiltGeom, dist, roi=None, timed=False):
         H, W = dist.shape
@@ -356,9 +372,22 @@ class ArtiaContext:
            dist_max=9999999999999.0, filter="point", timed=False):
         H, W = img.shape
         f = {"point": _abi.ARTIA_FILTER_POINT, "linear": _abi.ARTIA_FILTER_LINEAR}[filter]
-        return self._call("artia_bp", C.byref(vg), C.byref(tg), _ptr(vol, "vol", vg.nx * vg.ny * vg.nz),
-                          _ptr(img, "img"), W, H, W, float(lambda_), int(oversampling), float(dist_min),
-                          float(dist_max), f, timed=timed)
+        vp, sfx = _vol(vol, vg)
+        return self._call("artia_bp" + sfx, C.byref(vg), C.byref(tg), vp, _ptr(img, "img"), W, H, W, float(lambda_),
+                          int(oversampling), float(dist_min), float(dist_max), f, timed=timed)
+
+    def volume_f16_to_f32(self, vol, out, z0=0, count=None, timed=False):
+        """convertVolumeFP16ToFP32: out[k] = -vol[z0 + k] as float32 for k < count (all slices from z0
+        by default); vol (nz, ny, nx) float16, out (count, ny, nx) float32."""
+        import torch
+
+        if not (isinstance(vol, torch.Tensor) and vol.is_cuda and vol.dtype == torch.float16 and vol.is_contiguous()
+                and vol.dim() == 3):
+            raise ArtiaError("vol must be a contiguous (nz, ny, nx) float16 CUDA tensor")
+        nz, ny, nx = vol.shape
+        count = nz - z0 if count is None else int(count)
+        return self._call("artia_volume_f16_to_f32", C.c_void_p(vol.data_ptr()), nx, ny, nz, int(z0), count,
+                          _ptr(out, "out", max(count, 0) * ny * nx), nx, timed=timed)
 
     def make_square(self, img, out, border_x, border_y, mirror_y=False, fill_zero=True, timed=False):
         H, W = img.shape
@@ -466,11 +495,12 @@ class ArtiaContext:
         tg, hp, rl, il = acq.c_arrays()
         empty = C.c_int(1 if volume_is_empty else 0)
         ms = C.c_float() if timed else None
-        st = self.lib.artia_sart_run(self.h, C.byref(vg), tg, hp, rl, n, il,
-                                     _ptr(projections, "projections", n * acq.width * acq.height), acq.width,
-                                     acq.height, _ptr(vol, "vol", vg.nx * vg.ny * vg.nz), C.byref(params),
-                                     int(iterations), C.byref(empty), C.byref(ms) if timed else None)
-        _check(st, self.h, "artia_sart_run")
+        vp, sfx = _vol(vol, vg)
+        st = getattr(self.lib, "artia_sart_run" + sfx)(
+            self.h, C.byref(vg), tg, hp, rl, n, il, _ptr(projections, "projections", n * acq.width * acq.height),
+            acq.width, acq.height, vp, C.byref(params), int(iterations), C.byref(empty),
+            C.byref(ms) if timed else None)
+        _check(st, self.h, "artia_sart_run" + sfx)
         return (ms.value if timed else None), bool(empty.value)
 
     def sart_ctf_run(self, vg: VolGeom, acq: Acquisition, defocus, projections, vol, params: SartParams,
@@ -485,12 +515,12 @@ class ArtiaContext:
         df = (TiltDefocus * n)(*defocus)
         empty = C.c_int(1 if volume_is_empty else 0)
         ms = C.c_float() if timed else None
-        st = self.lib.artia_sart_ctf_run(self.h, C.byref(vg), tg, hp, rl, sr, df, n, il,
-                                         _ptr(projections, "projections", n * acq.width * acq.height), acq.width,
-                                         acq.height, _ptr(vol, "vol", vg.nx * vg.ny * vg.nz), C.byref(params),
-                                         C.byref(ctf), int(iterations), C.byref(empty),
-                                         C.byref(ms) if timed else None)
-        _check(st, self.h, "artia_sart_ctf_run")
+        vp, sfx = _vol(vol, vg)
+        st = getattr(self.lib, "artia_sart_ctf_run" + sfx)(
+            self.h, C.byref(vg), tg, hp, rl, sr, df, n, il,
+            _ptr(projections, "projections", n * acq.width * acq.height), acq.width, acq.height, vp, C.byref(params),
+            C.byref(ctf), int(iterations), C.byref(empty), C.byref(ms) if timed else None)
+        _check(st, self.h, "artia_sart_ctf_run" + sfx)
         return (ms.value if timed else None), bool(empty.value)
 
 

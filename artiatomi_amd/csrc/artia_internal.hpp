@@ -6,6 +6,7 @@
 
 #include <cstdarg>
 #include <cstddef>
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -76,6 +77,22 @@ struct VolG {
     int nx, ny, nz;
 };
 
+// A volume as the launchers take it: dense [z][y][x] fp32 voxels, or IEEE binary16 voxels (half,
+// the reference's FP16Volume storage).  Converts implicitly from a float pointer.
+struct VolData {
+    void* p = nullptr;
+    bool half = false;
+    VolData() = default;
+    VolData(float* f) : p(f) {}
+    VolData(const float* f) : p(const_cast<float*>(f)) {}
+    static VolData f16(const uint16_t* h) {
+        VolData d;
+        d.p = const_cast<uint16_t*>(h);
+        d.half = true;
+        return d;
+    }
+};
+
 TiltG to_device(const artia_tilt_geom& g);
 VolG to_device(const artia_vol_geom& g);
 
@@ -103,7 +120,7 @@ artia_status prof_collect(artia_ctx* ctx);
 bool valid_geometry(const artia_vol_geom* vg, const artia_tilt_geom* tg);
 
 // Kernel launchers used by the drivers (no argument validation, stream = ctx->stream)
-void launch_march(artia_ctx* ctx, const VolG& vg, const TiltG& tg, const float* vol, int W, int H,
+void launch_march(artia_ctx* ctx, const VolG& vg, const TiltG& tg, VolData vol, int W, int H,
                   size_t pitch, float* proj, float* dist, int x0, int y0, int x1, int y1);
 void launch_vol_traversal(artia_ctx* ctx, const VolG& vg, const TiltG& tg, int W, int H, size_t pitch,
                           float* dist, int x0, int y0, int x1, int y1);
@@ -111,21 +128,21 @@ void launch_compare(artia_ctx* ctx, int W, int H, size_t pitch, const float* rea
                     const float* dist, float realLength, const float cut[4], const float dim[4], float scale);
 void launch_crop_border(artia_ctx* ctx, int W, int H, size_t pitch, float* img, const float cut[2],
                         const float dim[2], const int p[8]);
-void launch_bp(artia_ctx* ctx, const VolG& vg, const TiltG& tg, float* vol, const float* img, int W, int H,
+void launch_bp(artia_ctx* ctx, const VolG& vg, const TiltG& tg, VolData vol, const float* img, int W, int H,
                size_t pitch, float lambda, int os, float distMin, float distMax, int filter);
 // exact-numerics BP of one tilt's CTF slabs in one launch: slab k's image at stack + k * stride, slab k
 // covering distances [bounds[k], bounds[k+1]) (the last [bounds[n-1], last_end)); bounds on the device
-void launch_bp_stack(artia_ctx* ctx, const VolG& vg, const TiltG& tg, float* vol, const float* stack, size_t stride,
+void launch_bp_stack(artia_ctx* ctx, const VolG& vg, const TiltG& tg, VolData vol, const float* stack, size_t stride,
                      int W, int H, size_t pitch, float lambda, int os, int filter, const float* bounds, int nslabs,
                      float last_end);
 void launch_fill(artia_ctx* ctx, float* p, size_t n, float value);
-void launch_slicer(artia_ctx* ctx, const VolG& vg, const TiltG& tg, const float* vol, int W, int H, size_t pitch,
+void launch_slicer(artia_ctx* ctx, const VolG& vg, const TiltG& tg, VolData vol, int W, int H, size_t pitch,
                    float* proj, float tmin, float tmax);
 // The slicer over slabs bounds[k] .. bounds[k] + step (k < nslab) into stack + k * stride; bounds
 // on the host and, as d_bounds, on the device.  With scratch of slicer_stack_scratch_bytes(W, H, n)
 // bytes the slabs go in passes of n (one volume pass each), otherwise one slicer launch per slab.
 size_t slicer_stack_scratch_bytes(int W, int H, int nslab);
-void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vol, int W, int H, size_t pitch,
+void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, int W, int H, size_t pitch,
                          float* stack, size_t stride, const float* bounds, const float* d_bounds, int nslab,
                          float step, void* scratch, size_t scratch_bytes);
 void launch_make_square(artia_ctx* ctx, int W, int H, size_t pitch, const float* in, int N, float* out, int bx, int by,

@@ -24,11 +24,34 @@
 namespace artia {
 namespace {
 
+// A quad's four voxels: one 16-byte access of fp32 voxels, one 8-byte access of half voxels (x is
+// a multiple of 4).  Half voxels are read exactly and written rounded to nearest even, as the
+// reference's backProjectionFP16 does (BackProjectionSquareOS.cu:785-1290: the FP32 kernel's
+// arithmetic between __half2float reads and __float2half_rn writes).
+typedef vhalf vhalf4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_quad(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ float4 load_quad(const vhalf* p) {
+    const vhalf4 h = *reinterpret_cast<const vhalf4*>(p);
+    return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
+}
+__device__ __forceinline__ void store_quad(float* p, const float o[4]) {
+    *reinterpret_cast<float4*>(p) = make_float4(o[0], o[1], o[2], o[3]);
+}
+__device__ __forceinline__ void store_quad(vhalf* p, const float o[4]) {
+    vhalf4 h;
+    h.x = (vhalf)o[0];
+    h.y = (vhalf)o[1];
+    h.z = (vhalf)o[2];
+    h.w = (vhalf)o[3];
+    *reinterpret_cast<vhalf4*>(p) = h;
+}
+
 // ---------------------------------------------------------------- reference numerics
 
 __device__ __forceinline__ float bp_src(float r, float n) { return (float)((double)r + 100000.0 * (double)n); }
 
-__global__ void __launch_bounds__(256) bp_ref_kernel(VolG v, TiltG g, float* __restrict__ vol,
+template <typename T>
+__global__ void __launch_bounds__(256) bp_ref_kernel(VolG v, TiltG g, T* __restrict__ vol,
                                                      const float* __restrict__ img, int W, int H, size_t pitch,
                                                      float lambda, int os, float osInv, float distMin, float distMax,
                                                      int filter) {
@@ -87,8 +110,8 @@ __global__ void __launch_bounds__(256) bp_ref_kernel(VolG v, TiltG g, float* __r
     py0 = (int)fminf(fmaxf((float)py0, 0.0f), (float)(H - 1));
     py1 = (int)fminf(fmaxf((float)py1, 0.0f), (float)(H - 1));
 
-    float* vp = vol + ((size_t)z * (size_t)v.ny + (size_t)y) * (size_t)v.nx + x;
-    float4 vb = *reinterpret_cast<float4*>(vp);
+    T* vp = vol + ((size_t)z * (size_t)v.ny + (size_t)y) * (size_t)v.nx + x;
+    float4 vb = load_quad(vp);
     float num[4] = {0, 0, 0, 0}, den[4] = {0, 0, 0, 0};
     const float yLo = v.bbMin[1] + (float)(unsigned)y * v.vs[1], yHi = v.bbMin[1] + (float)(unsigned)(y + 1) * v.vs[1];
     const float zLo = v.bbMin[2] + (float)(unsigned)z * v.vs[2], zHi = v.bbMin[2] + (float)(unsigned)(z + 1) * v.vs[2];
@@ -130,7 +153,7 @@ __global__ void __launch_bounds__(256) bp_ref_kernel(VolG v, TiltG g, float* __r
         den[k] = fmaxf(0.0f, den[k]);
         if (den[k] != 0.0f) out[k] += (lambda * num[k]) / den[k];
     }
-    *reinterpret_cast<float4*>(vp) = make_float4(out[0], out[1], out[2], out[3]);
+    store_quad(vp, out);
 }
 
 // ---------------------------------------------------------------- exact numerics
@@ -139,8 +162,8 @@ __global__ void __launch_bounds__(256) bp_ref_kernel(VolG v, TiltG g, float* __r
 // 4 slices in z.  kOS >= 0 selects the LDS-tile path (POINT sampling, no magnification
 // anisotropy, affine footprint): the block first stages the projection pixels its quads can
 // reach.  kOS < 0 is the global-memory path.
-template <int kOS, int kPM>
-__global__ void __launch_bounds__(256) bp_exact_kernel(VolG v, TiltG g, BPExact e, float* __restrict__ vol,
+template <int kOS, int kPM, typename T>
+__global__ void __launch_bounds__(256) bp_exact_kernel(VolG v, TiltG g, BPExact e, T* __restrict__ vol,
                                                        const float* __restrict__ img, int W, int H, size_t pitch,
                                                        float lambda, int os, double distMin, double distMax,
                                                        int filter) {
@@ -181,8 +204,8 @@ __global__ void __launch_bounds__(256) bp_exact_kernel(VolG v, TiltG g, BPExact 
     const int O = kOS > 0 ? kOS : os;
     const int nI = (px1 - px0) * O, nJ = (py1 - py0) * O;
     if (nI <= 0 || nJ <= 0) return;
-    float* vp = vol + ((size_t)z * (size_t)v.ny + (size_t)y) * (size_t)v.nx + x;
-    const float4 vb = *reinterpret_cast<const float4*>(vp);
+    T* vp = vol + ((size_t)z * (size_t)v.ny + (size_t)y) * (size_t)v.nx + x;
+    const float4 vb = load_quad(vp);
     float out[4] = {vb.x, vb.y, vb.z, vb.w};
     const float osf = (float)O, delta = 1.0f / osf;
     // voxel corner projections relative to the quad footprint origin (small: exact in float to ~1e-6)
@@ -211,7 +234,7 @@ __global__ void __launch_bounds__(256) bp_exact_kernel(VolG v, TiltG g, BPExact 
             any = true;
         }
     }
-    if (any) *reinterpret_cast<float4*>(vp) = make_float4(out[0], out[1], out[2], out[3]);
+    if (any) store_quad(vp, out);
 }
 
 // The CTF backprojection of one tilt in one launch: every voxel takes its slab's filtered image
@@ -220,7 +243,8 @@ __global__ void __launch_bounds__(256) bp_exact_kernel(VolG v, TiltG g, BPExact 
 // update, in the same arithmetic, as in the per-slab launches.  For LINEAR sampling without
 // magnification anisotropy (affine footprint) the block stages its footprint of every slab its
 // quads fall in, when that fits the LDS tile, and fetches from it with tex2d's arithmetic.
-__global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact e, float* __restrict__ vol,
+template <typename T>
+__global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact e, T* __restrict__ vol,
                                                        const float* __restrict__ stack, size_t stride, int W, int H,
                                                        size_t pitch, float lambda, int os, int filter, SlabArgs sl) {
     __shared__ float tile[kTileFloats];
@@ -281,8 +305,8 @@ __global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact 
     // the quad's fetches read pixels max(p0 - 1, 0) .. p1 on each axis
     const bool fromTile = tiled && max(px0 - 1, 0) >= tx0 && px1 <= tx1 && max(py0 - 1, 0) >= ty0 && py1 <= ty1;
     const float* __restrict__ slabTile = tile + (fromTile ? (slab - k0) * plane : 0);
-    float* vp = vol + ((size_t)z * (size_t)v.ny + (size_t)y) * (size_t)v.nx + x;
-    const float4 vb = *reinterpret_cast<const float4*>(vp);
+    T* vp = vol + ((size_t)z * (size_t)v.ny + (size_t)y) * (size_t)v.nx + x;
+    const float4 vb = load_quad(vp);
     float out[4] = {vb.x, vb.y, vb.z, vb.w};
     const float osf = (float)os, delta = 1.0f / osf;
     const float du0 = (float)(e.pu0 + (double)x * e.pui + (double)y * e.puj + (double)z * e.puk - (double)px0);
@@ -309,42 +333,31 @@ __global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact 
             any = true;
         }
     }
-    if (any) *reinterpret_cast<float4*>(vp) = make_float4(out[0], out[1], out[2], out[3]);
+    if (any) store_quad(vp, out);
 }
 
-template <int kOS>
-void launch_tile(artia_ctx* ctx, dim3 grid, const VolG& v, const TiltG& g, const BPExact& e, float* vol,
+template <int kOS, typename T>
+void launch_tile(artia_ctx* ctx, dim3 grid, const VolG& v, const TiltG& g, const BPExact& e, T* vol,
                  const float* img, int W, int H, size_t pitch, float lambda, int os, double dmin, double dmax,
                  int filter) {
     if (e.parallel == 0)
-        hipLaunchKernelGGL((bp_exact_kernel<kOS, 0>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H, pitch,
-                           lambda, os, dmin, dmax, filter);
+        hipLaunchKernelGGL((bp_exact_kernel<kOS, 0, T>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H,
+                           pitch, lambda, os, dmin, dmax, filter);
     else if (e.parallel == 2)
-        hipLaunchKernelGGL((bp_exact_kernel<kOS, 2>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H, pitch,
-                           lambda, os, dmin, dmax, filter);
+        hipLaunchKernelGGL((bp_exact_kernel<kOS, 2, T>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H,
+                           pitch, lambda, os, dmin, dmax, filter);
     else
-        hipLaunchKernelGGL((bp_exact_kernel<kOS, -1>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H,
+        hipLaunchKernelGGL((bp_exact_kernel<kOS, -1, T>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H,
                            pitch, lambda, os, dmin, dmax, filter);
 }
 
-}  // namespace
-
-void launch_bp_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, float* vol, const float* stack, size_t stride,
-                     int W, int H, size_t pitch, float lambda, int os, int filter, const float* bounds, int nslabs,
-                     float last_end) {
-    const BPExact e = prepare_exact(v, g);
-    const int nq = v.nx / 4;
-    const dim3 grid((unsigned)((nq + 15) / 16), (unsigned)((v.ny + 3) / 4), (unsigned)((v.nz + 3) / 4));
-    hipLaunchKernelGGL(bp_stack_kernel, grid, dim3(256), 0, ctx->stream, v, g, e, vol, stack, stride, W, H, pitch,
-                       lambda, os, filter, SlabArgs{bounds, nslabs, last_end});
-}
-
-void launch_bp(artia_ctx* ctx, const VolG& v, const TiltG& g, float* vol, const float* img, int W, int H, size_t pitch,
-               float lambda, int os, float distMin, float distMax, int filter) {
+template <typename T>
+void launch_bp_t(artia_ctx* ctx, const VolG& v, const TiltG& g, T* vol, const float* img, int W, int H, size_t pitch,
+                 float lambda, int os, float distMin, float distMax, int filter) {
     const int nq = v.nx / 4;
     if (ctx->numerics == ARTIA_NUMERICS_REFERENCE) {
         const dim3 block(64, 4), grid((unsigned)((nq + 63) / 64), (unsigned)((v.ny + 3) / 4), (unsigned)v.nz);
-        hipLaunchKernelGGL(bp_ref_kernel, grid, block, 0, ctx->stream, v, g, vol, img, W, H, pitch, lambda, os,
+        hipLaunchKernelGGL((bp_ref_kernel<T>), grid, block, 0, ctx->stream, v, g, vol, img, W, H, pitch, lambda, os,
                            1.0f / (float)os, distMin, distMax, filter);
         return;
     }
@@ -354,7 +367,7 @@ void launch_bp(artia_ctx* ctx, const VolG& v, const TiltG& g, float* vol, const 
                       (long long)(e.tileW + 2) * (long long)(e.tileH + 2) <= kTileFloats;
     const double dmin = distMin, dmax = distMax;
     if (!tile)
-        hipLaunchKernelGGL((bp_exact_kernel<-1, -1>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H,
+        hipLaunchKernelGGL((bp_exact_kernel<-1, -1, T>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, img, W, H,
                            pitch, lambda, os, dmin, dmax, filter);
     else if (os == 1)
         launch_tile<1>(ctx, grid, v, g, e, vol, img, W, H, pitch, lambda, os, dmin, dmax, filter);
@@ -366,30 +379,74 @@ void launch_bp(artia_ctx* ctx, const VolG& v, const TiltG& g, float* vol, const 
         launch_tile<0>(ctx, grid, v, g, e, vol, img, W, H, pitch, lambda, os, dmin, dmax, filter);
 }
 
+}  // namespace
+
+void launch_bp_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, const float* stack, size_t stride,
+                     int W, int H, size_t pitch, float lambda, int os, int filter, const float* bounds, int nslabs,
+                     float last_end) {
+    const BPExact e = prepare_exact(v, g);
+    const int nq = v.nx / 4;
+    const dim3 grid((unsigned)((nq + 15) / 16), (unsigned)((v.ny + 3) / 4), (unsigned)((v.nz + 3) / 4));
+    const SlabArgs sl{bounds, nslabs, last_end};
+    if (vol.half)
+        hipLaunchKernelGGL(bp_stack_kernel<vhalf>, grid, dim3(256), 0, ctx->stream, v, g, e,
+                           static_cast<vhalf*>(vol.p), stack, stride, W, H, pitch, lambda, os, filter, sl);
+    else
+        hipLaunchKernelGGL(bp_stack_kernel<float>, grid, dim3(256), 0, ctx->stream, v, g, e,
+                           static_cast<float*>(vol.p), stack, stride, W, H, pitch, lambda, os, filter, sl);
+}
+
+void launch_bp(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, const float* img, int W, int H,
+               size_t pitch, float lambda, int os, float distMin, float distMax, int filter) {
+    if (vol.half)
+        launch_bp_t(ctx, v, g, static_cast<vhalf*>(vol.p), img, W, H, pitch, lambda, os, distMin, distMax, filter);
+    else
+        launch_bp_t(ctx, v, g, static_cast<float*>(vol.p), img, W, H, pitch, lambda, os, distMin, distMax, filter);
+}
+
 }  // namespace artia
 
 using namespace artia;
+
+namespace {
+artia_status bp_entry(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                      VolData vol, const float* img, int width, int height, size_t pitch, float lambda,
+                      int oversampling, float dist_min, float dist_max, int filter, float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!valid_geometry(vg, tg) || !vol.p || !img)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: null pointer or invalid geometry", fn);
+    if (vg->nx % 4 != 0)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: volume x dimension %d is not a multiple of 4", fn, vg->nx);
+    // a quad is one 16-byte (fp32) or 8-byte (half) access
+    const uintptr_t align = vol.half ? 7 : 15;
+    if ((reinterpret_cast<uintptr_t>(vol.p) & align) != 0)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: volume pointer must be %d-byte aligned", fn,
+                    (int)align + 1);
+    if (width <= 0 || height <= 0 || pitch < (size_t)width || oversampling <= 0 || oversampling > 64)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: bad image size or oversampling", fn);
+    if (filter != ARTIA_FILTER_POINT && filter != ARTIA_FILTER_LINEAR)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: unknown filter %d", fn, filter);
+    Timing t(ctx, ms);
+    launch_bp(ctx, to_device(*vg), to_device(*tg), vol, img, width, height, pitch, lambda, oversampling, dist_min,
+              dist_max, filter);
+    return t.finish();
+}
+}  // namespace
 
 extern "C" {
 
 artia_status artia_bp(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, float* vol,
                       const float* img, int width, int height, size_t pitch, float lambda, int oversampling,
                       float dist_min, float dist_max, int filter, float* ms) {
-    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
-    if (!valid_geometry(vg, tg) || !vol || !img)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_bp: null pointer or invalid geometry");
-    if (vg->nx % 4 != 0)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_bp: volume x dimension %d is not a multiple of 4", vg->nx);
-    if ((reinterpret_cast<uintptr_t>(vol) & 15) != 0)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_bp: volume pointer must be 16-byte aligned");
-    if (width <= 0 || height <= 0 || pitch < (size_t)width || oversampling <= 0 || oversampling > 64)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_bp: bad image size or oversampling");
-    if (filter != ARTIA_FILTER_POINT && filter != ARTIA_FILTER_LINEAR)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_bp: unknown filter %d", filter);
-    Timing t(ctx, ms);
-    launch_bp(ctx, to_device(*vg), to_device(*tg), vol, img, width, height, pitch, lambda, oversampling, dist_min,
-              dist_max, filter);
-    return t.finish();
+    return bp_entry(ctx, "artia_bp", vg, tg, VolData(vol), img, width, height, pitch, lambda, oversampling, dist_min,
+                    dist_max, filter, ms);
+}
+
+artia_status artia_bp_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, uint16_t* vol,
+                          const float* img, int width, int height, size_t pitch, float lambda, int oversampling,
+                          float dist_min, float dist_max, int filter, float* ms) {
+    return bp_entry(ctx, "artia_bp_f16", vg, tg, VolData::f16(vol), img, width, height, pitch, lambda, oversampling,
+                    dist_min, dist_max, filter, ms);
 }
 
 }  // extern "C"

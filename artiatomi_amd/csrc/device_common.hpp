@@ -14,8 +14,17 @@ namespace artia {
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 __device__ __forceinline__ float lerpf(float p, float q, float w) { return __builtin_fmaf(w, q - p, p); }
 
-// Linear-filtered, clamp-addressed read of a dense [z][y][x] fp32 volume.
-__device__ __forceinline__ float tex3d_linear(const float* __restrict__ vol, int nx, int ny, int nz, float x,
+// Volume element types: fp32, or IEEE binary16, the reference's FP16Volume storage (a half-float
+// CUDA array behind Volume<unsigned short>, EmSart.cpp:354-374, BackProjectionSquareOS.cu:785-1290).
+// A voxel is read as float, exactly; a half voxel is written back rounded to nearest even
+// (__float2half_rn).
+typedef _Float16 vhalf;
+__device__ __forceinline__ float vox(const float* p, size_t i) { return p[i]; }
+__device__ __forceinline__ float vox(const vhalf* p, size_t i) { return (float)p[i]; }
+
+// Linear-filtered, clamp-addressed read of a dense [z][y][x] volume of fp32 or half voxels.
+template <typename T>
+__device__ __forceinline__ float tex3d_linear(const T* __restrict__ vol, int nx, int ny, int nz, float x,
                                               float y, float z) {
     x = fminf(fmaxf(x, -1.0f), (float)(nx + 1));
     y = fminf(fmaxf(y, -1.0f), (float)(ny + 1));
@@ -29,14 +38,14 @@ __device__ __forceinline__ float tex3d_linear(const float* __restrict__ vol, int
     j0 = clampi(j0, 0, ny - 1);
     k0 = clampi(k0, 0, nz - 1);
     const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
-    const float* p00 = vol + (size_t)k0 * sz + (size_t)j0 * sy;
-    const float* p10 = vol + (size_t)k0 * sz + (size_t)j1 * sy;
-    const float* p01 = vol + (size_t)k1 * sz + (size_t)j0 * sy;
-    const float* p11 = vol + (size_t)k1 * sz + (size_t)j1 * sy;
-    const float c00 = lerpf(p00[i0], p00[i1], a);
-    const float c10 = lerpf(p10[i0], p10[i1], a);
-    const float c01 = lerpf(p01[i0], p01[i1], a);
-    const float c11 = lerpf(p11[i0], p11[i1], a);
+    const T* p00 = vol + (size_t)k0 * sz + (size_t)j0 * sy;
+    const T* p10 = vol + (size_t)k0 * sz + (size_t)j1 * sy;
+    const T* p01 = vol + (size_t)k1 * sz + (size_t)j0 * sy;
+    const T* p11 = vol + (size_t)k1 * sz + (size_t)j1 * sy;
+    const float c00 = lerpf(vox(p00, i0), vox(p00, i1), a);
+    const float c10 = lerpf(vox(p10, i0), vox(p10, i1), a);
+    const float c01 = lerpf(vox(p01, i0), vox(p01, i1), a);
+    const float c11 = lerpf(vox(p11, i0), vox(p11, i1), a);
     return lerpf(lerpf(c00, c10, b), lerpf(c01, c11, b), c);
 }
 

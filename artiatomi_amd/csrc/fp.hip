@@ -50,8 +50,8 @@ __device__ __forceinline__ void slab_test(const VolG& v, const TiltG& g, const f
 
 // ---------------------------------------------------------------- reference numerics
 
-template <bool kSlicer>
-__global__ void __launch_bounds__(256) march_ref_kernel(VolG v, TiltG g, const float* __restrict__ vol, Roi roi,
+template <bool kSlicer, typename T>
+__global__ void __launch_bounds__(256) march_ref_kernel(VolG v, TiltG g, const T* __restrict__ vol, Roi roi,
                                                         size_t pitch, float* __restrict__ proj,
                                                         float* __restrict__ dist, float tminDef, float tmaxDef) {
     const int x = roi.x0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -113,8 +113,8 @@ __global__ void __launch_bounds__(256) vol_traversal_ref_kernel(VolG v, TiltG g,
 
 // Sum of the trilinear samples q0 + k*dq, k = 0..count-1 (coordinates shifted by -0.5, texel j
 // centred at j), cell by cell in closed form (cell_sum, fp_common.hpp).
-template <bool kBig>
-__device__ __forceinline__ double march_sum(const VolG& v, const float* __restrict__ vol, const float q0[3],
+template <bool kBig, typename T>
+__device__ __forceinline__ double march_sum(const VolG& v, const T* __restrict__ vol, const float q0[3],
                                             const float dq[3], int count) {
     double total = 0.0;
     if (kBig) {
@@ -123,18 +123,17 @@ __device__ __forceinline__ double march_sum(const VolG& v, const float* __restri
             cell_corners<true>(v, cell, o);
             float val[8];
 #pragma unroll
-            for (int i = 0; i < 8; i++) val[i] = vol[o[i]];
+            for (int i = 0; i < 8; i++) val[i] = vox(vol, o[i]);
             total += cell_sum_values(val, cell_moments(q0, dq, cell, k, mi), mi);
         });
     } else {
         const __amdgpu_buffer_rsrc_t rs = volume_rsrc(vol, v);
         for_each_cell(q0, dq, count, [&](const int cell[3], int k, int mi) {
             uint32_t o[8];
-            cell_corners_bytes(v, cell, o);
+            cell_corners_bytes<(uint32_t)sizeof(T)>(v, cell, o);
             float val[8];
 #pragma unroll
-            for (int i = 0; i < 8; i++)   // the b32 load returns the word's bits
-                val[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)o[i], 0, 0));
+            for (int i = 0; i < 8; i++) val[i] = buffer_vox(rs, o[i], vol);
             total += cell_sum_values(val, cell_moments(q0, dq, cell, k, mi), mi);
         });
     }
@@ -151,7 +150,8 @@ struct MarchLane {
 };
 
 // The slicer's first sample, at the detector point c of the ray (ForwardProjectionSlicer.cu:129-133)
-__device__ __forceinline__ float detector_sample(const VolG& v, const float* __restrict__ vol, const RayD& r) {
+template <typename T>
+__device__ __forceinline__ float detector_sample(const VolG& v, const T* __restrict__ vol, const RayD& r) {
     return tex3d_linear(vol, v.nx, v.ny, v.nz,
                         (float)((r.c[0] - (double)v.bbMin[0]) * ((double)v.rcp[0] * (double)v.dim[0])),
                         (float)((r.c[1] - (double)v.bbMin[1]) * ((double)v.rcp[1] * (double)v.dim[1])),
@@ -176,8 +176,8 @@ __device__ __forceinline__ void sample_start(const VolG& v, const TiltG& g, cons
     }
 }
 
-template <bool kSlicer>
-__device__ __forceinline__ MarchLane march_lane(const VolG& v, const TiltG& g, const float* __restrict__ vol,
+template <bool kSlicer, typename T>
+__device__ __forceinline__ MarchLane march_lane(const VolG& v, const TiltG& g, const T* __restrict__ vol,
                                                 bool active, int x, int y, int sub, double step, float tminDef,
                                                 float tmaxDef) {
     MarchLane L;
@@ -245,8 +245,8 @@ __device__ __forceinline__ void march_store(double acc, double len, double hits,
 
 // One lane per sub-ray: a wave covers 4x4 pixels x 4 sub-rays, a 256-thread block 8x8 pixels,
 // so the rays of a wave stay within a few voxels of each other through the whole volume.
-template <bool kSlicer, bool kBig>
-__global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const float* __restrict__ vol, Roi roi,
+template <bool kSlicer, bool kBig, typename T>
+__global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const T* __restrict__ vol, Roi roi,
                                                           size_t pitch, float* __restrict__ proj,
                                                           float* __restrict__ dist, float tminDef, float tmaxDef) {
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
@@ -279,8 +279,8 @@ __global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const
 // paused at slab boundaries), so the projection is bit-identical.  Six waves per SIMD (80
 // registers): the walk is VALU bound, and the occupancy hides the LDS latency better than the few
 // registers it spills.
-template <bool kSlicer>
-__global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, const float* __restrict__ vol, Roi roi,
+template <bool kSlicer, typename T>
+__global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, const T* __restrict__ vol, Roi roi,
                                                         size_t pitch, float* __restrict__ proj,
                                                         float* __restrict__ dist, float tminDef, float tmaxDef,
                                                         TubePlan tp) {
@@ -374,7 +374,8 @@ struct SlicerSegs {
     }
 };
 
-__global__ void __launch_bounds__(256, 6) slicer_segments_lds_kernel(VolG v, TiltG g, const float* __restrict__ vol,
+template <typename T>
+__global__ void __launch_bounds__(256, 6) slicer_segments_lds_kernel(VolG v, TiltG g, const T* __restrict__ vol,
                                                                   int W, int H, const float* __restrict__ bounds,
                                                                   int kbeg, int kend, float step_f,
                                                                   double* __restrict__ part, size_t pstride,
@@ -504,17 +505,18 @@ dim3 grid_for(const Roi& r) {
     return dim3((unsigned)((w + 63) / 64), (unsigned)((h + 3) / 4));
 }
 
-void launch_march_impl(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vol, const Roi& r, size_t pitch,
-                       float* proj, float* dist, bool slicer, float tmin, float tmax) {
+template <typename T>
+void launch_march_t(artia_ctx* ctx, const VolG& v, const TiltG& g, const T* vol, const Roi& r, size_t pitch,
+                    float* proj, float* dist, bool slicer, float tmin, float tmax) {
     if (r.x1 <= r.x0 || r.y1 <= r.y0) return;
     if (ctx->numerics == ARTIA_NUMERICS_REFERENCE) {
         const dim3 grid = grid_for(r), block(64, 4);
         if (slicer)
-            hipLaunchKernelGGL(march_ref_kernel<true>, grid, block, 0, ctx->stream, v, g, vol, r, pitch, proj, dist,
-                               tmin, tmax);
+            hipLaunchKernelGGL((march_ref_kernel<true, T>), grid, block, 0, ctx->stream, v, g, vol, r, pitch, proj,
+                               dist, tmin, tmax);
         else
-            hipLaunchKernelGGL(march_ref_kernel<false>, grid, block, 0, ctx->stream, v, g, vol, r, pitch, proj, dist,
-                               tmin, tmax);
+            hipLaunchKernelGGL((march_ref_kernel<false, T>), grid, block, 0, ctx->stream, v, g, vol, r, pitch, proj,
+                               dist, tmin, tmax);
         return;
     }
     const dim3 grid((unsigned)((r.x1 - r.x0 + 7) / 8), (unsigned)((r.y1 - r.y0 + 7) / 8)), block(256);
@@ -529,39 +531,47 @@ void launch_march_impl(artia_ctx* ctx, const VolG& v, const TiltG& g, const floa
         const int groups = tp.groups_x * ((tp.tiles_y + kXcdGroup - 1) / kXcdGroup);
         const dim3 grid1((unsigned)(((groups + 7) / 8) * 8 * kXcdGroup * kXcdGroup));
         if (slicer)
-            hipLaunchKernelGGL((march_lds_kernel<true>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch, proj,
+            hipLaunchKernelGGL((march_lds_kernel<true, T>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch, proj,
                                dist, tmin, tmax, tp);
         else
-            hipLaunchKernelGGL((march_lds_kernel<false>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch, proj,
+            hipLaunchKernelGGL((march_lds_kernel<false, T>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch, proj,
                                dist, tmin, tmax, tp);
         return;
     }
     if (slicer) {
         if (big)
-            hipLaunchKernelGGL((march_exact_kernel<true, true>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+            hipLaunchKernelGGL((march_exact_kernel<true, true, T>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
                                proj, dist, tmin, tmax);
         else
-            hipLaunchKernelGGL((march_exact_kernel<true, false>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+            hipLaunchKernelGGL((march_exact_kernel<true, false, T>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
                                proj, dist, tmin, tmax);
     } else {
         if (big)
-            hipLaunchKernelGGL((march_exact_kernel<false, true>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
+            hipLaunchKernelGGL((march_exact_kernel<false, true, T>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
                                proj, dist, tmin, tmax);
         else
-            hipLaunchKernelGGL((march_exact_kernel<false, false>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
-                               proj, dist, tmin, tmax);
+            hipLaunchKernelGGL((march_exact_kernel<false, false, T>), grid, block, 0, ctx->stream, v, g, vol, r,
+                               pitch, proj, dist, tmin, tmax);
     }
+}
+
+void launch_march_impl(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, const Roi& r, size_t pitch,
+                       float* proj, float* dist, bool slicer, float tmin, float tmax) {
+    if (vol.half)
+        launch_march_t(ctx, v, g, static_cast<const vhalf*>(vol.p), r, pitch, proj, dist, slicer, tmin, tmax);
+    else
+        launch_march_t(ctx, v, g, static_cast<const float*>(vol.p), r, pitch, proj, dist, slicer, tmin, tmax);
 }
 
 }  // namespace
 
-void launch_march(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vol, int W, int H, size_t pitch,
+void launch_march(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, int W, int H, size_t pitch,
                   float* proj, float* dist, int x0, int y0, int x1, int y1) {
     const int roi[4] = {x0, y0, x1, y1};
     launch_march_impl(ctx, v, g, vol, make_roi(roi, W, H), pitch, proj, dist, false, 0.0f, 0.0f);
 }
 
-void launch_slicer(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vol, int W, int H, size_t pitch,
+void launch_slicer(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, int W, int H, size_t pitch,
                    float* proj, float tmin, float tmax) {
     const int roi[4] = {0, 0, W, H};
     launch_march_impl(ctx, v, g, vol, make_roi(roi, W, H), pitch, proj, nullptr, true, tmin, tmax);
@@ -572,7 +582,7 @@ size_t slicer_stack_scratch_bytes(int W, int H, int nslab) {
     return npix * 4 * sizeof(int2) + (size_t)nslab * npix * 4 * sizeof(double);
 }
 
-void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, const float* vol, int W, int H, size_t pitch,
+void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, int W, int H, size_t pitch,
                          float* stack, size_t stride, const float* bounds, const float* d_bounds, int nslab,
                          float step, void* scratch, size_t scratch_bytes) {
     if (nslab <= 0) return;
@@ -595,8 +605,14 @@ void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, const fl
         const double dstep = (double)v.vs[0] * 0.15;
         for (int kb = 0; kb < nslab; kb += per_pass) {
             const int ke = std::min(nslab, kb + per_pass);
-            hipLaunchKernelGGL(slicer_segments_lds_kernel, grid1, dim3(256), 0, ctx->stream, v, g, vol, W, H,
-                               d_bounds, kb, ke, step, part, npix * 4, range, tp);
+            if (vol.half)
+                hipLaunchKernelGGL(slicer_segments_lds_kernel<vhalf>, grid1, dim3(256), 0, ctx->stream, v, g,
+                                   static_cast<const vhalf*>(vol.p), W, H, d_bounds, kb, ke, step, part, npix * 4,
+                                   range, tp);
+            else
+                hipLaunchKernelGGL(slicer_segments_lds_kernel<float>, grid1, dim3(256), 0, ctx->stream, v, g,
+                                   static_cast<const float*>(vol.p), W, H, d_bounds, kb, ke, step, part, npix * 4,
+                                   range, tp);
             hipLaunchKernelGGL(slicer_combine_kernel, dim3((unsigned)((W + 255) / 256), (unsigned)H, (unsigned)(ke - kb)),
                                dim3(256), 0, ctx->stream, part, npix * 4, range, W, H, pitch, stack, stride, kb, dstep);
         }
@@ -632,35 +648,50 @@ artia_status check_image(artia_ctx* ctx, const char* fn, int W, int H, size_t pi
         return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: bad image size %dx%d pitch %zu", fn, W, H, pitch);
     return ARTIA_OK;
 }
+
+// artia_fp_march (slicer false: proj and dist) and artia_fp_slicer (proj only)
+artia_status fp_entry(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                      VolData vol, int width, int height, size_t pitch, float* proj, float* dist, const int* roi,
+                      bool slicer, float tmin, float tmax, float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!valid_geometry(vg, tg) || !vol.p || !proj || (!slicer && !dist))
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: null pointer or invalid geometry", fn);
+    artia_status st = check_image(ctx, fn, width, height, pitch);
+    if (st != ARTIA_OK) return st;
+    Timing t(ctx, ms);
+    launch_march_impl(ctx, to_device(*vg), to_device(*tg), vol, make_roi(roi, width, height), pitch, proj, dist,
+                      slicer, tmin, tmax);
+    return t.finish();
+}
 }  // namespace
 
 extern "C" {
 
 artia_status artia_fp_march(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, const float* vol,
                             int width, int height, size_t pitch, float* proj, float* dist, const int* roi, float* ms) {
-    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
-    if (!valid_geometry(vg, tg) || !vol || !proj || !dist)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_fp_march: null pointer or invalid geometry");
-    artia_status st = check_image(ctx, "artia_fp_march", width, height, pitch);
-    if (st != ARTIA_OK) return st;
-    Timing t(ctx, ms);
-    launch_march_impl(ctx, to_device(*vg), to_device(*tg), vol, make_roi(roi, width, height), pitch, proj, dist,
-                      false, 0.0f, 0.0f);
-    return t.finish();
+    return fp_entry(ctx, "artia_fp_march", vg, tg, VolData(vol), width, height, pitch, proj, dist, roi, false, 0.0f,
+                    0.0f, ms);
+}
+
+artia_status artia_fp_march_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                                const uint16_t* vol, int width, int height, size_t pitch, float* proj, float* dist,
+                                const int* roi, float* ms) {
+    return fp_entry(ctx, "artia_fp_march_f16", vg, tg, VolData::f16(vol), width, height, pitch, proj, dist, roi,
+                    false, 0.0f, 0.0f, ms);
 }
 
 artia_status artia_fp_slicer(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, const float* vol,
                              int width, int height, size_t pitch, float* proj, float tmin, float tmax, const int* roi,
                              float* ms) {
-    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
-    if (!valid_geometry(vg, tg) || !vol || !proj)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_fp_slicer: null pointer or invalid geometry");
-    artia_status st = check_image(ctx, "artia_fp_slicer", width, height, pitch);
-    if (st != ARTIA_OK) return st;
-    Timing t(ctx, ms);
-    launch_march_impl(ctx, to_device(*vg), to_device(*tg), vol, make_roi(roi, width, height), pitch, proj, nullptr,
-                      true, tmin, tmax);
-    return t.finish();
+    return fp_entry(ctx, "artia_fp_slicer", vg, tg, VolData(vol), width, height, pitch, proj, nullptr, roi, true,
+                    tmin, tmax, ms);
+}
+
+artia_status artia_fp_slicer_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                                 const uint16_t* vol, int width, int height, size_t pitch, float* proj, float tmin,
+                                 float tmax, const int* roi, float* ms) {
+    return fp_entry(ctx, "artia_fp_slicer_f16", vg, tg, VolData::f16(vol), width, height, pitch, proj, nullptr, roi,
+                    true, tmin, tmax, ms);
 }
 
 artia_status artia_vol_traversal(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, int width,

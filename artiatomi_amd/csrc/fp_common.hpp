@@ -244,14 +244,15 @@ __device__ __forceinline__ void cell_corners(const VolG& v, const int cell[3], s
 
 // The same offsets in 32 bits, in bytes, for volumes below 2^30 voxels with ny * nz < 2^24 (buffer
 // loads; the row index z * ny + y then fits the 24-bit multiplier).
+template <uint32_t kElem = 4>
 __device__ __forceinline__ void cell_corners_bytes(const VolG& v, const int cell[3], uint32_t off[8]) {
     const int x0 = clampi(cell[0], 0, v.nx - 1), y0 = clampi(cell[1], 0, v.ny - 1), z0 = clampi(cell[2], 0, v.nz - 1);
-    const uint32_t sy = (uint32_t)v.nx * 4u, sz = (uint32_t)v.nx * (uint32_t)v.ny * 4u;
-    const uint32_t dx = (cell[0] + 1 > x0 && x0 + 1 < v.nx) ? 4u : 0u;
+    const uint32_t sy = (uint32_t)v.nx * kElem, sz = (uint32_t)v.nx * (uint32_t)v.ny * kElem;
+    const uint32_t dx = (cell[0] + 1 > x0 && x0 + 1 < v.nx) ? kElem : 0u;
     const uint32_t dy = (cell[1] + 1 > y0 && y0 + 1 < v.ny) ? sy : 0u;
     const uint32_t dz = (cell[2] + 1 > z0 && z0 + 1 < v.nz) ? sz : 0u;
     const uint32_t row = __umul24((uint32_t)z0, (uint32_t)v.ny) + (uint32_t)y0;
-    off[0] = (__umul24(row, (uint32_t)v.nx) + (uint32_t)x0) * 4u;
+    off[0] = (__umul24(row, (uint32_t)v.nx) + (uint32_t)x0) * kElem;
     off[1] = off[0] + dx;
     off[2] = off[0] + dy;
     off[3] = off[2] + dx;
@@ -263,9 +264,19 @@ __device__ __forceinline__ void cell_corners_bytes(const VolG& v, const int cell
 
 // Buffer resource of a volume for 32-bit byte offsets (raw buffer, gfx9 descriptor word 3); loads
 // beyond the volume's bytes return 0.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t volume_rsrc(const float* vol, const VolG& v) {
-    const uint32_t bytes = (uint32_t)v.nx * (uint32_t)v.ny * (uint32_t)v.nz * 4u;
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t volume_rsrc(const T* vol, const VolG& v) {
+    const uint32_t bytes = (uint32_t)v.nx * (uint32_t)v.ny * (uint32_t)v.nz * (uint32_t)sizeof(T);
     return __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, (int)bytes, 0x00020000);
+}
+
+// One voxel through a buffer resource at a byte offset: the b32 load returns the word's bits, the
+// b16 load a half's bits.
+__device__ __forceinline__ float buffer_vox(const __amdgpu_buffer_rsrc_t rs, uint32_t off, const float*) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0));
+}
+__device__ __forceinline__ float buffer_vox(const __amdgpu_buffer_rsrc_t rs, uint32_t off, const vhalf*) {
+    return (float)__builtin_bit_cast(vhalf, __builtin_amdgcn_raw_buffer_load_b16(rs, (int)off, 0, 0));
 }
 
 // Sum of one cell's m samples, m f(centre) + S2c corr, rounded once to float (relative error

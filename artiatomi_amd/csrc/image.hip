@@ -1,5 +1,5 @@
 // Per-pixel image operators on gfx950: residual (compare), projected-volume border crop
-// (cropBorder) and pad-to-square (makeSquare).  These are bandwidth-trivial; one thread per
+// (cropBorder), pad-to-square (makeSquare) and the half-volume slice export (convertVolume).  These are bandwidth-trivial; one thread per
 // pixel, 64x4 blocks so a wave covers 64 consecutive pixels of a row.
 #include "device_common.hpp"
 #include "image_ops.hpp"
@@ -79,6 +79,18 @@ __global__ void __launch_bounds__(256) make_square_kernel(int W, int H, size_t p
     out[(size_t)y * N + x] = val;
 }
 
+// convertVolumeFP16ToFP32 (BackProjectionSquareOS.cu:1294-1323): slices z0 .. z0 + gridDim.z - 1
+// of a half volume as negated floats (the sign EmSART writes volumes with), slice z0 + k at
+// out + k * ny * out_pitch.
+__global__ void __launch_bounds__(256) volume_f16_to_f32_kernel(const vhalf* __restrict__ vol, int nx, int ny, int z0,
+                                                                float* __restrict__ out, size_t out_pitch) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)blockIdx.y, k = (int)blockIdx.z;
+    if (x >= nx) return;
+    const size_t src = ((size_t)(z0 + k) * (size_t)ny + (size_t)y) * (size_t)nx + (size_t)x;
+    out[((size_t)k * (size_t)ny + (size_t)y) * out_pitch + (size_t)x] = -(float)vol[src];
+}
+
 dim3 grid2d(int w, int h) { return dim3((unsigned)((w + 63) / 64), (unsigned)((h + 3) / 4)); }
 
 }  // namespace
@@ -136,6 +148,21 @@ artia_status artia_make_square(artia_ctx* ctx, int width, int height, size_t pit
         return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_make_square: invalid arguments");
     Timing t(ctx, ms);
     launch_make_square(ctx, width, height, pitch, in, n, out, border_x, border_y, mirror_y, fill_zero);
+    return t.finish();
+}
+
+artia_status artia_volume_f16_to_f32(artia_ctx* ctx, const uint16_t* vol, int nx, int ny, int nz, int z0, int count,
+                                     float* out, size_t out_pitch, float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!vol || !out || nx <= 0 || ny <= 0 || nz <= 0 || out_pitch < (size_t)nx || z0 < 0 || count < 0 ||
+        count > nz - z0)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_volume_f16_to_f32: invalid arguments");
+    if (ny > 65535 || count > 65535)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_volume_f16_to_f32: at most 65535 rows and slices per call");
+    Timing t(ctx, ms);
+    if (count > 0)
+        hipLaunchKernelGGL(volume_f16_to_f32_kernel, dim3((unsigned)((nx + 255) / 256), (unsigned)ny, (unsigned)count),
+                           dim3(256), 0, ctx->stream, reinterpret_cast<const vhalf*>(vol), nx, ny, z0, out, out_pitch);
     return t.finish();
 }
 

@@ -14,28 +14,25 @@ using namespace artia;
 namespace {
 // Upper bound on CTF slabs per tilt: a slab loop longer than this means a degenerate range/step.
 constexpr double kMaxSlabs = 1 << 16;
-}  // namespace
 
-extern "C" {
-
-artia_status artia_sart_run(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
-                            const int* hitpoints, const float* real_lengths, int ntilts, const int* index_list,
-                            const float* projections, int width, int height, float* vol, const artia_sart_params* sp,
-                            int iterations, int* volume_is_empty, float* ms) {
+artia_status sart_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                      const int* hitpoints, const float* real_lengths, int ntilts, const int* index_list,
+                      const float* projections, int width, int height, VolData vol, const artia_sart_params* sp,
+                      int iterations, int* volume_is_empty, float* ms) {
     if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
-    if (!vg || !tilts || !hitpoints || !real_lengths || !index_list || !projections || !vol || !sp ||
+    if (!vg || !tilts || !hitpoints || !real_lengths || !index_list || !projections || !vol.p || !sp ||
         !volume_is_empty || ntilts <= 0 || width <= 0 || height <= 0 || iterations < 0)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_run: invalid arguments");
-    if (vg->nx % 4 != 0 || (reinterpret_cast<uintptr_t>(vol) & 15) != 0)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_run: volume x dimension must be a multiple of 4 "
-                                                     "and the volume 16-byte aligned");
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: invalid arguments", fn);
+    if (vg->nx % 4 != 0 || (reinterpret_cast<uintptr_t>(vol.p) & (vol.half ? 7u : 15u)) != 0)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: volume x dimension must be a multiple of 4 "
+                                                     "and the volume 16-byte (half: 8-byte) aligned", fn);
     if (sp->sirt_count <= 0 || sp->oversampling <= 0 || sp->oversampling > 64)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_run: bad SIRTCount or OverSampling");
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: bad SIRTCount or OverSampling", fn);
     for (int i = 0; i < ntilts; i++) {
         if (index_list[i] < 0 || index_list[i] >= ntilts)
-            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_run: index list entry %d out of range", i);
+            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: index list entry %d out of range", fn, i);
         if (!valid_geometry(vg, &tilts[i]))
-            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_run: invalid geometry for tilt %d", i);
+            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: invalid geometry for tilt %d", fn, i);
     }
     const int S = sp->sirt_count;
     const size_t npix = (size_t)width * (size_t)height;
@@ -94,37 +91,35 @@ artia_status artia_sart_run(artia_ctx* ctx, const artia_vol_geom* vg, const arti
     return prof_collect(ctx);
 }
 
-artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
-                                const int* hitpoints, const float* real_lengths, const float* slab_range,
-                                const artia_tilt_defocus* defocus, int ntilts, const int* index_list,
-                                const float* projections, int width, int height, float* vol,
-                                const artia_sart_params* sp, const artia_ctf_params* ctf, int iterations,
-                                int* volume_is_empty, float* ms) {
+artia_status sart_ctf_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                          const int* hitpoints, const float* real_lengths, const float* slab_range,
+                          const artia_tilt_defocus* defocus, int ntilts, const int* index_list,
+                          const float* projections, int width, int height, VolData vol, const artia_sart_params* sp,
+                          const artia_ctf_params* ctf, int iterations, int* volume_is_empty, float* ms) {
     if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
     if (!vg || !tilts || !hitpoints || !real_lengths || !slab_range || !defocus || !index_list || !projections ||
-        !vol || !sp || !ctf || !volume_is_empty || ntilts <= 0 || width <= 0 || height <= 0 || iterations < 0)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: invalid arguments");
-    if (vg->nx % 4 != 0 || (reinterpret_cast<uintptr_t>(vol) & 15) != 0)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: volume x dimension must be a multiple "
-                                                     "of 4 and the volume 16-byte aligned");
+        !vol.p || !sp || !ctf || !volume_is_empty || ntilts <= 0 || width <= 0 || height <= 0 || iterations < 0)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: invalid arguments", fn);
+    if (vg->nx % 4 != 0 || (reinterpret_cast<uintptr_t>(vol.p) & (vol.half ? 7u : 15u)) != 0)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: volume x dimension must be a multiple "
+                                                     "of 4 and the volume 16-byte (half: 8-byte) aligned", fn);
     if (sp->sirt_count <= 0 || sp->oversampling <= 0 || sp->oversampling > 64)
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: bad SIRTCount or OverSampling");
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: bad SIRTCount or OverSampling", fn);
     if (!(ctf->slice_thickness_nm > 0.0f) || !(ctf->pixelsize_nm > 0.0f) || !(ctf->voltage_kV > 0.0f) ||
         !(ctf->wiener >= 0.0f))
-        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: bad CTF parameters");
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: bad CTF parameters", fn);
     // Reconstructor.cpp:514: the slab step in pixels, accumulated in float as the reference does
     const float step = ctf->slice_thickness_nm / ctf->pixelsize_nm;
     for (int i = 0; i < ntilts; i++) {
         if (index_list[i] < 0 || index_list[i] >= ntilts)
-            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: index list entry %d out of range", i);
+            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: index list entry %d out of range", fn, i);
         if (!valid_geometry(vg, &tilts[i]))
-            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: invalid geometry for tilt %d", i);
+            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: invalid geometry for tilt %d", fn, i);
         const float a = slab_range[2 * i], b = slab_range[2 * i + 1];
         if (!std::isfinite(a) || !std::isfinite(b) || !(step > 0.0f) || (double)(b - a) / step > (double)kMaxSlabs ||
             (b > a && a + step == a))
-            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_sart_ctf_run: slab range of tilt %d holds too many "
-                                                         "slabs",
-                        i);
+            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: slab range of tilt %d holds too many "
+                                                         "slabs", fn, i);
     }
     const int S = sp->sirt_count;
     const size_t npix = (size_t)width * (size_t)height;
@@ -291,6 +286,48 @@ artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const 
     st = t.finish();
     if (st != ARTIA_OK) return st;
     return prof_collect(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+artia_status artia_sart_run(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                            const int* hitpoints, const float* real_lengths, int ntilts, const int* index_list,
+                            const float* projections, int width, int height, float* vol, const artia_sart_params* sp,
+                            int iterations, int* volume_is_empty, float* ms) {
+    return sart_run(ctx, "artia_sart_run", vg, tilts, hitpoints, real_lengths, ntilts, index_list, projections, width,
+                    height, VolData(vol), sp, iterations, volume_is_empty, ms);
+}
+
+artia_status artia_sart_run_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                                const int* hitpoints, const float* real_lengths, int ntilts, const int* index_list,
+                                const float* projections, int width, int height, uint16_t* vol,
+                                const artia_sart_params* sp, int iterations, int* volume_is_empty, float* ms) {
+    return sart_run(ctx, "artia_sart_run_f16", vg, tilts, hitpoints, real_lengths, ntilts, index_list, projections,
+                    width, height, VolData::f16(vol), sp, iterations, volume_is_empty, ms);
+}
+
+artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                                const int* hitpoints, const float* real_lengths, const float* slab_range,
+                                const artia_tilt_defocus* defocus, int ntilts, const int* index_list,
+                                const float* projections, int width, int height, float* vol,
+                                const artia_sart_params* sp, const artia_ctf_params* ctf, int iterations,
+                                int* volume_is_empty, float* ms) {
+    return sart_ctf_run(ctx, "artia_sart_ctf_run", vg, tilts, hitpoints, real_lengths, slab_range, defocus, ntilts,
+                        index_list, projections, width, height, VolData(vol), sp, ctf, iterations, volume_is_empty,
+                        ms);
+}
+
+artia_status artia_sart_ctf_run_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                                    const int* hitpoints, const float* real_lengths, const float* slab_range,
+                                    const artia_tilt_defocus* defocus, int ntilts, const int* index_list,
+                                    const float* projections, int width, int height, uint16_t* vol,
+                                    const artia_sart_params* sp, const artia_ctf_params* ctf, int iterations,
+                                    int* volume_is_empty, float* ms) {
+    return sart_ctf_run(ctx, "artia_sart_ctf_run_f16", vg, tilts, hitpoints, real_lengths, slab_range, defocus,
+                        ntilts, index_list, projections, width, height, VolData::f16(vol), sp, ctf, iterations,
+                        volume_is_empty, ms);
 }
 
 }  // extern "C"

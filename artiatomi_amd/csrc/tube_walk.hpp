@@ -114,7 +114,8 @@ __device__ __forceinline__ TubeBox tube_box(const VolG& v, const TiltG& g, const
 // clamped indices.  The LDS reads are issued unconditionally (at offset 0 when outside, which stays
 // inside the box) and overwritten on the rare global path: branching between the two sources would
 // let the compiler merge them into generic loads that wait on both memory counters.
-__device__ __forceinline__ void chunk_corners(const VolG& v, const float* __restrict__ vol,
+template <typename T>
+__device__ __forceinline__ void chunk_corners(const VolG& v, const T* __restrict__ vol,
                                               const float* __restrict__ chunk, bool staged, int X0, int Y0, int Z0,
                                               int CX, int CY, int CZ, const int cell[3], float val[8]) {
     const int cx = cell[0] - X0, cy = cell[1] - Y0, cz = cell[2] - Z0;
@@ -135,7 +136,7 @@ __device__ __forceinline__ void chunk_corners(const VolG& v, const float* __rest
         size_t g[8];
         cell_corners<true>(v, cell, g);
 #pragma unroll
-        for (int i = 0; i < 8; i++) val[i] = vol[g[i]];
+        for (int i = 0; i < 8; i++) val[i] = vox(vol, g[i]);
     }
 }
 
@@ -156,8 +157,8 @@ __device__ __forceinline__ size_t box_voxel(const VolG& v, int X0, int Y0, int Z
 //   scatter (kScatter true): f(float wgt[8], cell, k, m) returns eight corner weights, which are
 //     added to the slab's box in LDS (shared-memory atomics) and then to grad with one global
 //     atomic per box voxel, instead of eight global atomics per cell.
-template <bool kScatter, class F>
-__device__ __forceinline__ void tube_slabs(const VolG& v, const float* __restrict__ vol, float* __restrict__ grad,
+template <bool kScatter, typename T, class F>
+__device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ vol, float* __restrict__ grad,
                                            float* __restrict__ chunk, int* zr, const TubePlan& tp, const TubeBox& tb,
                                            const TubeDir& td, const float q0[3], int count, F&& f) {
     // cell layers the block's sub-rays cross
@@ -208,7 +209,7 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const float* __restric
                 if (kScatter)
                     chunk[e] = 0.0f;
                 else
-                    chunk[e] = vol[box_voxel(v, X0, Y0, Z0, CX, CY, e)];
+                    chunk[e] = vox(vol, box_voxel(v, X0, Y0, Z0, CX, CY, e));
             }
         }
         __syncthreads();
@@ -293,7 +294,7 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const float* __restric
                 size_t o[8];
                 cell_corners<true>(v, cell, o);
 #pragma unroll
-                for (int i = 0; i < 8; i++) val[i] = vol[o[i]];
+                for (int i = 0; i < 8; i++) val[i] = vox(vol, o[i]);
                 f(val, cell, k, mi);
             }
         });
@@ -308,8 +309,8 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const float* __restric
 // are cell layers bounding every sample of the lane (za > zb: none).  Each segment is walked with
 // tube_slabs' CellWalker steps, so its cells, their order and their values are those of a
 // tube_walk of that segment alone.  Must be called by all 256 threads of the block.
-template <class Seg, class F>
-__device__ __forceinline__ void tube_walk_segments(const VolG& v, const float* __restrict__ vol,
+template <typename T, class Seg, class F>
+__device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __restrict__ vol,
                                                    float* __restrict__ chunk, int* zr, const TubePlan& tp,
                                                    const TubeBox& tb, const TubeDir& td, int za, int zb, Seg& seg,
                                                    F&& f) {
@@ -347,7 +348,7 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const float* _
         if (!tail) {
             __syncthreads();
             if (staged)
-                for (int e = (int)threadIdx.x; e < n; e += 256) chunk[e] = vol[box_voxel(v, X0, Y0, Z0, CX, CY, e)];
+                for (int e = (int)threadIdx.x; e < n; e += 256) chunk[e] = vox(vol, box_voxel(v, X0, Y0, Z0, CX, CY, e));
             __syncthreads();
         }
         for (;;) {
@@ -398,8 +399,8 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const float* _
 }
 
 // The gather walk (see tube_slabs).
-template <class F>
-__device__ __forceinline__ void tube_walk(const VolG& v, const float* __restrict__ vol, float* __restrict__ chunk,
+template <typename T, class F>
+__device__ __forceinline__ void tube_walk(const VolG& v, const T* __restrict__ vol, float* __restrict__ chunk,
                                           int* zr, const TubePlan& tp, const TubeBox& tb, const TubeDir& td,
                                           const float q0[3], int count, F&& f) {
     tube_slabs<false>(v, vol, nullptr, chunk, zr, tp, tb, td, q0, count, f);
@@ -410,7 +411,7 @@ template <class F>
 __device__ __forceinline__ void tube_scatter(const VolG& v, float* __restrict__ grad, float* __restrict__ chunk,
                                              int* zr, const TubePlan& tp, const TubeBox& tb, const TubeDir& td,
                                              const float q0[3], int count, F&& f) {
-    tube_slabs<true>(v, nullptr, grad, chunk, zr, tp, tb, td, q0, count, f);
+    tube_slabs<true>(v, static_cast<const float*>(nullptr), grad, chunk, zr, tp, tb, td, q0, count, f);
 }
 
 }  // namespace artia

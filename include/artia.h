@@ -30,6 +30,7 @@
 #define ARTIA_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -411,6 +412,47 @@ artia_status artia_sart_ctf_run(artia_ctx* ctx, const artia_vol_geom* vg, const 
                                 const float* projections, int width, int height, float* vol,
                                 const artia_sart_params* sp, const artia_ctf_params* ctf, int iterations,
                                 int* volume_is_empty, float* ms);
+
+/* ---------------------------------------------------------------- half volume storage
+ * EmSART's FP16Volume mode (EmSart.cpp:354-374, Config FP16Volume): the volume is stored as IEEE
+ * binary16 (uint16_t bits, dense [z][y][x], half the bytes of fp32).  Forward projection filters the
+ * half voxels as floats (the reference's texture over a half-float CUDA array); backprojection is
+ * BPKernel(modBP, fp16 = true), backProjectionFP16 (BackProjectionSquareOS.cu:785-1290): the fp32
+ * kernel's arithmetic between a read of the quad's half voxels and a write rounded to nearest even.
+ * Each call behaves as its fp32 namesake on the float values of the half voxels, so a half volume
+ * projects bit-identically to the fp32 volume holding the same values, and a backprojection into a
+ * half volume equals the fp32 backprojection rounded to half.  The volume pointer is 8-byte
+ * aligned. */
+artia_status artia_fp_march_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                                const uint16_t* vol, int width, int height, size_t pitch, float* proj,
+                                float* dist, const int* roi, float* ms);
+artia_status artia_fp_slicer_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                                 const uint16_t* vol, int width, int height, size_t pitch, float* proj,
+                                 float tmin, float tmax, const int* roi, float* ms);
+artia_status artia_bp_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, uint16_t* vol,
+                          const float* img, int width, int height, size_t pitch, float lambda,
+                          int oversampling, float dist_min, float dist_max, int filter, float* ms);
+/* artia_sart_run / artia_sart_ctf_run over a half volume (EmSart.cpp:599-667 with FP16Volume) */
+artia_status artia_sart_run_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                                const int* hitpoints, const float* real_lengths, int ntilts,
+                                const int* index_list, const float* projections, int width, int height,
+                                uint16_t* vol, const artia_sart_params* sp, int iterations,
+                                int* volume_is_empty, float* ms);
+artia_status artia_sart_ctf_run_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                                    const int* hitpoints, const float* real_lengths, const float* slab_range,
+                                    const artia_tilt_defocus* defocus, int ntilts, const int* index_list,
+                                    const float* projections, int width, int height, uint16_t* vol,
+                                    const artia_sart_params* sp, const artia_ctf_params* ctf, int iterations,
+                                    int* volume_is_empty, float* ms);
+/* ConvVolKernel / convertVolumeFP16ToFP32 (Kernels.cpp:1134-1180, BackProjectionSquareOS.cu:1294-1323),
+ * as Reconstructor::ConvertVolumeFP16 (Reconstructor.cpp:1734-1745) uses it slice by slice for the
+ * output file: slices z0 .. z0+count-1 of an nx x ny x nz half volume as negated floats (EmSART
+ * writes volumes with this sign; the fp32 path negates with Volume::Invert, EmSart.cpp:712), slice
+ * z0+k at out + k * ny * out_pitch.  (The reference's convertVolume3DFP16ToFP32,
+ * BackProjectionSquareOS.cu:1325-1355, stores every slice into the same plane; count > 1 here
+ * gives the slices one plane each.) */
+artia_status artia_volume_f16_to_f32(artia_ctx* ctx, const uint16_t* vol, int nx, int ny, int nz, int z0,
+                                     int count, float* out, size_t out_pitch, float* ms);
 
 /* ---------------------------------------------------------------- sub-volume WBP
  * EmSARTSubVolumes (EmSartSubVolumes.cpp:611-707): every motive-list sub-volume is reconstructed

@@ -179,13 +179,58 @@ def test_sart_ctf_parity(gpu):
         ctx.close()
 
 
+def per_slab_sart_ctf(ctx, vg, acq, tdef, d_proj, vol, sp, ctf, iterations):
+    """The reference's per-slab CTF-SART loop (Reconstructor.cpp:496-645 forward, :920-969 backward)
+    driven through the per-operator entry points, S = 1: per tilt, slab by slab, fp_slicer ->
+    ctf_slab_filter (accumulated), compare, then per slab ctf_slab_filter -> gated bp.  vol (float32
+    or float16) is updated in place."""
+    f32 = np.float32
+    H, W = d_proj.shape[1:]
+    step = f32(ctf.slice_thickness_nm) / f32(ctf.pixelsize_nm)
+    frame = (0, H - 1, W - 1, H - 1, 0, 0, W - 1, 0)
+    proj = torch.zeros((H, W), device="cuda")
+    dist = torch.zeros((H, W), device="cuda")
+    cut, dim = tuple(sp.cut_length), tuple(sp.dim_length)
+
+    def slabs(idx):
+        ray = f32(acq.slab_ranges[idx][0])
+        while ray < f32(acq.slab_ranges[idx][1]):
+            yield ray, f32(ray + step)
+            ray = f32(ray + step)
+
+    empty = True
+    for _ in range(iterations):
+        for idx in acq.index_list:
+            g, d = acq.geoms[idx], tdef[idx]
+            proj.zero_()
+            dist.zero_()
+            if not empty:
+                for a, b in slabs(idx):
+                    dmin, dmax = A.defocus_minmax(float(a), d, ctf)
+                    dist.zero_()
+                    ctx.fp_slicer(vg, g, vol, dist, float(a), float(b))
+                    ctx.ctf_slab_filter(dist, proj, frame, ctf, dmin, dmax, d.astig_angle_deg, for_fp=True,
+                                        accumulate=True, cut_length=cut, dim_length=dim)
+                dist.zero_()
+            ctx.vol_traversal(vg, g, dist)
+            ctx.compare(d_proj[idx], proj, dist, acq.real_lengths[idx], tuple(sp.crop), tuple(sp.crop_dim),
+                        sp.proj_scale)
+            empty = False
+            for a, b in slabs(idx):
+                dmin, dmax = A.defocus_minmax(float(a), d, ctf)
+                ctx.ctf_slab_filter(proj, dist, acq.hitpoints[idx], ctf, dmin, dmax, d.astig_angle_deg,
+                                    for_fp=False, accumulate=False, cut_length=cut, dim_length=dim)
+                ctx.bp(vg, g, vol, dist, float(f32(sp.lambda_)), oversampling=sp.oversampling, dist_min=float(a),
+                       dist_max=float(b), filter="linear")
+    torch.cuda.synchronize()
+
+
 def test_sart_ctf_equals_per_slab_composition(gpu):
     """artia_sart_ctf_run backprojects a tilt's CTF slabs in one launch over a stack of the filtered
     slab images; driving the reference's per-slab loop (Reconstructor.cpp:920-969) through the
     per-operator entry points (gated artia_bp per slab) gives the same volume bit for bit."""
     dims, vg, acq, projs, defocus, ctf = ctf_case()
     nx, ny, nz = dims
-    H, W = projs.shape[1:]
     sp = A.sart_params(lambda_=0.3, oversampling=2, cut_length=(1, 1), dim_length=(2, 2))
     tdef = [A.TiltDefocus(*d) for d in defocus]
     ctx = A.ArtiaContext(0)
@@ -194,44 +239,7 @@ def test_sart_ctf_equals_per_slab_composition(gpu):
         vol = torch.zeros((nz, ny, nx), device="cuda")
         ctx.sart_ctf_run(vg, acq, tdef, d_proj, vol, sp, ctf, iterations=2)
         ref = torch.zeros_like(vol)
-        f32 = np.float32
-        step = f32(ctf.slice_thickness_nm) / f32(ctf.pixelsize_nm)
-        frame = (0, H - 1, W - 1, H - 1, 0, 0, W - 1, 0)
-        proj = torch.zeros((H, W), device="cuda")
-        dist = torch.zeros((H, W), device="cuda")
-        cut, dim = tuple(sp.cut_length), tuple(sp.dim_length)
-
-        def slabs(idx):
-            ray = f32(acq.slab_ranges[idx][0])
-            while ray < f32(acq.slab_ranges[idx][1]):
-                yield ray, f32(ray + step)
-                ray = f32(ray + step)
-
-        empty = True
-        for _ in range(2):
-            for idx in acq.index_list:
-                g, d = acq.geoms[idx], tdef[idx]
-                proj.zero_()
-                dist.zero_()
-                if not empty:
-                    for a, b in slabs(idx):
-                        dmin, dmax = A.defocus_minmax(float(a), d, ctf)
-                        dist.zero_()
-                        ctx.fp_slicer(vg, g, ref, dist, float(a), float(b))
-                        ctx.ctf_slab_filter(dist, proj, frame, ctf, dmin, dmax, d.astig_angle_deg, for_fp=True,
-                                            accumulate=True, cut_length=cut, dim_length=dim)
-                    dist.zero_()
-                ctx.vol_traversal(vg, g, dist)
-                ctx.compare(d_proj[idx], proj, dist, acq.real_lengths[idx], tuple(sp.crop), tuple(sp.crop_dim),
-                            sp.proj_scale)
-                empty = False
-                for a, b in slabs(idx):
-                    dmin, dmax = A.defocus_minmax(float(a), d, ctf)
-                    ctx.ctf_slab_filter(proj, dist, acq.hitpoints[idx], ctf, dmin, dmax, d.astig_angle_deg,
-                                        for_fp=False, accumulate=False, cut_length=cut, dim_length=dim)
-                    ctx.bp(vg, g, ref, dist, float(f32(sp.lambda_)), oversampling=2, dist_min=float(a),
-                           dist_max=float(b), filter="linear")
-        torch.cuda.synchronize()
+        per_slab_sart_ctf(ctx, vg, acq, tdef, d_proj, ref, sp, ctf, iterations=2)
         assert float(ref.abs().max()) > 0
         assert torch.equal(vol, ref)
     finally:
