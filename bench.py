@@ -257,7 +257,7 @@ def cpu_baseline(vg, acq, vol_dev, projs_dev, w):
     W, H = acq.width, acq.height
     ti = len(acq.geoms) // 4   # a tilt of about -30 deg
     g = acq.geoms[ti]
-    vol = vol_dev.detach().cpu().numpy()
+    vol = vol_dev.detach().float().cpu().numpy()
     rows = 8 if W >= 1024 else 32
     y0 = H // 2 - rows // 2
     t0 = time.perf_counter()
@@ -784,6 +784,8 @@ def main() -> int:
                     sorted(REFINE_WORKLOADS) + sorted(PREP_WORKLOADS))
     ap.add_argument("--numerics", default="exact", choices=["exact", "reference"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--volume-f16", action="store_true",
+                    help="store the SART volume as IEEE half (the reference's FP16Volume mode)")
     ap.add_argument("--no-fp-staging", action="store_true",
                     help="forward-project with direct volume reads instead of LDS-staged slabs")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -829,7 +831,8 @@ def main() -> int:
     nx, ny, nz = w["dims"]
     V, T = nx * ny * nz, len(acq.geoms)
     sp = A.sart_params(lambda_=0.3, sirt_count=1, oversampling=w["os"])
-    vol = torch.zeros((nz, ny, nx), device="cuda")
+    vol = torch.zeros((nz, ny, nx), device="cuda", dtype=torch.float16 if args.volume_f16 else torch.float32)
+    vbytes = 2.0 if args.volume_f16 else 4.0   # bytes per stored voxel
 
     empty = True
     for i in range(args.warmup):
@@ -853,8 +856,8 @@ def main() -> int:
     for name, (ms, n) in prof.items():
         if n:
             per_kernel[name] = {"avg_ms": ms / n, "launches": n, "share": ms / (elapsed * 1e3)}
-    bytes_bp = 8.0 * V + 4.0 * W * H
-    bytes_fp = 4.0 * V + 8.0 * W * H
+    bytes_bp = 2 * vbytes * V + 4.0 * W * H
+    bytes_fp = vbytes * V + 8.0 * W * H
     bp_avg = prof["bp"][0] / max(1, prof["bp"][1]) / 1e3
     fp_avg = prof["fp"][0] / max(1, prof["fp"][1]) / 1e3
     samples = fp_samples_per_launch(ctx, vg, acq, vol)
@@ -870,7 +873,7 @@ def main() -> int:
     roofline = {"kernel": "march" if dom == "fp" else "backProjection", "bound": "hbm", "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
     # the profile covers the default configuration's kernels only
-    default_kernels = args.numerics == "exact" and not args.no_fp_staging
+    default_kernels = args.numerics == "exact" and not args.no_fp_staging and not args.volume_f16
     measured = measured_traffic(args.workload, "march_lds_kernel<false>" if dom == "fp" else "bp_exact_kernel<2, 2>") \
         if default_kernels else None
     if measured:
@@ -901,7 +904,8 @@ def main() -> int:
         "data": "synthetic (seeded ellipsoid + gold-bead phantom, projections from the forward projector + noise)",
         "config": {"workload": args.workload, "volume": list(w["dims"]), "projection": [W, H], "tilts": T,
                    "tilt_range_deg": [-60, 60], "oversampling": w["os"], "sirt_count": 1, "lambda": 0.3,
-                   "numerics": args.numerics, "fp_staging": not args.no_fp_staging, "parallelism": "single GPU"},
+                   "numerics": args.numerics, "fp_staging": not args.no_fp_staging,
+                   "volume_storage": "f16" if args.volume_f16 else "f32", "parallelism": "single GPU"},
         "roofline": roofline,
         "kernels": per_kernel,
         "cpu_baseline": cpu,
