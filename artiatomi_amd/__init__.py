@@ -2,8 +2,8 @@
 backprojection, 3D-CTF slab correction, WBP weighting) as gfx950 HIP kernels behind the C ABI
 of include/artia.h.  This package is the Python view of that library; the compute path is the
 HIP library (libartia.so) and nothing else."""
-from ._abi import (CtfParams, GlobalParams, SartParams, Subvol, SubvolTuning, TiltDefocus, TiltGeom,  # noqa: F401
-                   TiltParams, VolGeom, load)
+from ._abi import (CtfParams, FileInfo, GlobalParams, SartParams, Subvol, SubvolTuning, TiltDefocus,  # noqa: F401
+                   TiltGeom, TiltParams, VolGeom, load)
 from .core import (  # noqa: F401
     Acquisition,
     ArtiaContext,
@@ -14,6 +14,9 @@ from .core import (  # noqa: F401
     prep_params,
     marker_tilt_params,
     ctf_file_defocus,
+    parse_file_header,
+    read_file,
+    volume_file_header,
     defocus_distances,
     defocus_minmax,
     footprint_window,
@@ -33,5 +36,6 @@ __all__ = [
     "subvol_wbp_plain", "subvolume_tilt_geometry", "GlobalParams", "TiltDefocus",
     "ctf_params", "defocus_distances", "defocus_minmax", "footprint_window", "SartParams", "TiltGeom", "TiltParams",
     "VolGeom", "hitpoints", "load", "projection_index_list", "real_length", "sart_params",
-    "subvolume_geometry", "tilt_geometry", "volume_geometry",
+    "subvolume_geometry", "tilt_geometry", "volume_geometry", "FileInfo", "parse_file_header", "read_file",
+    "volume_file_header", "marker_tilt_params", "ctf_file_defocus", "prep_params",
 ]

@@ -100,6 +100,18 @@ class TiltDefocus(C.Structure):
                 ("astig_angle_deg", C.c_float), ("tilt_deg", C.c_float)]
 
 
+class FileInfo(C.Structure):
+    """artia_file_info: element type, sizes and data offset of an EM or MRC file."""
+
+    _fields_ = [("dtype", C.c_int), ("raw_type", C.c_int), ("elem_bytes", C.c_int), ("nx", C.c_int),
+                ("ny", C.c_int), ("nz", C.c_int), ("data_offset", C.c_size_t), ("pixel_size_nm", C.c_float),
+                ("byte_swapped", C.c_int)]
+
+
+ARTIA_FILE_EM = 0
+ARTIA_FILE_MRC = 1
+
+
 class Subvol(C.Structure):
     """artia_subvol: sub-volume placement and its device volume."""
 
@@ -169,6 +181,8 @@ SIGNATURES = {
     "artia_bp_f16": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _P, _I, _I, _SZ, _F, _I, _F, _F, _I,
                           _PF]),
     "artia_volume_f16_to_f32": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _SZ, _PF]),
+    "artia_file_parse_header": (_I, [_I, _P, _SZ, C.POINTER(FileInfo)]),
+    "artia_file_volume_header": (_I, [_I, _I, _I, _I, _I, _F, _PF, _P, _SZ, C.POINTER(_SZ)]),
     "artia_ctf_apply": (_I, [_P, _P, _I, _SZ, C.POINTER(CtfParams), _F, _F, _F, _I, _PF]),
     "artia_wbp_weight": (_I, [_P, _P, _I, _SZ, _F, _I, _PF]),
     "artia_ctf_slab_filter": (_I, [_P, _P, _I, _I, _SZ, _P, _SZ, _I, _PI, _PF, _PF, C.POINTER(CtfParams), _F, _F,

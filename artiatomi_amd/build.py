@@ -17,6 +17,7 @@ LIB = PKG / "libartia.so"
 SOURCES = [
     "context.hip",
     "geometry.cpp",
+    "formats.cpp",
     "fp.hip",
     "bp.hip",
     "image.hip",

@@ -235,6 +235,49 @@ def ctf_file_defocus(ctf_data, index: int, tilt_deg: float) -> TiltDefocus:
     return d
 
 
+FILE_FORMATS = {"em": _abi.ARTIA_FILE_EM, "mrc": _abi.ARTIA_FILE_MRC}
+# numpy element types of the EM DataType / MRC MODE codes that have one
+_EM_NP = {1: "u1", 2: "i2", 4: "i4", 5: "f4", 8: "c8", 9: "f8", 10: "f2"}
+_MRC_NP = {0: "u1", 1: "i2", 2: "f4", 4: "c8", 6: "u2", 12: "f2", 64: "f2"}
+
+
+def parse_file_header(fmt: str, header: bytes) -> "_abi.FileInfo":
+    """Sizes, element type and data offset of an EM or MRC file from its first bytes (512 / 1024)."""
+    info = _abi.FileInfo()
+    buf = C.create_string_buffer(bytes(header), len(header))
+    st = _abi.load().artia_file_parse_header(FILE_FORMATS[fmt], buf, len(header), C.byref(info))
+    _check(st, what="artia_file_parse_header")
+    return info
+
+
+def read_file(path, fmt: Optional[str] = None):
+    """(FileInfo, array) of an EM or MRC stack or volume: the data as a read-only numpy memory map of
+    shape (nz, ny, nx) in the file's byte order.  fmt defaults to the file's suffix (.em, else MRC)."""
+    import numpy as np
+
+    fmt = fmt or ("em" if str(path).lower().endswith(".em") else "mrc")
+    with open(path, "rb") as f:
+        head = f.read(1024 if fmt == "mrc" else 512)
+    info = parse_file_header(fmt, head)
+    code = (_EM_NP if fmt == "em" else _MRC_NP).get(info.raw_type)
+    if code is None:
+        raise ArtiaError(f"{fmt} element type {info.raw_type} has no array type")
+    dt = np.dtype(code).newbyteorder(">" if info.byte_swapped else "<")
+    data = np.memmap(path, dtype=dt, mode="r", offset=info.data_offset, shape=(info.nz, info.ny, info.nx))
+    return info, data
+
+
+def volume_file_header(fmt: str, nx: int, ny: int, nz: int, pixel_size_nm: float, voxel_size=(1.0, 1.0, 1.0),
+                       half: bool = False) -> bytes:
+    """The header EmSART writes before a reconstructed volume (EmSart.cpp:717-759)."""
+    out = C.create_string_buffer(1024)
+    n = C.c_size_t()
+    st = _abi.load().artia_file_volume_header(FILE_FORMATS[fmt], int(bool(half)), nx, ny, nz, float(pixel_size_nm),
+                                              _abi.farr(voxel_size, 3), out, 1024, C.byref(n))
+    _check(st, what="artia_file_volume_header")
+    return out.raw[:n.value]
+
+
 def prep_params(dtype: str = "float32", mirror_y: bool = False, correct_bad_pixels: bool = True,
                 bad_pixel_value: float = 10.0, skip_filter: bool = False, lp: float = 800.0, lps: float = 200.0,
                 hp: float = 0.0, hps: float = 0.0, dose_weighting: bool = False, pixelsize_nm: float = 1.0,

@@ -454,6 +454,37 @@ artia_status artia_sart_ctf_run_f16(artia_ctx* ctx, const artia_vol_geom* vg, co
 artia_status artia_volume_f16_to_f32(artia_ctx* ctx, const uint16_t* vol, int nx, int ny, int nz, int z0,
                                      int count, float* out, size_t out_pitch, float* ms);
 
+/* ---------------------------------------------------------------- file headers
+ * EM (EmTools/FileIO/EmHeader.h, EmFile.cpp) and MRC (mrcHeader.h, MRCFile.cpp) stacks and
+ * volumes.  artia_file_parse_header reads a file's first bytes (EM: 512, MRC: 1024) and reports
+ * what a reader needs to hand the data to the device: the sizes, the element type and where the
+ * data starts (MRC: after NEXT extended-header bytes).  dtype is the artia_dtype the projection
+ * preparation converts from (EM short/int/float, MRC modes 1, 2, 6), -1 for the other types
+ * (bytes, complex, double, half, RGB); raw_type is the header's own code (EM DataType, MRC MODE).
+ * EM files must have PC machine coding (EmFile.cpp:75-78; else ARTIA_ERR_UNSUPPORTED).  An MRC
+ * header whose NX reads above 32000 is big-endian (MRCFile.cpp:84-88): its fields are swapped and
+ * byte_swapped = 1 tells the caller to swap the data.  pixel_size_nm: MRC Xlen / NX / 10
+ * (MRCFile::GetPixelsize, MRCFile.cpp:585-591, 0 when unset), EM 1 (TiltSeries::GetPixelSize).
+ * artia_file_volume_header writes the header EmSART writes before a reconstructed volume
+ * (EmSart.cpp:717-759): MRC mode 2 (64 for a half volume), Xlen/Ylen/Zlen = pixel size x dim x
+ * voxel size x 10, axes x, y, z, all else 0; or EM with PC coding, DataType float (10 for half) and
+ * ObjectPixelSize = (int)(pixel size x voxel size x 1000).  *header_bytes = 1024 (MRC) or 512 (EM).
+ * Host-only calls; no context. */
+typedef enum { ARTIA_FILE_EM = 0, ARTIA_FILE_MRC = 1 } artia_file_format;
+typedef struct {
+    int dtype;            /* artia_dtype, or -1 */
+    int raw_type;         /* EM DataType or MRC MODE */
+    int elem_bytes;       /* bytes per element */
+    int nx, ny, nz;
+    size_t data_offset;   /* bytes from the file's start to the first element */
+    float pixel_size_nm;
+    int byte_swapped;     /* MRC: the file is big-endian */
+} artia_file_info;
+artia_status artia_file_parse_header(int format, const void* header, size_t size, artia_file_info* out);
+artia_status artia_file_volume_header(int format, int half, int nx, int ny, int nz, float pixel_size_nm,
+                                      const float voxel_size[3], void* out, size_t capacity,
+                                      size_t* header_bytes);
+
 /* ---------------------------------------------------------------- sub-volume WBP
  * EmSARTSubVolumes (EmSartSubVolumes.cpp:611-707): every motive-list sub-volume is reconstructed
  * by backprojecting each tilt's WBP-filtered projection (artia_wbp_filter) with lambda 1 through
