@@ -385,9 +385,10 @@ __global__ void __launch_bounds__(256, 6) slicer_segments_lds_kernel(VolG v, Til
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     const int sub = lane & 3, pix = lane >> 2;
     const int xcd = (int)(blockIdx.x & 7u), slot = (int)(blockIdx.x >> 3);
-    const int grp = (slot / (kXcdGroup * kXcdGroup)) * 8 + xcd, mem = slot % (kXcdGroup * kXcdGroup);
-    const int gtx = (grp % tp.groups_x) * kXcdGroup + mem % kXcdGroup;
-    const int gty = (grp / tp.groups_x) * kXcdGroup + mem / kXcdGroup;
+    constexpr int kG = kSlicerXcdGroup;
+    const int grp = (slot / (kG * kG)) * 8 + xcd, mem = slot % (kG * kG);
+    const int gtx = (grp % tp.groups_x) * kG + mem % kG;
+    const int gty = (grp / tp.groups_x) * kG + mem / kG;
     if (gtx >= tp.tiles_x || gty >= tp.tiles_y) return;   // whole workgroup
     const int tx0 = gtx * 8, ty0 = gty * 8;
     const int x = tx0 + (wave & 1) * 4 + (pix & 3);
@@ -597,9 +598,9 @@ void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData 
     if (ctx->numerics == ARTIA_NUMERICS_EXACT && ctx->march_lds && per_pass > 0 && fits && tube_plan(v, g, tp)) {
         tp.tiles_x = (W + 7) / 8;
         tp.tiles_y = (H + 7) / 8;
-        tp.groups_x = (tp.tiles_x + kXcdGroup - 1) / kXcdGroup;
-        const int groups = tp.groups_x * ((tp.tiles_y + kXcdGroup - 1) / kXcdGroup);
-        const dim3 grid1((unsigned)(((groups + 7) / 8) * 8 * kXcdGroup * kXcdGroup));
+        tp.groups_x = (tp.tiles_x + kSlicerXcdGroup - 1) / kSlicerXcdGroup;
+        const int groups = tp.groups_x * ((tp.tiles_y + kSlicerXcdGroup - 1) / kSlicerXcdGroup);
+        const dim3 grid1((unsigned)(((groups + 7) / 8) * 8 * kSlicerXcdGroup * kSlicerXcdGroup));
         int2* range = static_cast<int2*>(scratch);
         double* part = reinterpret_cast<double*>(static_cast<char*>(scratch) + range_bytes);
         const double dstep = (double)v.vs[0] * 0.15;

@@ -378,7 +378,8 @@ def bench_ctf(args) -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "kernels": per_kernel,
     }
-    measured = measured_traffic(args.workload, "slicer_segments_lds_kernel" if dom == "fp" else "bp_stack_kernel")
+    dom_kernel = "slicer_segments_lds_kernel<float>" if dom == "fp" else "bp_stack_kernel<float>"
+    measured = measured_traffic(args.workload, dom_kernel)
     if measured:
         out["roofline"]["traffic"] = measured.pop("bytes")
         out["roofline"]["traffic_detail"] = measured
@@ -874,8 +875,8 @@ def main() -> int:
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
     # the profile covers the default configuration's kernels only
     default_kernels = args.numerics == "exact" and not args.no_fp_staging and not args.volume_f16
-    measured = measured_traffic(args.workload, "march_lds_kernel<false>" if dom == "fp" else "bp_exact_kernel<2, 2>") \
-        if default_kernels else None
+    dom_kernel = "march_lds_kernel<false, float>" if dom == "fp" else "bp_exact_kernel<2, 2, float>"
+    measured = measured_traffic(args.workload, dom_kernel) if default_kernels else None
     if measured:
         roofline["traffic"] = measured.pop("bytes")
         roofline["traffic_detail"] = measured
