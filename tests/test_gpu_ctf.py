@@ -179,7 +179,7 @@ def test_sart_ctf_parity(gpu):
         ctx.close()
 
 
-def per_slab_sart_ctf(ctx, vg, acq, tdef, d_proj, vol, sp, ctf, iterations):
+def per_slab_sart_ctf(ctx, vg, acq, tdef, d_proj, vol, sp, ctf, iterations, volume_is_empty=True):
     """The reference's per-slab CTF-SART loop (Reconstructor.cpp:496-645 forward, :920-969 backward)
     driven through the per-operator entry points, S = 1: per tilt, slab by slab, fp_slicer ->
     ctf_slab_filter (accumulated), compare, then per slab ctf_slab_filter -> gated bp.  vol (float32
@@ -198,7 +198,7 @@ def per_slab_sart_ctf(ctx, vg, acq, tdef, d_proj, vol, sp, ctf, iterations):
             yield ray, f32(ray + step)
             ray = f32(ray + step)
 
-    empty = True
+    empty = volume_is_empty
     for _ in range(iterations):
         for idx in acq.index_list:
             g, d = acq.geoms[idx], tdef[idx]

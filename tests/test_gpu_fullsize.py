@@ -1,0 +1,116 @@
+"""Parity at the benchmark sizes, where the oracle cannot run whole.
+
+SART workload (1024 x 1024 x 256 volume, 1024^2 detector, exact numerics):
+  - forward projection at three tilts against the FP64 oracle on 24 x 24-pixel windows (the
+    detector's centre, the edge of the volume's shadow, a corner), within 1e-5 relative L2;
+  - backprojection (OS 2) against the FP64 oracle on two-slice slabs at the bottom face, the middle
+    and the top face of the volume (the oracle runs on the slab alone: a voxel's update depends only
+    on its own position and the image).  For tilts about y the beam direction's y component is a
+    float residue (3.7e-9 at 47 deg), so a sub-ray whose detector position lies within ~1e-8 voxels
+    of a y face is inside or outside the whole voxel column by rounding alone, in any arithmetic: the
+    reference's own float evaluation moves such voxels by up to 3x against FP64.  At 1024^2 x 256 a
+    few such sub-rays occur; the gate is that at most 1e-5 of the slab's voxels differ by more than
+    1e-5 of the slab's largest update, and the rest agree within 1e-5 relative L2.
+CTF workload (2048 x 2048 x 512 volume, 2048^2 detector, 4 nm slabs at 0.25 nm/px): one CTF-SART
+pass over two tilts from a non-empty volume, stacked forward slabs and backprojection, equals the
+reference's per-slab loop through the operator entry points bit for bit.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import artiatomi_amd as A
+import oracle_lib as O
+from conftest import rel_l2
+from test_gpu_ctf import per_slab_sart_ctf
+
+pytestmark = pytest.mark.gpu
+
+DIMS, DET = (1024, 1024, 256), 1024
+
+
+@pytest.fixture(scope="module")
+def cfg2(gpu):
+    ctx = A.ArtiaContext(0)
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    nx, ny, nz = DIMS
+    vol = torch.rand((nz, ny, nx), generator=gen, device="cuda")
+    host = vol.cpu().numpy().astype(np.float64)
+    yield ctx, A.volume_geometry(*DIMS), vol, host
+    ctx.close()
+
+
+@pytest.mark.parametrize("tilt,psi,shift", [(0.0, 0.0, (0.0, 0.0)), (37.0, 2.5, (3.0, -2.0)),
+                                            (-58.0, -4.0, (-1.5, 4.0))])
+def test_march_windows_full_size(cfg2, tilt, psi, shift):
+    ctx, vg, vol, host = cfg2
+    g = A.tilt_geometry(tilt, DET, DET, psi_deg=psi, shift=shift)
+    proj = torch.zeros((DET, DET), device="cuda")
+    dist = torch.zeros((DET, DET), device="cuda")
+    ctx.fp_march(vg, g, vol, proj, dist)
+    p, d = proj.cpu().numpy(), dist.cpu().numpy()
+    for x0, y0 in ((500, 500), (16, 500), (1000, 1000)):
+        roi = (x0, y0, x0 + 24, y0 + 24)
+        print(f"march 1024^3/4 tilt={tilt} window {roi}", flush=True)
+        po, do = O.march(vg, g, host, DET, DET, dtype=np.float64, roi=roi)
+        sl = np.s_[y0:y0 + 24, x0:x0 + 24]
+        if np.abs(po[sl]).max() == 0:
+            assert np.abs(p[sl]).max() == 0
+            continue
+        assert rel_l2(p[sl], po[sl]) <= 1e-5
+        assert rel_l2(d[sl], do[sl]) <= 1e-5
+
+
+@pytest.mark.parametrize("tilt,psi,shift", [(0.0, 0.0, (0.0, 0.0)), (-47.0, 3.0, (2.0, 1.0))])
+def test_bp_slabs_full_size(cfg2, tilt, psi, shift):
+    ctx, vg, _, _ = cfg2
+    nx, ny, nz = DIMS
+    g = A.tilt_geometry(tilt, DET, DET, psi_deg=psi, shift=shift)
+    img = np.random.default_rng(5).normal(0.0, 1.0, (DET, DET)).astype(np.float32)
+    vol = torch.zeros((nz, ny, nx), device="cuda")
+    ctx.bp(vg, g, vol, torch.from_numpy(img).cuda(), 0.3, oversampling=2)
+    torch.cuda.synchronize()
+    for z0 in (0, nz // 2 - 1, nz - 2):
+        sub = A.VolGeom.from_buffer_copy(vg)
+        sub.bbox_min[2] = vg.bbox_min[2] + z0 * vg.voxel_size[2]
+        assert float(np.float32(sub.bbox_min[2])) == vg.bbox_min[2] + z0 * vg.voxel_size[2]   # exact
+        sub.nz = 2
+        print(f"bp 1024^3/4 tilt={tilt} slab z={z0}", flush=True)
+        ref = O.back_project(sub, g, np.zeros((2, ny, nx)), img, 0.3, os_=2, dtype=np.float64)
+        got = vol[z0:z0 + 2].cpu().numpy().astype(np.float64)
+        assert np.abs(ref).max() > 0
+        off = np.abs(got - ref) > 1e-5 * np.abs(ref).max()
+        print(f"  {int(off.sum())} of {off.size} voxels decided by a face tie; rest rel L2 "
+              f"{rel_l2(got[~off], ref[~off]):.2e}", flush=True)
+        assert off.sum() <= 1e-5 * off.size
+        assert rel_l2(got[~off], ref[~off]) <= 1e-5
+
+
+def test_sart_ctf_two_tilts_full_size_equals_per_slab(gpu):
+    nx, ny, nz = 2048, 2048, 512
+    W = H = 2048
+    vg = A.volume_geometry(nx, ny, nz)
+    tilts = [-36.0, 44.0]
+    acq = A.Acquisition.build(vg, W, H, tilts, psis_deg=[1.5, -2.0], shifts=[(2.0, -1.0), (-3.0, 0.5)])
+    ctf = A.ctf_params(pixelsize_nm=0.25, slice_thickness_nm=4.0, amp_contrast=0.07)
+    tdef = [A.TiltDefocus(3500.0, 3700.0, 30.0, tilts[0]), A.TiltDefocus(4200.0, 4400.0, -65.0, tilts[1])]
+    sp = A.sart_params(lambda_=0.3, oversampling=2, cut_length=(2, 2), dim_length=(4, 4))
+    ctx = A.ArtiaContext(0)
+    try:
+        gen = torch.Generator(device="cuda").manual_seed(21)
+        start = torch.rand((nz, ny, nx), generator=gen, device="cuda")
+        d_proj = torch.randn((2, H, W), generator=gen, device="cuda") * 50.0
+        mid = start[nz // 2].clone()
+        print("ctf 2048^2x512: stacked driver", flush=True)
+        vol = start.clone()
+        _, empty = ctx.sart_ctf_run(vg, acq, tdef, d_proj, vol, sp, ctf, iterations=1, volume_is_empty=False)
+        print("ctf 2048^2x512: per-slab loop", flush=True)
+        ref = start
+        per_slab_sart_ctf(ctx, vg, acq, tdef, d_proj, ref, sp, ctf, iterations=1, volume_is_empty=False)
+        assert not empty
+        assert not torch.equal(vol[nz // 2], mid)   # the two tilts changed the volume
+        assert torch.equal(vol, ref)
+    finally:
+        ctx.close()
