@@ -14,6 +14,8 @@ SART workload (1024 x 1024 x 256 volume, 1024^2 detector, exact numerics):
 CTF workload (2048 x 2048 x 512 volume, 2048^2 detector, 4 nm slabs at 0.25 nm/px): one CTF-SART
 pass over two tilts from a non-empty volume, stacked forward slabs and backprojection, equals the
 reference's per-slab loop through the operator entry points bit for bit.
+Sub-volume workload (4096^2 projections, 4096 x 4096 x 1024 field, 3D CTF): the batched plan equals
+the per-call schedule bit for bit on 40 cubes of the target mix over three tilts.
 """
 import ctypes as C
 
@@ -112,5 +114,47 @@ def test_sart_ctf_two_tilts_full_size_equals_per_slab(gpu):
         assert not empty
         assert not torch.equal(vol[nz // 2], mid)   # the two tilts changed the volume
         assert torch.equal(vol, ref)
+    finally:
+        ctx.close()
+
+
+def test_subvol_ctf_4096_batched_equals_plain(gpu):
+    """Sub-volume WBP at the target detector size: 40 cubes of 32/64/96/128 voxels (4:3:2:1) in a
+    4096 x 4096 x 1024 field, three 4096^2 projections, per-(cube, tilt) shifts, 3D CTF with 16-px
+    slabs; the batched plan equals the reference's per-call schedule bit for bit."""
+    rng = np.random.default_rng(17)
+    W = H = 4096
+    full = A.volume_geometry(4096, 4096, 1024)
+    tilts = [-50.0, 0.0, 38.0]
+    sizes = [32] * 16 + [64] * 12 + [96] * 8 + [128] * 4
+    subs = []
+    for n in sizes:
+        pos = rng.uniform([n, n, n / 2], [4096 - n, 4096 - n, 1024 - n / 2])
+        g = A.subvolume_geometry(n, 1.0, (1.0, 1.0, 1.0), full, pos.tolist())
+        subs.append((g, torch.rand((n, n, n), device="cuda")))
+    extra = rng.uniform(-4.0, 4.0, (len(sizes), len(tilts), 2))
+    geoms = [[A.subvolume_tilt_geometry(t, W, H, extra_shift=extra[s, i]) for i, t in enumerate(tilts)]
+             for s in range(len(sizes))]
+    images = torch.randn((len(tilts), H, W), device="cuda")
+    ctf = A.ctf_params(pixelsize_nm=0.25, slice_thickness_nm=4.0, amp_contrast=0.07)
+    base = [A.tilt_geometry(t, W, H) for t in tilts]
+    kw = dict(ctf=ctf, slab_ranges=[A.defocus_distances(full, g) for g in base],
+              defocus=[A.TiltDefocus(3000.0 + 500 * i, 3200.0 + 500 * i, 20.0 * i, t) for i, t in enumerate(tilts)])
+    ctx = A.ArtiaContext(0)
+    try:
+        plain = [(g, v.clone()) for g, v in subs]
+        print("subvol 4096^2 ctf: plain", flush=True)
+        A.subvol_wbp_plain(ctx, plain, geoms, images, 2, **kw)
+        print("subvol 4096^2 ctf: batched plan", flush=True)
+        batched = [(g, v.clone()) for g, v in subs]
+        plan = A.SubvolPlan(ctx, batched, geoms, images, 2, **kw)
+        plan.execute()
+        torch.cuda.synchronize()
+        plan.close()
+        changed = 0
+        for (_, a), (_, b), (_, v0) in zip(batched, plain, subs):
+            assert torch.equal(a, b)
+            changed += int(not torch.equal(a, v0))
+        assert changed == len(sizes)
     finally:
         ctx.close()
