@@ -461,21 +461,23 @@ __global__ void __launch_bounds__(256, 6) slicer_segments_lds_kernel(VolG v, Til
 __global__ void __launch_bounds__(256) slicer_combine_kernel(const double* __restrict__ part, size_t pstride,
                                                              const int2* __restrict__ range, int W, int H,
                                                              size_t pitch, float* __restrict__ stack, size_t stride,
-                                                             int kbeg, double step) {
+                                                             int kbeg, int kend, double step) {
     const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = (int)blockIdx.y;
     if (x >= W) return;
-    const int k = kbeg + (int)blockIdx.z;
+    // one thread per pixel over the pass's slabs: its sub-rays' slab ranges are read once
     const size_t p4 = 4 * ((size_t)y * (size_t)W + (size_t)x);
-    const double* pk = part + (size_t)blockIdx.z * pstride + p4;
-    double a[4];
+    int2 r[4];
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const int2 r = range[p4 + s];
-        a[s] = (k >= r.x && k <= r.y) ? pk[s] : 0.0;
+    for (int s = 0; s < 4; s++) r[s] = range[p4 + s];
+    for (int k = kbeg; k < kend; k++) {
+        const double* pk = part + (size_t)(k - kbeg) * pstride + p4;
+        double a[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) a[s] = (k >= r[s].x && k <= r[s].y) ? pk[s] : 0.0;
+        const double acc = (a[0] + a[1]) + (a[2] + a[3]);
+        stack[(size_t)k * stride + (size_t)y * pitch + (size_t)x] = 0.0f + (float)(acc * step * 0.25);
     }
-    const double acc = (a[0] + a[1]) + (a[2] + a[3]);
-    stack[(size_t)k * stride + (size_t)y * pitch + (size_t)x] = 0.0f + (float)(acc * step * 0.25);
 }
 
 __global__ void __launch_bounds__(256) vol_traversal_exact_kernel(VolG v, TiltG g, Roi roi, size_t pitch,
@@ -614,8 +616,8 @@ void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData 
                 hipLaunchKernelGGL(slicer_segments_lds_kernel<float>, grid1, dim3(256), 0, ctx->stream, v, g,
                                    static_cast<const float*>(vol.p), W, H, d_bounds, kb, ke, step, part, npix * 4,
                                    range, tp);
-            hipLaunchKernelGGL(slicer_combine_kernel, dim3((unsigned)((W + 255) / 256), (unsigned)H, (unsigned)(ke - kb)),
-                               dim3(256), 0, ctx->stream, part, npix * 4, range, W, H, pitch, stack, stride, kb, dstep);
+            hipLaunchKernelGGL(slicer_combine_kernel, dim3((unsigned)((W + 255) / 256), (unsigned)H), dim3(256), 0,
+                               ctx->stream, part, npix * 4, range, W, H, pitch, stack, stride, kb, ke, dstep);
         }
         return;
     }
