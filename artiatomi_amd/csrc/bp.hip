@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(256) bp_exact_kernel(VolG v, TiltG g, BPExact 
 // update, in the same arithmetic, as in the per-slab launches.  For LINEAR sampling without
 // magnification anisotropy (affine footprint) the block stages its footprint of every slab its
 // quads fall in, when that fits the LDS tile, and fetches from it with tex2d's arithmetic.
-template <typename T>
+template <int kPM, typename T>
 __global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact e, T* __restrict__ vol,
                                                        const float* __restrict__ stack, size_t stride, int W, int H,
                                                        size_t pitch, float lambda, int os, int filter, SlabArgs sl) {
@@ -323,8 +323,8 @@ __global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact 
         const int j1 = min(nJ - 1, (int)floorf((dv + e.vMax) * osf - 0.5f));
         float num = 0.0f, den = 0.0f;
         if (fromTile)
-            voxel_sums_linear_tile(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0, j1, num,
-                                   den);
+            voxel_sums_linear_tile<kPM>(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0, j1,
+                                        num, den);
         else
             voxel_sums_global(e, g, img, W, H, pitch, filter, px0, py0, os, delta, du, dv, i0, i1, j0, j1, num, den);
         den = fmaxf(0.0f, den);
@@ -379,6 +379,22 @@ void launch_bp_t(artia_ctx* ctx, const VolG& v, const TiltG& g, T* vol, const fl
         launch_tile<0>(ctx, grid, v, g, e, vol, img, W, H, pitch, lambda, os, dmin, dmax, filter);
 }
 
+// bp_stack_kernel with the parallel-axis mask fixed at compile time where it is 0 or 2 (launch_tile)
+template <typename T>
+void launch_stack_t(artia_ctx* ctx, dim3 grid, const VolG& v, const TiltG& g, const BPExact& e, T* vol,
+                    const float* stack, size_t stride, int W, int H, size_t pitch, float lambda, int os, int filter,
+                    const SlabArgs& sl) {
+    if (e.parallel == 0)
+        hipLaunchKernelGGL((bp_stack_kernel<0, T>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, stack, stride, W, H,
+                           pitch, lambda, os, filter, sl);
+    else if (e.parallel == 2)
+        hipLaunchKernelGGL((bp_stack_kernel<2, T>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, stack, stride, W, H,
+                           pitch, lambda, os, filter, sl);
+    else
+        hipLaunchKernelGGL((bp_stack_kernel<-1, T>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, stack, stride, W,
+                           H, pitch, lambda, os, filter, sl);
+}
+
 }  // namespace
 
 void launch_bp_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, const float* stack, size_t stride,
@@ -389,11 +405,11 @@ void launch_bp_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol,
     const dim3 grid((unsigned)((nq + 15) / 16), (unsigned)((v.ny + 3) / 4), (unsigned)((v.nz + 3) / 4));
     const SlabArgs sl{bounds, nslabs, last_end};
     if (vol.half)
-        hipLaunchKernelGGL(bp_stack_kernel<vhalf>, grid, dim3(256), 0, ctx->stream, v, g, e,
-                           static_cast<vhalf*>(vol.p), stack, stride, W, H, pitch, lambda, os, filter, sl);
+        launch_stack_t(ctx, grid, v, g, e, static_cast<vhalf*>(vol.p), stack, stride, W, H, pitch, lambda, os, filter,
+                       sl);
     else
-        hipLaunchKernelGGL(bp_stack_kernel<float>, grid, dim3(256), 0, ctx->stream, v, g, e,
-                           static_cast<float*>(vol.p), stack, stride, W, H, pitch, lambda, os, filter, sl);
+        launch_stack_t(ctx, grid, v, g, e, static_cast<float*>(vol.p), stack, stride, W, H, pitch, lambda, os, filter,
+                       sl);
 }
 
 void launch_bp(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, const float* img, int W, int H,

@@ -373,6 +373,7 @@ __device__ __forceinline__ float tile_linear(const float* __restrict__ t, int tw
 
 // voxel_sums_global for LINEAR sampling without magnification anisotropy (the fetch point is the
 // sub-ray's pixel position), reading the image from an LDS tile (tile_linear).
+template <int kPM>
 __device__ __forceinline__ void voxel_sums_linear_tile(const BPExact& e, const float* __restrict__ t, int tw, int tx0,
                                                        int ty0, int W, int H, int px0, int py0, float delta, float du,
                                                        float dv, int i0, int i1, int j0, int j1, float& num,
@@ -384,7 +385,7 @@ __device__ __forceinline__ void voxel_sums_linear_tile(const BPExact& e, const f
         const float bV2 = __builtin_fmaf(b, e.V[2], e.Smin[2]);
         const float a0 = __builtin_fmaf((float)i0 + 0.5f, delta, -du);
         for (int i = i0; i <= i1; i++) {
-            const float chord = chord_at<-1>(e, lattice_a(i, i0, delta, a0), b, bV0, bV1, bV2);
+            const float chord = chord_at<kPM>(e, lattice_a(i, i0, delta, a0), b, bV0, bV1, bV2);
             if (chord > 0.0f) {
                 const float pxf = (float)px0 + ((float)i + 0.5f) * delta;
                 const float pyf = (float)py0 + ((float)j + 0.5f) * delta;

@@ -101,6 +101,7 @@ __device__ __forceinline__ void quad_update(const PairDev& p, const float* __res
 
 // quad_update for a CTF pair whose quad lies in slab k: LINEAR fetches from the slab's LDS tile
 // (pixels [tx0, tx1] x [ty0, ty1]) when the quad's fetches stay inside it, else from global memory.
+template <int kPM>
 __device__ __forceinline__ void quad_update_stack(const PairDev& p, const float* __restrict__ slabTile, bool tiled,
                                                   int tw, int tx0, int ty0, int tx1, int ty1,
                                                   const float* __restrict__ src, int W, int H, size_t pitch, int os,
@@ -126,8 +127,8 @@ __device__ __forceinline__ void quad_update_stack(const PairDev& p, const float*
         const int j1 = min(nJ - 1, (int)floorf((dv + e.vMax) * osf - 0.5f));
         float num = 0.0f, den = 0.0f;
         if (fromTile)
-            voxel_sums_linear_tile(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0, j1, num,
-                                   den);
+            voxel_sums_linear_tile<kPM>(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0, j1,
+                                        num, den);
         else
             voxel_sums_global(e, p.g, src, W, H, pitch, ARTIA_FILTER_LINEAR, px0, py0, os, delta, du, dv, i0, i1, j0,
                               j1, num, den);
@@ -233,8 +234,17 @@ __global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__
                 for (int q = 0; q < kQPT; q++) {
                     if (ks[q] < 0) continue;
                     const unsigned z = (unsigned)(item.z0 + tz + q * item.tz);
-                    quad_update_stack(p, tile + (tiled ? (ks[q] - k0) * plane : 0), tiled, tw, tx0, ty0, tx1, ty1,
-                                      img + (size_t)ks[q] * imgStride, W, H, pitch, os, x, (unsigned)y, z, acc[q]);
+                    const float* st = tile + (tiled ? (ks[q] - k0) * plane : 0);
+                    const float* src = img + (size_t)ks[q] * imgStride;
+                    if (p.e.parallel == 0)
+                        quad_update_stack<0>(p, st, tiled, tw, tx0, ty0, tx1, ty1, src, W, H, pitch, os, x,
+                                             (unsigned)y, z, acc[q]);
+                    else if (p.e.parallel == 2)
+                        quad_update_stack<2>(p, st, tiled, tw, tx0, ty0, tx1, ty1, src, W, H, pitch, os, x,
+                                             (unsigned)y, z, acc[q]);
+                    else
+                        quad_update_stack<-1>(p, st, tiled, tw, tx0, ty0, tx1, ty1, src, W, H, pitch, os, x,
+                                              (unsigned)y, z, acc[q]);
                 }
                 __syncthreads();   // the tile and kr are reused by the next tilt or item
                 continue;
