@@ -304,6 +304,7 @@ __global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact 
     if (nI <= 0 || nJ <= 0) return;
     // the quad's fetches read pixels max(p0 - 1, 0) .. p1 on each axis
     const bool fromTile = tiled && max(px0 - 1, 0) >= tx0 && px1 <= tx1 && max(py0 - 1, 0) >= ty0 && py1 <= ty1;
+    const bool inner = px0 >= 1 && px1 <= W - 1 && py0 >= 1 && py1 <= H - 1;   // no fetch clamps
     const float* __restrict__ slabTile = tile + (fromTile ? (slab - k0) * plane : 0);
     T* vp = vol + ((size_t)z * (size_t)v.ny + (size_t)y) * (size_t)v.nx + x;
     const float4 vb = load_quad(vp);
@@ -322,9 +323,12 @@ __global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact 
         const int j0 = max(0, (int)ceilf((dv + e.vMin) * osf - 0.5f));
         const int j1 = min(nJ - 1, (int)floorf((dv + e.vMax) * osf - 0.5f));
         float num = 0.0f, den = 0.0f;
-        if (fromTile)
-            voxel_sums_linear_tile<kPM>(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0, j1,
-                                        num, den);
+        if (fromTile && inner)
+            voxel_sums_linear_tile<kPM, true>(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0,
+                                              j1, num, den);
+        else if (fromTile)
+            voxel_sums_linear_tile<kPM, false>(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0,
+                                               j1, num, den);
         else
             voxel_sums_global(e, g, img, W, H, pitch, filter, px0, py0, os, delta, du, dv, i0, i1, j0, j1, num, den);
         den = fmaxf(0.0f, den);

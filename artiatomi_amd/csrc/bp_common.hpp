@@ -371,9 +371,24 @@ __device__ __forceinline__ float tile_linear(const float* __restrict__ t, int tw
     return lerpf(r0, r1, b);
 }
 
+// tile_linear where every clamp is a no-op: fetch points with x in [1, W - 1] and y in [1, H - 1],
+// so the corner pixels floor(x - 0.5) and floor(x - 0.5) + 1 lie inside the image (same values).
+__device__ __forceinline__ float tile_linear_inner(const float* __restrict__ t, int tw, int tx0, int ty0, float x,
+                                                   float y) {
+    const float xb = x - 0.5f, yb = y - 0.5f;
+    const float fx = floorf(xb), fy = floorf(yb);
+    const float a = xb - fx, b = yb - fy;
+    const float* __restrict__ r = t + ((int)fy - ty0) * tw + ((int)fx - tx0);
+    const float r0 = lerpf(r[0], r[1], a);
+    const float r1 = lerpf(r[tw], r[tw + 1], a);
+    return lerpf(r0, r1, b);
+}
+
 // voxel_sums_global for LINEAR sampling without magnification anisotropy (the fetch point is the
-// sub-ray's pixel position), reading the image from an LDS tile (tile_linear).
-template <int kPM>
+// sub-ray's pixel position), reading the image from an LDS tile (tile_linear).  kInner: the quad's
+// footprint satisfies 1 <= px0, px1 <= W - 1, 1 <= py0, py1 <= H - 1, so every fetch point (in
+// [p0 + delta/2, p1 - delta/2]) is one tile_linear_inner takes.
+template <int kPM, bool kInner>
 __device__ __forceinline__ void voxel_sums_linear_tile(const BPExact& e, const float* __restrict__ t, int tw, int tx0,
                                                        int ty0, int W, int H, int px0, int py0, float delta, float du,
                                                        float dv, int i0, int i1, int j0, int j1, float& num,
@@ -389,7 +404,8 @@ __device__ __forceinline__ void voxel_sums_linear_tile(const BPExact& e, const f
             if (chord > 0.0f) {
                 const float pxf = (float)px0 + ((float)i + 0.5f) * delta;
                 const float pyf = (float)py0 + ((float)j + 0.5f) * delta;
-                const float val = tile_linear(t, tw, tx0, ty0, W, H, pxf, pyf);
+                const float val = kInner ? tile_linear_inner(t, tw, tx0, ty0, pxf, pyf)
+                                         : tile_linear(t, tw, tx0, ty0, W, H, pxf, pyf);
                 num = __builtin_fmaf(val, chord, num);
                 den += chord;
             }

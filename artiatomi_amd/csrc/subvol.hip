@@ -113,6 +113,7 @@ __device__ __forceinline__ void quad_update_stack(const PairDev& p, const float*
     if (nI <= 0 || nJ <= 0) return;
     // the quad's fetches read pixels max(p0 - 1, 0) .. p1 on each axis
     const bool fromTile = tiled && max(px0 - 1, 0) >= tx0 && px1 <= tx1 && max(py0 - 1, 0) >= ty0 && py1 <= ty1;
+    const bool inner = px0 >= 1 && px1 <= W - 1 && py0 >= 1 && py1 <= H - 1;   // no fetch clamps
     const float osf = (float)os, delta = 1.0f / osf;
     const float du0 = (float)(e.pu0 + (double)x * e.pui + (double)y * e.puj + (double)z * e.puk - (double)px0);
     const float dv0 = (float)(e.pv0 + (double)x * e.pvi + (double)y * e.pvj + (double)z * e.pvk - (double)py0);
@@ -126,9 +127,12 @@ __device__ __forceinline__ void quad_update_stack(const PairDev& p, const float*
         const int j0 = max(0, (int)ceilf((dv + e.vMin) * osf - 0.5f));
         const int j1 = min(nJ - 1, (int)floorf((dv + e.vMax) * osf - 0.5f));
         float num = 0.0f, den = 0.0f;
-        if (fromTile)
-            voxel_sums_linear_tile<kPM>(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0, j1,
-                                        num, den);
+        if (fromTile && inner)
+            voxel_sums_linear_tile<kPM, true>(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0,
+                                              j1, num, den);
+        else if (fromTile)
+            voxel_sums_linear_tile<kPM, false>(e, slabTile, tw, tx0, ty0, W, H, px0, py0, delta, du, dv, i0, i1, j0,
+                                               j1, num, den);
         else
             voxel_sums_global(e, p.g, src, W, H, pitch, ARTIA_FILTER_LINEAR, px0, py0, os, delta, du, dv, i0, i1, j0,
                               j1, num, den);
