@@ -227,19 +227,23 @@ __global__ void __launch_bounds__(256) wbp_weight_kernel(float2* __restrict__ sp
 }
 
 // out (+)= crop(x, y) * (square[by + y][bx + x] / N^2)  (nppiDivC + cropBorder + nppiAdd); over a
-// stack, square blockIdx.z into out + blockIdx.z * out_stride
-__global__ void __launch_bounds__(256) finish_kernel(const float* __restrict__ sq, int N, int bx, int by, int W, int H,
-                                                     float* __restrict__ out, size_t pitch, float norm, int accumulate,
-                                                     int crop, CropParams cp, size_t out_stride) {
+// stack of ns squares, square s into out + s * out_stride
+__global__ void __launch_bounds__(256) finish_kernel(const float* __restrict__ sq, int ns, int N, int bx, int by, int W,
+                                                     int H, float* __restrict__ out, size_t pitch, float norm,
+                                                     int accumulate, int crop, CropParams cp, size_t out_stride) {
     const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
     if (x >= W || y >= H) return;
-    sq += (size_t)blockIdx.z * (size_t)N * (size_t)N;
-    out += (size_t)blockIdx.z * out_stride;
-    float v = sq[(size_t)(by + y) * N + (bx + x)] / norm;
-    if (crop) v = crop_weight(x, y, W, H, cp) * v;
+    // the crop weight depends on the pixel only: evaluated once for the ns squares
+    const float w = crop ? crop_weight(x, y, W, H, cp) : 1.0f;
+    const size_t j = (size_t)(by + y) * N + (bx + x), plane = (size_t)N * (size_t)N;
     const size_t i = (size_t)y * pitch + x;
-    out[i] = accumulate ? out[i] + v : v;
+    for (int s = 0; s < ns; s++) {
+        float v = sq[(size_t)s * plane + j] / norm;
+        if (crop) v = w * v;
+        float* o = out + (size_t)s * out_stride;
+        o[i] = accumulate ? o[i] + v : v;
+    }
 }
 
 // out += crop(x, y) * (square_s[by + y][bx + x] / N^2) for s = 0 .. ns-1 in turn: the sum, rounded
@@ -456,7 +460,7 @@ artia_status ctf_slab_filter(artia_ctx* ctx, float* img, int W, int H, size_t pi
     if (hipfftExecC2R(ctx->fft.c2r, (hipfftComplex*)ctx->fft.spec, ctx->fft.square) != HIPFFT_SUCCESS)
         return fail(ctx, ARTIA_ERR_FFT, "hipfftExecC2R failed");
     const CropParams cp = corners ? make_crop_params(cut, dim, corners) : CropParams{};
-    hipLaunchKernelGGL(finish_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, ctx->fft.square, N, (N - W) / 2,
+    hipLaunchKernelGGL(finish_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, ctx->fft.square, 1, N, (N - W) / 2,
                        (N - H) / 2, W, H, out, out_pitch, (float)(N * N), accumulate, corners ? 1 : 0, cp, (size_t)0);
     return check_hip(ctx, hipGetLastError(), "slab filter");
 }
@@ -502,9 +506,8 @@ artia_status ctf_filter_slabs(artia_ctx* ctx, const float* src, size_t src_strid
             hipLaunchKernelGGL(finish_sum_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, fb.square, b, N, bx, by, W,
                                H, out, pitch, (float)(N * N), crop, cp);
         else
-            hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((W + 63) / 64), (unsigned)((H + 3) / 4), (unsigned)b),
-                               dim3(64, 4), 0, ctx->stream, fb.square, N, bx, by, W, H, out + (size_t)k0 * out_stride,
-                               pitch, (float)(N * N), 0, crop, cp, out_stride);
+            hipLaunchKernelGGL(finish_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, fb.square, b, N, bx, by, W, H,
+                               out + (size_t)k0 * out_stride, pitch, (float)(N * N), 0, crop, cp, out_stride);
         st = check_hip(ctx, hipGetLastError(), "stacked slab filter");
         if (st != ARTIA_OK) return st;
         k0 += b;
@@ -528,7 +531,7 @@ artia_status wbp_filter(artia_ctx* ctx, const float* img, int W, int H, size_t p
     if (hipfftExecC2R(ctx->fft.c2r, (hipfftComplex*)ctx->fft.spec, ctx->fft.square) != HIPFFT_SUCCESS)
         return fail(ctx, ARTIA_ERR_FFT, "hipfftExecC2R failed");
     const CropParams cp{};
-    hipLaunchKernelGGL(finish_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, ctx->fft.square, N, (N - W) / 2,
+    hipLaunchKernelGGL(finish_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, ctx->fft.square, 1, N, (N - W) / 2,
                        (N - H) / 2, W, H, out, out_pitch, (float)(N * N), 0, 0, cp, (size_t)0);
     return check_hip(ctx, hipGetLastError(), "wbp filter");
 }
