@@ -378,7 +378,7 @@ def bench_ctf(args) -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "kernels": per_kernel,
     }
-    dom_kernel = "slicer_segments_lds_kernel<float>" if dom == "fp" else "bp_stack_kernel<float>"
+    dom_kernel = "slicer_segments_lds_kernel<float>" if dom == "fp" else "bp_stack_kernel<2, float>"
     measured = measured_traffic(args.workload, dom_kernel)
     if measured:
         out["roofline"]["traffic"] = measured.pop("bytes")
