@@ -258,7 +258,7 @@ def cpu_baseline(vg, acq, vol_dev, projs_dev, w):
     ti = len(acq.geoms) // 4   # a tilt of about -30 deg
     g = acq.geoms[ti]
     vol = vol_dev.detach().float().cpu().numpy()
-    rows = 8 if W >= 1024 else 32
+    rows = H if W >= 1024 else 64   # one whole projection at 1024^2: about 6 s
     y0 = H // 2 - rows // 2
     t0 = time.perf_counter()
     proj, dist = O.march(vg, g, vol, W, H, dtype=np.float32, roi=(0, y0, W, y0 + rows))
@@ -269,7 +269,7 @@ def cpu_baseline(vg, acq, vol_dev, projs_dev, w):
     res = O.compare(real, np.zeros((H, W), np.float32), dist_full, acq.real_lengths[ti])
     res = O.crop_border(res, acq.hitpoints[ti])
     t_cmp = time.perf_counter() - t0
-    zs = 2 if nz >= 128 else 8
+    zs = 128 if nz >= 128 else 16   # half the slices at 1024^2 x 256: about 5.5 s
     z0 = nz // 2
     slab = type(vg)()
     C.memmove(C.byref(slab), C.byref(vg), C.sizeof(vg))
