@@ -70,6 +70,13 @@ public:
                 float* proj, float* dist, const int* roi = nullptr) {
         c_.check(artia_fp_march(c_.get(), &vg, &tg, vol, w, h, pitch, proj, dist, roi, nullptr), "FPKernel");
     }
+    // a half volume (FP16Volume): the volume pointer holds IEEE half bits
+    float operator()(const artia_vol_geom& vg, const artia_tilt_geom& tg, const uint16_t* vol, int w, int h,
+                     size_t pitch, float* proj, float* dist, const int* roi = nullptr) {
+        float ms = 0.0f;
+        c_.check(artia_fp_march_f16(c_.get(), &vg, &tg, vol, w, h, pitch, proj, dist, roi, &ms), "FPKernel");
+        return ms;
+    }
 
 private:
     Context& c_;
@@ -83,6 +90,13 @@ public:
                      size_t pitch, float* proj, float tmin, float tmax, const int* roi = nullptr) {
         float ms = 0.0f;
         c_.check(artia_fp_slicer(c_.get(), &vg, &tg, vol, w, h, pitch, proj, tmin, tmax, roi, &ms), "SlicerKernel");
+        return ms;
+    }
+    float operator()(const artia_vol_geom& vg, const artia_tilt_geom& tg, const uint16_t* vol, int w, int h,
+                     size_t pitch, float* proj, float tmin, float tmax, const int* roi = nullptr) {
+        float ms = 0.0f;
+        c_.check(artia_fp_slicer_f16(c_.get(), &vg, &tg, vol, w, h, pitch, proj, tmin, tmax, roi, &ms),
+                 "SlicerKernel");
         return ms;
     }
 
@@ -145,6 +159,16 @@ public:
         float ms = 0.0f;
         c_.check(artia_bp(c_.get(), &vg, &tg, vol, img, w, h, pitch, lambda, oversampling, dist_min, dist_max,
                           filter, &ms),
+                 "BPKernel");
+        return ms;
+    }
+    // BPKernel(module, fp16 = true), backProjectionFP16 (Kernels.cpp:327-334): a half volume
+    float operator()(const artia_vol_geom& vg, const artia_tilt_geom& tg, uint16_t* vol, const float* img, int w,
+                     int h, size_t pitch, float lambda, int oversampling, float dist_min, float dist_max,
+                     artia_filter filter) {
+        float ms = 0.0f;
+        c_.check(artia_bp_f16(c_.get(), &vg, &tg, vol, img, w, h, pitch, lambda, oversampling, dist_min, dist_max,
+                              filter, &ms),
                  "BPKernel");
         return ms;
     }
