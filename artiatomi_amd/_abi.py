@@ -164,7 +164,7 @@ SIGNATURES = {
     "artia_geom_hitpoints": (_I, [C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI]),
     "artia_geom_real_length": (_I, [C.POINTER(VolGeom), C.POINTER(TiltGeom), _PF]),
     "artia_projection_index_list": (_I, [_PF, _I, _PI]),
-    "artia_marker_tilt_params": (_I, [_PF, _I, _I, _I, _I, C.POINTER(TiltParams), _PI]),
+    "artia_marker_tilt_params": (_I, [_PF, _I, _I, _I, _I, _I, C.POINTER(TiltParams), _PI]),
     "artia_ctf_file_defocus": (_I, [_PF, _I, _I, _I, _F, C.POINTER(TiltDefocus)]),
     "artia_geom_defocus_distances": (_I, [C.POINTER(VolGeom), C.POINTER(TiltGeom), _PF, _PF]),
     "artia_geom_defocus_minmax": (_I, [_F, C.POINTER(TiltDefocus), C.POINTER(CtfParams), _PF, _PF]),
@@ -172,6 +172,10 @@ SIGNATURES = {
     "artia_fp_slicer": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _SZ, _P, _F, _F, _PI, _PF]),
     "artia_vol_traversal": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _I, _I, _SZ, _P, _PI, _PF]),
     "artia_compare": (_I, [_P, _I, _I, _SZ, _P, _P, _P, _F, _PF, _PF, _F, _PF]),
+    "artia_fp_residual": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _I, _SZ, _P, _F, _PF, _PF, _F,
+                               _PF, _PF, _PI, _P, _P, _PF]),
+    "artia_fp_residual_f16": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _I, _SZ, _P, _F, _PF, _PF,
+                                   _F, _PF, _PF, _PI, _P, _P, _PF]),
     "artia_crop_border": (_I, [_P, _I, _I, _SZ, _P, _PF, _PF, _PI, _PF]),
     "artia_bp": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _P, _I, _I, _SZ, _F, _I, _F, _F, _I, _PF]),
     "artia_make_square": (_I, [_P, _I, _I, _SZ, _P, _I, _P, _I, _I, _I, _I, _PF]),

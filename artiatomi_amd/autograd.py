@@ -58,6 +58,9 @@ class _Project(torch.autograd.Function):
     def forward(ctx, vol: torch.Tensor, shifts: torch.Tensor, setup: ProjectionSetup):
         if setup.ctx.numerics != "exact":
             raise core.ArtiaError("the differentiable projector needs an ARTIA_NUMERICS_EXACT context")
+        if vol.dtype != torch.float32:
+            # the adjoint and shift-gradient kernels read fp32 volumes only
+            raise core.ArtiaError(f"the differentiable projector takes a float32 volume, got {vol.dtype}")
         vol = vol.contiguous()
         geoms = setup.geometries(shifts)
         T = len(geoms)

@@ -8,11 +8,13 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libartia.so"
+OBJ = PKG / "_build"
 
 SOURCES = [
     "context.hip",
@@ -32,16 +34,16 @@ SOURCES = [
 # -ffp-contract=off: the reference-numerics kernels keep the reference's float operation order;
 # explicit fmaf is used where a fused multiply-add is intended.
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
-HIP_FLAGS = [
+COMPILE_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+LINK_FLAGS = ["--offload-arch=gfx950", "-shared", "-fPIC"]
 
 
 def hipcc() -> str:
@@ -51,20 +53,43 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found; the HIP library cannot be built")
 
 
-def build_library(verbose: bool = False) -> Path:
+def _compile(src: Path, obj: Path) -> tuple:
+    cmd = [hipcc(), *COMPILE_FLAGS, "-I", str(PKG.parent / "include"), "-c", "-o", str(obj), str(src)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    return src.name, res.returncode, res.stdout + res.stderr, " ".join(cmd)
+
+
+def build_library(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> Path:
+    """Compiles every source to an object under artiatomi_amd/_build (in parallel), then links
+    libartia.so.  incremental=True recompiles only sources newer than their object, or all of them
+    when a header is newer than the oldest object."""
+    OBJ.mkdir(exist_ok=True)
+    todo = []
+    headers = max(p.stat().st_mtime for p in list(CSRC.glob("*.hpp")) + list((PKG.parent / "include").glob("*.h")))
+    for s in SOURCES:
+        src, obj = CSRC / s, OBJ / (s + ".o")
+        if (not incremental or not obj.exists() or obj.stat().st_mtime < src.stat().st_mtime
+                or obj.stat().st_mtime < headers):
+            todo.append((src, obj))
+    jobs = jobs or min(len(todo), os.cpu_count() or 4, 16) or 1
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for name, rc, out, cmd in pool.map(lambda a: _compile(*a), todo):
+            if verbose:
+                print(cmd, flush=True)
+            if rc != 0:
+                sys.stderr.write(out)
+                raise RuntimeError(f"hipcc failed on {name} with exit code {rc}")
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIP_FLAGS, "-I", str(PKG.parent / "include"), "-o", str(tmp)]
-    cmd += [str(CSRC / s) for s in SOURCES]
-    cmd += ["-lhipfft"]
+    cmd = [hipcc(), *LINK_FLAGS, "-o", str(tmp)] + [str(OBJ / (s + ".o")) for s in SOURCES] + ["-lhipfft"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+        raise RuntimeError(f"hipcc link failed with exit code {res.returncode}")
     os.replace(tmp, LIB)
     return LIB
 
 
 if __name__ == "__main__":
-    print(build_library(verbose=True))
+    print(build_library(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

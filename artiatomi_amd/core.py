@@ -214,8 +214,8 @@ def marker_tilt_params(marker_data, proj: int, ref_marker: int = 0):
         raise ArtiaError("marker data must be (markers, projections, items)")
     tp, good = TiltParams(), C.c_int()
     lib = _abi.load()
-    st = lib.artia_marker_tilt_params(a.ctypes.data_as(C.POINTER(C.c_float)), a.shape[2], a.shape[1], ref_marker,
-                                      proj, C.byref(tp), C.byref(good))
+    st = lib.artia_marker_tilt_params(a.ctypes.data_as(C.POINTER(C.c_float)), a.shape[2], a.shape[1], a.shape[0],
+                                      ref_marker, proj, C.byref(tp), C.byref(good))
     _check(st, what="artia_marker_tilt_params")
     return tp, bool(good.value)
 
@@ -405,6 +405,23 @@ class ArtiaContext:
         return self._call("artia_compare", W, H, W, _ptr(real, "real", W * H), _ptr(virt, "virt"),
                           _ptr(dist, "dist", W * H), float(real_length), _abi.farr(crop, 4),
                           _abi.farr(crop_dim, 4), float(scale), timed=timed)
+
+    def fp_residual(self, vg: VolGeom, tg: TiltGeom, vol, real, residual, real_length, crop=(0, 0, 0, 0),
+                    crop_dim=(0, 0, 0, 0), scale=1.0, points=None, cut_length=(0, 0), dim_length=(0, 0),
+                    dist=None, volume_is_empty=False, timed=False):
+        """residual = cropBorder(compare(real, FP(vol), mean chord)) in the forward projector's
+        epilogue (artia_fp_residual); points=None skips cropBorder.  dist (H, W) receives the chord
+        for an empty volume or under reference numerics."""
+        H, W = residual.shape
+        if vol is None:
+            vp, sfx = None, ""
+        else:
+            vp, sfx = _vol(vol, vg)
+        return self._call("artia_fp_residual" + sfx, C.byref(vg), C.byref(tg), vp, int(bool(volume_is_empty)), W, H, W,
+                          _ptr(real, "real", W * H), float(real_length), _abi.farr(crop, 4), _abi.farr(crop_dim, 4),
+                          float(scale), _abi.farr(cut_length, 2), _abi.farr(dim_length, 2),
+                          None if points is None else _abi.iarr(points, 8), _ptr(residual, "residual"),
+                          None if dist is None else _ptr(dist, "dist", W * H), timed=timed)
 
     def crop_border(self, img, points, cut_length=(0, 0), dim_length=(0, 0), timed=False):
         H, W = img.shape

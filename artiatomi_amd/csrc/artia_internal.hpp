@@ -122,6 +122,14 @@ bool valid_geometry(const artia_vol_geom* vg, const artia_tilt_geom* tg);
 // Kernel launchers used by the drivers (no argument validation, stream = ctx->stream)
 void launch_march(artia_ctx* ctx, const VolG& vg, const TiltG& tg, VolData vol, int W, int H,
                   size_t pitch, float* proj, float* dist, int x0, int y0, int x1, int y1);
+// Forward projection from zero straight into the SART residual: compare (real, Crop, CropDim,
+// ProjectionScaleFactor) and, when corners != nullptr, cropBorder (cut, dim, corners), bit-identical
+// to fill + march + compare + crop_border.  Exact numerics need no dist image (dist may be null);
+// reference numerics run those launches and need dist.
+void launch_march_residual(artia_ctx* ctx, const VolG& vg, const TiltG& tg, VolData vol, int W, int H, size_t pitch,
+                           const float* real, float realLength, const float crop[4], const float cropDim[4],
+                           float scale, const float cut[2], const float dim[2], const int* corners, float* resid,
+                           float* dist);
 void launch_vol_traversal(artia_ctx* ctx, const VolG& vg, const TiltG& tg, int W, int H, size_t pitch,
                           float* dist, int x0, int y0, int x1, int y1);
 void launch_compare(artia_ctx* ctx, int W, int H, size_t pitch, const float* real, float* virt,

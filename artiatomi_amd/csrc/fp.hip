@@ -19,6 +19,7 @@
 
 #include "device_common.hpp"
 #include "fp_common.hpp"
+#include "image_ops.hpp"
 #include "tube_walk.hpp"
 
 namespace artia {
@@ -213,14 +214,19 @@ __device__ __forceinline__ MarchLane march_lane(const VolG& v, const TiltG& g, c
     return L;
 }
 
-// The pixel's mean chord over its four sub-rays (lanes 4p .. 4p+3) into dist.
-__device__ __forceinline__ void march_store_dist(double len, double hits, bool active, int sub, int x, int y,
-                                                 size_t pitch, float* __restrict__ dist) {
+// The pixel's mean chord over its four sub-rays (lanes 4p .. 4p+3), as dist holds it.
+__device__ __forceinline__ float pixel_dist(double len, double hits) {
     len += __shfl_xor(len, 1);
     len += __shfl_xor(len, 2);
     hits += __shfl_xor(hits, 1);
     hits += __shfl_xor(hits, 2);
-    if (active && sub == 0) dist[(size_t)y * pitch + (size_t)x] = hits > 0.0 ? (float)(len / hits) : 0.0f;
+    return hits > 0.0 ? (float)(len / hits) : 0.0f;
+}
+
+__device__ __forceinline__ void march_store_dist(double len, double hits, bool active, int sub, int x, int y,
+                                                 size_t pitch, float* __restrict__ dist) {
+    const float d = pixel_dist(len, hits);
+    if (active && sub == 0) dist[(size_t)y * pitch + (size_t)x] = d;
 }
 
 // Sum of a pixel's four sub-rays (lanes 4p .. 4p+3) and the pixel's outputs.
@@ -230,25 +236,32 @@ __device__ __forceinline__ void march_store(double acc, double len, double hits,
                                             float* __restrict__ dist) {
     acc += __shfl_xor(acc, 1);
     acc += __shfl_xor(acc, 2);
-    if (!kSlicer) {
-        len += __shfl_xor(len, 1);
-        len += __shfl_xor(len, 2);
-        hits += __shfl_xor(hits, 1);
-        hits += __shfl_xor(hits, 2);
-    }
+    const float d = kSlicer ? 0.0f : pixel_dist(len, hits);
     if (active && sub == 0) {
         const size_t i = (size_t)y * pitch + (size_t)x;
         proj[i] += (float)(acc * step * 0.25);
-        if (!kSlicer) dist[i] = hits > 0.0 ? (float)(len / hits) : 0.0f;
+        if (!kSlicer) dist[i] = d;
     }
+}
+
+// The residual epilogue: the pixel's projection (from zero) and mean chord go straight into
+// compare and cropBorder (residual_value); out receives the residual instead of the projection.
+__device__ __forceinline__ void march_store_resid(double acc, float dist, bool active, int sub, int x, int y,
+                                                  size_t pitch, double step, const ResidEpi& e, int W, int H,
+                                                  float* __restrict__ out) {
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    if (active && sub == 0)
+        out[(size_t)y * pitch + (size_t)x] = residual_value(e, x, y, W, H, pitch, (float)(acc * step * 0.25), dist);
 }
 
 // One lane per sub-ray: a wave covers 4x4 pixels x 4 sub-rays, a 256-thread block 8x8 pixels,
 // so the rays of a wave stay within a few voxels of each other through the whole volume.
-template <bool kSlicer, bool kBig, typename T>
+template <bool kSlicer, bool kBig, bool kResid, typename T>
 __global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const T* __restrict__ vol, Roi roi,
                                                           size_t pitch, float* __restrict__ proj,
-                                                          float* __restrict__ dist, float tminDef, float tmaxDef) {
+                                                          float* __restrict__ dist, float tminDef, float tmaxDef,
+                                                          ResidEpi epi, int W, int H) {
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     const int sub = lane & 3, pix = lane >> 2;
     const int x = roi.x0 + (int)blockIdx.x * 8 + (wave & 1) * 4 + (pix & 3);
@@ -271,7 +284,10 @@ __global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const
             acc += tot;
         }
     }
-    march_store<kSlicer>(acc, L.len, L.hits, active, sub, x, y, pitch, step, proj, dist);
+    if (kResid)
+        march_store_resid(acc, pixel_dist(L.len, L.hits), active, sub, x, y, pitch, step, epi, W, H, proj);
+    else
+        march_store<kSlicer>(acc, L.len, L.hits, active, sub, x, y, pitch, step, proj, dist);
 }
 
 // LDS-staged form of march_exact_kernel (tube_walk.hpp) for volumes of at least 2 voxels per
@@ -279,11 +295,11 @@ __global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const
 // paused at slab boundaries), so the projection is bit-identical.  Six waves per SIMD (80
 // registers): the walk is VALU bound, and the occupancy hides the LDS latency better than the few
 // registers it spills.
-template <bool kSlicer, typename T>
+template <bool kSlicer, bool kResid, typename T>
 __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, const T* __restrict__ vol, Roi roi,
                                                         size_t pitch, float* __restrict__ proj,
                                                         float* __restrict__ dist, float tminDef, float tmaxDef,
-                                                        TubePlan tp) {
+                                                        TubePlan tp, ResidEpi epi, int W, int H) {
     __shared__ float chunk[kChunkFloats];
     __shared__ int zr[2];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
@@ -302,7 +318,10 @@ __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, cons
     const bool active = x < roi.x1 && y < roi.y1;
     const double step = (double)v.vs[0] * 0.15;
     const MarchLane L = march_lane<kSlicer>(v, g, vol, active, x, y, sub, step, tminDef, tmaxDef);
-    if (!kSlicer) march_store_dist(L.len, L.hits, active, sub, x, y, pitch, dist);   // before the walk
+    // the mean chord before the walk: stored, or kept for the residual epilogue
+    float dval = 0.0f;
+    if (kResid) dval = pixel_dist(L.len, L.hits);
+    else if (!kSlicer) march_store_dist(L.len, L.hits, active, sub, x, y, pitch, dist);
     const TubeDir td = tube_dir(v, g, step);
     const TubeBox tb = tube_box(v, g, tp, (double)tx0 + 0.25, (double)tx0 + 7.75, (double)ty0 + 0.25,
                                 (double)ty0 + 7.75, 0.0, 0.0);
@@ -312,7 +331,10 @@ __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, cons
     });
     double acc = L.acc;
     if (L.count > 0) acc += total;
-    march_store<true>(acc, 0.0, 0.0, active, sub, x, y, pitch, step, proj, nullptr);   // proj only
+    if (kResid)
+        march_store_resid(acc, dval, active, sub, x, y, pitch, step, epi, W, H, proj);
+    else
+        march_store<true>(acc, 0.0, 0.0, active, sub, x, y, pitch, step, proj, nullptr);   // proj only
 }
 
 // Every CTF slab of a tilt in one pass: slab k (kbeg <= k < kend) covers bounds[k] .. bounds[k] +
@@ -508,6 +530,45 @@ dim3 grid_for(const Roi& r) {
     return dim3((unsigned)((w + 63) / 64), (unsigned)((h + 3) / 4));
 }
 
+template <bool kResid, typename T>
+void launch_exact_t(artia_ctx* ctx, const VolG& v, const TiltG& g, const T* vol, const Roi& r, size_t pitch,
+                    float* proj, float* dist, bool slicer, float tmin, float tmax, const ResidEpi& epi, int W, int H) {
+    const dim3 grid((unsigned)((r.x1 - r.x0 + 7) / 8), (unsigned)((r.y1 - r.y0 + 7) / 8)), block(256);
+    // 32-bit buffer offsets need fewer than 2^30 voxels and a row index z * ny + y below 2^24
+    const bool big = (uint64_t)v.nx * (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 30) ||
+                     (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 24) || v.nx >= (1 << 24);
+    TubePlan tp;
+    if (ctx->march_lds && tube_plan(v, g, tp)) {
+        tp.tiles_x = (int)grid.x;
+        tp.tiles_y = (int)grid.y;
+        tp.groups_x = (tp.tiles_x + kXcdGroup - 1) / kXcdGroup;
+        const int groups = tp.groups_x * ((tp.tiles_y + kXcdGroup - 1) / kXcdGroup);
+        const dim3 grid1((unsigned)(((groups + 7) / 8) * 8 * kXcdGroup * kXcdGroup));
+        if (slicer)
+            hipLaunchKernelGGL((march_lds_kernel<true, false, T>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch,
+                               proj, dist, tmin, tmax, tp, epi, W, H);
+        else
+            hipLaunchKernelGGL((march_lds_kernel<false, kResid, T>), grid1, block, 0, ctx->stream, v, g, vol, r,
+                               pitch, proj, dist, tmin, tmax, tp, epi, W, H);
+        return;
+    }
+    if (slicer) {
+        if (big)
+            hipLaunchKernelGGL((march_exact_kernel<true, true, false, T>), grid, block, 0, ctx->stream, v, g, vol, r,
+                               pitch, proj, dist, tmin, tmax, epi, W, H);
+        else
+            hipLaunchKernelGGL((march_exact_kernel<true, false, false, T>), grid, block, 0, ctx->stream, v, g, vol, r,
+                               pitch, proj, dist, tmin, tmax, epi, W, H);
+    } else {
+        if (big)
+            hipLaunchKernelGGL((march_exact_kernel<false, true, kResid, T>), grid, block, 0, ctx->stream, v, g, vol, r,
+                               pitch, proj, dist, tmin, tmax, epi, W, H);
+        else
+            hipLaunchKernelGGL((march_exact_kernel<false, false, kResid, T>), grid, block, 0, ctx->stream, v, g, vol,
+                               r, pitch, proj, dist, tmin, tmax, epi, W, H);
+    }
+}
+
 template <typename T>
 void launch_march_t(artia_ctx* ctx, const VolG& v, const TiltG& g, const T* vol, const Roi& r, size_t pitch,
                     float* proj, float* dist, bool slicer, float tmin, float tmax) {
@@ -522,40 +583,7 @@ void launch_march_t(artia_ctx* ctx, const VolG& v, const TiltG& g, const T* vol,
                                dist, tmin, tmax);
         return;
     }
-    const dim3 grid((unsigned)((r.x1 - r.x0 + 7) / 8), (unsigned)((r.y1 - r.y0 + 7) / 8)), block(256);
-    // 32-bit buffer offsets need fewer than 2^30 voxels and a row index z * ny + y below 2^24
-    const bool big = (uint64_t)v.nx * (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 30) ||
-                     (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 24) || v.nx >= (1 << 24);
-    TubePlan tp;
-    if (ctx->march_lds && tube_plan(v, g, tp)) {
-        tp.tiles_x = (int)grid.x;
-        tp.tiles_y = (int)grid.y;
-        tp.groups_x = (tp.tiles_x + kXcdGroup - 1) / kXcdGroup;
-        const int groups = tp.groups_x * ((tp.tiles_y + kXcdGroup - 1) / kXcdGroup);
-        const dim3 grid1((unsigned)(((groups + 7) / 8) * 8 * kXcdGroup * kXcdGroup));
-        if (slicer)
-            hipLaunchKernelGGL((march_lds_kernel<true, T>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch, proj,
-                               dist, tmin, tmax, tp);
-        else
-            hipLaunchKernelGGL((march_lds_kernel<false, T>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch, proj,
-                               dist, tmin, tmax, tp);
-        return;
-    }
-    if (slicer) {
-        if (big)
-            hipLaunchKernelGGL((march_exact_kernel<true, true, T>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
-                               proj, dist, tmin, tmax);
-        else
-            hipLaunchKernelGGL((march_exact_kernel<true, false, T>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
-                               proj, dist, tmin, tmax);
-    } else {
-        if (big)
-            hipLaunchKernelGGL((march_exact_kernel<false, true, T>), grid, block, 0, ctx->stream, v, g, vol, r, pitch,
-                               proj, dist, tmin, tmax);
-        else
-            hipLaunchKernelGGL((march_exact_kernel<false, false, T>), grid, block, 0, ctx->stream, v, g, vol, r,
-                               pitch, proj, dist, tmin, tmax);
-    }
+    launch_exact_t<false>(ctx, v, g, vol, r, pitch, proj, dist, slicer, tmin, tmax, ResidEpi{}, 0, 0);
 }
 
 void launch_march_impl(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, const Roi& r, size_t pitch,
@@ -572,6 +600,35 @@ void launch_march(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, in
                   float* proj, float* dist, int x0, int y0, int x1, int y1) {
     const int roi[4] = {x0, y0, x1, y1};
     launch_march_impl(ctx, v, g, vol, make_roi(roi, W, H), pitch, proj, dist, false, 0.0f, 0.0f);
+}
+
+void launch_march_residual(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, int W, int H, size_t pitch,
+                           const float* real, float realLength, const float crop[4], const float cropDim[4],
+                           float scale, const float cut[2], const float dim[2], const int* corners, float* resid,
+                           float* dist) {
+    if (ctx->numerics == ARTIA_NUMERICS_REFERENCE) {
+        // the reference's three launches (the kernel bodies keep their own float order)
+        launch_fill(ctx, resid, (size_t)H * pitch, 0.0f);
+        launch_march(ctx, v, g, vol, W, H, pitch, resid, dist, 0, 0, W, H);
+        launch_compare(ctx, W, H, pitch, real, resid, dist, realLength, crop, cropDim, scale);
+        if (corners) launch_crop_border(ctx, W, H, pitch, resid, cut, dim, corners);
+        return;
+    }
+    ResidEpi e{};
+    e.real = real;
+    e.L = realLength;
+    e.cut = F4{crop[0], crop[1], crop[2], crop[3]};
+    e.dim = F4{cropDim[0], cropDim[1], cropDim[2], cropDim[3]};
+    e.S = scale;
+    e.crop_on = corners ? 1 : 0;
+    if (corners) e.crop = make_crop_params(cut, dim, corners);
+    const Roi r{0, 0, W, H};
+    if (vol.half)
+        launch_exact_t<true>(ctx, v, g, static_cast<const vhalf*>(vol.p), r, pitch, resid, nullptr, false, 0.0f, 0.0f,
+                             e, W, H);
+    else
+        launch_exact_t<true>(ctx, v, g, static_cast<const float*>(vol.p), r, pitch, resid, nullptr, false, 0.0f, 0.0f,
+                             e, W, H);
 }
 
 void launch_slicer(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, int W, int H, size_t pitch,
@@ -695,6 +752,59 @@ artia_status artia_fp_slicer_f16(artia_ctx* ctx, const artia_vol_geom* vg, const
                                  float tmax, const int* roi, float* ms) {
     return fp_entry(ctx, "artia_fp_slicer_f16", vg, tg, VolData::f16(vol), width, height, pitch, proj, nullptr, roi,
                     true, tmin, tmax, ms);
+}
+
+namespace {
+artia_status fp_residual_entry(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                               VolData vol, int volume_is_empty, int width, int height, size_t pitch, const float* real,
+                               float real_length, const float crop[4], const float crop_dim[4], float proj_scale,
+                               const float cut_length[2], const float dim_length[2], const int* hitpoints,
+                               float* residual, float* dist, float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!valid_geometry(vg, tg) || (!vol.p && !volume_is_empty) || !real || !crop || !crop_dim || !residual ||
+        (hitpoints && (!cut_length || !dim_length)))
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: null pointer or invalid geometry", fn);
+    if (!dist && (volume_is_empty || ctx->numerics == ARTIA_NUMERICS_REFERENCE))
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: an empty volume or reference numerics need the dist image",
+                    fn);
+    artia_status st = check_image(ctx, fn, width, height, pitch);
+    if (st != ARTIA_OK) return st;
+    Timing t(ctx, ms);
+    const VolG v = to_device(*vg);
+    const TiltG g = to_device(*tg);
+    if (volume_is_empty) {
+        // EmSart.cpp:598-601: no projection of an empty volume, the centre-ray traversal length only
+        launch_fill(ctx, residual, (size_t)height * pitch, 0.0f);
+        launch_fill(ctx, dist, (size_t)height * pitch, 0.0f);
+        launch_vol_traversal(ctx, v, g, width, height, pitch, dist, 0, 0, width, height);
+        launch_compare(ctx, width, height, pitch, real, residual, dist, real_length, crop, crop_dim, proj_scale);
+        if (hitpoints) launch_crop_border(ctx, width, height, pitch, residual, cut_length, dim_length, hitpoints);
+    } else {
+        launch_march_residual(ctx, v, g, vol, width, height, pitch, real, real_length, crop, crop_dim, proj_scale,
+                              cut_length, dim_length, hitpoints, residual, dist);
+    }
+    return t.finish();
+}
+}  // namespace
+
+artia_status artia_fp_residual(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, const float* vol,
+                               int volume_is_empty, int width, int height, size_t pitch, const float* real,
+                               float real_length, const float crop[4], const float crop_dim[4], float proj_scale,
+                               const float cut_length[2], const float dim_length[2], const int hitpoints[8],
+                               float* residual, float* dist, float* ms) {
+    return fp_residual_entry(ctx, "artia_fp_residual", vg, tg, VolData(vol), volume_is_empty, width, height, pitch,
+                             real, real_length, crop, crop_dim, proj_scale, cut_length, dim_length, hitpoints,
+                             residual, dist, ms);
+}
+
+artia_status artia_fp_residual_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                                   const uint16_t* vol, int volume_is_empty, int width, int height, size_t pitch,
+                                   const float* real, float real_length, const float crop[4], const float crop_dim[4],
+                                   float proj_scale, const float cut_length[2], const float dim_length[2],
+                                   const int hitpoints[8], float* residual, float* dist, float* ms) {
+    return fp_residual_entry(ctx, "artia_fp_residual_f16", vg, tg, VolData::f16(vol), volume_is_empty, width, height,
+                             pitch, real, real_length, crop, crop_dim, proj_scale, cut_length, dim_length, hitpoints,
+                             residual, dist, ms);
 }
 
 artia_status artia_vol_traversal(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, int width,

@@ -279,9 +279,10 @@ artia_status artia_geom_defocus_minmax(float ray, const artia_tilt_defocus* d, c
 
 // Fisher-Yates with glibc's rand() after srand(0) (additive feedback generator, degree 31,
 // separation 3, 310 warm-up draws), then the smallest |tilt| moved to the front.
-artia_status artia_marker_tilt_params(const float* data, int dim_x, int dim_y, int ref_marker, int proj,
-                                      artia_tilt_params* out, int* good) {
-    if (!data || !out || !good || dim_x < 10 || dim_y <= 0 || ref_marker < 0 || proj < 0 || proj >= dim_y)
+artia_status artia_marker_tilt_params(const float* data, int dim_x, int dim_y, int dim_z, int ref_marker,
+                                      int proj, artia_tilt_params* out, int* good) {
+    if (!data || !out || !good || dim_x < 10 || dim_y <= 0 || dim_z <= 0 || ref_marker < 0 || ref_marker >= dim_z ||
+        proj < 0 || proj >= dim_y)
         return ARTIA_ERR_INVALID_ARGUMENT;
     const float* m = data + (size_t)ref_marker * (size_t)dim_x * (size_t)dim_y + (size_t)proj * (size_t)dim_x;
     // MarkerFileItem_enum: 0 tilt, 1 x, 2 y, 3 deviation, 4 x shift, 5 y shift, 6/7 projected x/y,

@@ -7,13 +7,7 @@
 namespace artia {
 namespace {
 
-struct F4 {
-    float x, y, z, w;
-};
-
-// Compare.cu:49-130: r = dist >= 1 ? ((b - (p/s)/dist*L)/L)*s : 0, times the border dimming.
-// The right/top branch evaluates a non-positive w and so zeroes a band of CropDim + Crop
-// pixels (reference behaviour kept).
+// Compare.cu:49-130 (arithmetic in image_ops.hpp, shared with the fused forward-projection epilogue)
 __global__ void __launch_bounds__(256) compare_kernel(int W, int H, size_t pitch, const float* __restrict__ real,
                                                       float* __restrict__ virt, const float* __restrict__ dist,
                                                       float L, F4 cut, F4 dim, float S) {
@@ -21,31 +15,7 @@ __global__ void __launch_bounds__(256) compare_kernel(int W, int H, size_t pitch
     const unsigned y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= (unsigned)W || y >= (unsigned)H) return;
     const size_t i = (size_t)y * pitch + x;
-    const float val = dist[i];
-    float error = 0.0f;
-    if (val >= 1.0f) error = ((real[i] - ((virt[i] / S) / val * L)) / L) * S;
-    float yb = 1.0f, ya = 1.0f, xb = 1.0f, xa = 1.0f;
-    if ((float)y < cut.z + dim.z) {
-        float w = ((float)y - cut.z) / dim.z;
-        if (w < 0) w = 0;
-        yb = border_weight(w);
-    }
-    if ((float)y > (((float)H - dim.w) - cut.w) - 1.0f) {
-        float w = (((float)((unsigned)H - y - 1u) - cut.w) - (float)(H - 1)) / dim.w;
-        if (w < 0) w = 0.0f;
-        ya = border_weight(w);
-    }
-    if ((float)x < cut.y + dim.y) {
-        float w = ((float)x - cut.y) / dim.y;
-        if (w < 0) w = 0;
-        xb = border_weight(w);
-    }
-    if ((float)x > (((float)W - dim.x) - cut.x) - 1.0f) {
-        float w = (((float)((unsigned)W - x - 1u) - cut.x) - (float)(W - 1)) / dim.x;
-        if (w < 0) w = 0.0f;
-        xa = border_weight(w);
-    }
-    virt[i] = (((xa * xb) * ya) * yb) * error;
+    virt[i] = compare_value(x, y, W, H, real[i], virt[i], dist[i], L, cut, dim, S);
 }
 
 // Compare.cu:170-277 (weight in image_ops.hpp)

@@ -6,6 +6,10 @@
 
 namespace artia {
 
+struct F4 {
+    float x, y, z, w;
+};
+
 // Crop parameters: cut and dim lengths and the corners p1..p4 as (x, y) pairs.
 struct CropParams {
     float cx, cy, dx, dy;
@@ -87,6 +91,57 @@ __device__ __forceinline__ float crop_weight(int x, int y, int W, int H, const C
         }
     }
     return wx * wy;
+}
+
+// Compare.cu:49-130: r = dist >= 1 ? ((b - (p/s)/dist*L)/L)*s : 0, times the border dimming.
+// The right/top branch evaluates a non-positive w and so zeroes a band of CropDim + Crop pixels
+// (reference behaviour kept).  Shared by compare_kernel and the forward projector's residual
+// epilogue, so both give the same bits.
+__device__ __forceinline__ float compare_value(unsigned x, unsigned y, int W, int H, float real, float virt, float val,
+                                               float L, const F4& cut, const F4& dim, float S) {
+    float error = 0.0f;
+    if (val >= 1.0f) error = ((real - ((virt / S) / val * L)) / L) * S;
+    float yb = 1.0f, ya = 1.0f, xb = 1.0f, xa = 1.0f;
+    if ((float)y < cut.z + dim.z) {
+        float w = ((float)y - cut.z) / dim.z;
+        if (w < 0) w = 0;
+        yb = border_weight(w);
+    }
+    if ((float)y > (((float)H - dim.w) - cut.w) - 1.0f) {
+        float w = (((float)((unsigned)H - y - 1u) - cut.w) - (float)(H - 1)) / dim.w;
+        if (w < 0) w = 0.0f;
+        ya = border_weight(w);
+    }
+    if ((float)x < cut.y + dim.y) {
+        float w = ((float)x - cut.y) / dim.y;
+        if (w < 0) w = 0;
+        xb = border_weight(w);
+    }
+    if ((float)x > (((float)W - dim.x) - cut.x) - 1.0f) {
+        float w = (((float)((unsigned)W - x - 1u) - cut.x) - (float)(W - 1)) / dim.x;
+        if (w < 0) w = 0.0f;
+        xa = border_weight(w);
+    }
+    return (((xa * xb) * ya) * yb) * error;
+}
+
+// The SART residual of one tilt as the forward projector's epilogue computes it: compare of the
+// measured image against the projection and its mean chord, then (when crop_on) cropBorder's
+// weight, each in the arithmetic of the separate kernels.
+struct ResidEpi {
+    const float* real;   // measured projection, same pitch as the output
+    float L;             // real traversal length (Reconstructor.cpp:1390-1393)
+    F4 cut, dim;         // Crop, CropDim
+    float S;             // ProjectionScaleFactor
+    CropParams crop;     // cropBorder (CutLength, DimLength, hit points)
+    int crop_on;
+};
+
+__device__ __forceinline__ float residual_value(const ResidEpi& e, int x, int y, int W, int H, size_t pitch, float p,
+                                                float dist) {
+    const size_t i = (size_t)y * pitch + (size_t)x;
+    const float r = compare_value((unsigned)x, (unsigned)y, W, H, e.real[i], p, dist, e.L, e.cut, e.dim, e.S);
+    return e.crop_on ? crop_weight(x, y, W, H, e.crop) * r : r;
 }
 
 }  // namespace artia

@@ -53,19 +53,30 @@ artia_status sart_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, 
                 const int idx = index_list[step * S + i];
                 const TiltG g = to_device(tilts[idx]);
                 float* proj = residual + (size_t)i * npix;
+                const float* real = projections + (size_t)idx * npix;
+                const int* corners = hitpoints + 8 * (size_t)idx;
+                if (!empty) {
+                    // projection, compare and cropBorder in one launch (the residual epilogue)
+                    const int pr = prof_start(ctx, ARTIA_PROF_FP);
+                    launch_march_residual(ctx, v, g, vol, width, height, pitch, real, real_lengths[idx], sp->crop,
+                                          sp->crop_dim, sp->proj_scale, sp->cut_length, sp->dim_length, corners,
+                                          proj, dist);
+                    prof_stop(ctx, pr);
+                    continue;
+                }
                 int pr = prof_start(ctx, ARTIA_PROF_FILL);
                 launch_fill(ctx, proj, npix, 0.0f);
                 launch_fill(ctx, dist, npix, 0.0f);
                 prof_stop(ctx, pr);
                 pr = prof_start(ctx, ARTIA_PROF_FP);
-                if (!empty)
-                    launch_march(ctx, v, g, vol, width, height, pitch, proj, dist, 0, 0, width, height);
-                else
-                    launch_vol_traversal(ctx, v, g, width, height, pitch, dist, 0, 0, width, height);
+                launch_vol_traversal(ctx, v, g, width, height, pitch, dist, 0, 0, width, height);
                 prof_stop(ctx, pr);
                 pr = prof_start(ctx, ARTIA_PROF_COMPARE);
-                launch_compare(ctx, width, height, pitch, projections + (size_t)idx * npix, proj, dist,
-                               real_lengths[idx], sp->crop, sp->crop_dim, sp->proj_scale);
+                launch_compare(ctx, width, height, pitch, real, proj, dist, real_lengths[idx], sp->crop, sp->crop_dim,
+                               sp->proj_scale);
+                prof_stop(ctx, pr);
+                pr = prof_start(ctx, ARTIA_PROF_CROP);
+                launch_crop_border(ctx, width, height, pitch, proj, sp->cut_length, sp->dim_length, corners);
                 prof_stop(ctx, pr);
             }
             for (int i = 0; i < S; i++) {
@@ -73,12 +84,8 @@ artia_status sart_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, 
                 empty = false;
                 const int idx = index_list[step * S + i];
                 const TiltG g = to_device(tilts[idx]);
-                float* res = residual + (size_t)i * npix;
-                int pr = prof_start(ctx, ARTIA_PROF_CROP);
-                launch_crop_border(ctx, width, height, pitch, res, sp->cut_length, sp->dim_length,
-                                   hitpoints + 8 * (size_t)idx);
-                prof_stop(ctx, pr);
-                pr = prof_start(ctx, ARTIA_PROF_BP);
+                float* res = residual + (size_t)i * npix;   // cropped already
+                const int pr = prof_start(ctx, ARTIA_PROF_BP);
                 launch_bp(ctx, v, g, vol, res, width, height, pitch, lam, sp->oversampling, 0.0f, 9999999999999.0f,
                           ARTIA_FILTER_POINT);
                 prof_stop(ctx, pr);

@@ -100,3 +100,27 @@ def gaussian_noise(shape, sigma: float, seed: int) -> np.ndarray:
 def tilt_series(lo: float, hi: float, step: float) -> list:
     n = int(round((hi - lo) / step)) + 1
     return [lo + i * step for i in range(n)]
+
+
+def blob_subvolumes(n: int, count: int, seed: int):
+    """count seeded n^3 sub-volumes of three Gaussian blobs each (centres U[0.38, 0.62] of the edge,
+    amplitudes U[0.5, 1], width 0.09 of the edge), below 1e-6 at the faces, so the projection of a
+    sub-volume is smooth in a detector shift (zero margins).  Returned as one flat float32 CUDA
+    buffer, sub-volume k at [k * n^3, (k + 1) * n^3)."""
+    import torch
+
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    g = (torch.arange(n, device="cuda", dtype=torch.float32) + 0.5) / n
+    z, y, x = torch.meshgrid(g, g, g, indexing="ij")
+    out = torch.empty((count, n, n, n), device="cuda")
+    for c0 in range(0, count, 50):
+        c1 = min(count, c0 + 50)
+        cen = 0.38 + 0.24 * torch.rand((c1 - c0, 3, 3), generator=gen, device="cuda")
+        amp = 0.5 + 0.5 * torch.rand((c1 - c0, 3), generator=gen, device="cuda")
+        acc = torch.zeros((c1 - c0, n, n, n), device="cuda")
+        for b in range(3):
+            d2 = ((x[None] - cen[:, b, 0, None, None, None]) ** 2 + (y[None] - cen[:, b, 1, None, None, None]) ** 2
+                  + (z[None] - cen[:, b, 2, None, None, None]) ** 2)
+            acc += amp[:, b, None, None, None] * torch.exp(-d2 / 0.09 ** 2)
+        out[c0:c1] = acc
+    return out.reshape(-1)

@@ -539,28 +539,6 @@ def bench_subvol(args) -> int:
     return 0
 
 
-def refine_phantoms(n: int, count: int, seed: int):
-    """count seeded n^3 phantoms of three Gaussian blobs each, < 1e-6 at the faces (zero margins),
-    as one flat float32 CUDA buffer."""
-    import torch
-
-    gen = torch.Generator(device="cuda").manual_seed(seed)
-    g = (torch.arange(n, device="cuda", dtype=torch.float32) + 0.5) / n
-    z, y, x = torch.meshgrid(g, g, g, indexing="ij")
-    out = torch.empty((count, n, n, n), device="cuda")
-    for c0 in range(0, count, 50):
-        c1 = min(count, c0 + 50)
-        cen = 0.38 + 0.24 * torch.rand((c1 - c0, 3, 3), generator=gen, device="cuda")
-        amp = 0.5 + 0.5 * torch.rand((c1 - c0, 3), generator=gen, device="cuda")
-        acc = torch.zeros((c1 - c0, n, n, n), device="cuda")
-        for b in range(3):
-            d2 = ((x[None] - cen[:, b, 0, None, None, None]) ** 2 + (y[None] - cen[:, b, 1, None, None, None]) ** 2
-                  + (z[None] - cen[:, b, 2, None, None, None]) ** 2)
-            acc += amp[:, b, None, None, None] * torch.exp(-d2 / 0.09 ** 2)
-        out[c0:c1] = acc
-    return out.reshape(-1)
-
-
 def plain_gradient_sample(ctx, ps, vols, subs, targets, shifts, geoms, W, H, npairs):
     """One gradient evaluation as per-pair calls (artia_fp_march + residual + artia_fp_shift_grad
     on the pair's window), the schedule without pair sets, timed over the first npairs pairs."""
@@ -632,7 +610,7 @@ def bench_refine(args) -> int:
     full = A.volume_geometry(FX, FY, FZ)
     ctx = A.ArtiaContext(0)
     t0 = time.time()
-    vols = refine_phantoms(n, K, w["seed"])
+    vols = phantom.blob_subvolumes(n, K, w["seed"])
     base = [A.tilt_geometry(t, W, H) for t in tilts]
     subs, pairs, geoms = [], [], []
     for k in range(K):

@@ -180,9 +180,10 @@ artia_status artia_projection_index_list(const float* tilt_deg, int count, int* 
  * (dim_x items per projection, at least 10; dim_y projections), read as EmSART reads them: tilt
  * MFI_TiltAngle, psi MFI_RotationPsi, magnification MFI_Magnifiaction, shifts MFI_X_Shift and
  * MFI_Y_Shift (MarkerFile.h:34-46).  *good = CheckIfProjIndexIsGood (MarkerFile.cpp:143-146):
- * both coordinates of the reference marker above -1. */
-artia_status artia_marker_tilt_params(const float* data, int dim_x, int dim_y, int ref_marker, int proj,
-                                      artia_tilt_params* out, int* good);
+ * both coordinates of the reference marker above -1.  dim_z is the marker count
+ * (MarkerFile::GetMarkerCount); ref_marker must lie below it. */
+artia_status artia_marker_tilt_params(const float* data, int dim_x, int dim_y, int dim_z, int ref_marker,
+                                      int proj, artia_tilt_params* out, int* good);
 /* CtfFile (CtfFile.cpp:37-56): projection index's defocus from a CTF file's float data
  * (dim_x columns, dim_y >= 5 rows): minimum row 1, maximum row 2 (nm), astigmatism angle row 4 in
  * radians, returned in degrees; tilt_deg is stored as given. */
@@ -212,6 +213,22 @@ artia_status artia_fp_slicer(artia_ctx* ctx, const artia_vol_geom* vg, const art
 artia_status artia_vol_traversal(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
                                  int width, int height, size_t pitch, float* dist, const int* roi,
                                  float* ms);
+/* Reconstructor::ForwardProjectionNoCTF, Reconstructor::Compare and the cropBorder that
+ * BackProjectionNoCTF applies first (Reconstructor.cpp:429-490, 1386-1405, 871-880), fused into the
+ * forward projector's epilogue: residual = cropBorder(compare(real, FP(vol), mean chord)), with the
+ * projection taken from zero (the reference zeroes proj_d first).  The projection and the mean
+ * chord never reach memory; the residual equals fill + artia_fp_march + artia_compare +
+ * artia_crop_border bit for bit.  hitpoints == NULL skips cropBorder (its corners as
+ * artia_geom_hitpoints gives them, cut_length and dim_length as artia_crop_border takes them).
+ * volume_is_empty != 0 takes EmSART's first-projection rule (EmSart.cpp:598-601): no projection,
+ * the centre-ray traversal length (artia_vol_traversal) as the chord; vol may then be NULL.  dist
+ * (W x H, same pitch) is written with the chord only for an empty volume or under reference
+ * numerics, which run the separate launches; it may be NULL otherwise. */
+artia_status artia_fp_residual(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, const float* vol,
+                               int volume_is_empty, int width, int height, size_t pitch, const float* real,
+                               float real_length, const float crop[4], const float crop_dim[4], float proj_scale,
+                               const float cut_length[2], const float dim_length[2], const int hitpoints[8],
+                               float* residual, float* dist, float* ms);
 /* CompKernel / compare (Kernels.cpp:216-270, Compare.cu:49-130): virt <- residual */
 artia_status artia_compare(artia_ctx* ctx, int width, int height, size_t pitch, const float* real,
                            float* virt, const float* dist, float real_length, const float crop[4],
@@ -429,6 +446,11 @@ artia_status artia_fp_march_f16(artia_ctx* ctx, const artia_vol_geom* vg, const 
 artia_status artia_fp_slicer_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
                                  const uint16_t* vol, int width, int height, size_t pitch, float* proj,
                                  float tmin, float tmax, const int* roi, float* ms);
+artia_status artia_fp_residual_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg,
+                                   const uint16_t* vol, int volume_is_empty, int width, int height, size_t pitch,
+                                   const float* real, float real_length, const float crop[4], const float crop_dim[4],
+                                   float proj_scale, const float cut_length[2], const float dim_length[2],
+                                   const int hitpoints[8], float* residual, float* dist, float* ms);
 artia_status artia_bp_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, uint16_t* vol,
                           const float* img, int width, int height, size_t pitch, float lambda,
                           int oversampling, float dist_min, float dist_max, int filter, float* ms);

@@ -5,12 +5,14 @@ SART workload (1024 x 1024 x 256 volume, 1024^2 detector, exact numerics):
     detector's centre, the edge of the volume's shadow, a corner), within 1e-5 relative L2;
   - backprojection (OS 2) against the FP64 oracle on two-slice slabs at the bottom face, the middle
     and the top face of the volume (the oracle runs on the slab alone: a voxel's update depends only
-    on its own position and the image).  For tilts about y the beam direction's y component is a
-    float residue (3.7e-9 at 47 deg), so a sub-ray whose detector position lies within ~1e-8 voxels
-    of a y face is inside or outside the whole voxel column by rounding alone, in any arithmetic: the
-    reference's own float evaluation moves such voxels by up to 3x against FP64.  At 1024^2 x 256 a
-    few such sub-rays occur; the gate is that at most 1e-5 of the slab's voxels differ by more than
-    1e-5 of the slab's largest update, and the rest agree within 1e-5 relative L2.
+    on its own position and the image).  A sub-ray parallel to a voxel face (every tilt about y has
+    a beam direction whose y component is zero or a float residue, 3.7e-9 at 47 deg) that lies
+    within rounding of that face is inside or outside the whole voxel column by rounding alone, in
+    any arithmetic: the reference's own float evaluation moves such voxels by up to 3x against
+    FP64.  Those voxels are found geometrically, in FP64 on the host (tie_voxels: a sub-ray through
+    the voxel's cross-section within 1e-6 voxels of a face it runs parallel to, which covers the
+    float rounding of a sub-ray's offset, ~5e-7); every other voxel agrees with the oracle within
+    1e-5 of the slab's largest update, and within 1e-5 relative L2 over the slab.
 CTF workload (2048 x 2048 x 512 volume, 2048^2 detector, 4 nm slabs at 0.25 nm/px): one CTF-SART
 pass over two tilts from a non-empty volume, stacked forward slabs and backprojection, equals the
 reference's per-slab loop through the operator entry points bit for bit.
@@ -65,6 +67,52 @@ def test_march_windows_full_size(cfg2, tilt, psi, shift):
         assert rel_l2(d[sl], do[sl]) <= 1e-5
 
 
+def tie_voxels(vg, g, os_, eps):
+    """Voxels of vg crossed by a sub-ray of the OS lattice (detector points ((i + 0.5)/OS,
+    (j + 0.5)/OS)) that runs parallel to one of the voxel's faces (|n_c| < 1e-6) and lies within
+    eps voxels of it, in FP64.  Rays are d + X u + Y v + t n (the tilt's pitch vectors, as the
+    backprojector's sub-rays); "crosses" means a positive chord through the voxel's extent along the
+    other axes.  Returns a boolean [nz, ny, nx] mask."""
+    d, u, v, n = (np.array(a, dtype=np.float64) for a in (g.d, g.u, g.v, g.n))
+    n = n / np.linalg.norm(n)
+    s = np.array(vg.voxel_size, dtype=np.float64)
+    binv = np.linalg.inv(np.stack([u, v, n], axis=1))
+    par = [c for c in range(3) if abs(n[c]) < 1e-6]
+    nonpar = [c for c in range(3) if c not in par]
+    tol = eps + 3.0 * float(s.max()) * max([abs(n[c]) for c in par] + [0.0])   # drift along the ray
+    z, y, x = np.meshgrid(np.arange(vg.nz), np.arange(vg.ny), np.arange(vg.nx), indexing="ij")
+    corner = np.stack([vg.bbox_min[c] + (x, y, z)[c] * s[c] for c in range(3)], axis=-1) - d
+    pu = corner @ binv[0]
+    pv = corner @ binv[1]
+    # the voxel's projected box relative to its corner's projection, from its 8 corners
+    offs = np.array([[(k >> 2) & 1, (k >> 1) & 1, k & 1] for k in range(8)], dtype=np.float64) * s
+    du, dv = offs @ binv[0], offs @ binv[1]
+    ni = int(np.ceil((du.max() - du.min()) * os_)) + 3
+    nj = int(np.ceil((dv.max() - dv.min()) * os_)) + 3
+    i0 = np.ceil((pu + du.min()) * os_ - 0.5) - 1
+    j0 = np.ceil((pv + dv.min()) * os_ - 0.5) - 1
+    mask = np.zeros(pu.shape, dtype=bool)
+    for di in range(ni):
+        a = (i0 + di + 0.5) / os_ - pu
+        for dj in range(nj):
+            b = (j0 + dj + 0.5) / os_ - pv
+            w = {c: a * u[c] + b * v[c] for c in range(3)}
+            tin = np.full(pu.shape, -np.inf)
+            tout = np.full(pu.shape, np.inf)
+            for c in nonpar:
+                t0, t1 = -w[c] / n[c], (s[c] - w[c]) / n[c]
+                tin = np.maximum(tin, np.minimum(t0, t1))
+                tout = np.minimum(tout, np.maximum(t0, t1))
+            crosses = tout > tin
+            for c in par:
+                near = (np.abs(w[c]) < tol) | (np.abs(w[c] - s[c]) < tol)
+                for c2 in par:
+                    if c2 != c:
+                        near &= (w[c2] > -tol) & (w[c2] < s[c2] + tol)
+                mask |= near & crosses
+    return mask
+
+
 @pytest.mark.parametrize("tilt,psi,shift", [(0.0, 0.0, (0.0, 0.0)), (-47.0, 3.0, (2.0, 1.0))])
 def test_bp_slabs_full_size(cfg2, tilt, psi, shift):
     ctx, vg, _, _ = cfg2
@@ -83,11 +131,14 @@ def test_bp_slabs_full_size(cfg2, tilt, psi, shift):
         ref = O.back_project(sub, g, np.zeros((2, ny, nx)), img, 0.3, os_=2, dtype=np.float64)
         got = vol[z0:z0 + 2].cpu().numpy().astype(np.float64)
         assert np.abs(ref).max() > 0
-        off = np.abs(got - ref) > 1e-5 * np.abs(ref).max()
-        print(f"  {int(off.sum())} of {off.size} voxels decided by a face tie; rest rel L2 "
-              f"{rel_l2(got[~off], ref[~off]):.2e}", flush=True)
-        assert off.sum() <= 1e-5 * off.size
-        assert rel_l2(got[~off], ref[~off]) <= 1e-5
+        tie = tie_voxels(sub, g, 2, 1e-6)
+        bad = (np.abs(got - ref) > 1e-5 * np.abs(ref).max()) & ~tie
+        print(f"  {int(tie.sum())} of {tie.size} voxels have a sub-ray within 1e-6 of a parallel face; "
+              f"{int(bad.sum())} others off by > 1e-5 of the max; rel L2 of the others "
+              f"{rel_l2(got[~tie], ref[~tie]):.2e}", flush=True)
+        assert tie.sum() <= 1e-4 * tie.size
+        assert not bad.any(), np.argwhere(bad)[:10]
+        assert rel_l2(got[~tie], ref[~tie]) <= 1e-5
 
 
 def test_sart_ctf_two_tilts_full_size_equals_per_slab(gpu):

@@ -63,6 +63,8 @@ def test_readers_reject_bad_shapes():
     with pytest.raises(A.ArtiaError):
         A.marker_tilt_params(np.zeros((1, 4, 10), np.float32), 4)  # projection out of range
     with pytest.raises(A.ArtiaError):
+        A.marker_tilt_params(np.zeros((2, 4, 10), np.float32), 0, ref_marker=2)  # marker out of range
+    with pytest.raises(A.ArtiaError):
         A.ctf_file_defocus(np.zeros((4, 3), np.float32), 0, 0.0)   # needs the angle row
 
 
