@@ -138,6 +138,11 @@ void launch_crop_border(artia_ctx* ctx, int W, int H, size_t pitch, float* img, 
                         const float dim[2], const int p[8]);
 void launch_bp(artia_ctx* ctx, const VolG& vg, const TiltG& tg, VolData vol, const float* img, int W, int H,
                size_t pitch, float lambda, int os, float distMin, float distMax, int filter);
+// The SART backprojections of one ordered subset (POINT sampling, gate [0, 1e13)): tilt k's residual at
+// imgs[k], in tilt order.  Exact numerics run up to four tilts per launch, adding their updates in
+// registers; the result equals n launch_bp calls.
+void launch_bp_subset(artia_ctx* ctx, const VolG& vg, const TiltG* g, const float* const* imgs, int n, int W, int H,
+                      size_t pitch, float lambda, int os, VolData vol);
 // exact-numerics BP of one tilt's CTF slabs in one launch: slab k's image at stack + k * stride, slab k
 // covering distances [bounds[k], bounds[k+1]) (the last [bounds[n-1], last_end)); bounds on the device
 void launch_bp_stack(artia_ctx* ctx, const VolG& vg, const TiltG& tg, VolData vol, const float* stack, size_t stride,

@@ -19,6 +19,7 @@
 #include <cmath>
 
 #include "bp_common.hpp"
+#include "bp_window.hpp"
 #include "device_common.hpp"
 
 namespace artia {
@@ -237,6 +238,70 @@ __global__ void __launch_bounds__(256) bp_exact_kernel(VolG v, TiltG g, BPExact 
     if (any) store_quad(vp, out);
 }
 
+// ---------------------------------------------------------------- exact numerics, window form
+
+// Up to kWinTilts tilts of an ordered subset in one launch (bp_window.hpp): every quad adds the
+// tilts' updates to its voxels in tilt order in registers and writes them once, which is what the
+// same number of single-tilt launches write (each launch reads the previous one's result).
+constexpr int kWinTilts = 4;
+struct BPWinSet {
+    BPWin w[kWinTilts];
+    const float* img[kWinTilts];
+    int n;
+    float lambda;
+    double distMin, distMax;
+};
+
+// Block = 16 quads x 4 rows x 4 slices, as bp_exact_kernel; per tilt the block stages its
+// footprint (one pixel of margin) in LDS.  Blocks whose footprint lies inside the image walk the
+// windows unclamped; blocks at the image edge clamp each quad's lattice to its clamped footprint.
+template <int kOS, int kNI, int kNJ, typename T>
+__global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __restrict__ vol, int W, int H,
+                                                     size_t pitch) {
+    __shared__ float tile[kTileFloats];
+    const int tid = (int)threadIdx.x;
+    const int xq = (int)blockIdx.x * 16 + (tid & 15);
+    const int y = (int)blockIdx.y * 4 + ((tid >> 4) & 3);
+    const int z = (int)blockIdx.z * 4 + (tid >> 6);
+    const bool inVol = xq < v.nx / 4 && y < v.ny && z < v.nz;
+    const unsigned x = (unsigned)xq * 4u;
+    T* vp = vol + ((size_t)(inVol ? z : 0) * (size_t)v.ny + (size_t)(inVol ? y : 0)) * (size_t)v.nx + (inVol ? x : 0u);
+    float out[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (inVol) {
+        const float4 vb = load_quad(vp);
+        out[0] = vb.x;
+        out[1] = vb.y;
+        out[2] = vb.z;
+        out[3] = vb.w;
+    }
+    bool any = false;
+    const double bx = (double)blockIdx.x * 64.0, by = (double)blockIdx.y * 4.0, bz = (double)blockIdx.z * 4.0;
+    const double qx = (double)x, qy = (double)y, qz = (double)z;
+    for (int t = 0; t < S.n; t++) {
+        const BPWin& w = S.w[t];
+        const float* __restrict__ img = S.img[t];
+        const double base0 = w.G[0][0] * bx + w.G[0][1] * by + w.G[0][2] * bz;
+        const double base1 = w.G[1][0] * bx + w.G[1][1] * by + w.G[1][2] * bz;
+        const int ux0 = (int)floor(base0 + w.tlo[0]) - 1, ux1 = (int)ceil(base0 + w.thi[0]) + 1;
+        const int uy0 = (int)floor(base1 + w.tlo[1]) - 1, uy1 = (int)ceil(base1 + w.thi[1]) + 1;
+        const bool interior = ux0 >= 0 && uy0 >= 0 && ux1 <= W - 1 && uy1 <= H - 1;
+        const int tx0 = max(0, ux0), ty0 = max(0, uy0);
+        const int tw = max(0, min(W - 1, ux1) - tx0 + 1), th = max(0, min(H - 1, uy1) - ty0 + 1);
+        if (t > 0) __syncthreads();   // the previous tilt's readers are done
+        for (int idx = tid; idx < tw * th; idx += 256) {
+            const int r = idx / tw, c = idx - r * tw;
+            tile[idx] = img[(size_t)(ty0 + r) * pitch + (size_t)(tx0 + c)];
+        }
+        __syncthreads();
+        if (!inVol) continue;
+        const double gt = fabs(fma(w.gn[0], qx, fma(w.gn[1], qy, fma(w.gn[2], qz, w.g0))));
+        if (!(gt >= S.distMin && gt < S.distMax)) continue;
+        any |= window_quad<kOS, kNI, kNJ>(w, tile, tw, tx0, ty0, interior, W, H, x, (unsigned)y, (unsigned)z,
+                                          S.lambda, out);
+    }
+    if (any) store_quad(vp, out);
+}
+
 // The CTF backprojection of one tilt in one launch: every voxel takes its slab's filtered image
 // from the stack (slab k at stack + k * stride) instead of one gated launch per slab
 // (Reconstructor.cpp:920-969).  A voxel lies in exactly one slab, so it receives the same single
@@ -340,6 +405,44 @@ __global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact 
     if (any) store_quad(vp, out);
 }
 
+template <int kOS, int kNI, typename T>
+void launch_win_nj(artia_ctx* ctx, dim3 grid, const VolG& v, const BPWinSet& S, int nj, T* vol, int W, int H,
+                   size_t pitch) {
+    constexpr int kLo = kOS == 1 ? 1 : 2;
+    if (nj == kLo)
+        hipLaunchKernelGGL((bp_win_kernel<kOS, kNI, kLo, T>), grid, dim3(256), 0, ctx->stream, v, S, vol, W, H, pitch);
+    else if (nj == kLo + 1)
+        hipLaunchKernelGGL((bp_win_kernel<kOS, kNI, kLo + 1, T>), grid, dim3(256), 0, ctx->stream, v, S, vol, W, H,
+                           pitch);
+    else
+        hipLaunchKernelGGL((bp_win_kernel<kOS, kNI, kLo + 2, T>), grid, dim3(256), 0, ctx->stream, v, S, vol, W, H,
+                           pitch);
+}
+
+template <int kOS, typename T>
+void launch_win_os(artia_ctx* ctx, dim3 grid, const VolG& v, const BPWinSet& S, int ni, int nj, T* vol, int W, int H,
+                   size_t pitch) {
+    constexpr int kLo = kOS == 1 ? 1 : 2;
+    if (ni == kLo)
+        launch_win_nj<kOS, kLo>(ctx, grid, v, S, nj, vol, W, H, pitch);
+    else if (ni == kLo + 1)
+        launch_win_nj<kOS, kLo + 1>(ctx, grid, v, S, nj, vol, W, H, pitch);
+    else
+        launch_win_nj<kOS, kLo + 2>(ctx, grid, v, S, nj, vol, W, H, pitch);
+}
+
+// One launch of the window form over S.n tilts sharing the window size (ni, nj).
+template <typename T>
+void launch_win_t(artia_ctx* ctx, const VolG& v, const BPWinSet& S, int os, int ni, int nj, T* vol, int W, int H,
+                  size_t pitch) {
+    const int nq = v.nx / 4;
+    const dim3 grid((unsigned)((nq + 15) / 16), (unsigned)((v.ny + 3) / 4), (unsigned)((v.nz + 3) / 4));
+    if (os == 1)
+        launch_win_os<1>(ctx, grid, v, S, ni, nj, vol, W, H, pitch);
+    else
+        launch_win_os<2>(ctx, grid, v, S, ni, nj, vol, W, H, pitch);
+}
+
 template <int kOS, typename T>
 void launch_tile(artia_ctx* ctx, dim3 grid, const VolG& v, const TiltG& g, const BPExact& e, T* vol,
                  const float* img, int W, int H, size_t pitch, float lambda, int os, double dmin, double dmax,
@@ -363,6 +466,16 @@ void launch_bp_t(artia_ctx* ctx, const VolG& v, const TiltG& g, T* vol, const fl
         const dim3 block(64, 4), grid((unsigned)((nq + 63) / 64), (unsigned)((v.ny + 3) / 4), (unsigned)v.nz);
         hipLaunchKernelGGL((bp_ref_kernel<T>), grid, block, 0, ctx->stream, v, g, vol, img, W, H, pitch, lambda, os,
                            1.0f / (float)os, distMin, distMax, filter);
+        return;
+    }
+    BPWinSet S{};
+    if (filter == ARTIA_FILTER_POINT && window_plan(v, g, os, S.w[0])) {
+        S.img[0] = img;
+        S.n = 1;
+        S.lambda = lambda;
+        S.distMin = distMin;
+        S.distMax = distMax;
+        launch_win_t(ctx, v, S, os, S.w[0].ni, S.w[0].nj, vol, W, H, pitch);
         return;
     }
     const BPExact e = prepare_exact(v, g);
@@ -422,6 +535,35 @@ void launch_bp(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, const
         launch_bp_t(ctx, v, g, static_cast<vhalf*>(vol.p), img, W, H, pitch, lambda, os, distMin, distMax, filter);
     else
         launch_bp_t(ctx, v, g, static_cast<float*>(vol.p), img, W, H, pitch, lambda, os, distMin, distMax, filter);
+}
+
+void launch_bp_subset(artia_ctx* ctx, const VolG& v, const TiltG* g, const float* const* imgs, int n, int W, int H,
+                      size_t pitch, float lambda, int os, VolData vol) {
+    int t = 0;
+    while (t < n) {
+        BPWinSet S{};
+        if (ctx->numerics == ARTIA_NUMERICS_EXACT && window_plan(v, g[t], os, S.w[0])) {
+            S.img[0] = imgs[t];
+            S.n = 1;
+            // the following tilts with the same window size join the launch
+            while (S.n < kWinTilts && t + S.n < n && window_plan(v, g[t + S.n], os, S.w[S.n]) &&
+                   S.w[S.n].ni == S.w[0].ni && S.w[S.n].nj == S.w[0].nj) {
+                S.img[S.n] = imgs[t + S.n];
+                S.n++;
+            }
+            S.lambda = lambda;
+            S.distMin = 0.0;
+            S.distMax = 9999999999999.0;
+            if (vol.half)
+                launch_win_t(ctx, v, S, os, S.w[0].ni, S.w[0].nj, static_cast<vhalf*>(vol.p), W, H, pitch);
+            else
+                launch_win_t(ctx, v, S, os, S.w[0].ni, S.w[0].nj, static_cast<float*>(vol.p), W, H, pitch);
+            t += S.n;
+        } else {
+            launch_bp(ctx, v, g[t], vol, imgs[t], W, H, pitch, lambda, os, 0.0f, 9999999999999.0f, ARTIA_FILTER_POINT);
+            t++;
+        }
+    }
 }
 
 }  // namespace artia

@@ -1,0 +1,257 @@
+// Fixed-window form of the exact backprojection (BackProjectionSquareOS.cu:263-782) for tilts
+// whose beam has no y component (every tilt about the y axis: n_y is zero or a float residue of
+// the rotation), POINT sampling, no magnification anisotropy.
+//
+// A voxel's sub-rays are the OS lattice points (i + 0.5) / OS of the detector inside the voxel's
+// projected box.  The box holds at most kNI x kNJ lattice points for every voxel of a tilt, so the
+// window starting at the box's first lattice point is walked fully unrolled, without per-voxel
+// bounds; lattice points of the window outside the voxel's projection get a chord <= 0 and add
+// nothing, as in the per-box loop.  With the sub-ray offset (a, b) from the projection of the
+// voxel's min corner (detector pixels along u and v), the chord through the voxel is
+//     chord = max(0, min(m, P, SP - P, Y, SY - Y))
+// with P and Y affine in (a, b).  Evaluated as med3(min(SP/2 - |P - SP/2|, SY/2 - |Y - SY/2|), 0, m),
+// which is the same clamp (m > 0), in 8 operations per lattice point:
+//   - n_x != 0: the x and z slabs' entries e_x, e_z are affine in (a, b) and the chord
+//     min(e_x + S_x, e_z + S_z) - max(e_x, e_z) equals min(min(S_x, S_z), S_x - d, S_z + d) for
+//     d = e_z - e_x; P = S_x - d, SP = S_x + S_z, m = min(S_x, S_z);
+//   - n_x == 0 (tilt 0): the ray runs inside the x slab when 0 < w_x < s_x (w_x the sub-ray's x
+//     offset a u_x + b v_x); P = B w_x, SP = B s_x, m = S_z, with B = 2^100;
+//   - y (parallel): Y = B w_y, SY = B s_y; min(Y, SY - Y) > 0 exactly when 0 < w_y < s_y, and
+//     B is large enough that a sub-ray inside the slab by more than 1e-30 voxels is not clipped.
+// For a float-residue n_y (~1e-9) the sub-ray's y offset drifts by less than 1e-8 voxels across a
+// voxel, so the parallel form differs from the slab form only for sub-rays within rounding of a y
+// face, where any evaluation is decided by rounding.
+#pragma once
+
+#include <cmath>
+
+#include "bp_common.hpp"
+#include "device_common.hpp"
+
+namespace artia {
+
+constexpr float kWinBig = 1.2676506002282294e30f;   // 2^100
+
+// Per-tilt constants of the window form, prepared on the host in double.
+struct BPWin {
+    float PU, PV, PC;        // Pc = P - SP/2 = a PU + b PV + PC
+    float YU, YV, YC;        // Yc = Y - SY/2 = a YU + b YV + YC
+    float PUd, PVd, YUd, YVd;   // the same per lattice step (times 1/OS)
+    float SPh, SYh, m;       // SP/2, SY/2, m
+    float uMin, vMin;        // the voxel's projected box starts (with guard) relative to its corner
+    double pu0, pui, puj, puk, pv0, pvi, pvj, pvk;   // corner projection (detector pixels)
+    double G[2][3], lo[2], hi[2], tlo[2], thi[2];     // quad and block footprints (see BPExact)
+    double gn[3], g0;                                 // dist gate: |gn . q + g0|
+    int ni, nj;              // window size in lattice points
+};
+
+// Whether the window form applies to a tilt and its constants, for oversampling os and a window of
+// at most maxN lattice points per axis.  Requires the affine footprint of prepare_exact (passed in).
+inline bool prepare_window(const VolG& v, const TiltG& g, const BPExact& e, int os, int maxN, BPWin& w) {
+    if (!e.affine || !e.identityAniso || os <= 0) return false;
+    const double nf[3] = {g.n[0], g.n[1], g.n[2]};
+    const double nl = std::sqrt(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]);
+    const double n[3] = {nf[0] / nl, nf[1] / nl, nf[2] / nl};
+    if (!(std::fabs(n[1]) <= 1e-6) || !(std::fabs(n[2]) > 1e-3)) return false;
+    const double u[3] = {g.u[0], g.u[1], g.u[2]}, wv[3] = {g.v[0], g.v[1], g.v[2]};
+    const double vs[3] = {v.vs[0], v.vs[1], v.vs[2]};
+    const double B = (double)kWinBig;
+    auto slab = [&](int c, double& U, double& V, double& Smin, double& Sabs) {
+        U = -u[c] / n[c];
+        V = -wv[c] / n[c];
+        const double S = vs[c] / n[c];
+        Smin = std::fmin(0.0, S);
+        Sabs = std::fabs(S);
+    };
+    double Uz, Vz, Sminz, Sabsz;
+    slab(2, Uz, Vz, Sminz, Sabsz);
+    if (std::fabs(n[0]) > 1e-6) {
+        double Ux, Vx, Sminx, Sabsx;
+        slab(0, Ux, Vx, Sminx, Sabsx);
+        // P = S_x - d, d = e_z - e_x
+        w.PU = (float)(-(Uz - Ux));
+        w.PV = (float)(-(Vz - Vx));
+        w.PC = (float)((Sabsx - (Sminz - Sminx)) - 0.5 * (Sabsx + Sabsz));
+        w.SPh = (float)(0.5 * (Sabsx + Sabsz));
+        w.m = (float)std::fmin(Sabsx, Sabsz);
+    } else {
+        w.PU = (float)(B * u[0]);
+        w.PV = (float)(B * wv[0]);
+        w.PC = (float)(-0.5 * B * vs[0]);
+        w.SPh = (float)(0.5 * B * vs[0]);
+        w.m = (float)Sabsz;
+    }
+    w.YU = (float)(B * u[1]);
+    w.YV = (float)(B * wv[1]);
+    w.YC = (float)(-0.5 * B * vs[1]);
+    w.SYh = (float)(0.5 * B * vs[1]);
+    w.PUd = (float)((double)w.PU / os);
+    w.PVd = (float)((double)w.PV / os);
+    w.YUd = (float)((double)w.YU / os);
+    w.YVd = (float)((double)w.YV / os);
+    w.uMin = e.uMin;
+    w.vMin = e.vMin;
+    w.pu0 = e.pu0;
+    w.pui = e.pui;
+    w.puj = e.puj;
+    w.puk = e.puk;
+    w.pv0 = e.pv0;
+    w.pvi = e.pvi;
+    w.pvj = e.pvj;
+    w.pvk = e.pvk;
+    for (int r = 0; r < 2; r++) {
+        for (int k = 0; k < 3; k++) w.G[r][k] = e.G[r][k];
+        w.lo[r] = e.lo[r];
+        w.hi[r] = e.hi[r];
+        w.tlo[r] = e.tlo[r];
+        w.thi[r] = e.thi[r];
+    }
+    for (int k = 0; k < 3; k++) w.gn[k] = e.gn[k];
+    w.g0 = e.g0;
+    // lattice points in a box of length L: at most floor(L * os) + 1 (the box's float rounding is
+    // far inside the 1e-4 guard already in uMin/uMax)
+    w.ni = (int)std::floor(((double)e.uMax - (double)e.uMin) * os) + 1;
+    w.nj = (int)std::floor(((double)e.vMax - (double)e.vMin) * os) + 1;
+    return w.ni >= 1 && w.nj >= 1 && w.ni <= maxN && w.nj <= maxN;
+}
+
+// The chord-weighted sums of one voxel over its kNI x kNJ window, read from the block's LDS tile
+// of image pixels (row pitch tw).  (i0, j0): the window's first lattice point, global; (a0, b0):
+// its offset from the voxel's min-corner projection; tile origin pixel (tx0, ty0).  kClamp: only
+// lattice points in [li0, li1) x [lj0, lj1) count (the quad footprint clamped at the image edge).
+template <int kOS, int kNI, int kNJ, bool kClamp>
+__device__ __forceinline__ void window_sums(const BPWin& w, const float* __restrict__ tile, int tw, int tx0, int ty0,
+                                            int i0, int j0, float a0, float b0, int li0, int li1, int lj0, int lj1,
+                                            float& num, float& den) {
+    const float P0 = __builtin_fmaf(a0, w.PU, __builtin_fmaf(b0, w.PV, w.PC));
+    const float Y0 = __builtin_fmaf(a0, w.YU, __builtin_fmaf(b0, w.YV, w.YC));
+    // pixel of lattice point i: (i0 + di) / kOS = c0 + (ph + di) / kOS
+    const int c0 = kOS == 1 ? i0 : (i0 >> 1), ph = kOS == 1 ? 0 : (i0 & 1);
+    const int r0 = kOS == 1 ? j0 : (j0 >> 1), qh = kOS == 1 ? 0 : (j0 & 1);
+    const int base = (r0 - ty0) * tw + (c0 - tx0);
+    float n = 0.0f, d = 0.0f;
+#pragma unroll
+    for (int dj = 0; dj < kNJ; dj++) {
+        const float Pr = dj == 0 ? P0 : (dj == 1 ? P0 + w.PVd : __builtin_fmaf((float)dj, w.PVd, P0));
+        const float Yr = dj == 0 ? Y0 : (dj == 1 ? Y0 + w.YVd : __builtin_fmaf((float)dj, w.YVd, Y0));
+        const int rowOff = kOS == 1 ? dj * tw : ((dj >> 1) + ((dj & 1) ? qh : 0)) * tw;
+        const int rowBase = base + rowOff;
+        const int rowOdd = rowBase + ph;   // odd di of the 2x lattice: pixel c0 + ph + (di - 1) / 2
+        bool rowOk = true;
+        if (kClamp) rowOk = (unsigned)(j0 + dj - lj0) < (unsigned)(lj1 - lj0);
+#pragma unroll
+        for (int di = 0; di < kNI; di++) {
+            const float Pc = di == 0 ? Pr : (di == 1 ? Pr + w.PUd : __builtin_fmaf((float)di, w.PUd, Pr));
+            const float Yc = di == 0 ? Yr : (di == 1 ? Yr + w.YUd : __builtin_fmaf((float)di, w.YUd, Yr));
+            // min(m, P, SP - P, Y, SY - Y) clamped at 0
+            const float A = w.SPh - fabsf(Pc), Bq = w.SYh - fabsf(Yc);
+            float chord = __builtin_amdgcn_fmed3f(fminf(A, Bq), 0.0f, w.m);
+            int addr = kOS == 1 ? rowBase + di : ((di & 1) ? rowOdd + (di >> 1) : rowBase + (di >> 1));
+            if (kClamp) {
+                const bool ok = rowOk && (unsigned)(i0 + di - li0) < (unsigned)(li1 - li0);
+                chord = ok ? chord : 0.0f;
+                addr = ok ? addr : 0;
+            }
+            const float val = tile[addr];
+            if (di == 0 && dj == 0) {
+                n = val * chord;
+                d = chord;
+            } else {
+                n = __builtin_fmaf(val, chord, n);
+                d += chord;
+            }
+        }
+    }
+    num = n;
+    den = d;
+}
+
+// The window form's plan for one tilt: the form applies, the footprint of a 64 x 4 x 4-voxel block
+// fits the LDS tile and the window size has a compiled instance (OS 1: 1-3, OS 2: 2-4 lattice
+// points per axis).  The per-call and the batched sub-volume backprojections take the same decision,
+// so they stay bit-identical.
+inline bool window_plan(const VolG& v, const TiltG& g, int os, BPWin& w) {
+    if (os != 1 && os != 2) return false;
+    const BPExact e = prepare_exact(v, g);
+    if ((long long)(e.tileW + 2) * (long long)(e.tileH + 2) > kTileFloats) return false;
+    if (!prepare_window(v, g, e, os, os == 1 ? 3 : 4, w)) return false;
+    return os == 1 ? true : (w.ni >= 2 && w.nj >= 2);
+}
+
+// The update of one quad (voxels x .. x+3 of row y, slice z) for one tilt: out[k] += lambda *
+// num / den over each voxel's window, from an image tile of row pitch tw whose pixel (tx0, ty0) is
+// tile[0] (the whole image with tw = pitch and tx0 = ty0 = 0).  interior: the quad's footprint is
+// not clamped by the image edges (then every window point with a positive chord lies inside it);
+// otherwise the lattice is clamped to the clamped footprint, as the per-box loop does.  Returns
+// whether a voxel changed.
+template <int kOS, int kNI, int kNJ>
+__device__ __forceinline__ bool window_quad(const BPWin& w, const float* __restrict__ tile, int tw, int tx0, int ty0,
+                                            bool interior, int W, int H, unsigned x, unsigned y, unsigned z,
+                                            float lambda, float out[4]) {
+    constexpr float kDelta = 1.0f / (float)kOS;
+    const double qx = (double)x, qy = (double)y, qz = (double)z;
+    int li0 = 0, li1 = 0, lj0 = 0, lj1 = 0;
+    if (!interior) {
+        // the quad's footprint clamped to [0, W-1] x [0, H-1] (footprint_affine): lattice points
+        // [px0 * OS, px1 * OS) x [py0 * OS, py1 * OS)
+        const double a0 = fma(w.G[0][0], qx, fma(w.G[0][1], qy, w.G[0][2] * qz));
+        const double a1 = fma(w.G[1][0], qx, fma(w.G[1][1], qy, w.G[1][2] * qz));
+        li0 = kOS * (int)fmin(fmax(floor(a0 + w.lo[0]), 0.0), (double)(W - 1));
+        li1 = kOS * (int)fmin(fmax(ceil(a0 + w.hi[0]), 0.0), (double)(W - 1));
+        lj0 = kOS * (int)fmin(fmax(floor(a1 + w.lo[1]), 0.0), (double)(H - 1));
+        lj1 = kOS * (int)fmin(fmax(ceil(a1 + w.hi[1]), 0.0), (double)(H - 1));
+        if (li1 <= li0 || lj1 <= lj0) return false;
+    }
+    // corner projection of the quad's first voxel, split into an integer pixel and a float offset
+    const double Uq = fma(w.pui, qx, fma(w.puj, qy, fma(w.puk, qz, w.pu0)));
+    const double Vq = fma(w.pvi, qx, fma(w.pvj, qy, fma(w.pvk, qz, w.pv0)));
+    const double fUq = floor(Uq), fVq = floor(Vq);
+    const int iq = (int)fUq * kOS, jq = (int)fVq * kOS;
+    const float fu = (float)(Uq - fUq), fv = (float)(Vq - fVq);
+    const float dui = (float)w.pui, dvi = (float)w.pvi;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float du = __builtin_fmaf((float)k, dui, fu);
+        const float dv = __builtin_fmaf((float)k, dvi, fv);
+        const int i0 = (int)ceilf((du + w.uMin) * (float)kOS - 0.5f);
+        const int j0 = (int)ceilf((dv + w.vMin) * (float)kOS - 0.5f);
+        const float a0 = __builtin_fmaf((float)i0 + 0.5f, kDelta, -du);
+        const float b0 = __builtin_fmaf((float)j0 + 0.5f, kDelta, -dv);
+        float num, den;
+        if (interior)
+            window_sums<kOS, kNI, kNJ, false>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, 0, 0, 0, 0, num, den);
+        else
+            window_sums<kOS, kNI, kNJ, true>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, li0, li1, lj0, lj1, num,
+                                             den);
+        if (den > 0.0f) {
+            out[k] += (lambda * num) * __builtin_amdgcn_rcpf(den);
+            any = true;
+        }
+    }
+    return any;
+}
+
+// window_quad with the window size chosen at run time (w.ni, w.nj) among the compiled instances.
+template <int kOS>
+__device__ __forceinline__ bool window_quad_any(const BPWin& w, const float* __restrict__ tile, int tw, int tx0,
+                                                int ty0, bool interior, int W, int H, unsigned x, unsigned y,
+                                                unsigned z, float lambda, float out[4]) {
+    constexpr int kLo = kOS == 1 ? 1 : 2;
+#define ARTIA_WIN_CASE(NI, NJ)                                                                                   \
+    if (w.ni == (NI) && w.nj == (NJ))                                                                           \
+        return window_quad<kOS, (NI), (NJ)>(w, tile, tw, tx0, ty0, interior, W, H, x, y, z, lambda, out);
+    ARTIA_WIN_CASE(kLo + 1, kLo + 1)
+    ARTIA_WIN_CASE(kLo, kLo)
+    ARTIA_WIN_CASE(kLo, kLo + 1)
+    ARTIA_WIN_CASE(kLo + 1, kLo)
+    ARTIA_WIN_CASE(kLo + 2, kLo + 1)
+    ARTIA_WIN_CASE(kLo + 1, kLo + 2)
+    ARTIA_WIN_CASE(kLo + 2, kLo + 2)
+    ARTIA_WIN_CASE(kLo, kLo + 2)
+    ARTIA_WIN_CASE(kLo + 2, kLo)
+#undef ARTIA_WIN_CASE
+    return false;
+}
+
+}  // namespace artia

@@ -44,6 +44,8 @@ artia_status sart_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, 
     const size_t pitch = (size_t)width;
     const float lam = sp->lambda / (float)S;
     bool empty = *volume_is_empty != 0;
+    std::vector<TiltG> sub_geoms((size_t)S);
+    std::vector<const float*> sub_imgs((size_t)S);
 
     Timing t(ctx, ms);
     for (int iter = 0; iter < iterations; iter++) {
@@ -79,15 +81,17 @@ artia_status sart_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, 
                 launch_crop_border(ctx, width, height, pitch, proj, sp->cut_length, sp->dim_length, corners);
                 prof_stop(ctx, pr);
             }
-            for (int i = 0; i < S; i++) {
-                if (step * S + i >= ntilts) continue;
+            // the subset's backprojections (residuals cropped already), in subset order
+            const int nsub = std::min(S, ntilts - step * S);
+            for (int i = 0; i < nsub; i++) {
+                sub_geoms[(size_t)i] = to_device(tilts[index_list[step * S + i]]);
+                sub_imgs[(size_t)i] = residual + (size_t)i * npix;
+            }
+            if (nsub > 0) {
                 empty = false;
-                const int idx = index_list[step * S + i];
-                const TiltG g = to_device(tilts[idx]);
-                float* res = residual + (size_t)i * npix;   // cropped already
                 const int pr = prof_start(ctx, ARTIA_PROF_BP);
-                launch_bp(ctx, v, g, vol, res, width, height, pitch, lam, sp->oversampling, 0.0f, 9999999999999.0f,
-                          ARTIA_FILTER_POINT);
+                launch_bp_subset(ctx, v, sub_geoms.data(), sub_imgs.data(), nsub, width, height, pitch, lam,
+                                 sp->oversampling, vol);
                 prof_stop(ctx, pr);
             }
         }

@@ -22,6 +22,7 @@
 
 #include "artia_internal.hpp"
 #include "bp_common.hpp"
+#include "bp_window.hpp"
 
 namespace artia {
 namespace {
@@ -36,9 +37,11 @@ struct SubvolDev {
 // LINEAR fetches (affine footprint, no magnification anisotropy).
 struct PairDev {
     BPExact e;
+    BPWin w;     // the window form (bp_window.hpp) when win != 0, as the per-call backprojection takes it
     TiltG g;
     VolG v;
     int mode;
+    int win;
 };
 
 // One block's work: sub-volume s, tile origin (quad x, y, z) and tile shape (quads x rows x slices
@@ -276,6 +279,31 @@ __global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__
                 tw = max(0, tx1 - tx0 + 1);
                 th = max(0, ty1 - ty0 + 1);
                 useTile = tw * th <= kTileFloats;
+            }
+            if ((kOS == 1 || kOS == 2) && p.win) {
+                // the per-call backprojection's window form (POINT, lambda 1), from the LDS tile or,
+                // when the item's footprint does not fit it, from the image with clamped lattices
+                const bool interior = useTile && tx0 >= 1 && ty0 >= 1 && tx0 + tw <= W - 1 && ty0 + th <= H - 1;
+                if (useTile) {
+                    for (int idx = tid; idx < tw * th; idx += 256) {
+                        const int r = idx / tw, c = idx - r * tw;
+                        tile[idx] = img[(size_t)(ty0 + r) * pitch + (size_t)(tx0 + c)];
+                    }
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int q = 0; q < kQPT; q++) {
+                    if (!in[q]) continue;
+                    const unsigned z = (unsigned)(item.z0 + tz + q * item.tz);
+                    if (useTile)
+                        window_quad_any<(kOS == 1 ? 1 : 2)>(p.w, tile, tw, tx0, ty0, interior, W, H, x, (unsigned)y, z,
+                                                           1.0f, acc[q]);
+                    else
+                        window_quad_any<(kOS == 1 ? 1 : 2)>(p.w, img, (int)pitch, 0, 0, false, W, H, x, (unsigned)y,
+                                                           z, 1.0f, acc[q]);
+                }
+                if (useTile) __syncthreads();
+                continue;
             }
             if (useTile) {
                 for (int idx = tid; idx < tw * th; idx += 256) {
@@ -747,6 +775,9 @@ artia_status artia_subvol_plan_create(artia_ctx* ctx, const artia_subvol* subs, 
             pd.g = to_device(geoms[i]);
             pd.e = prepare_exact(v, pd.g);
             pd.mode = (pd.e.affine && pd.e.identityAniso) ? (ctf ? 3 : 1) : 2;
+            // (the image read with 32-bit offsets when an item's footprint does not fit the LDS tile)
+            pd.win = (!ctf && pd.mode == 1 && (size_t)height * pitch < ((size_t)1 << 31) &&
+                      window_plan(v, pd.g, oversampling, pd.w)) ? 1 : 0;
         }
     }
     if ((st = upload(ctx, (void**)&p->d_pairs, pairs.data(), pairs.size() * sizeof(PairDev), "pair table")) !=

@@ -80,3 +80,51 @@ def test_sart_parity(gpu, name):
               f"f32-vs-f64 {rel_l2(ref32, ref64b):.2e}", flush=True)
     finally:
         ctx.close()
+
+
+def per_launch_sart(ctx, vg, acq, d_proj, vol, sp, S, lam, empty, border):
+    """One SART iteration of EmSART's ordered-subset loop (EmSart.cpp:579-675) composed from the
+    operator entry points: per subset member the fused residual (artia_fp_residual, or the
+    traversal length while the volume is empty), then one artia_bp launch per member."""
+    H, W = d_proj.shape[1:]
+    T = len(acq.geoms)
+    dist = torch.zeros((H, W), device="cuda")
+    for step in range((T + S - 1) // S):
+        members = acq.index_list[step * S:(step + 1) * S]
+        res = []
+        for idx in members:
+            r = torch.empty((H, W), device="cuda")
+            ctx.fp_residual(vg, acq.geoms[idx], None if empty else vol, d_proj[idx], r, acq.real_lengths[idx],
+                            crop=border.get("crop", (0, 0, 0, 0)), crop_dim=border.get("crop_dim", (0, 0, 0, 0)),
+                            points=acq.hitpoints[idx], cut_length=border.get("cut_length", (0, 0)),
+                            dim_length=border.get("dim_length", (0, 0)), dist=dist, volume_is_empty=empty)
+            res.append(r)
+        for idx, r in zip(members, res):
+            ctx.bp(vg, acq.geoms[idx], vol, r, lam, oversampling=sp.oversampling)
+        empty = False
+    return empty
+
+
+@pytest.mark.parametrize("sirt,os_", [(3, 2), (5, 1), (1, 2)])
+def test_sart_subset_equals_per_launch_loop(gpu, sirt, os_):
+    """The driver backprojects up to four tilts of a subset per launch, adding their updates in
+    registers; the volume equals one backprojection launch per tilt, bit for bit."""
+    border = dict(crop=(2, 2, 2, 2), crop_dim=(3, 3, 3, 3), cut_length=(1, 1), dim_length=(3, 3))
+    vg, acq, projs = make_case((40, 36, 12), 48, 44, phantom.tilt_series(-57, 60, 9), 21, psi_range=4.0,
+                               shift_range=2.0)
+    sp = A.sart_params(lambda_=0.3, sirt_count=sirt, oversampling=os_, **border)
+    ctx = A.ArtiaContext(0)
+    try:
+        d_proj = torch.from_numpy(projs).cuda()
+        vol = torch.zeros((12, 36, 40), device="cuda")
+        _, empty = ctx.sart_run(vg, acq, d_proj, vol, sp, iterations=2)
+        ref = torch.zeros_like(vol)
+        lam = float(np.float32(0.3) / np.float32(sirt))   # Lambda / SIRTCount in float
+        e = per_launch_sart(ctx, vg, acq, d_proj, ref, sp, sirt, lam, True, border)
+        per_launch_sart(ctx, vg, acq, d_proj, ref, sp, sirt, lam, e, border)
+        torch.cuda.synchronize()
+        assert not empty
+        assert float(vol.abs().max()) > 0
+        assert torch.equal(vol, ref)
+    finally:
+        ctx.close()
