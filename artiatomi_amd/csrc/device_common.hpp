@@ -11,7 +11,8 @@
 
 namespace artia {
 
-__device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+// clamp to [lo, hi] (lo <= hi): one v_med3_i32
+__device__ __forceinline__ int clampi(int i, int lo, int hi) { return min(max(i, lo), hi); }
 __device__ __forceinline__ float lerpf(float p, float q, float w) { return __builtin_fmaf(w, q - p, p); }
 
 // Volume element types: fp32, or IEEE binary16, the reference's FP16Volume storage (a half-float

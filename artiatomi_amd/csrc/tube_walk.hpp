@@ -142,13 +142,66 @@ __device__ __forceinline__ void chunk_corners(const VolG& v, const T* __restrict
     }
 }
 
-// Global index of a padded box entry (clamp-addressed: -1 and n map to the edge voxels).
-__device__ __forceinline__ size_t box_voxel(const VolG& v, int X0, int Y0, int Z0, int CX, int CY, int e) {
-    const int row = e / CX, xx = e - row * CX;
-    const int zz = row / CY, yy = row - zz * CY;
-    const size_t gx = (size_t)clampi(X0 + xx, 0, v.nx - 1), gy = (size_t)clampi(Y0 + yy, 0, v.ny - 1);
-    const size_t gz = (size_t)clampi(Z0 + zz, 0, v.nz - 1);
-    return (gz * (size_t)v.ny + gy) * (size_t)v.nx + gx;
+// Box entry e = (zz * CY + yy) * CX + xx of the thread's strided pass over a padded box (entries
+// threadIdx.x, threadIdx.x + 256, ...), stepped incrementally instead of dividing per entry.
+struct BoxCursor {
+    int xx, yy, zz;
+    int sx, sy, sz;   // the step of 256 entries, as (x, y, z) increments before carries
+    int CX, CY;
+    __device__ __forceinline__ BoxCursor(int e, int cx, int cy) : CX(cx), CY(cy) {
+        const int row = e / cx;
+        xx = e - row * cx;
+        zz = row / cy;
+        yy = row - zz * cy;
+        const int rs = 256 / cx;
+        sx = 256 - rs * cx;
+        sz = rs / cy;
+        sy = rs - sz * cy;
+    }
+    __device__ __forceinline__ void next() {
+        xx += sx;
+        yy += sy;
+        zz += sz;
+        if (xx >= CX) {
+            xx -= CX;
+            yy++;
+        }
+        if (yy >= CY) {
+            yy -= CY;
+            zz++;
+        }
+    }
+    // global index of the entry (clamp-addressed: -1 and n map to the edge voxels)
+    // (rows z * ny + y of the volume number below 2^31)
+    __device__ __forceinline__ size_t voxel(const VolG& v, int X0, int Y0, int Z0) const {
+        const int gx = clampi(X0 + xx, 0, v.nx - 1), gy = clampi(Y0 + yy, 0, v.ny - 1);
+        const int gz = clampi(Z0 + zz, 0, v.nz - 1);
+        return (size_t)(unsigned)(gz * v.ny + gy) * (size_t)(unsigned)v.nx + (size_t)(unsigned)gx;
+    }
+};
+
+// Stages the padded box [X0, X0 + CX) x [Y0, ..) x [Z0, ..) (n = CX * CY * CZ entries) into chunk:
+// four loads in flight per thread before their LDS writes.
+template <typename T>
+__device__ __forceinline__ void stage_box(const VolG& v, const T* __restrict__ vol, float* __restrict__ chunk, int X0,
+                                          int Y0, int Z0, int CX, int CY, int n) {
+    int e = (int)threadIdx.x;
+    if (e >= n) return;
+    BoxCursor c(e, CX, CY);
+    for (; e + 3 * 256 < n; e += 4 * 256) {
+        float val[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            val[k] = vox(vol, c.voxel(v, X0, Y0, Z0));
+            c.next();
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) chunk[e + k * 256] = val[k];
+    }
+    for (; e < n; e += 256) {
+        chunk[e] = vox(vol, c.voxel(v, X0, Y0, Z0));
+        c.next();
+    }
 }
 
 // Walks every lane's samples q0 + k*dq (k < count) cell by cell, slab by slab through LDS.  Must be
@@ -207,12 +260,10 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
         const int n = CX * CY * CZ;
         __syncthreads();   // the previous slab's reads (or flush) are done
         if (staged) {
-            for (int e = (int)threadIdx.x; e < n; e += 256) {
-                if (kScatter)
-                    chunk[e] = 0.0f;
-                else
-                    chunk[e] = vox(vol, box_voxel(v, X0, Y0, Z0, CX, CY, e));
-            }
+            if (kScatter)
+                for (int e = (int)threadIdx.x; e < n; e += 256) chunk[e] = 0.0f;
+            else
+                stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, n);
         }
         __syncthreads();
         // A lane whose cells in this slab provably lie in the box (its samples here run from the
@@ -279,9 +330,13 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
         }
         if (kScatter && staged) {
             __syncthreads();   // the box is complete
-            for (int e = (int)threadIdx.x; e < n; e += 256) {
-                const float a = chunk[e];
-                if (a != 0.0f) unsafeAtomicAdd(grad + box_voxel(v, X0, Y0, Z0, CX, CY, e), a);
+            if ((int)threadIdx.x < n) {
+                BoxCursor c((int)threadIdx.x, CX, CY);
+                for (int e = (int)threadIdx.x; e < n; e += 256) {
+                    const float a = chunk[e];
+                    if (a != 0.0f) unsafeAtomicAdd(grad + c.voxel(v, X0, Y0, Z0), a);
+                    c.next();
+                }
             }
         }
     }
@@ -349,8 +404,7 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
         const int n = CX * CY * CZ;
         if (!tail) {
             __syncthreads();
-            if (staged)
-                for (int e = (int)threadIdx.x; e < n; e += 256) chunk[e] = vox(vol, box_voxel(v, X0, Y0, Z0, CX, CY, e));
+            if (staged) stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, n);
             __syncthreads();
         }
         for (;;) {
