@@ -14,6 +14,10 @@ from .core import (  # noqa: F401
     prep_params,
     marker_tilt_params,
     ctf_file_defocus,
+    MotiveList,
+    shift_file_get,
+    shift_file_set,
+    subvolume_job,
     parse_file_header,
     read_file,
     volume_file_header,
@@ -37,5 +41,6 @@ __all__ = [
     "ctf_params", "defocus_distances", "defocus_minmax", "footprint_window", "SartParams", "TiltGeom", "TiltParams",
     "VolGeom", "hitpoints", "load", "projection_index_list", "real_length", "sart_params",
     "subvolume_geometry", "tilt_geometry", "volume_geometry", "FileInfo", "parse_file_header", "read_file",
-    "volume_file_header", "marker_tilt_params", "ctf_file_defocus", "prep_params",
+    "volume_file_header", "marker_tilt_params", "ctf_file_defocus", "prep_params", "MotiveList", "shift_file_get",
+    "shift_file_set", "subvolume_job",
 ]

@@ -112,6 +112,20 @@ ARTIA_FILE_EM = 0
 ARTIA_FILE_MRC = 1
 
 
+class Motive(C.Structure):
+    """artia_motive: one motive-list row (struct motive, MotiveListe.h:81-106), binnings applied."""
+
+    _fields_ = [(n, C.c_float) for n in (
+        "cc_coeff", "x_coord_2d", "y_coord_2d", "part_nr", "tomo_nr", "part_nr_in_tomo", "wedge_idx", "x", "y", "z",
+        "x_shift", "y_shift", "z_shift", "x_shift_before", "y_shift_before", "z_shift_before", "phi", "psi", "theta",
+        "class_no")]
+
+
+ARTIA_GROUP_BY_CLASS = 0
+ARTIA_GROUP_MAX_DIST = 1
+ARTIA_GROUP_MAX_COUNT = 2
+
+
 class Subvol(C.Structure):
     """artia_subvol: sub-volume placement and its device volume."""
 
@@ -187,6 +201,11 @@ SIGNATURES = {
     "artia_volume_f16_to_f32": (_I, [_P, _P, _I, _I, _I, _I, _I, _P, _SZ, _PF]),
     "artia_file_parse_header": (_I, [_I, _P, _SZ, C.POINTER(FileInfo)]),
     "artia_file_volume_header": (_I, [_I, _I, _I, _I, _I, _F, _PF, _P, _SZ, C.POINTER(_SZ)]),
+    "artia_shift_file_get": (_I, [_PF, _I, _I, _I, _I, _I, _PF]),
+    "artia_shift_file_set": (_I, [_PF, _I, _I, _I, _I, _I, _PF]),
+    "artia_motive_get": (_I, [_PF, _I, _I, _I, _F, _F, C.POINTER(Motive)]),
+    "artia_motive_group_count": (_I, [_PF, _I, _I, _I, _PI]),
+    "artia_motive_group": (_I, [_PF, _I, _I, _I, _I, _F, _I, _F, _F, _PI, _I, _PI]),
     "artia_ctf_apply": (_I, [_P, _P, _I, _SZ, C.POINTER(CtfParams), _F, _F, _F, _I, _PF]),
     "artia_wbp_weight": (_I, [_P, _P, _I, _SZ, _F, _I, _PF]),
     "artia_ctf_slab_filter": (_I, [_P, _P, _I, _I, _SZ, _P, _SZ, _I, _PI, _PF, _PF, C.POINTER(CtfParams), _F, _F,

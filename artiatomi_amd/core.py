@@ -235,6 +235,108 @@ def ctf_file_defocus(ctf_data, index: int, tilt_deg: float) -> TiltDefocus:
     return d
 
 
+def _as_f32(a, ndim: int, what: str):
+    import numpy as np
+
+    a = np.ascontiguousarray(a, np.float32)
+    if a.ndim != ndim:
+        raise ArtiaError(f"{what} must have {ndim} dimensions")
+    return a
+
+
+def shift_file_get(shift_data, proj: int, motive: int) -> tuple:
+    """ShiftFile::operator(): the (negated) extra shift of motive at projection proj.  shift_data is
+    the shift file's float array, shape (2, projections, motives)."""
+    a = _as_f32(shift_data, 3, "shift file data")
+    out = (C.c_float * 2)()
+    st = _abi.load().artia_shift_file_get(a.ctypes.data_as(C.POINTER(C.c_float)), a.shape[2], a.shape[1], a.shape[0],
+                                          proj, motive, out)
+    _check(st, what="artia_shift_file_get")
+    return float(out[0]), float(out[1])
+
+
+def shift_file_set(shift_data, proj: int, motive: int, value) -> None:
+    """ShiftFile::SetValue into a writable float32 array of shape (2, projections, motives)."""
+    import numpy as np
+
+    if not (isinstance(shift_data, np.ndarray) and shift_data.dtype == np.float32 and shift_data.ndim == 3
+            and shift_data.flags.c_contiguous and shift_data.flags.writeable):
+        raise ArtiaError("shift file data must be a writable C-contiguous float32 array (2, projections, motives)")
+    st = _abi.load().artia_shift_file_set(shift_data.ctypes.data_as(C.POINTER(C.c_float)), shift_data.shape[2],
+                                          shift_data.shape[1], shift_data.shape[0], proj, motive,
+                                          _abi.farr(value, 2))
+    _check(st, what="artia_shift_file_set")
+
+
+class MotiveList:
+    """A motive list (MotiveListe.h:33-106) over the file's float array of shape (particles, 20),
+    with the position and shift binnings EmSART's ScaleMotivelistPosition/-Shift apply."""
+
+    GROUP_MODES = {"by_class": _abi.ARTIA_GROUP_BY_CLASS, "max_dist": _abi.ARTIA_GROUP_MAX_DIST,
+                   "max_count": _abi.ARTIA_GROUP_MAX_COUNT}
+
+    def __init__(self, data, binning_click: float = 1.0, binning_shift: float = 1.0):
+        self.data = _as_f32(data, 2, "motive list data")
+        if self.data.shape[1] != 20:
+            raise ArtiaError("a motive list row has 20 floats")
+        self.click, self.shift = float(binning_click), float(binning_shift)
+        self._p = self.data.ctypes.data_as(C.POINTER(C.c_float))
+
+    @classmethod
+    def from_file(cls, path, binning_click: float = 1.0, binning_shift: float = 1.0) -> "MotiveList":
+        info, data = read_file(path, "em")
+        return cls(data.reshape(info.ny, info.nx), binning_click, binning_shift)
+
+    def __len__(self) -> int:
+        return self.data.shape[0]
+
+    def __getitem__(self, index: int) -> "_abi.Motive":
+        m = _abi.Motive()
+        _check(_abi.load().artia_motive_get(self._p, 20, len(self), int(index), self.click, self.shift, C.byref(m)),
+               what="artia_motive_get")
+        return m
+
+    def group_count(self, mode: str) -> int:
+        n = C.c_int()
+        _check(_abi.load().artia_motive_group_count(self._p, 20, len(self), self.GROUP_MODES[mode], C.byref(n)),
+               what="artia_motive_group_count")
+        return n.value
+
+    def group(self, mode: str, group: int, max_distance: float = 0.0, group_size: int = 0) -> list:
+        """Row indices of the particles of group `group` (GetNeighbours), the group's first particle first."""
+        out = (C.c_int * len(self))()
+        n = C.c_int()
+        _check(_abi.load().artia_motive_group(self._p, 20, len(self), self.GROUP_MODES[mode], int(group),
+                                              float(max_distance), int(group_size), self.click, self.shift, out,
+                                              len(self), C.byref(n)), what="artia_motive_group")
+        return list(out[:n.value])
+
+
+def subvolume_job(full: VolGeom, acq: "Acquisition", motives: MotiveList, shift_data, size: int,
+                  voxel_size_sub: float, voxel_size_full: Sequence[float] = (1.0, 1.0, 1.0),
+                  max_shift: float = 1e30, glob: Optional[GlobalParams] = None):
+    """EmSARTSubVolumes' inputs for the sub-volume WBP (EmSartSubVolumes.cpp:611-662), from the motive
+    list and the shift file: per motive the sub-volume geometry (PositionInSpace with the motive's
+    position and shift), per (motive, tilt) the tilt geometry with the shift file's extra shift,
+    and whether the pair is backprojected (|extra shift| < MaxShift - 0.5).  Returns (volume
+    geometries, tilt geometries [s * ntilts + t], active flags [s * ntilts + t])."""
+    import numpy as np
+
+    shifts = _as_f32(shift_data, 3, "shift file data") if shift_data is not None else None
+    vgs, geoms, active = [], [], []
+    for s in range(len(motives)):
+        m = motives[s]
+        vgs.append(subvolume_geometry(size, voxel_size_sub, voxel_size_full, full, (m.x, m.y, m.z),
+                                      (m.x_shift, m.y_shift, m.z_shift)))
+        for t in range(len(acq.geoms)):
+            es = shift_file_get(shifts, t, s) if shifts is not None else (0.0, 0.0)
+            ex, ey = np.float32(es[0]), np.float32(es[1])
+            active.append(bool(np.sqrt(ex * ex + ey * ey) < np.float32(max_shift) - np.float32(0.5)))
+            geoms.append(subvolume_tilt_geometry(acq.tilts_deg[t], acq.width, acq.height, psi_deg=acq.psis_deg[t],
+                                                 shift=acq.shifts[t], extra_shift=(float(ex), float(ey)), glob=glob))
+    return vgs, geoms, active
+
+
 FILE_FORMATS = {"em": _abi.ARTIA_FILE_EM, "mrc": _abi.ARTIA_FILE_MRC}
 # numpy element types of the EM DataType / MRC MODE codes that have one
 _EM_NP = {1: "u1", 2: "i2", 4: "i4", 5: "f4", 8: "c8", 9: "f8", 10: "f2"}

@@ -3,8 +3,12 @@
 // (no struct overlay), so the result does not depend on this compiler's struct packing; the fields
 // are little-endian, and an MRC header whose NX reads above 32000 is taken as big-endian and
 // byte-swapped, as MRCFile::OpenAndRead does (MRCFile.cpp:84-88).
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 #include "../../include/artia.h"
 
@@ -55,6 +59,43 @@ bool mrc_type(uint32_t m, int& bytes, int& dtype) {
     case 64: bytes = 2; dtype = -1; return true;                // half, as EmSART writes FP16 volumes
     default: return false;
     }
+}
+
+// struct motive (MotiveListe.h:81-106): 20 floats per particle row
+constexpr int kMotiveColumns = 20;
+
+// MotiveList::GetAt (MotiveList.cpp:76-90): the row, positions and shifts scaled by the binnings
+artia_motive motive_at(const float* data, int index, float click, float shift) {
+    artia_motive m;
+    std::memcpy(&m, data + (size_t)index * kMotiveColumns, sizeof(m));
+    m.x_coord_2d *= click;
+    m.y_coord_2d *= click;
+    m.x *= click;
+    m.y *= click;
+    m.z *= click;
+    m.x_shift *= shift;
+    m.y_shift *= shift;
+    m.z_shift *= shift;
+    return m;
+}
+
+// MotiveList::GetDistance (MotiveList.cpp:126-135), in float
+float motive_distance(const artia_motive& a, const artia_motive& b) {
+    const float x = a.x - b.x, y = a.y - b.y, z = a.z - b.z;
+    return std::sqrt(x * x + y * y + z * z);
+}
+
+// The distinct class numbers in file order, kept as int (MotiveList.cpp:57-73: an int list
+// compared against the float class, appended with the float truncated)
+std::vector<int> class_numbers(const float* data, int rows) {
+    std::vector<int> cls;
+    for (int i = 0; i < rows; i++) {
+        const float c = data[(size_t)i * kMotiveColumns + 19];
+        bool seen = false;
+        for (int k : cls) seen = seen || (float)k == c;
+        if (!seen) cls.push_back((int)c);
+    }
+    return cls;
 }
 
 }  // namespace
@@ -136,6 +177,94 @@ artia_status artia_file_volume_header(int format, int half, int nx, int ny, int 
         return ARTIA_OK;
     }
     return ARTIA_ERR_INVALID_ARGUMENT;
+}
+
+// ---------------------------------------------------------------- ShiftFile
+
+artia_status artia_shift_file_get(const float* data, int dim_x, int dim_y, int dim_z, int proj, int motive,
+                                  float out[2]) {
+    if (!data || !out || dim_x <= 0 || dim_y <= 0 || dim_z < 2 || proj < 0 || proj >= dim_y || motive < 0 ||
+        motive >= dim_x)
+        return ARTIA_ERR_INVALID_ARGUMENT;
+    const size_t plane = (size_t)dim_x * (size_t)dim_y, at = (size_t)proj * (size_t)dim_x + (size_t)motive;
+    out[0] = -data[at];
+    out[1] = -data[plane + at];
+    return ARTIA_OK;
+}
+
+artia_status artia_shift_file_set(float* data, int dim_x, int dim_y, int dim_z, int proj, int motive,
+                                  const float value[2]) {
+    if (!data || !value || dim_x <= 0 || dim_y <= 0 || dim_z < 2 || proj < 0 || proj >= dim_y || motive < 0 ||
+        motive >= dim_x)
+        return ARTIA_ERR_INVALID_ARGUMENT;
+    const size_t plane = (size_t)dim_x * (size_t)dim_y, at = (size_t)proj * (size_t)dim_x + (size_t)motive;
+    data[at] = value[0];
+    data[plane + at] = value[1];
+    return ARTIA_OK;
+}
+
+// ---------------------------------------------------------------- MotiveList
+
+artia_status artia_motive_get(const float* data, int dim_x, int dim_y, int index, float binning_click,
+                              float binning_shift, artia_motive* out) {
+    static_assert(sizeof(artia_motive) == kMotiveColumns * sizeof(float), "artia_motive is 20 floats");
+    if (!data || !out || dim_x != kMotiveColumns || dim_y <= 0 || index < 0 || index >= dim_y)
+        return ARTIA_ERR_INVALID_ARGUMENT;
+    *out = motive_at(data, index, binning_click, binning_shift);
+    return ARTIA_OK;
+}
+
+artia_status artia_motive_group_count(const float* data, int dim_x, int dim_y, int mode, int* count) {
+    if (!data || !count || dim_x != kMotiveColumns || dim_y <= 0) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (mode == ARTIA_GROUP_BY_CLASS) {
+        *count = (int)class_numbers(data, dim_y).size();
+        return ARTIA_OK;
+    }
+    if (mode == ARTIA_GROUP_MAX_DIST || mode == ARTIA_GROUP_MAX_COUNT) {
+        *count = dim_y;
+        return ARTIA_OK;
+    }
+    return ARTIA_ERR_INVALID_ARGUMENT;
+}
+
+artia_status artia_motive_group(const float* data, int dim_x, int dim_y, int mode, int group, float max_distance,
+                                int group_size, float binning_click, float binning_shift, int* out, int capacity,
+                                int* count) {
+    if (!data || !count || dim_x != kMotiveColumns || dim_y <= 0 || capacity < 0 || (capacity > 0 && !out))
+        return ARTIA_ERR_INVALID_ARGUMENT;
+    std::vector<int> members;
+    if (mode == ARTIA_GROUP_BY_CLASS) {
+        // GetGroupIdx: -1 (no members) outside the class list; a negative class number also
+        // yields no members (MotiveList.cpp:206-207)
+        const std::vector<int> classes = class_numbers(data, dim_y);
+        if (group >= 0 && group < (int)classes.size() && classes[(size_t)group] >= 0) {
+            const float cls = (float)classes[(size_t)group];
+            for (int i = 0; i < dim_y; i++)
+                if (data[(size_t)i * kMotiveColumns + 19] == cls) members.push_back(i);
+        }
+    } else if (mode == ARTIA_GROUP_MAX_DIST || mode == ARTIA_GROUP_MAX_COUNT) {
+        if (group < 0 || group >= dim_y) return ARTIA_ERR_INVALID_ARGUMENT;
+        const artia_motive c = motive_at(data, group, binning_click, binning_shift);
+        if (mode == ARTIA_GROUP_MAX_DIST) {
+            members.push_back(group);
+            for (int i = 0; i < dim_y; i++)
+                if (i != group && motive_distance(motive_at(data, i, binning_click, binning_shift), c) <= max_distance)
+                    members.push_back(i);
+        } else {
+            std::vector<std::pair<float, int>> d((size_t)dim_y);
+            for (int i = 0; i < dim_y; i++)
+                d[(size_t)i] = {motive_distance(motive_at(data, i, binning_click, binning_shift), c), i};
+            std::sort(d.begin(), d.end());
+            const int n = std::min(group_size, dim_y);
+            for (int i = 0; i < n; i++) members.push_back(d[(size_t)i].second);
+        }
+    } else {
+        return ARTIA_ERR_INVALID_ARGUMENT;
+    }
+    *count = (int)members.size();
+    if ((int)members.size() > capacity) return ARTIA_ERR_INVALID_ARGUMENT;
+    std::copy(members.begin(), members.end(), out);
+    return ARTIA_OK;
 }
 
 }  // extern "C"

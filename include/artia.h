@@ -507,6 +507,45 @@ artia_status artia_file_volume_header(int format, int half, int nx, int ny, int 
                                       const float voxel_size[3], void* out, size_t capacity,
                                       size_t* header_bytes);
 
+/* ShiftFile (EmTools/FileIO/ShiftFile.cpp:68-84): an EM float file of dim_x motives by dim_y
+ * projections by dim_z = 2 components, component c of (proj, motive) at
+ * data[c * dim_x * dim_y + proj * dim_x + motive].  artia_shift_file_get is operator(): both
+ * components negated (the shift EmSARTSubVolumes applies, EmSartSubVolumes.cpp:651-653);
+ * artia_shift_file_set is SetValue: stored as given. */
+artia_status artia_shift_file_get(const float* data, int dim_x, int dim_y, int dim_z, int proj, int motive,
+                                  float out[2]);
+artia_status artia_shift_file_set(float* data, int dim_x, int dim_y, int dim_z, int proj, int motive,
+                                  const float value[2]);
+
+/* MotiveList (EmTools/FileIO/MotiveListe.h:33-106, MotiveList.cpp): an EM float file of
+ * dim_x = 20 columns (struct motive, in this order) by dim_y particles.  artia_motive_get is GetAt:
+ * the row of particle `index`, with the five coordinates scaled by binning_click and the three
+ * shifts by binning_shift (ScaleMotivelistPosition, ScaleMotivelistShift). */
+typedef struct {
+    float cc_coeff, x_coord_2d, y_coord_2d, part_nr, tomo_nr, part_nr_in_tomo, wedge_idx;
+    float x, y, z;                      /* x_Coord, y_Coord, z_Coord: position, voxels, from 1 */
+    float x_shift, y_shift, z_shift;
+    float x_shift_before, y_shift_before, z_shift_before;
+    float phi, psi, theta, class_no;
+} artia_motive;
+artia_status artia_motive_get(const float* data, int dim_x, int dim_y, int index, float binning_click,
+                              float binning_shift, artia_motive* out);
+/* The particle groups EmSARTRefine iterates over (EmSartRefine.cpp:697-710):
+ *   ARTIA_GROUP_BY_CLASS  MotiveList::GetNeighbours(int groupNr) (MotiveList.cpp:202-219): group g
+ *                         is the particles whose class equals the g-th distinct class number in
+ *                         file order (class numbers are kept as int, MotiveList.cpp:57-73);
+ *   ARTIA_GROUP_MAX_DIST  GetNeighbours(index, maxDist) (:180-193): particle g first, then every
+ *                         other particle within max_distance (scaled coordinates), in file order;
+ *   ARTIA_GROUP_MAX_COUNT GetNeighbours(index, count) (:147-178): the group_size nearest particles
+ *                         by (distance, index), particle g itself first.
+ * artia_motive_group_count is GetGroupCount (:221-233).  artia_motive_group writes the group's
+ * particle indices (global rows) to out (capacity entries) and their number to *count. */
+typedef enum { ARTIA_GROUP_BY_CLASS = 0, ARTIA_GROUP_MAX_DIST = 1, ARTIA_GROUP_MAX_COUNT = 2 } artia_group_mode;
+artia_status artia_motive_group_count(const float* data, int dim_x, int dim_y, int mode, int* count);
+artia_status artia_motive_group(const float* data, int dim_x, int dim_y, int mode, int group, float max_distance,
+                                int group_size, float binning_click, float binning_shift, int* out, int capacity,
+                                int* count);
+
 /* ---------------------------------------------------------------- sub-volume WBP
  * EmSARTSubVolumes (EmSartSubVolumes.cpp:611-707): every motive-list sub-volume is reconstructed
  * by backprojecting each tilt's WBP-filtered projection (artia_wbp_filter) with lambda 1 through

@@ -171,3 +171,115 @@ def test_read_file_em_and_mrc(tmp_path):
     r.write_bytes(mrc_header(4, 3, 2, 6, order=">") + c.tobytes())
     info, d = A.read_file(r)
     assert info.byte_swapped == 1 and np.array_equal(d.astype(np.uint16), c.astype(np.uint16))
+
+
+# ---------------------------------------------------------------- ShiftFile and MotiveList
+# ShiftFile (ShiftFile.cpp:68-84): EM float data of DimX = motives, DimY = projections, DimZ = 2;
+# operator() negates.  MotiveList (MotiveListe.h:81-106, MotiveList.cpp): rows of 20 floats, GetAt
+# scales the 2D and 3D coordinates by the click binning and the shifts by the shift binning.
+
+MOTIVE_FIELDS = ["cc_coeff", "x_coord_2d", "y_coord_2d", "part_nr", "tomo_nr", "part_nr_in_tomo", "wedge_idx",
+                 "x", "y", "z", "x_shift", "y_shift", "z_shift", "x_shift_before", "y_shift_before",
+                 "z_shift_before", "phi", "psi", "theta", "class_no"]
+
+
+def test_shift_file_layout_and_sign(tmp_path):
+    P, M = 7, 5
+    rng = np.random.default_rng(11)
+    sx, sy = rng.uniform(-4, 4, (P, M)).astype("<f4"), rng.uniform(-4, 4, (P, M)).astype("<f4")
+    p = tmp_path / "shifts.em"
+    p.write_bytes(em_header(M, P, 2, 5) + sx.tobytes() + sy.tobytes())   # DimX motives, DimY projections
+    info, d = A.read_file(p)
+    assert (info.nx, info.ny, info.nz) == (M, P, 2)
+    for proj, mot in ((0, 0), (3, 4), (6, 2)):
+        assert A.shift_file_get(d, proj, mot) == (-float(sx[proj, mot]), -float(sy[proj, mot]))
+    out = np.zeros((2, P, M), np.float32)
+    A.shift_file_set(out, 5, 1, (1.25, -2.5))
+    assert out[0, 5, 1] == 1.25 and out[1, 5, 1] == -2.5 and np.count_nonzero(out) == 2
+    assert A.shift_file_get(out, 5, 1) == (-1.25, 2.5)
+    for bad in ((P, 0), (0, M), (-1, 0)):
+        with pytest.raises(A.ArtiaError):
+            A.shift_file_get(d, *bad)
+    with pytest.raises(A.ArtiaError):
+        A.shift_file_get(np.zeros((1, P, M), np.float32), 0, 0)   # one component only
+
+
+def motive_rows(n, seed, classes=None):
+    rng = np.random.default_rng(seed)
+    rows = rng.uniform(-20, 20, (n, 20)).astype(np.float32)
+    rows[:, 7:10] = rng.uniform(1, 100, (n, 3)).astype(np.float32)
+    rows[:, 19] = classes if classes is not None else rng.integers(0, 3, n)
+    return rows
+
+
+def test_motive_list_row_layout_and_binning(tmp_path):
+    rows = motive_rows(9, 4)
+    p = tmp_path / "motl.em"
+    p.write_bytes(em_header(20, 9, 1, 5) + rows.astype("<f4").tobytes())
+    ml = A.MotiveList.from_file(p, binning_click=2.0, binning_shift=0.5)
+    assert len(ml) == 9
+    f32 = np.float32
+    for i in (0, 4, 8):
+        m = ml[i]
+        for k, name in enumerate(MOTIVE_FIELDS):
+            scale = 2.0 if name in ("x_coord_2d", "y_coord_2d", "x", "y", "z") else \
+                0.5 if name in ("x_shift", "y_shift", "z_shift") else 1.0
+            assert getattr(m, name) == float(f32(rows[i, k]) * f32(scale)), name
+    with pytest.raises(A.ArtiaError):
+        ml[9]
+    with pytest.raises(A.ArtiaError):
+        A.MotiveList(np.zeros((3, 19), np.float32))
+
+
+def float_dist(a, b):
+    d = (a - b).astype(np.float32)
+    return np.float32(np.sqrt(np.float32(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])))
+
+
+def test_motive_groups_by_class():
+    # class numbers in file order 2, 0, 2.5 (truncated to 2 but compared as float: a class of its own
+    # whose members are those equal to 2.0), -1 (no members)
+    cls = np.array([2, 0, 2, 2.5, -1, 0, 2], np.float32)
+    ml = A.MotiveList(motive_rows(len(cls), 5, cls))
+    assert ml.group_count("by_class") == 4
+    assert ml.group("by_class", 0) == [0, 2, 6]
+    assert ml.group("by_class", 1) == [1, 5]
+    assert ml.group("by_class", 2) == [0, 2, 6]
+    assert ml.group("by_class", 3) == []
+    assert ml.group("by_class", 7) == []
+
+
+def test_motive_groups_by_distance_and_count():
+    rows = motive_rows(40, 6)
+    rows[17, 7:10] = rows[3, 7:10]   # a particle at the same place as particle 3
+    ml = A.MotiveList(rows, binning_click=1.5)
+    pos = rows[:, 7:10] * np.float32(1.5)
+    assert ml.group_count("max_dist") == 40 and ml.group_count("max_count") == 40
+    for g in (0, 3, 17, 39):
+        d = np.array([float_dist(pos[i], pos[g]) for i in range(40)], np.float32)
+        want = [g] + [i for i in range(40) if i != g and d[i] <= np.float32(30.0)]
+        assert ml.group("max_dist", g, max_distance=30.0) == want
+        order = sorted(range(40), key=lambda i: (d[i], i))
+        assert ml.group("max_count", g, group_size=6) == order[:6]
+    # particle 3 and 17 coincide: the (distance, index) order puts 3 first in both groups
+    assert ml.group("max_count", 17, group_size=2) == [3, 17]
+    assert ml.group("max_count", 5, group_size=100) == sorted(range(40), key=lambda i: (
+        float_dist(pos[i], pos[5]), i))
+
+
+def test_subvolume_job_from_motive_list_and_shift_file():
+    full = A.volume_geometry(256, 256, 64)
+    acq = A.Acquisition.build(full, 128, 128, [-30.0, 0.0, 30.0], shifts=[(1.0, -2.0), (0, 0), (0.5, 0.25)])
+    rows = motive_rows(4, 7)
+    rows[:, 10:13] = np.float32([[1, 0, 0], [0, -1, 0], [0, 0, 2], [0.5, 0.5, 0.5]])
+    ml = A.MotiveList(rows)
+    shifts = np.zeros((2, 3, 4), np.float32)
+    shifts[0, 1, 2], shifts[1, 1, 2] = 3.0, 4.0    # |shift| = 5 for motive 2 at projection 1
+    vgs, geoms, active = A.subvolume_job(full, acq, ml, shifts, 32, 1.0, max_shift=5.4)
+    assert len(vgs) == 4 and len(geoms) == 12
+    assert active == [True] * 7 + [False] + [True] * 4       # 5 < 5.4 - 0.5 fails for pair (2, 1)
+    m = ml[1]
+    ref = A.subvolume_geometry(32, 1.0, (1.0, 1.0, 1.0), full, (m.x, m.y, m.z), (m.x_shift, m.y_shift, m.z_shift))
+    assert bytes(vgs[1]) == bytes(ref)
+    g = A.subvolume_tilt_geometry(0.0, 128, 128, shift=(0.0, 0.0), extra_shift=(-3.0, -4.0))
+    assert bytes(geoms[2 * 3 + 1]) == bytes(g)
