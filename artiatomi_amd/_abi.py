@@ -233,6 +233,14 @@ SIGNATURES = {
     "artia_pairs_cc_search": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _PF]),
     "artia_profile_enable": (_I, [_P, _I]),
     "artia_profile_read": (_I, [_P, _I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "artia_fp_ctf": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _I, _SZ, _F, _F,
+                          C.POINTER(TiltDefocus), C.POINTER(CtfParams), _PF, _PF, _P, _P, _PF]),
+    "artia_fp_ctf_f16": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _I, _SZ, _F, _F,
+                              C.POINTER(TiltDefocus), C.POINTER(CtfParams), _PF, _PF, _P, _P, _PF]),
+    "artia_bp_ctf": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _P, _I, _I, _SZ, _PI, _F, _F,
+                          C.POINTER(TiltDefocus), C.POINTER(CtfParams), _PF, _PF, _F, _I, _PF]),
+    "artia_bp_ctf_f16": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _P, _I, _I, _SZ, _PI, _F, _F,
+                              C.POINTER(TiltDefocus), C.POINTER(CtfParams), _PF, _PF, _F, _I, _PF]),
     "artia_sart_run": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _I, _PI, _P, _I, _I, _P,
                             C.POINTER(SartParams), _I, _PI, _PF]),
     "artia_sart_ctf_run": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _PF, C.POINTER(TiltDefocus),

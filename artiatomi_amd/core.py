@@ -650,6 +650,31 @@ class ArtiaContext:
             out[name] = (ms.value, n.value)
         return out
 
+    def fp_ctf(self, vg: VolGeom, tg: TiltGeom, vol, slab_range, defocus: TiltDefocus, ctf: CtfParams, proj, dist,
+               cut_length=(0, 0), dim_length=(0, 0), volume_is_empty=False, timed=False):
+        """ForwardProjectionCTF for one tilt (artia_fp_ctf): proj += the CTF-filtered slab projections
+        over slab_range = (t_in, t_out); dist = the centre-ray traversal length."""
+        H, W = proj.shape
+        if vol is None:
+            vp, sfx = None, ""
+        else:
+            vp, sfx = _vol(vol, vg)
+        return self._call("artia_fp_ctf" + sfx, C.byref(vg), C.byref(tg), vp, int(bool(volume_is_empty)), W, H, W,
+                          float(slab_range[0]), float(slab_range[1]), C.byref(defocus), C.byref(ctf),
+                          _abi.farr(cut_length, 2), _abi.farr(dim_length, 2), _ptr(proj, "proj"),
+                          _ptr(dist, "dist", W * H), timed=timed)
+
+    def bp_ctf(self, vg: VolGeom, tg: TiltGeom, vol, residual, points, slab_range, defocus: TiltDefocus,
+               ctf: CtfParams, lambda_, oversampling=1, cut_length=(0, 0), dim_length=(0, 0), timed=False):
+        """BackProjectionCTF for one tilt (artia_bp_ctf): every slab of slab_range Wiener-filtered and
+        backprojected inside its distance gate; the residual is left cropped once per slab."""
+        H, W = residual.shape
+        vp, sfx = _vol(vol, vg)
+        return self._call("artia_bp_ctf" + sfx, C.byref(vg), C.byref(tg), vp, _ptr(residual, "residual"), W, H, W,
+                          _abi.iarr(points, 8), float(slab_range[0]), float(slab_range[1]), C.byref(defocus),
+                          C.byref(ctf), _abi.farr(cut_length, 2), _abi.farr(dim_length, 2), float(lambda_),
+                          int(oversampling), timed=timed)
+
     def sart_run(self, vg: VolGeom, acq: Acquisition, projections, vol, params: SartParams, iterations=1,
                  volume_is_empty: bool = True, timed=False):
         """Runs `iterations` SART iterations; returns (ms or None, volume_is_empty)."""

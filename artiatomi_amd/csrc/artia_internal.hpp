@@ -134,8 +134,10 @@ void launch_vol_traversal(artia_ctx* ctx, const VolG& vg, const TiltG& tg, int W
                           float* dist, int x0, int y0, int x1, int y1);
 void launch_compare(artia_ctx* ctx, int W, int H, size_t pitch, const float* real, float* virt,
                     const float* dist, float realLength, const float cut[4], const float dim[4], float scale);
+// cropBorder in place, `repeat` times over (the weight multiplied in that many times, as that many
+// launches would)
 void launch_crop_border(artia_ctx* ctx, int W, int H, size_t pitch, float* img, const float cut[2],
-                        const float dim[2], const int p[8]);
+                        const float dim[2], const int p[8], int repeat = 1);
 void launch_bp(artia_ctx* ctx, const VolG& vg, const TiltG& tg, VolData vol, const float* img, int W, int H,
                size_t pitch, float lambda, int os, float distMin, float distMax, int filter);
 // The SART backprojections of one ordered subset (POINT sampling, gate [0, 1e13)): tilt k's residual at

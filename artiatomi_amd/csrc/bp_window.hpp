@@ -43,6 +43,12 @@ struct BPWin {
     double G[2][3], lo[2], hi[2], tlo[2], thi[2];     // quad and block footprints (see BPExact)
     double gn[3], g0;                                 // dist gate: |gn . q + g0|
     int ni, nj;              // window size in lattice points
+    // The reference sums only the sub-rays inside a quad's footprint, the box spanned by two
+    // projected corners of its macro-cell (BackProjectionSquareOS.cu:365-509, x bounds swapped
+    // when n_x <= 0), which need not cover a voxel's projected box.  covers != 0: it covers the
+    // projected boxes of all four voxels of every quad of the volume, so a quad whose footprint
+    // the image edges do not clamp may walk its windows without the per-point footprint test.
+    int covers;
 };
 
 // Whether the window form applies to a tilt and its constants, for oversampling os and a window of
@@ -112,6 +118,25 @@ inline bool prepare_window(const VolG& v, const TiltG& g, const BPExact& e, int 
     // far inside the 1e-4 guard already in uMin/uMax)
     w.ni = (int)std::floor(((double)e.uMax - (double)e.uMin) * os) + 1;
     w.nj = (int)std::floor(((double)e.vMax - (double)e.vMin) * os) + 1;
+    // Footprint [G_r . q + lo_r, G_r . q + hi_r] against the union of the quad's voxel boxes
+    // [p_r(q) + min(0, 3 p_ri) + min_r, p_r(q) + max(0, 3 p_ri) + max_r] (guarded, so conservative):
+    // both ends are affine in the quad origin q, so the volume's corner quads decide.  A lattice
+    // point inside a box then lies in [floor(lower) * os, ceil(upper) * os) as the clamp requires.
+    w.covers = 1;
+    const double qx[2] = {0.0, (double)std::max(0, v.nx - 4)}, qy[2] = {0.0, (double)(v.ny - 1)},
+                 qz[2] = {0.0, (double)(v.nz - 1)};
+    for (int c = 0; c < 8; c++) {
+        const double q[3] = {qx[c & 1], qy[(c >> 1) & 1], qz[(c >> 2) & 1]};
+        for (int r = 0; r < 2; r++) {
+            const double gq = e.G[r][0] * q[0] + e.G[r][1] * q[1] + e.G[r][2] * q[2];
+            const double p = r == 0 ? e.pu0 + e.pui * q[0] + e.puj * q[1] + e.puk * q[2]
+                                    : e.pv0 + e.pvi * q[0] + e.pvj * q[1] + e.pvk * q[2];
+            const double pi = r == 0 ? e.pui : e.pvi;
+            const double boxLo = p + std::fmin(0.0, 3.0 * pi) + (double)(r == 0 ? e.uMin : e.vMin);
+            const double boxHi = p + std::fmax(0.0, 3.0 * pi) + (double)(r == 0 ? e.uMax : e.vMax);
+            if (!(gq + e.lo[r] <= boxLo && boxHi <= gq + e.hi[r])) w.covers = 0;
+        }
+    }
     return w.ni >= 1 && w.nj >= 1 && w.ni <= maxN && w.nj <= maxN;
 }
 
@@ -181,8 +206,8 @@ inline bool window_plan(const VolG& v, const TiltG& g, int os, BPWin& w) {
 // The update of one quad (voxels x .. x+3 of row y, slice z) for one tilt: out[k] += lambda *
 // num / den over each voxel's window, from an image tile of row pitch tw whose pixel (tx0, ty0) is
 // tile[0] (the whole image with tw = pitch and tx0 = ty0 = 0).  interior: the quad's footprint is
-// not clamped by the image edges (then every window point with a positive chord lies inside it);
-// otherwise the lattice is clamped to the clamped footprint, as the per-box loop does.  Returns
+// not clamped by the image edges; with w.covers every window point with a positive chord then lies
+// inside it.  Otherwise the lattice is clamped to the clamped footprint, as the per-box loop does.  Returns
 // whether a voxel changed.
 template <int kOS, int kNI, int kNJ>
 __device__ __forceinline__ bool window_quad(const BPWin& w, const float* __restrict__ tile, int tw, int tx0, int ty0,
@@ -191,6 +216,7 @@ __device__ __forceinline__ bool window_quad(const BPWin& w, const float* __restr
     constexpr float kDelta = 1.0f / (float)kOS;
     const double qx = (double)x, qy = (double)y, qz = (double)z;
     int li0 = 0, li1 = 0, lj0 = 0, lj1 = 0;
+    interior = interior && w.covers;
     if (!interior) {
         // the quad's footprint clamped to [0, W-1] x [0, H-1] (footprint_affine): lattice points
         // [px0 * OS, px1 * OS) x [py0 * OS, py1 * OS)

@@ -18,14 +18,18 @@ __global__ void __launch_bounds__(256) compare_kernel(int W, int H, size_t pitch
     virt[i] = compare_value(x, y, W, H, real[i], virt[i], dist[i], L, cut, dim, S);
 }
 
-// Compare.cu:170-277 (weight in image_ops.hpp)
+// Compare.cu:170-277 (weight in image_ops.hpp), applied `repeat` times in place as that many
+// launches would apply it
 __global__ void __launch_bounds__(256) crop_border_kernel(int W, int H, size_t pitch, float* __restrict__ img,
-                                                          CropParams c) {
+                                                          CropParams c, int repeat) {
     const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
     if (x >= W || y >= H) return;
     const size_t i = (size_t)y * pitch + x;
-    img[i] = crop_weight(x, y, W, H, c) * img[i];
+    const float w = crop_weight(x, y, W, H, c);
+    float val = img[i];
+    for (int r = 0; r < repeat; r++) val = w * val;
+    img[i] = val;
 }
 
 // CopyToSquare.cu:37-92: W x H -> N x N, centred zero fill or mirror reflection, optional y flip.
@@ -73,9 +77,10 @@ void launch_compare(artia_ctx* ctx, int W, int H, size_t pitch, const float* rea
 }
 
 void launch_crop_border(artia_ctx* ctx, int W, int H, size_t pitch, float* img, const float cut[2], const float dim[2],
-                        const int p[8]) {
+                        const int p[8], int repeat) {
+    if (repeat <= 0) return;
     hipLaunchKernelGGL(crop_border_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, W, H, pitch, img,
-                       make_crop_params(cut, dim, p));
+                       make_crop_params(cut, dim, p), repeat);
 }
 
 void launch_make_square(artia_ctx* ctx, int W, int H, size_t pitch, const float* in, int N, float* out, int bx, int by,

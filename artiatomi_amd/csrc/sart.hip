@@ -15,6 +15,225 @@ namespace {
 // Upper bound on CTF slabs per tilt: a slab loop longer than this means a degenerate range/step.
 constexpr double kMaxSlabs = 1 << 16;
 
+// One tilt's CTF slabs: [bounds[k], bounds[k] + step) for k < n, stepped in float as the
+// reference's loops step them (Reconstructor.cpp:514, 934), with each slab's defocus range
+// (GetDefocusMinMax, Reconstructor.cpp:522, 942); the last slab ends at last_end.
+struct CtfSlabs {
+    const float* bounds;     // host
+    const float* d_bounds;   // device copy (stacked forms)
+    const float* dmin;
+    const float* dmax;
+    int n;
+    float step, last_end;
+};
+
+// Workspace of the stacked forms: a tilt's slab images at stack + k * stride and the stacked
+// slicer's sub-ray sums.  on == false: one launch per slab (reference numerics, or no room).
+struct CtfStackWs {
+    bool on;
+    float* stack;
+    size_t stride;
+    void* slicer_scratch;
+    size_t slicer_bytes;
+};
+
+// Reconstructor::ForwardProjectionCTF's slab loop (Reconstructor.cpp:514-584) for one tilt:
+// proj += cropBorder(ctf-filter(cropBorder(slicer(slab)))) over the slabs in slab order, with the
+// image corners as the crop quadrilateral.  Stacked: every slab's slicer image in one volume pass,
+// then the batched filter sums them in slab order; otherwise one slicer launch and one filter per
+// slab through `scratch` (W x H).  Both leave the same bits in proj.
+artia_status ctf_forward_slabs(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, int W, int H, size_t pitch,
+                               float* proj, float* scratch, const CtfSlabs& s, const CtfStackWs& ws,
+                               const artia_ctf_params& ctf, float astig_deg, const float cut[2], const float dim[2]) {
+    // image corners (Reconstructor.cpp:557-565): pA, pB, pC, pD
+    const int frame[8] = {0, H - 1, W - 1, H - 1, 0, 0, W - 1, 0};
+    if (ws.on) {
+        int pr = prof_start(ctx, ARTIA_PROF_FP);
+        launch_slicer_stack(ctx, v, g, vol, W, H, pitch, ws.stack, ws.stride, s.bounds, s.d_bounds, s.n, s.step,
+                            ws.slicer_scratch, ws.slicer_bytes);
+        prof_stop(ctx, pr);
+        pr = prof_start(ctx, ARTIA_PROF_FFT);
+        const artia_status st = ctf_filter_slabs(ctx, ws.stack, ws.stride, s.n, W, H, pitch, proj, 0, frame, cut, dim,
+                                                 ctf, s.dmin, s.dmax, astig_deg, 1);
+        prof_stop(ctx, pr);
+        return st;
+    }
+    const size_t plane = (size_t)(H - 1) * pitch + (size_t)W;
+    for (int k = 0; k < s.n; k++) {
+        int pr = prof_start(ctx, ARTIA_PROF_FILL);
+        launch_fill(ctx, scratch, plane, 0.0f);
+        prof_stop(ctx, pr);
+        pr = prof_start(ctx, ARTIA_PROF_FP);
+        launch_slicer(ctx, v, g, vol, W, H, pitch, scratch, s.bounds[k], s.bounds[k] + s.step);
+        prof_stop(ctx, pr);
+        pr = prof_start(ctx, ARTIA_PROF_FFT);
+        const artia_status st = ctf_slab_filter(ctx, scratch, W, H, pitch, proj, pitch, 1, frame, cut, dim, ctf,
+                                                s.dmin[k], s.dmax[k], astig_deg, 1);
+        prof_stop(ctx, pr);
+        if (st != ARTIA_OK) return st;
+    }
+    return ARTIA_OK;
+}
+
+// Reconstructor::BackProjectionCTF's slab loop (Reconstructor.cpp:934-966) for one tilt: per slab,
+// cropBorder(res) in place, the Wiener/phase-flip filter, cropBorder, and the slab-gated LINEAR
+// backprojection.  Stacked: the slabs are filtered in batches into the stack (res unchanged) and
+// backprojected in one launch; otherwise one filter (res cropped in place) and one gated launch
+// per slab through `scratch`.  Both leave the same bits in the volume.
+artia_status ctf_backward_slabs(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, int W, int H, size_t pitch,
+                                float* res, float* scratch, const int* corners, const CtfSlabs& s,
+                                const CtfStackWs& ws, const artia_ctf_params& ctf, float astig_deg, const float cut[2],
+                                const float dim[2], float lambda, int os) {
+    if (ws.on) {
+        int pr = prof_start(ctx, ARTIA_PROF_FFT);
+        const artia_status st = ctf_filter_slabs(ctx, res, 0, s.n, W, H, pitch, ws.stack, ws.stride, corners, cut,
+                                                 dim, ctf, s.dmin, s.dmax, astig_deg, 0);
+        prof_stop(ctx, pr);
+        if (st != ARTIA_OK || s.n == 0) return st;
+        pr = prof_start(ctx, ARTIA_PROF_BP);
+        launch_bp_stack(ctx, v, g, vol, ws.stack, ws.stride, W, H, pitch, lambda, os, ARTIA_FILTER_LINEAR, s.d_bounds,
+                        s.n, s.last_end);
+        prof_stop(ctx, pr);
+        return ARTIA_OK;
+    }
+    for (int k = 0; k < s.n; k++) {
+        int pr = prof_start(ctx, ARTIA_PROF_FFT);
+        const artia_status st = ctf_slab_filter(ctx, res, W, H, pitch, scratch, pitch, 0, corners, cut, dim, ctf,
+                                                s.dmin[k], s.dmax[k], astig_deg, 0);
+        prof_stop(ctx, pr);
+        if (st != ARTIA_OK) return st;
+        pr = prof_start(ctx, ARTIA_PROF_BP);
+        launch_bp(ctx, v, g, vol, scratch, W, H, pitch, lambda, os, s.bounds[k], s.bounds[k] + s.step,
+                  ARTIA_FILTER_LINEAR);
+        prof_stop(ctx, pr);
+    }
+    return ARTIA_OK;
+}
+
+// Bytes for the stacked slicer's sub-ray sums next to a stack of stack_floats: every slab of a tilt
+// in one volume pass while they fit 16 GiB (or what the context's stack limit leaves), passes of
+// fewer slabs otherwise; 0 (one slicer launch per slab) when not even one slab's sums fit.
+size_t slicer_scratch_for(const artia_ctx* ctx, int W, int H, int max_slabs, size_t stack_floats) {
+    const size_t cap = ctx->stack_limit ? ctx->stack_limit - stack_floats * sizeof(float) : ((size_t)16 << 30);
+    int pass = max_slabs;
+    while (pass > 1 && slicer_stack_scratch_bytes(W, H, pass) > cap) pass = (pass + 1) / 2;
+    return slicer_stack_scratch_bytes(W, H, pass) <= cap ? slicer_stack_scratch_bytes(W, H, pass) : 0;
+}
+
+// Whether a stack of max_slabs images of `plane` floats takes the stacked forms: exact numerics and
+// the stack within the context's stack limit (32 GiB by default).
+bool stack_fits(const artia_ctx* ctx, int max_slabs, size_t plane) {
+    const double cap = ctx->stack_limit ? (double)ctx->stack_limit : 34359738368.0;
+    return ctx->numerics == ARTIA_NUMERICS_EXACT && max_slabs > 0 && (double)max_slabs * (double)plane * 4.0 <= cap;
+}
+
+// The slabs and workspace of one per-tilt CTF call: the slab loop of GetDefocusDistances' range
+// [a, b) stepped by CTFSliceThickness / pixel size, each slab's defocus range, and (stacked) the
+// slab stack, its device bounds and the slicer's sums; `scratch` is one image (with_scratch).
+struct TiltCtfPlan {
+    std::vector<float> bounds, dmin, dmax;
+    CtfSlabs s{};
+    CtfStackWs ws{};
+    float* scratch = nullptr;
+};
+
+artia_status plan_tilt_ctf(artia_ctx* ctx, const char* fn, float a, float b, const artia_tilt_defocus& d,
+                           const artia_ctf_params& ctf, int W, int H, size_t pitch, bool with_scratch,
+                           TiltCtfPlan& p) {
+    const float step = ctf.slice_thickness_nm / ctf.pixelsize_nm;
+    if (!std::isfinite(a) || !std::isfinite(b) || !(step > 0.0f) || (double)(b - a) / step > kMaxSlabs ||
+        (b > a && a + step == a))
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: the slab range holds too many slabs", fn);
+    float ray = a;
+    for (; ray < b; ray += step) p.bounds.push_back(ray);
+    const int n = (int)p.bounds.size();
+    p.dmin.resize((size_t)n);
+    p.dmax.resize((size_t)n);
+    for (int k = 0; k < n; k++) artia_geom_defocus_minmax(p.bounds[(size_t)k], &d, &ctf, &p.dmin[(size_t)k], &p.dmax[(size_t)k]);
+    const size_t plane = pitch * (size_t)H;
+    const bool stacked = stack_fits(ctx, n, plane);
+    const size_t stack_floats = stacked ? (size_t)n * plane : 0;
+    const size_t slicer_bytes = stacked ? slicer_scratch_for(ctx, W, H, n, stack_floats) : 0;
+    const size_t scratch_floats = with_scratch ? plane : 0;
+    const size_t head = (scratch_floats + stack_floats + (stacked ? (size_t)n : 0) + 63) & ~(size_t)63;
+    artia_status st = ensure_workspace(ctx, head + (slicer_bytes + sizeof(float) - 1) / sizeof(float));
+    if (st != ARTIA_OK) return st;
+    p.scratch = with_scratch ? ctx->ws : nullptr;
+    float* stack = ctx->ws + scratch_floats;
+    float* d_bounds = stack + stack_floats;
+    if (stacked && n > 0) {
+        // earlier work on the stream may still read the workspace
+        st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        if (st == ARTIA_OK)
+            st = check_hip(ctx, hipMemcpy(d_bounds, p.bounds.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice),
+                           "hipMemcpy(slab bounds)");
+        if (st != ARTIA_OK) return st;
+    }
+    p.s = CtfSlabs{p.bounds.data(), d_bounds, p.dmin.data(), p.dmax.data(), n, step, ray};
+    p.ws = CtfStackWs{stacked, stack, plane, ctx->ws + head, slicer_bytes};
+    return ARTIA_OK;
+}
+
+bool valid_ctf_params(const artia_ctf_params* ctf) {
+    return ctf && ctf->slice_thickness_nm > 0.0f && ctf->pixelsize_nm > 0.0f && ctf->voltage_kV > 0.0f &&
+           ctf->wiener >= 0.0f;
+}
+
+artia_status fp_ctf(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, const artia_tilt_geom* tg, VolData vol,
+                    int volume_is_empty, int width, int height, size_t pitch, float slab_start, float slab_end,
+                    const artia_tilt_defocus* defocus, const artia_ctf_params* ctf, const float cut_length[2],
+                    const float dim_length[2], float* proj, float* dist, float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!vg || !tg || (!vol.p && !volume_is_empty) || !defocus || !cut_length || !dim_length || !proj || !dist ||
+        width <= 0 || height <= 0 || pitch < (size_t)width || !valid_ctf_params(ctf))
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: invalid arguments", fn);
+    if (!valid_geometry(vg, tg)) return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: invalid geometry", fn);
+    const VolG v = to_device(*vg);
+    const TiltG g = to_device(*tg);
+    TiltCtfPlan p;
+    if (!volume_is_empty) {
+        const artia_status st = plan_tilt_ctf(ctx, fn, slab_start, slab_end, *defocus, *ctf, width, height, pitch,
+                                              false, p);
+        if (st != ARTIA_OK) return st;
+    }
+    Timing t(ctx, ms);
+    if (!volume_is_empty) {
+        // dist is the per-slab form's slab image
+        const artia_status st = ctf_forward_slabs(ctx, v, g, vol, width, height, pitch, proj, dist, p.s, p.ws, *ctf,
+                                                  defocus->astig_angle_deg, cut_length, dim_length);
+        if (st != ARTIA_OK) return st;
+    }
+    launch_fill(ctx, dist, (size_t)(height - 1) * pitch + (size_t)width, 0.0f);
+    launch_vol_traversal(ctx, v, g, width, height, pitch, dist, 0, 0, width, height);
+    return t.finish();
+}
+
+artia_status bp_ctf(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, const artia_tilt_geom* tg, VolData vol,
+                    float* residual, int width, int height, size_t pitch, const int hitpoints[8], float slab_start,
+                    float slab_end, const artia_tilt_defocus* defocus, const artia_ctf_params* ctf,
+                    const float cut_length[2], const float dim_length[2], float lambda, int oversampling, float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    if (!vg || !tg || !vol.p || !residual || !hitpoints || !defocus || !cut_length || !dim_length || width <= 0 ||
+        height <= 0 || pitch < (size_t)width || oversampling <= 0 || oversampling > 64 || !valid_ctf_params(ctf))
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: invalid arguments", fn);
+    if (!valid_geometry(vg, tg)) return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: invalid geometry", fn);
+    if (vg->nx % 4 != 0 || (reinterpret_cast<uintptr_t>(vol.p) & (vol.half ? 7u : 15u)) != 0)
+        return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: volume x dimension must be a multiple of 4 "
+                                                     "and the volume 16-byte (half: 8-byte) aligned", fn);
+    TiltCtfPlan p;
+    artia_status st = plan_tilt_ctf(ctx, fn, slab_start, slab_end, *defocus, *ctf, width, height, pitch, true, p);
+    if (st != ARTIA_OK) return st;
+    const VolG v = to_device(*vg);
+    const TiltG g = to_device(*tg);
+    Timing t(ctx, ms);
+    st = ctf_backward_slabs(ctx, v, g, vol, width, height, pitch, residual, p.scratch, hitpoints, p.s, p.ws, *ctf,
+                            defocus->astig_angle_deg, cut_length, dim_length, lambda, oversampling);
+    if (st != ARTIA_OK) return st;
+    // the per-slab loop crops the residual in place once per slab; the stacked form reads it unchanged
+    if (p.ws.on) launch_crop_border(ctx, width, height, pitch, residual, cut_length, dim_length, hitpoints, p.s.n);
+    return t.finish();
+}
+
 artia_status sart_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
                       const int* hitpoints, const float* real_lengths, int ntilts, const int* index_list,
                       const float* projections, int width, int height, VolData vol, const artia_sart_params* sp,
@@ -148,23 +367,11 @@ artia_status sart_ctf_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* 
         last_end[(size_t)i] = ray;
         max_slabs = std::max(max_slabs, nslab[(size_t)i]);
     }
-    // one launch per tilt over a stack of its filtered slab images, while the stack fits 32 GiB
-    // the slab stack (32 GiB at most, or the context's stack limit) and the stacked slicer's sub-ray
-    // sums: every slab of a tilt in one volume pass while they fit 16 GiB (or what the limit leaves),
-    // in passes of fewer slabs otherwise; without room for the stack, one launch per slab
-    const double stack_cap = ctx->stack_limit ? (double)ctx->stack_limit : 34359738368.0;
-    const bool stacked = ctx->numerics == ARTIA_NUMERICS_EXACT && max_slabs > 0 &&
-                         (double)max_slabs * (double)npix * 4.0 <= stack_cap;
+    // one launch per tilt over a stack of its filtered slab images while the stack fits, one launch
+    // per slab otherwise
+    const bool stacked = stack_fits(ctx, max_slabs, npix);
     const size_t stack_floats = stacked ? (size_t)max_slabs * npix : 0;
-    size_t slicer_bytes = 0;
-    if (stacked) {
-        const size_t cap = ctx->stack_limit ? ctx->stack_limit - stack_floats * sizeof(float) : ((size_t)16 << 30);
-        int pass = max_slabs;
-        while (pass > 1 && slicer_stack_scratch_bytes(width, height, pass) > cap) pass = (pass + 1) / 2;
-        // no room for one slab's sums: the slicer goes slab by slab
-        slicer_bytes = slicer_stack_scratch_bytes(width, height, pass) <= cap ? slicer_stack_scratch_bytes(width, height, pass)
-                                                                              : 0;
-    }
+    const size_t slicer_bytes = stacked ? slicer_scratch_for(ctx, width, height, max_slabs, stack_floats) : 0;
     const size_t head = (npix * (size_t)(1 + S) + stack_floats + (stacked ? bounds.size() : 0) + 63) & ~(size_t)63;
     artia_status st = ensure_workspace(ctx, head + (slicer_bytes + sizeof(float) - 1) / sizeof(float));
     if (st != ARTIA_OK) return st;
@@ -192,8 +399,12 @@ artia_status sart_ctf_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* 
     const VolG v = to_device(*vg);
     const size_t pitch = (size_t)width;
     const float lam = sp->lambda / (float)S;
-    // image corners for the forward slabs (Reconstructor.cpp:557-565): pA, pB, pC, pD
-    const int frame[8] = {0, height - 1, width - 1, height - 1, 0, 0, width - 1, 0};
+    const CtfStackWs ws{stacked, stack, npix, slicer_scratch, slicer_bytes};
+    auto slabs_of = [&](int i) {
+        const size_t f = first[(size_t)i];
+        return CtfSlabs{bounds.data() + f, d_bounds + f, sdmin.data() + f, sdmax.data() + f, nslab[(size_t)i], step,
+                        last_end[(size_t)i]};
+    };
     bool empty = *volume_is_empty != 0;
 
     Timing t(ctx, ms);
@@ -208,39 +419,10 @@ artia_status sart_ctf_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* 
                 launch_fill(ctx, proj, npix, 0.0f);
                 launch_fill(ctx, dist, npix, 0.0f);
                 prof_stop(ctx, pr);
-                if (!empty && stacked) {
-                    // every slab's slicer image into the stack, then the batched filter sums them in
-                    // slab order
-                    const size_t f = first[(size_t)idx];
-                    const int ns = nslab[(size_t)idx];
-                    pr = prof_start(ctx, ARTIA_PROF_FP);
-                    launch_slicer_stack(ctx, v, g, vol, width, height, pitch, stack, npix, &bounds[f], d_bounds + f, ns,
-                                        step, slicer_scratch, slicer_bytes);
-                    prof_stop(ctx, pr);
-                    pr = prof_start(ctx, ARTIA_PROF_FFT);
-                    st = ctf_filter_slabs(ctx, stack, npix, ns, width, height, pitch, proj, 0, frame, sp->cut_length,
-                                          sp->dim_length, *ctf, &sdmin[f], &sdmax[f], defocus[idx].astig_angle_deg, 1);
-                    prof_stop(ctx, pr);
+                if (!empty) {
+                    st = ctf_forward_slabs(ctx, v, g, vol, width, height, pitch, proj, dist, slabs_of(idx), ws, *ctf,
+                                           defocus[idx].astig_angle_deg, sp->cut_length, sp->dim_length);
                     if (st != ARTIA_OK) return st;
-                    pr = prof_start(ctx, ARTIA_PROF_FILL);
-                    launch_fill(ctx, dist, npix, 0.0f);
-                    prof_stop(ctx, pr);
-                } else if (!empty) {
-                    for (float ray = slab_range[2 * idx]; ray < slab_range[2 * idx + 1]; ray += step) {
-                        float dmin, dmax;
-                        artia_geom_defocus_minmax(ray, &defocus[idx], ctf, &dmin, &dmax);
-                        pr = prof_start(ctx, ARTIA_PROF_FILL);
-                        launch_fill(ctx, dist, npix, 0.0f);
-                        prof_stop(ctx, pr);
-                        pr = prof_start(ctx, ARTIA_PROF_FP);
-                        launch_slicer(ctx, v, g, vol, width, height, pitch, dist, ray, ray + step);
-                        prof_stop(ctx, pr);
-                        pr = prof_start(ctx, ARTIA_PROF_FFT);
-                        st = ctf_slab_filter(ctx, dist, width, height, pitch, proj, pitch, 1, frame, sp->cut_length,
-                                             sp->dim_length, *ctf, dmin, dmax, defocus[idx].astig_angle_deg, 1);
-                        prof_stop(ctx, pr);
-                        if (st != ARTIA_OK) return st;
-                    }
                     pr = prof_start(ctx, ARTIA_PROF_FILL);
                     launch_fill(ctx, dist, npix, 0.0f);
                     prof_stop(ctx, pr);
@@ -259,37 +441,10 @@ artia_status sart_ctf_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* 
                 const int idx = index_list[sstep * S + i];
                 const TiltG g = to_device(tilts[idx]);
                 float* res = residual + (size_t)i * npix;
-                const int* hp = hitpoints + 8 * (size_t)idx;
-                if (stacked) {
-                    // every slab's filtered image into the stack, then one launch
-                    const size_t f = first[(size_t)idx];
-                    int pr = prof_start(ctx, ARTIA_PROF_FFT);
-                    st = ctf_filter_slabs(ctx, res, 0, nslab[(size_t)idx], width, height, pitch, stack, npix, hp,
-                                          sp->cut_length, sp->dim_length, *ctf, &sdmin[f], &sdmax[f],
-                                          defocus[idx].astig_angle_deg, 0);
-                    prof_stop(ctx, pr);
-                    if (st != ARTIA_OK) return st;
-                    if (nslab[(size_t)idx] > 0) {
-                        pr = prof_start(ctx, ARTIA_PROF_BP);
-                        launch_bp_stack(ctx, v, g, vol, stack, npix, width, height, pitch, lam, sp->oversampling,
-                                        ARTIA_FILTER_LINEAR, d_bounds + f, nslab[(size_t)idx], last_end[(size_t)idx]);
-                        prof_stop(ctx, pr);
-                    }
-                    continue;
-                }
-                for (float ray = slab_range[2 * idx]; ray < slab_range[2 * idx + 1]; ray += step) {
-                    float dmin, dmax;
-                    artia_geom_defocus_minmax(ray, &defocus[idx], ctf, &dmin, &dmax);
-                    int pr = prof_start(ctx, ARTIA_PROF_FFT);
-                    st = ctf_slab_filter(ctx, res, width, height, pitch, dist, pitch, 0, hp, sp->cut_length,
-                                         sp->dim_length, *ctf, dmin, dmax, defocus[idx].astig_angle_deg, 0);
-                    prof_stop(ctx, pr);
-                    if (st != ARTIA_OK) return st;
-                    pr = prof_start(ctx, ARTIA_PROF_BP);
-                    launch_bp(ctx, v, g, vol, dist, width, height, pitch, lam, sp->oversampling, ray, ray + step,
-                              ARTIA_FILTER_LINEAR);
-                    prof_stop(ctx, pr);
-                }
+                st = ctf_backward_slabs(ctx, v, g, vol, width, height, pitch, res, dist, hitpoints + 8 * (size_t)idx,
+                                        slabs_of(idx), ws, *ctf, defocus[idx].astig_angle_deg, sp->cut_length,
+                                        sp->dim_length, lam, sp->oversampling);
+                if (st != ARTIA_OK) return st;
             }
         }
     }
@@ -339,6 +494,45 @@ artia_status artia_sart_ctf_run_f16(artia_ctx* ctx, const artia_vol_geom* vg, co
     return sart_ctf_run(ctx, "artia_sart_ctf_run_f16", vg, tilts, hitpoints, real_lengths, slab_range, defocus,
                         ntilts, index_list, projections, width, height, VolData::f16(vol), sp, ctf, iterations,
                         volume_is_empty, ms);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+artia_status artia_fp_ctf(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, const float* vol,
+                          int volume_is_empty, int width, int height, size_t pitch, float slab_start, float slab_end,
+                          const artia_tilt_defocus* defocus, const artia_ctf_params* ctf, const float cut_length[2],
+                          const float dim_length[2], float* proj, float* dist, float* ms) {
+    return fp_ctf(ctx, "artia_fp_ctf", vg, tg, VolData(vol), volume_is_empty, width, height, pitch, slab_start,
+                  slab_end, defocus, ctf, cut_length, dim_length, proj, dist, ms);
+}
+
+artia_status artia_fp_ctf_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, const uint16_t* vol,
+                              int volume_is_empty, int width, int height, size_t pitch, float slab_start,
+                              float slab_end, const artia_tilt_defocus* defocus, const artia_ctf_params* ctf,
+                              const float cut_length[2], const float dim_length[2], float* proj, float* dist,
+                              float* ms) {
+    return fp_ctf(ctx, "artia_fp_ctf_f16", vg, tg, VolData::f16(vol), volume_is_empty, width, height, pitch,
+                  slab_start, slab_end, defocus, ctf, cut_length, dim_length, proj, dist, ms);
+}
+
+artia_status artia_bp_ctf(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, float* vol,
+                          float* residual, int width, int height, size_t pitch, const int hitpoints[8],
+                          float slab_start, float slab_end, const artia_tilt_defocus* defocus,
+                          const artia_ctf_params* ctf, const float cut_length[2], const float dim_length[2],
+                          float lambda, int oversampling, float* ms) {
+    return bp_ctf(ctx, "artia_bp_ctf", vg, tg, VolData(vol), residual, width, height, pitch, hitpoints, slab_start,
+                  slab_end, defocus, ctf, cut_length, dim_length, lambda, oversampling, ms);
+}
+
+artia_status artia_bp_ctf_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, uint16_t* vol,
+                              float* residual, int width, int height, size_t pitch, const int hitpoints[8],
+                              float slab_start, float slab_end, const artia_tilt_defocus* defocus,
+                              const artia_ctf_params* ctf, const float cut_length[2], const float dim_length[2],
+                              float lambda, int oversampling, float* ms) {
+    return bp_ctf(ctx, "artia_bp_ctf_f16", vg, tg, VolData::f16(vol), residual, width, height, pitch, hitpoints,
+                  slab_start, slab_end, defocus, ctf, cut_length, dim_length, lambda, oversampling, ms);
 }
 
 }  // extern "C"

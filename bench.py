@@ -68,7 +68,8 @@ SUBVOL_WORKLOADS = {
 # U[-3, 3] px per (sub-volume, tilt); 50 fused loss-and-gradient evaluations against the
 # integer cross-correlation search over the maxShift disk.
 REFINE_WORKLOADS = {
-    "refine_500subvols_64_512x512x128_41tilts": dict(field=(512, 512, 128), W=512, H=512, n=64, cubes=500,
+    "refine_500subvols_64_512x512x128_41tilts": dict(name="refine_500subvols_64_512x512x128_41tilts",
+                                                     field=(512, 512, 128), W=512, H=512, n=64, cubes=500,
                                                      step=3.0, true_shift=3.0, max_shift=5, margin=8, evals=50,
                                                      seed=9),
 }
@@ -389,13 +390,20 @@ def bench_ctf(args) -> int:
 
 
 def bench_subvol(args) -> int:
+    print(json.dumps(run_subvol(args.workload, args.steps)), flush=True)
+    return 0
+
+
+def run_subvol(name: str, steps: int) -> dict:
+    """The batched sub-volume plan against the per-call plain schedule on one workload; returns the
+    benchmark line (value = steady-state batched Gvoxel-updates/s)."""
     import numpy as np
     import torch
 
     import artiatomi_amd as A
     from artiatomi_amd import phantom
 
-    w = SUBVOL_WORKLOADS[args.workload]
+    w = SUBVOL_WORKLOADS[name]
     rng = np.random.default_rng(w["seed"])
     FX, FY, FZ = w["field"]
     W, H = w["W"], w["H"]
@@ -435,7 +443,7 @@ def bench_subvol(args) -> int:
                            for d, a, t in zip(dz, rng.uniform(-90, 90, T), tilts)])
     torch.cuda.synchronize()
     vox = sum(n ** 3 for n in sizes)
-    log(f"workload {args.workload}: {len(sizes)} cubes ({vox / 1e6:.0f} M voxels), {T} tilts of {W}x{H}, "
+    log(f"workload {name}: {len(sizes)} cubes ({vox / 1e6:.0f} M voxels), {T} tilts of {W}x{H}, "
         f"inputs in {time.time() - t0:.1f} s")
     every = w.get("plain_every", 1)
     plain_info = {}
@@ -502,7 +510,7 @@ def bench_subvol(args) -> int:
     if every == 1:
         equal = all(torch.equal(a, b) for (_, a), (_, b) in zip(subs_a, subs_b))
     # further replays accumulate on top; timed for the steady-state rate
-    reps = max(1, args.steps)
+    reps = max(1, steps)
     t0 = time.perf_counter()
     for _ in range(reps):
         plan.execute()
@@ -523,7 +531,7 @@ def bench_subvol(args) -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded cube placements and shifts, random projections)",
-        "config": {"workload": args.workload, "field": list(w["field"]), "projection": [W, H], "tilts": T,
+        "config": {"workload": name, "field": list(w["field"]), "projection": [W, H], "tilts": T,
                    "cubes": len(sizes), "cube_sizes": n_classes, "cube_counts": counts, "oversampling": w["os"],
                    "ctf": w["ctf"], "weighting": w.get("weighting"), "numerics": "exact",
                    "parallelism": "single GPU"},
@@ -534,9 +542,9 @@ def bench_subvol(args) -> int:
         "tuning": {"tile_policy": tu.tile_policy, "quads_per_thread": tu.quads_per_thread,
                    "blocks_per_cu": tu.blocks_per_cu, "autotuned": bool(tu.autotune)},
     }
-    print(json.dumps(out), flush=True)
+    del subs_b, images
     ctx.close()
-    return 0
+    return out
 
 
 def plain_gradient_sample(ctx, ps, vols, subs, targets, shifts, geoms, W, H, npairs):
@@ -594,6 +602,14 @@ def plain_cc_sample(ps, models, targets, square, max_shift, npairs):
 
 
 def bench_refine(args) -> int:
+    print(json.dumps(run_refine(args.workload, args.steps)), flush=True)
+    return 0
+
+
+def run_refine(name: str, steps: int, lite: bool = False) -> dict:
+    """The fused pair-set loss-and-gradient launch against per-pair calls, the dot test of the
+    whole-volume projector and adjoint, and (not lite) the gradient refinement, the CC search and
+    the adjoint launches; returns the benchmark line."""
     import numpy as np
     import torch
 
@@ -601,7 +617,7 @@ def bench_refine(args) -> int:
     from artiatomi_amd import autograd as AG
     from artiatomi_amd import phantom
 
-    w = REFINE_WORKLOADS[args.workload]
+    w = REFINE_WORKLOADS[name]
     rng = np.random.default_rng(w["seed"])
     FX, FY, FZ = w["field"]
     W, H, n, K = w["W"], w["H"], w["n"], w["cubes"]
@@ -626,14 +642,16 @@ def bench_refine(args) -> int:
     s_true = torch.from_numpy(rng.uniform(-w["true_shift"], w["true_shift"], (P, 2))).cuda()
     targets = ps.project(vols, s_true)
     torch.cuda.synchronize()
-    log(f"workload {args.workload}: {K} sub-volumes of {n}^3, {T} tilts, {P} pairs, {ps.n_pixels / 1e6:.1f} M "
+    log(f"workload {name}: {K} sub-volumes of {n}^3, {T} tilts, {P} pairs, {ps.n_pixels / 1e6:.1f} M "
         f"window pixels; inputs in {time.time() - t0:.1f} s")
 
     # fused loss + shift gradient: kernel time (HIP events) and the 50-evaluation refinement
     zero = torch.zeros((P, 2), device="cuda", dtype=torch.float64)
     ps.loss_grad(vols, zero, targets)
-    ms = [ps.loss_grad(vols, zero, targets, timed=True)[2] for _ in range(max(1, args.steps))]
+    ms = [ps.loss_grad(vols, zero, targets, timed=True)[2] for _ in range(max(1, steps))]
     t_eval = sum(ms) / len(ms) / 1e3
+    if lite:
+        return refine_lite(ctx, ps, vols, subs, targets, zero, geoms, base, w, t_eval, K, n, T, P)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     s_grad = AG.refine_pair_shifts(vols, targets, ps, steps=w["evals"])
@@ -669,22 +687,13 @@ def bench_refine(args) -> int:
         f"full-square FFT search {t_plain_cc:.2f} s")
 
     # dot-product test of the whole-volume projector and its adjoint at this size
+    dot_rel = dot_test(ctx, FX, FY, FZ, W, H, base, w["seed"])
+    log(f"dot test {FX}x{FY}x{FZ}, {T} tilts: rel {dot_rel:.2e}")
     vg_full = A.volume_geometry(FX, FY, FZ)
-    gen = torch.Generator(device="cuda").manual_seed(w["seed"])
+    gen = torch.Generator(device="cuda").manual_seed(w["seed"] + 1)
     xv = torch.randn((FZ, FY, FX), generator=gen, device="cuda")
-    lhs = rhs = 0.0
     proj = torch.empty((H, W), device="cuda")
     dist = torch.empty((H, W), device="cuda")
-    for g in base:
-        yv = torch.randn((H, W), generator=gen, device="cuda")
-        proj.zero_()
-        ctx.fp_march(vg_full, g, xv, proj, dist)
-        pty = torch.zeros_like(xv)
-        ctx.fp_adjoint(vg_full, g, yv, pty)
-        lhs += float((proj.double() * yv.double()).sum())
-        rhs += float((xv.double() * pty.double()).sum())
-    dot_rel = abs(lhs - rhs) / abs(lhs)
-    log(f"dot test {FX}x{FY}x{FZ}, {T} tilts: rel {dot_rel:.2e}")
     # adjoint and forward launch times on the whole volume, LDS-staged and direct
     yv = torch.randn((H, W), generator=gen, device="cuda")
     pty = torch.zeros_like(xv)
@@ -724,7 +733,7 @@ def bench_refine(args) -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded zero-margin blob sub-volumes, targets projected at seeded true shifts, no noise)",
-        "config": {"workload": args.workload, "field": list(w["field"]), "projection": [W, H], "tilts": T,
+        "config": {"workload": name, "field": list(w["field"]), "projection": [W, H], "tilts": T,
                    "subvolumes": K, "subvolume_edge": n, "pairs": P, "true_shift_px": w["true_shift"],
                    "max_shift": w["max_shift"], "numerics": "exact", "parallelism": "single GPU"},
         "roofline": {"kernel": "pair_march_kernel<loss+grad>", "bound": "hbm", "achieved": achieved,
@@ -747,10 +756,52 @@ def bench_refine(args) -> int:
         "pair_adjoint_launch_ms": {"staged": ms_pair_adj, "direct": ms_pair_adj_direct},
         "pair_loss_grad_launch_direct_ms": ms_pair_lg_direct,
     }
-    print(json.dumps(out), flush=True)
     ps.close()
     ctx.close()
-    return 0
+    return out
+
+
+def dot_test(ctx, FX, FY, FZ, W, H, base, seed):
+    """<P x, y> against <x, P^T y> over a tilt series for random x and y, the whole-volume projector
+    and its adjoint at FX x FY x FZ; the relative difference."""
+    import torch
+
+    import artiatomi_amd as A
+
+    vg_full = A.volume_geometry(FX, FY, FZ)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    xv = torch.randn((FZ, FY, FX), generator=gen, device="cuda")
+    lhs = rhs = 0.0
+    proj = torch.empty((H, W), device="cuda")
+    dist = torch.empty((H, W), device="cuda")
+    for g in base:
+        yv = torch.randn((H, W), generator=gen, device="cuda")
+        proj.zero_()
+        ctx.fp_march(vg_full, g, xv, proj, dist)
+        pty = torch.zeros_like(xv)
+        ctx.fp_adjoint(vg_full, g, yv, pty)
+        lhs += float((proj.double() * yv.double()).sum())
+        rhs += float((xv.double() * pty.double()).sum())
+    return abs(lhs - rhs) / abs(lhs)
+
+
+def refine_lite(ctx, ps, vols, subs, targets, zero, geoms, base, w, t_eval, K, n, T, P) -> dict:
+    """The adjoint/gradient form against the plain per-pair schedule: one fused loss + shift-gradient
+    launch over every (sub-volume, tilt) pair against per-pair projection, residual and
+    shift-gradient calls (timed on a sample of pairs, scaled to all), plus the dot test."""
+    FX, FY, FZ = w["field"]
+    W, H = w["W"], w["H"]
+    sample = min(P, 10 * T)
+    t_plain = plain_gradient_sample(ctx, ps, vols, subs, targets, zero, geoms, W, H, sample) * P / sample
+    dot = dot_test(ctx, FX, FY, FZ, W, H, base, w["seed"])
+    log(f"adjoint vs plain: fused {t_eval * 1e3:.1f} ms, per-pair {t_plain:.2f} s (from {sample} pairs); "
+        f"dot test {dot:.2e}")
+    ps.close()
+    ctx.close()
+    return {"workload": w["name"], "pairs": P, "subvolumes": K, "subvolume_edge": n, "tilts": T,
+            "fused_launch_ms": t_eval * 1e3, "Gvoxel_updates_per_s": K * n ** 3 * T / t_eval / 1e9,
+            "plain_per_pair_s": t_plain, "plain_extrapolated_from_pairs": sample,
+            "speedup_vs_plain": t_plain / t_eval, "dot_test_rel": dot}
 
 
 def main() -> int:
@@ -767,6 +818,9 @@ def main() -> int:
                     help="store the SART volume as IEEE half (the reference's FP16Volume mode)")
     ap.add_argument("--no-fp-staging", action="store_true",
                     help="forward-project with direct volume reads instead of LDS-staged slabs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="SART line only: skip the reference-numerics iteration and the batched and adjoint "
+                         "comparisons (for profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only an elementwise copy of 1 GiB (three launches) to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -853,7 +907,9 @@ def main() -> int:
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
     # the profile covers the default configuration's kernels only
     default_kernels = args.numerics == "exact" and not args.no_fp_staging and not args.volume_f16
-    dom_kernel = "march_lds_kernel<false, float>" if dom == "fp" else "bp_exact_kernel<2, 2, float>"
+    # the driver's forward projection is the residual form of the march (compare and cropBorder in its
+    # epilogue); the backprojection runs the window form, one instance per window size
+    dom_kernel = "march_lds_kernel<false, true, float>" if dom == "fp" else "bp_win_kernel"
     measured = measured_traffic(args.workload, dom_kernel) if default_kernels else None
     if measured:
         roofline["traffic"] = measured.pop("bytes")
@@ -868,6 +924,22 @@ def main() -> int:
             cpu = {"error": repr(exc)}
 
     value = V * T * args.steps / elapsed / 1e9
+    # the whole iteration against the roofline: per tilt the forward projection reads the volume,
+    # the backprojection reads and writes it, and the images move 28 bytes per pixel
+    # (FP 8, compare 16, BP 4)
+    iter_bytes = T * (3.0 * vbytes * V + 28.0 * W * H)
+    iter_gbs = iter_bytes / (elapsed / args.steps) / 1e9
+    ref_numerics = None
+    if not args.no_extras and args.numerics == "exact":
+        ref_numerics = reference_numerics_iteration(ctx, vg, acq, projs, vol, sp)
+    del projs, vol
+    torch.cuda.empty_cache()
+    copy = copy_bandwidth()
+    if cpu is not None and "error" not in cpu:
+        cpu["cpu_model"] = cpu_model()
+    extras = {}
+    if not args.no_extras:
+        extras = extra_comparisons()
     out = {
         "metric": "Gvoxel-updates/s per SART iteration",
         "value": value,
@@ -886,12 +958,96 @@ def main() -> int:
                    "numerics": args.numerics, "fp_staging": not args.no_fp_staging,
                    "volume_storage": "f16" if args.volume_f16 else "f32", "parallelism": "single GPU"},
         "roofline": roofline,
+        "iteration_roofline": {"algorithmic_bytes": iter_bytes, "achieved_GBs": iter_gbs, "peak": HBM_PEAK_GBS,
+                               "frac": iter_gbs / HBM_PEAK_GBS, "frac_of_measured_copy": iter_gbs / copy["GBs"]},
+        "copy_bandwidth": copy,
         "kernels": per_kernel,
+        "reference_numerics": ref_numerics,
+        **extras,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (the first `model name` of /proc/cpuinfo, which lscpu reports)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def copy_bandwidth() -> dict:
+    """Device-to-device copy of 1 GiB (1 GiB read + 1 GiB written per copy), timed with events over
+    five copies: the achievable HBM bandwidth on this GPU."""
+    import torch
+
+    n = 1 << 28
+    src = torch.ones(n, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    del src, dst
+    return {"GBs": 8.0 * n / ms / 1e6, "bytes_per_copy": 8 * n, "ms_per_copy": ms,
+            "how": "torch copy_ of a 1 GiB float32 tensor, 5 copies between HIP events"}
+
+
+def reference_numerics_iteration(ctx, vg, acq, projs, vol, sp) -> dict:
+    """One SART iteration in the reference's float operation order (ARTIA_NUMERICS_REFERENCE) from a
+    copy of the current volume: the price of bit-faithfulness to the reference's kernel bodies."""
+    import torch
+
+    v = vol.clone()
+    ctx.set_numerics("reference")
+    try:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.sart_run(vg, acq, projs, v, sp, iterations=1, volume_is_empty=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    finally:
+        ctx.set_numerics("exact")
+    V = vg.nx * vg.ny * vg.nz
+    log(f"reference numerics: one iteration in {dt:.3f} s")
+    del v
+    return {"seconds_per_iteration": dt, "Gvoxel_updates_per_s": V * len(acq.geoms) / dt / 1e9}
+
+
+def extra_comparisons() -> dict:
+    """The rest of the headline metric, measured in the same run: the batched sub-volume WBP against
+    per-call plain WBP (2000 cubes, 61 tilts of 4096^2; without CTF, and with 3D CTF where the plain
+    schedule is timed on every 10th tilt and scaled by the slab count), and the fused adjoint/gradient
+    form against per-pair calls (500 sub-volumes of 64^3, 41 tilts of 512^2)."""
+    import torch
+
+    out = {"batched_vs_plain": {}, "adjoint_vs_plain": None}
+    for key, name in (("no_ctf", "subvol_wbp_2000cubes_4096_61tilts"),
+                      ("ctf", "subvol_wbp_ctf_2000cubes_4096_61tilts")):
+        t0 = time.time()
+        r = run_subvol(name, 2)
+        torch.cuda.empty_cache()
+        out["batched_vs_plain"][key] = {
+            "workload": name, "speedup_vs_plain": r["speedup_vs_plain"], "batched_s": r["ms_per_step"] / 1e3,
+            "plain_s": r["plain"]["seconds"], "plain_extrapolated": bool(r["plain"].get("extrapolated", False)),
+            "batched_Gvoxel_updates_per_s": r["value"], "bitwise_equal_to_plain": r["bitwise_equal_to_plain"],
+            "wall_s": time.time() - t0}
+    t0 = time.time()
+    out["adjoint_vs_plain"] = run_refine("refine_500subvols_64_512x512x128_41tilts", 3, lite=True)
+    out["adjoint_vs_plain"]["wall_s"] = time.time() - t0
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":

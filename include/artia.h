@@ -406,6 +406,34 @@ artia_status artia_pairs_adjoint(artia_pairs* ps, const double* shifts, const fl
 artia_status artia_pairs_cc_search(artia_pairs* ps, const float* models, const float* targets, int square,
                                    int max_shift, int* peak, float* cc_value, float* cc_map, float* ms);
 
+/* ---------------------------------------------------------------- per-tilt 3D-CTF steps
+ * Reconstructor::ForwardProjectionCTF (Reconstructor.cpp:496-644) for one tilt: for every slab
+ * ray = slab_start, slab_start + T/ps, ... < slab_end (T = ctf->slice_thickness_nm,
+ * ps = ctf->pixelsize_nm, stepped in float; slab_start/slab_end as artia_geom_defocus_distances
+ * gives them), the slicer image of [ray, ray + T/ps), cropBorder with the image corners and
+ * cut_length/dim_length, the CTF multiply at the slab's defocus (artia_geom_defocus_minmax with
+ * *defocus), cropBorder again, added into proj in slab order: proj accumulates, as proj_d does
+ * after ResetProjectionsDevice.  dist (W x H, same pitch) is overwritten with the centre-ray
+ * traversal length (volTraversalLength).  volume_is_empty != 0: only dist, and vol may be NULL.
+ * Exact numerics run every slab of the tilt in one volume pass and filter the slabs in batched
+ * FFTs; the result equals the per-slab loop bit for bit. */
+artia_status artia_fp_ctf(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, const float* vol,
+                          int volume_is_empty, int width, int height, size_t pitch, float slab_start, float slab_end,
+                          const artia_tilt_defocus* defocus, const artia_ctf_params* ctf, const float cut_length[2],
+                          const float dim_length[2], float* proj, float* dist, float* ms);
+/* Reconstructor::BackProjectionCTF (Reconstructor.cpp:920-969) for one tilt: for every slab (as
+ * artia_fp_ctf steps them), cropBorder of the residual in place with hitpoints (artia_geom_hitpoints),
+ * the Wiener (or phase-flip) filter at the slab's defocus, cropBorder, and the LINEAR-sampled
+ * backprojection with `lambda` (Lambda / SIRTCount) gated to the slab's distances.  On return the
+ * residual is cropped once per slab, as the reference's loop leaves proj_d.  Exact numerics filter
+ * the slabs in batched FFTs into a stack and backproject all of them in one launch, bit for bit
+ * the per-slab loop. */
+artia_status artia_bp_ctf(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, float* vol,
+                          float* residual, int width, int height, size_t pitch, const int hitpoints[8],
+                          float slab_start, float slab_end, const artia_tilt_defocus* defocus,
+                          const artia_ctf_params* ctf, const float cut_length[2], const float dim_length[2],
+                          float lambda, int oversampling, float* ms);
+
 /* ---------------------------------------------------------------- drivers
  * EmSART's ordered-subset SART loop (EmSart.cpp:579-675) without CTF, device resident:
  * for every subset, per tilt FP (volTraversalLength while the volume is empty) -> compare
@@ -466,6 +494,16 @@ artia_status artia_sart_ctf_run_f16(artia_ctx* ctx, const artia_vol_geom* vg, co
                                     const float* projections, int width, int height, uint16_t* vol,
                                     const artia_sart_params* sp, const artia_ctf_params* ctf, int iterations,
                                     int* volume_is_empty, float* ms);
+artia_status artia_fp_ctf_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, const uint16_t* vol,
+                              int volume_is_empty, int width, int height, size_t pitch, float slab_start,
+                              float slab_end, const artia_tilt_defocus* defocus, const artia_ctf_params* ctf,
+                              const float cut_length[2], const float dim_length[2], float* proj, float* dist,
+                              float* ms);
+artia_status artia_bp_ctf_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tg, uint16_t* vol,
+                              float* residual, int width, int height, size_t pitch, const int hitpoints[8],
+                              float slab_start, float slab_end, const artia_tilt_defocus* defocus,
+                              const artia_ctf_params* ctf, const float cut_length[2], const float dim_length[2],
+                              float lambda, int oversampling, float* ms);
 /* ConvVolKernel / convertVolumeFP16ToFP32 (Kernels.cpp:1134-1180, BackProjectionSquareOS.cu:1294-1323),
  * as Reconstructor::ConvertVolumeFP16 (Reconstructor.cpp:1734-1745) uses it slice by slice for the
  * output file: slices z0 .. z0+count-1 of an nx x ny x nz half volume as negated floats (EmSART

@@ -177,6 +177,36 @@ private:
     Context& c_;
 };
 
+// Reconstructor::ForwardProjectionCTF / BackProjectionCTF for one tilt (Reconstructor.cpp:496-644,
+// 920-969): the slab loop over [slab_start, slab_end) with the slicer, the CTF filters and the
+// slab-gated backprojection, as one call each.
+class CtfTiltSteps {
+public:
+    explicit CtfTiltSteps(Context& c) : c_(c) {}
+    float forward(const artia_vol_geom& vg, const artia_tilt_geom& tg, const float* vol, bool volume_is_empty, int w,
+                  int h, size_t pitch, float slab_start, float slab_end, const artia_tilt_defocus& defocus,
+                  const artia_ctf_params& ctf, const float cut[2], const float dim[2], float* proj, float* dist) {
+        float ms = 0.0f;
+        c_.check(artia_fp_ctf(c_.get(), &vg, &tg, vol, volume_is_empty, w, h, pitch, slab_start, slab_end, &defocus,
+                              &ctf, cut, dim, proj, dist, &ms),
+                 "ForwardProjectionCTF");
+        return ms;
+    }
+    float backward(const artia_vol_geom& vg, const artia_tilt_geom& tg, float* vol, float* residual, int w, int h,
+                   size_t pitch, const int hitpoints[8], float slab_start, float slab_end,
+                   const artia_tilt_defocus& defocus, const artia_ctf_params& ctf, const float cut[2],
+                   const float dim[2], float lambda, int oversampling) {
+        float ms = 0.0f;
+        c_.check(artia_bp_ctf(c_.get(), &vg, &tg, vol, residual, w, h, pitch, hitpoints, slab_start, slab_end,
+                              &defocus, &ctf, cut, dim, lambda, oversampling, &ms),
+                 "BackProjectionCTF");
+        return ms;
+    }
+
+private:
+    Context& c_;
+};
+
 // CopyToSquareKernel (Kernels.h:104-111)
 class CopyToSquareKernel {
 public:
