@@ -246,6 +246,8 @@ SIGNATURES = {
     "artia_sart_ctf_run": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _PF, C.POINTER(TiltDefocus),
                                 _I, _PI, _P, _I, _I, _P, C.POINTER(SartParams), C.POINTER(CtfParams), _I, _PI,
                                 _PF]),
+    "artia_sart_run_noalloc": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _I, _PI, _P, _I, _I, _P,
+                                    C.POINTER(SartParams), _I, _PI, _PF]),
     "artia_sart_run_f16": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _I, _PI, _P, _I, _I, _P,
                                 C.POINTER(SartParams), _I, _PI, _PF]),
     "artia_sart_ctf_run_f16": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _PI, _PF, _PF,

@@ -676,13 +676,19 @@ class ArtiaContext:
                           int(oversampling), timed=timed)
 
     def sart_run(self, vg: VolGeom, acq: Acquisition, projections, vol, params: SartParams, iterations=1,
-                 volume_is_empty: bool = True, timed=False):
-        """Runs `iterations` SART iterations; returns (ms or None, volume_is_empty)."""
+                 volume_is_empty: bool = True, timed=False, noalloc=False):
+        """Runs `iterations` SART iterations; returns (ms or None, volume_is_empty).  noalloc=True takes
+        the allocation-free entry point (fp32 volumes; its workspace is the library's fixed per-GPU
+        array)."""
         n = len(acq.geoms)
         tg, hp, rl, il = acq.c_arrays()
         empty = C.c_int(1 if volume_is_empty else 0)
         ms = C.c_float() if timed else None
         vp, sfx = _vol(vol, vg)
+        if noalloc:
+            if sfx:
+                raise ArtiaError("the allocation-free SART entry point takes float32 volumes")
+            sfx = "_noalloc"
         st = getattr(self.lib, "artia_sart_run" + sfx)(
             self.h, C.byref(vg), tg, hp, rl, n, il, _ptr(projections, "projections", n * acq.width * acq.height),
             acq.width, acq.height, vp, C.byref(params), int(iterations), C.byref(empty),

@@ -111,6 +111,12 @@ private:
 
 artia_status ensure_workspace(artia_ctx* ctx, size_t floats);
 
+// The allocation-free SART entry point's workspace: a device array of kNoAllocScratchFloats floats
+// (64 MiB) per GPU, held for the life of the process.  noalloc_scratch gives the context device's
+// address (looked up by artia_create), or an error when the lookup failed.
+constexpr size_t kNoAllocScratchFloats = (size_t)16 << 20;
+artia_status noalloc_scratch(artia_ctx* ctx, float** out);
+
 // Records a start event for kernel class `kind` when profiling is on; returns the pair index
 // (or -1).  prof_stop records the matching stop event; prof_collect waits for the last event
 // and folds all pairs into the per-class totals.

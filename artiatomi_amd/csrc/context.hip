@@ -1,11 +1,52 @@
 // Context lifecycle, error reporting, timing and workspace for libartia.
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <new>
 
 #include "artia_internal.hpp"
 
+namespace {
+// Workspace of the allocation-free SART entry point (artia_sart_run_noalloc): a file-scope device
+// array, so that entry point never allocates.  Every GPU the library runs on holds its own copy
+// until the process ends.  Host code never uses the array's name (a host-side stand-in, not a
+// device address): artia_create looks up each device's address with hipGetSymbolAddress, with that
+// device current, and everything else takes the pointer kept here.
+static __device__ __align__(256) float scratch[artia::kNoAllocScratchFloats];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_done[kMaxScratchDevices] = {};
+hipError_t g_scratch_err[kMaxScratchDevices] = {};
+float* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Once per device, with that device current (artia_create's set-up).
+void resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void* p = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = err;
+    g_scratch_ptr[device] = err == hipSuccess ? static_cast<float*>(p) : nullptr;
+    g_scratch_done[device] = true;
+}
+}  // namespace
+
 namespace artia {
+
+artia_status noalloc_scratch(artia_ctx* ctx, float** out) {
+    const int d = ctx->device;
+    if (d < 0 || d >= kMaxScratchDevices)
+        return fail(ctx, ARTIA_ERR_UNSUPPORTED, "allocation-free workspace: device %d beyond the %d supported", d,
+                    kMaxScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[d] || g_scratch_err[d] != hipSuccess || !g_scratch_ptr[d])
+        return fail(ctx, ARTIA_ERR_DEVICE, "allocation-free workspace of device %d unavailable: %s", d,
+                    g_scratch_done[d] ? hipGetErrorString(g_scratch_err[d]) : "not looked up");
+    *out = g_scratch_ptr[d];
+    return ARTIA_OK;
+}
 
 TiltG to_device(const artia_tilt_geom& g) {
     TiltG t;
@@ -169,6 +210,8 @@ artia_status artia_create(int device, artia_ctx** out) {
         delete ctx;
         return ARTIA_ERR_DEVICE;
     }
+    // the allocation-free entry point's workspace address on this device (device current)
+    resolve_scratch(device);
     ctx->stream = nullptr;   // the null stream until the caller sets one
     *out = ctx;
     return ARTIA_OK;

@@ -234,10 +234,17 @@ artia_status bp_ctf(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, co
     return t.finish();
 }
 
+// The subset's tilts go to the backprojection in chunks of this many (a fixed host array: the
+// driver's loop allocates nothing); chunking does not change the result.
+constexpr int kSubsetChunk = 16;
+
+// fixed_ws != nullptr: the allocation-free form, with (1 + SIRTCount) images of workspace taken
+// from fixed_ws (fixed_floats floats) instead of the context's workspace.
 artia_status sart_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
                       const int* hitpoints, const float* real_lengths, int ntilts, const int* index_list,
                       const float* projections, int width, int height, VolData vol, const artia_sart_params* sp,
-                      int iterations, int* volume_is_empty, float* ms) {
+                      int iterations, int* volume_is_empty, float* ms, float* fixed_ws = nullptr,
+                      size_t fixed_floats = 0) {
     if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
     if (!vg || !tilts || !hitpoints || !real_lengths || !index_list || !projections || !vol.p || !sp ||
         !volume_is_empty || ntilts <= 0 || width <= 0 || height <= 0 || iterations < 0)
@@ -255,16 +262,31 @@ artia_status sart_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, 
     }
     const int S = sp->sirt_count;
     const size_t npix = (size_t)width * (size_t)height;
-    artia_status st = ensure_workspace(ctx, npix * (size_t)(1 + S));
-    if (st != ARTIA_OK) return st;
-    float* dist = ctx->ws;
-    float* residual = ctx->ws + npix;   // S images; the forward projection is built in place
+    const size_t need = npix * (size_t)(1 + S);
+    artia_status st = ARTIA_OK;
+    float* ws = nullptr;
+    if (fixed_ws) {
+        if (need > fixed_floats)
+            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: %d x %d images with SIRTCount %d need %zu floats of "
+                        "workspace; the allocation-free entry point holds at most %zu", fn, width, height, S, need,
+                        fixed_floats);
+        if (ctx->prof)
+            return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "%s: kernel profiling is on, and its event pool "
+                                                         "allocates; turn it off first", fn);
+        ws = fixed_ws;
+    } else {
+        st = ensure_workspace(ctx, need);
+        if (st != ARTIA_OK) return st;
+        ws = ctx->ws;
+    }
+    float* dist = ws;
+    float* residual = ws + npix;   // S images; the forward projection is built in place
     const VolG v = to_device(*vg);
     const size_t pitch = (size_t)width;
     const float lam = sp->lambda / (float)S;
     bool empty = *volume_is_empty != 0;
-    std::vector<TiltG> sub_geoms((size_t)S);
-    std::vector<const float*> sub_imgs((size_t)S);
+    TiltG sub_geoms[kSubsetChunk];
+    const float* sub_imgs[kSubsetChunk];
 
     Timing t(ctx, ms);
     for (int iter = 0; iter < iterations; iter++) {
@@ -302,15 +324,15 @@ artia_status sart_run(artia_ctx* ctx, const char* fn, const artia_vol_geom* vg, 
             }
             // the subset's backprojections (residuals cropped already), in subset order
             const int nsub = std::min(S, ntilts - step * S);
-            for (int i = 0; i < nsub; i++) {
-                sub_geoms[(size_t)i] = to_device(tilts[index_list[step * S + i]]);
-                sub_imgs[(size_t)i] = residual + (size_t)i * npix;
-            }
-            if (nsub > 0) {
+            for (int c0 = 0; c0 < nsub; c0 += kSubsetChunk) {
+                const int nc = std::min(kSubsetChunk, nsub - c0);
+                for (int i = 0; i < nc; i++) {
+                    sub_geoms[i] = to_device(tilts[index_list[step * S + c0 + i]]);
+                    sub_imgs[i] = residual + (size_t)(c0 + i) * npix;
+                }
                 empty = false;
                 const int pr = prof_start(ctx, ARTIA_PROF_BP);
-                launch_bp_subset(ctx, v, sub_geoms.data(), sub_imgs.data(), nsub, width, height, pitch, lam,
-                                 sp->oversampling, vol);
+                launch_bp_subset(ctx, v, sub_geoms, sub_imgs, nc, width, height, pitch, lam, sp->oversampling, vol);
                 prof_stop(ctx, pr);
             }
         }
@@ -464,6 +486,20 @@ artia_status artia_sart_run(artia_ctx* ctx, const artia_vol_geom* vg, const arti
                             int iterations, int* volume_is_empty, float* ms) {
     return sart_run(ctx, "artia_sart_run", vg, tilts, hitpoints, real_lengths, ntilts, index_list, projections, width,
                     height, VolData(vol), sp, iterations, volume_is_empty, ms);
+}
+
+artia_status artia_sart_run_noalloc(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                                    const int* hitpoints, const float* real_lengths, int ntilts,
+                                    const int* index_list, const float* projections, int width, int height,
+                                    float* vol, const artia_sart_params* sp, int iterations, int* volume_is_empty,
+                                    float* ms) {
+    if (!ctx) return ARTIA_ERR_INVALID_ARGUMENT;
+    float* ws = nullptr;
+    const artia_status st = noalloc_scratch(ctx, &ws);
+    if (st != ARTIA_OK) return st;
+    return sart_run(ctx, "artia_sart_run_noalloc", vg, tilts, hitpoints, real_lengths, ntilts, index_list,
+                    projections, width, height, VolData(vol), sp, iterations, volume_is_empty, ms, ws,
+                    kNoAllocScratchFloats);
 }
 
 artia_status artia_sart_run_f16(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,

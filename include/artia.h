@@ -446,6 +446,19 @@ artia_status artia_sart_run(artia_ctx* ctx, const artia_vol_geom* vg, const arti
                             const int* index_list, const float* projections, int width, int height,
                             float* vol, const artia_sart_params* sp, int iterations,
                             int* volume_is_empty, float* ms);
+/* artia_sart_run without any allocation, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It has no counterpart
+ * in the reference.  Same arguments and result as artia_sart_run (bit for bit); the (1 + SIRTCount)
+ * projection-sized workspace images come from a 64 MiB device array the library holds on every
+ * GPU it runs on for the life of the process, so (1 + sirt_count) * width * height must not exceed
+ * 16 Mi floats (2048 x 2048 with SIRTCount <= 3; ARTIA_ERR_INVALID_ARGUMENT beyond).  Calls that
+ * overlap on one GPU share that array and race; kernel profiling (artia_profile_enable) must be
+ * off. */
+artia_status artia_sart_run_noalloc(artia_ctx* ctx, const artia_vol_geom* vg, const artia_tilt_geom* tilts,
+                                    const int* hitpoints, const float* real_lengths, int ntilts,
+                                    const int* index_list, const float* projections, int width, int height,
+                                    float* vol, const artia_sart_params* sp, int iterations, int* volume_is_empty,
+                                    float* ms);
 /* The same loop with 3D-CTF correction (Reconstructor.cpp:496-645 forward, :920-969 backward):
  * per tilt, slabs ray = t_in, t_in + T/ps, ... < t_out (slab_range[2*tilt] = t_in, t_out from
  * artia_geom_defocus_distances), each sliced, CTF-filtered and summed into the forward
