@@ -172,7 +172,9 @@ def measured_traffic(workload: str, kernel: str):
         return None
     for line in text.splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 7 and cells[0] == f"`{kernel}`":
+        name = cells[0].strip("`") if cells else ""
+        # an exact name, or a kernel name without template arguments matching any instance
+        if len(cells) == 7 and (name == kernel or ("<" not in kernel and name.startswith(kernel + "<"))):
             try:
                 return {"bytes": float(cells[6]) * 1e9, "read_bytes": float(cells[4]) * 1e9,
                         "written_bytes": float(cells[5]) * 1e9, "source": f"profiles/{workload}.md"}
