@@ -238,7 +238,7 @@ artia_status artia_fp_adjoint(artia_ctx* ctx, const artia_vol_geom* vg, const ar
     const Roi r = clip_roi(roi, width, height);
     Timing t(ctx, ms);
     const VolG vd = to_device(*vg);
-    const TiltG gd = to_device(*tg);
+    const TiltG gd = flat_beam(to_device(*tg));
     TubePlan tp;
     if (r.x1 > r.x0 && r.y1 > r.y0 && ctx->march_lds && tube_plan(vd, gd, tp)) {
         const dim3 grid = grid_of(r);
@@ -253,11 +253,11 @@ artia_status artia_fp_adjoint(artia_ctx* ctx, const artia_vol_geom* vg, const ar
     {
         const bool big = (double)vg->nx * (double)vg->ny * (double)vg->nz >= 1073741824.0;
         if (big)
-            hipLaunchKernelGGL(fp_adjoint_kernel<true>, grid_of(r), dim3(256), 0, ctx->stream, to_device(*vg),
-                               to_device(*tg), resid, pitch, r, scale, vol_grad);
+            hipLaunchKernelGGL(fp_adjoint_kernel<true>, grid_of(r), dim3(256), 0, ctx->stream, vd,
+                               gd, resid, pitch, r, scale, vol_grad);
         else
-            hipLaunchKernelGGL(fp_adjoint_kernel<false>, grid_of(r), dim3(256), 0, ctx->stream, to_device(*vg),
-                               to_device(*tg), resid, pitch, r, scale, vol_grad);
+            hipLaunchKernelGGL(fp_adjoint_kernel<false>, grid_of(r), dim3(256), 0, ctx->stream, vd,
+                               gd, resid, pitch, r, scale, vol_grad);
     }
     st = check_hip(ctx, hipGetLastError(), "artia_fp_adjoint launch");
     if (st != ARTIA_OK) return st;
@@ -275,7 +275,7 @@ artia_status artia_fp_shift_grad(artia_ctx* ctx, const artia_vol_geom* vg, const
     Timing t(ctx, ms);
     if (r.x1 > r.x0 && r.y1 > r.y0)
         hipLaunchKernelGGL(fp_shift_grad_kernel, grid_of(r), dim3(256), 0, ctx->stream, to_device(*vg),
-                           to_device(*tg), vol, grad_proj, pitch, r, out);
+                           flat_beam(to_device(*tg)), vol, grad_proj, pitch, r, out);
     st = check_hip(ctx, hipGetLastError(), "artia_fp_shift_grad launch");
     if (st != ARTIA_OK) return st;
     return t.finish();

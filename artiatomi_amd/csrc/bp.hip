@@ -250,11 +250,13 @@ struct BPWinSet {
     int n;
     float lambda;
     double distMin, distMax;
+    int gated;   // 0: the dist gate passes every quad of the volume for every tilt (gate_passes_all)
 };
 
 // Block = 16 quads x 4 rows x 4 slices, as bp_exact_kernel; per tilt the block stages its
-// footprint (one pixel of margin) in LDS.  Blocks whose footprint lies inside the image walk the
-// windows unclamped; blocks at the image edge clamp each quad's lattice to its clamped footprint.
+// footprint (one pixel of margin) in LDS.  Blocks whose footprint lies inside the image apply the
+// quad footprint as a chord cap (or not at all where it covers the voxel boxes); blocks at the
+// image edge clamp each quad's lattice to its clamped footprint.
 template <int kOS, int kNI, int kNJ, typename T>
 __global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __restrict__ vol, int W, int H,
                                                      size_t pitch) {
@@ -288,14 +290,27 @@ __global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __re
         const int tx0 = max(0, ux0), ty0 = max(0, uy0);
         const int tw = max(0, min(W - 1, ux1) - tx0 + 1), th = max(0, min(H - 1, uy1) - ty0 + 1);
         if (t > 0) __syncthreads();   // the previous tilt's readers are done
-        for (int idx = tid; idx < tw * th; idx += 256) {
-            const int r = idx / tw, c = idx - r * tw;
-            tile[idx] = img[(size_t)(ty0 + r) * pitch + (size_t)(tx0 + c)];
+        // tile entry idx = r * tw + c, strided by 256 over the block: one division per thread,
+        // then the row and column are stepped (tiles are often a few columns wide and tall)
+        if (tid < tw * th) {
+            int r = tid / tw, c = tid - r * tw;
+            const int dr = 256 / tw, dc = 256 - dr * tw;
+            for (int idx = tid; idx < tw * th; idx += 256) {
+                tile[idx] = img[(size_t)(ty0 + r) * pitch + (size_t)(tx0 + c)];
+                r += dr;
+                c += dc;
+                if (c >= tw) {
+                    c -= tw;
+                    r++;
+                }
+            }
         }
         __syncthreads();
         if (!inVol) continue;
-        const double gt = fabs(fma(w.gn[0], qx, fma(w.gn[1], qy, fma(w.gn[2], qz, w.g0))));
-        if (!(gt >= S.distMin && gt < S.distMax)) continue;
+        if (S.gated) {
+            const double gt = fabs(fma(w.gn[0], qx, fma(w.gn[1], qy, fma(w.gn[2], qz, w.g0))));
+            if (!(gt >= S.distMin && gt < S.distMax)) continue;
+        }
         any |= window_quad<kOS, kNI, kNJ>(w, tile, tw, tx0, ty0, interior, W, H, x, (unsigned)y, (unsigned)z,
                                           S.lambda, out);
     }
@@ -475,6 +490,7 @@ void launch_bp_t(artia_ctx* ctx, const VolG& v, const TiltG& g, T* vol, const fl
         S.lambda = lambda;
         S.distMin = distMin;
         S.distMax = distMax;
+        S.gated = gate_passes_all(v, S.w[0], S.distMin, S.distMax) ? 0 : 1;
         launch_win_t(ctx, v, S, os, S.w[0].ni, S.w[0].nj, vol, W, H, pitch);
         return;
     }
@@ -554,6 +570,9 @@ void launch_bp_subset(artia_ctx* ctx, const VolG& v, const TiltG* g, const float
             S.lambda = lambda;
             S.distMin = 0.0;
             S.distMax = 9999999999999.0;
+            S.gated = 0;
+            for (int k = 0; k < S.n; k++)
+                if (!gate_passes_all(v, S.w[k], S.distMin, S.distMax)) S.gated = 1;
             if (vol.half)
                 launch_win_t(ctx, v, S, os, S.w[0].ni, S.w[0].nj, static_cast<vhalf*>(vol.p), W, H, pitch);
             else

@@ -49,6 +49,10 @@ struct BPWin {
     // projected boxes of all four voxels of every quad of the volume, so a quad whose footprint
     // the image edges do not clamp may walk its windows without the per-point footprint test.
     int covers;
+    // masked != 0: the voxel boxes stick out of the quad footprints by less than 1 - 1/os pixels,
+    // so every window of a block lies in its footprint tile with a one-pixel margin, and a block
+    // away from the image edges can apply the footprint test as a per-point chord cap (0 outside).
+    int masked;
 };
 
 // Whether the window form applies to a tilt and its constants, for oversampling os and a window of
@@ -122,7 +126,11 @@ inline bool prepare_window(const VolG& v, const TiltG& g, const BPExact& e, int 
     // [p_r(q) + min(0, 3 p_ri) + min_r, p_r(q) + max(0, 3 p_ri) + max_r] (guarded, so conservative):
     // both ends are affine in the quad origin q, so the volume's corner quads decide.  A lattice
     // point inside a box then lies in [floor(lower) * os, ceil(upper) * os) as the clamp requires.
+    // The window of a voxel ends less than 1/os past its box, so with boxes sticking out of the
+    // footprint by less than 1 - 1/os pixels its pixels lie within one pixel of the footprint's
+    // floor and ceiling: the block tile's margin.
     w.covers = 1;
+    double stick = 0.0;   // largest distance a voxel box sticks out of its quad footprint
     const double qx[2] = {0.0, (double)std::max(0, v.nx - 4)}, qy[2] = {0.0, (double)(v.ny - 1)},
                  qz[2] = {0.0, (double)(v.nz - 1)};
     for (int c = 0; c < 8; c++) {
@@ -135,16 +143,37 @@ inline bool prepare_window(const VolG& v, const TiltG& g, const BPExact& e, int 
             const double boxLo = p + std::fmin(0.0, 3.0 * pi) + (double)(r == 0 ? e.uMin : e.vMin);
             const double boxHi = p + std::fmax(0.0, 3.0 * pi) + (double)(r == 0 ? e.uMax : e.vMax);
             if (!(gq + e.lo[r] <= boxLo && boxHi <= gq + e.hi[r])) w.covers = 0;
+            stick = std::fmax(stick, std::fmax((gq + e.lo[r]) - boxLo, boxHi - (gq + e.hi[r])));
         }
     }
+    w.masked = stick + 1.0 / os < 0.95 ? 1 : 0;
     return w.ni >= 1 && w.nj >= 1 && w.ni <= maxN && w.nj <= maxN;
+}
+
+// Whether the dist gate [distMin, distMax) passes every quad of the volume: |gn . q + g0| is the
+// absolute value of an affine form of one sign over the volume (the affine footprint), so its
+// extremes lie at the corner quads.
+inline bool gate_passes_all(const VolG& v, const BPWin& w, double distMin, double distMax) {
+    const double qx[2] = {0.0, (double)std::max(0, v.nx - 4)}, qy[2] = {0.0, (double)(v.ny - 1)},
+                 qz[2] = {0.0, (double)(v.nz - 1)};
+    for (int c = 0; c < 8; c++) {
+        const double q[3] = {qx[c & 1], qy[(c >> 1) & 1], qz[(c >> 2) & 1]};
+        const double t = std::fabs(std::fma(w.gn[0], q[0], std::fma(w.gn[1], q[1], std::fma(w.gn[2], q[2], w.g0))));
+        if (!(t >= distMin && t < distMax)) return false;
+    }
+    return true;
 }
 
 // The chord-weighted sums of one voxel over its kNI x kNJ window, read from the block's LDS tile
 // of image pixels (row pitch tw).  (i0, j0): the window's first lattice point, global; (a0, b0):
-// its offset from the voxel's min-corner projection; tile origin pixel (tx0, ty0).  kClamp: only
-// lattice points in [li0, li1) x [lj0, lj1) count (the quad footprint clamped at the image edge).
-template <int kOS, int kNI, int kNJ, bool kClamp>
+// its offset from the voxel's min-corner projection; tile origin pixel (tx0, ty0).  Only lattice
+// points in the quad footprint [li0, li1) x [lj0, lj1) count, which kMode applies as:
+//   kWinOpen:   not at all (the footprint covers every voxel box: w.covers);
+//   kWinMasked: as a chord cap of 0 outside it (every window pixel lies in the tile);
+//   kWinClamp:  by skipping the point and its read (blocks at the image edges).
+// The three give the same sums for finite images: a point outside adds fma(val, 0, n) = n.
+constexpr int kWinOpen = 0, kWinMasked = 1, kWinClamp = 2;
+template <int kOS, int kNI, int kNJ, int kMode>
 __device__ __forceinline__ void window_sums(const BPWin& w, const float* __restrict__ tile, int tw, int tx0, int ty0,
                                             int i0, int j0, float a0, float b0, int li0, int li1, int lj0, int lj1,
                                             float& num, float& den) {
@@ -154,6 +183,11 @@ __device__ __forceinline__ void window_sums(const BPWin& w, const float* __restr
     const int c0 = kOS == 1 ? i0 : (i0 >> 1), ph = kOS == 1 ? 0 : (i0 & 1);
     const int r0 = kOS == 1 ? j0 : (j0 >> 1), qh = kOS == 1 ? 0 : (j0 & 1);
     const int base = (r0 - ty0) * tw + (c0 - tx0);
+    // masked: the chord cap of each window column (m inside the footprint's columns, else 0)
+    float cap[kNI];
+#pragma unroll
+    for (int di = 0; di < kNI; di++)
+        cap[di] = (kMode == kWinMasked && !((unsigned)(i0 + di - li0) < (unsigned)(li1 - li0))) ? 0.0f : w.m;
     float n = 0.0f, d = 0.0f;
 #pragma unroll
     for (int dj = 0; dj < kNJ; dj++) {
@@ -163,16 +197,17 @@ __device__ __forceinline__ void window_sums(const BPWin& w, const float* __restr
         const int rowBase = base + rowOff;
         const int rowOdd = rowBase + ph;   // odd di of the 2x lattice: pixel c0 + ph + (di - 1) / 2
         bool rowOk = true;
-        if (kClamp) rowOk = (unsigned)(j0 + dj - lj0) < (unsigned)(lj1 - lj0);
+        if (kMode != kWinOpen) rowOk = (unsigned)(j0 + dj - lj0) < (unsigned)(lj1 - lj0);
 #pragma unroll
         for (int di = 0; di < kNI; di++) {
             const float Pc = di == 0 ? Pr : (di == 1 ? Pr + w.PUd : __builtin_fmaf((float)di, w.PUd, Pr));
             const float Yc = di == 0 ? Yr : (di == 1 ? Yr + w.YUd : __builtin_fmaf((float)di, w.YUd, Yr));
-            // min(m, P, SP - P, Y, SY - Y) clamped at 0
+            // min(m, P, SP - P, Y, SY - Y) clamped at 0 (cap 0: exactly 0)
             const float A = w.SPh - fabsf(Pc), Bq = w.SYh - fabsf(Yc);
-            float chord = __builtin_amdgcn_fmed3f(fminf(A, Bq), 0.0f, w.m);
+            const float mcap = kMode == kWinMasked ? (rowOk ? cap[di] : 0.0f) : w.m;
+            float chord = __builtin_amdgcn_fmed3f(fminf(A, Bq), 0.0f, mcap);
             int addr = kOS == 1 ? rowBase + di : ((di & 1) ? rowOdd + (di >> 1) : rowBase + (di >> 1));
-            if (kClamp) {
+            if (kMode == kWinClamp) {
                 const bool ok = rowOk && (unsigned)(i0 + di - li0) < (unsigned)(li1 - li0);
                 chord = ok ? chord : 0.0f;
                 addr = ok ? addr : 0;
@@ -205,10 +240,11 @@ inline bool window_plan(const VolG& v, const TiltG& g, int os, BPWin& w) {
 
 // The update of one quad (voxels x .. x+3 of row y, slice z) for one tilt: out[k] += lambda *
 // num / den over each voxel's window, from an image tile of row pitch tw whose pixel (tx0, ty0) is
-// tile[0] (the whole image with tw = pitch and tx0 = ty0 = 0).  interior: the quad's footprint is
-// not clamped by the image edges; with w.covers every window point with a positive chord then lies
-// inside it.  Otherwise the lattice is clamped to the clamped footprint, as the per-box loop does.  Returns
-// whether a voxel changed.
+// tile[0] (the whole image with tw = pitch and tx0 = ty0 = 0).  Only lattice points in the quad's
+// footprint clamped to [0, W-1] x [0, H-1] count, as in the per-box loop.  interior: the block's
+// footprint lies in the image with a pixel to spare and the tile holds it with a one-pixel margin;
+// then the footprint test is skipped (w.covers) or applied as a chord cap (w.masked), otherwise
+// each point outside is skipped with its read.  Returns whether a voxel changed.
 template <int kOS, int kNI, int kNJ>
 __device__ __forceinline__ bool window_quad(const BPWin& w, const float* __restrict__ tile, int tw, int tx0, int ty0,
                                             bool interior, int W, int H, unsigned x, unsigned y, unsigned z,
@@ -216,16 +252,23 @@ __device__ __forceinline__ bool window_quad(const BPWin& w, const float* __restr
     constexpr float kDelta = 1.0f / (float)kOS;
     const double qx = (double)x, qy = (double)y, qz = (double)z;
     int li0 = 0, li1 = 0, lj0 = 0, lj1 = 0;
-    interior = interior && w.covers;
-    if (!interior) {
+    const int mode = interior && w.covers ? kWinOpen : (interior && w.masked ? kWinMasked : kWinClamp);
+    if (mode != kWinOpen) {
         // the quad's footprint clamped to [0, W-1] x [0, H-1] (footprint_affine): lattice points
-        // [px0 * OS, px1 * OS) x [py0 * OS, py1 * OS)
+        // [px0 * OS, px1 * OS) x [py0 * OS, py1 * OS); in an interior block the clamps are no-ops
         const double a0 = fma(w.G[0][0], qx, fma(w.G[0][1], qy, w.G[0][2] * qz));
         const double a1 = fma(w.G[1][0], qx, fma(w.G[1][1], qy, w.G[1][2] * qz));
-        li0 = kOS * (int)fmin(fmax(floor(a0 + w.lo[0]), 0.0), (double)(W - 1));
-        li1 = kOS * (int)fmin(fmax(ceil(a0 + w.hi[0]), 0.0), (double)(W - 1));
-        lj0 = kOS * (int)fmin(fmax(floor(a1 + w.lo[1]), 0.0), (double)(H - 1));
-        lj1 = kOS * (int)fmin(fmax(ceil(a1 + w.hi[1]), 0.0), (double)(H - 1));
+        if (mode == kWinMasked) {
+            li0 = kOS * (int)floor(a0 + w.lo[0]);
+            li1 = kOS * (int)ceil(a0 + w.hi[0]);
+            lj0 = kOS * (int)floor(a1 + w.lo[1]);
+            lj1 = kOS * (int)ceil(a1 + w.hi[1]);
+        } else {
+            li0 = kOS * (int)fmin(fmax(floor(a0 + w.lo[0]), 0.0), (double)(W - 1));
+            li1 = kOS * (int)fmin(fmax(ceil(a0 + w.hi[0]), 0.0), (double)(W - 1));
+            lj0 = kOS * (int)fmin(fmax(floor(a1 + w.lo[1]), 0.0), (double)(H - 1));
+            lj1 = kOS * (int)fmin(fmax(ceil(a1 + w.hi[1]), 0.0), (double)(H - 1));
+        }
         if (li1 <= li0 || lj1 <= lj0) return false;
     }
     // corner projection of the quad's first voxel, split into an integer pixel and a float offset
@@ -245,11 +288,15 @@ __device__ __forceinline__ bool window_quad(const BPWin& w, const float* __restr
         const float a0 = __builtin_fmaf((float)i0 + 0.5f, kDelta, -du);
         const float b0 = __builtin_fmaf((float)j0 + 0.5f, kDelta, -dv);
         float num, den;
-        if (interior)
-            window_sums<kOS, kNI, kNJ, false>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, 0, 0, 0, 0, num, den);
+        if (mode == kWinOpen)
+            window_sums<kOS, kNI, kNJ, kWinOpen>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, 0, 0, 0, 0, num,
+                                                 den);
+        else if (mode == kWinMasked)
+            window_sums<kOS, kNI, kNJ, kWinMasked>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, li0, li1, lj0,
+                                                   lj1, num, den);
         else
-            window_sums<kOS, kNI, kNJ, true>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, li0, li1, lj0, lj1, num,
-                                             den);
+            window_sums<kOS, kNI, kNJ, kWinClamp>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, li0, li1, lj0,
+                                                  lj1, num, den);
         if (den > 0.0f) {
             out[k] += (lambda * num) * __builtin_amdgcn_rcpf(den);
             any = true;

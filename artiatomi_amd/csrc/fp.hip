@@ -114,28 +114,29 @@ __global__ void __launch_bounds__(256) vol_traversal_ref_kernel(VolG v, TiltG g,
 
 // Sum of the trilinear samples q0 + k*dq, k = 0..count-1 (coordinates shifted by -0.5, texel j
 // centred at j), cell by cell in closed form (cell_sum, fp_common.hpp).
-template <bool kBig, typename T>
+// kFlat: dq[1] == 0 (the walk, moments and cell sums without the y terms; the same values).
+template <bool kBig, bool kFlat, typename T>
 __device__ __forceinline__ double march_sum(const VolG& v, const T* __restrict__ vol, const float q0[3],
                                             const float dq[3], int count) {
     double total = 0.0;
     if (kBig) {
-        for_each_cell(q0, dq, count, [&](const int cell[3], int k, int mi) {
+        for_each_cell<kFlat>(q0, dq, count, [&](const int cell[3], int k, int mi) {
             size_t o[8];
             cell_corners<true>(v, cell, o);
             float val[8];
 #pragma unroll
             for (int i = 0; i < 8; i++) val[i] = vox(vol, o[i]);
-            total += cell_sum_values(val, cell_moments(q0, dq, cell, k, mi), mi);
+            total += cell_sum_values<kFlat>(val, cell_moments<kFlat>(q0, dq, cell, k, mi), mi);
         });
     } else {
         const __amdgpu_buffer_rsrc_t rs = volume_rsrc(vol, v);
-        for_each_cell(q0, dq, count, [&](const int cell[3], int k, int mi) {
+        for_each_cell<kFlat>(q0, dq, count, [&](const int cell[3], int k, int mi) {
             uint32_t o[8];
             cell_corners_bytes<(uint32_t)sizeof(T)>(v, cell, o);
             float val[8];
 #pragma unroll
             for (int i = 0; i < 8; i++) val[i] = buffer_vox(rs, o[i], vol);
-            total += cell_sum_values(val, cell_moments(q0, dq, cell, k, mi), mi);
+            total += cell_sum_values<kFlat>(val, cell_moments<kFlat>(q0, dq, cell, k, mi), mi);
         });
     }
     return total;
@@ -257,7 +258,7 @@ __device__ __forceinline__ void march_store_resid(double acc, float dist, bool a
 
 // One lane per sub-ray: a wave covers 4x4 pixels x 4 sub-rays, a 256-thread block 8x8 pixels,
 // so the rays of a wave stay within a few voxels of each other through the whole volume.
-template <bool kSlicer, bool kBig, bool kResid, typename T>
+template <bool kSlicer, bool kBig, bool kResid, typename T, bool kFlat = false>
 __global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const T* __restrict__ vol, Roi roi,
                                                           size_t pitch, float* __restrict__ proj,
                                                           float* __restrict__ dist, float tminDef, float tmaxDef,
@@ -272,7 +273,7 @@ __global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const
     double acc = L.acc;
     if (L.count > 0) {
         if (v.nx >= 2 && v.ny >= 2 && v.nz >= 2) {
-            acc += march_sum<kBig>(v, vol, L.q0, L.dq, L.count);
+            acc += march_sum<kBig, kFlat>(v, vol, L.q0, L.dq, L.count);
         } else {
             double tot = 0.0;
             for (int k = 0; k < L.count; k++) {
@@ -295,7 +296,7 @@ __global__ void __launch_bounds__(256) march_exact_kernel(VolG v, TiltG g, const
 // paused at slab boundaries), so the projection is bit-identical.  Six waves per SIMD (80
 // registers): the walk is VALU bound, and the occupancy hides the LDS latency better than the few
 // registers it spills.
-template <bool kSlicer, bool kResid, typename T>
+template <bool kSlicer, bool kResid, typename T, bool kFlat = false>
 __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, const T* __restrict__ vol, Roi roi,
                                                         size_t pitch, float* __restrict__ proj,
                                                         float* __restrict__ dist, float tminDef, float tmaxDef,
@@ -326,9 +327,10 @@ __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, cons
     const TubeBox tb = tube_box(v, g, tp, (double)tx0 + 0.25, (double)tx0 + 7.75, (double)ty0 + 0.25,
                                 (double)ty0 + 7.75, 0.0, 0.0);
     double total = 0.0;   // summed as march_sum does, then added to the slicer's first sample
-    tube_walk(v, vol, chunk, zr, tp, tb, td, L.q0, L.count, [&](const float val[8], const int cell[3], int k, int mi) {
-        total += cell_sum_values(val, cell_moments(L.q0, td.dq, cell, k, mi), mi);
-    });
+    tube_walk<kFlat>(v, vol, chunk, zr, tp, tb, td, L.q0, L.count,
+                     [&](const float val[8], const int cell[3], int k, int mi) {
+                         total += cell_sum_values<kFlat>(val, cell_moments<kFlat>(L.q0, td.dq, cell, k, mi), mi);
+                     });
     double acc = L.acc;
     if (L.count > 0) acc += total;
     if (kResid)
@@ -531,12 +533,16 @@ dim3 grid_for(const Roi& r) {
 }
 
 template <bool kResid, typename T>
-void launch_exact_t(artia_ctx* ctx, const VolG& v, const TiltG& g, const T* vol, const Roi& r, size_t pitch,
+void launch_exact_t(artia_ctx* ctx, const VolG& v, const TiltG& g_in, const T* vol, const Roi& r, size_t pitch,
                     float* proj, float* dist, bool slicer, float tmin, float tmax, const ResidEpi& epi, int W, int H) {
+    const TiltG g = flat_beam(g_in);
     const dim3 grid((unsigned)((r.x1 - r.x0 + 7) / 8), (unsigned)((r.y1 - r.y0 + 7) / 8)), block(256);
     // 32-bit buffer offsets need fewer than 2^30 voxels and a row index z * ny + y below 2^24
     const bool big = (uint64_t)v.nx * (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 30) ||
                      (uint64_t)v.ny * (uint64_t)v.nz >= (1ull << 24) || v.nx >= (1 << 24);
+    // rays without a y component (every tilt about the y axis whose beam direction has n_y == 0):
+    // the walk without the y axis, the same projections
+    const bool flat = g.n[1] == 0.0f;
     TubePlan tp;
     if (ctx->march_lds && tube_plan(v, g, tp)) {
         tp.tiles_x = (int)grid.x;
@@ -547,6 +553,9 @@ void launch_exact_t(artia_ctx* ctx, const VolG& v, const TiltG& g, const T* vol,
         if (slicer)
             hipLaunchKernelGGL((march_lds_kernel<true, false, T>), grid1, block, 0, ctx->stream, v, g, vol, r, pitch,
                                proj, dist, tmin, tmax, tp, epi, W, H);
+        else if (flat)
+            hipLaunchKernelGGL((march_lds_kernel<false, kResid, T, true>), grid1, block, 0, ctx->stream, v, g, vol, r,
+                               pitch, proj, dist, tmin, tmax, tp, epi, W, H);
         else
             hipLaunchKernelGGL((march_lds_kernel<false, kResid, T>), grid1, block, 0, ctx->stream, v, g, vol, r,
                                pitch, proj, dist, tmin, tmax, tp, epi, W, H);
@@ -559,6 +568,13 @@ void launch_exact_t(artia_ctx* ctx, const VolG& v, const TiltG& g, const T* vol,
         else
             hipLaunchKernelGGL((march_exact_kernel<true, false, false, T>), grid, block, 0, ctx->stream, v, g, vol, r,
                                pitch, proj, dist, tmin, tmax, epi, W, H);
+    } else if (flat) {
+        if (big)
+            hipLaunchKernelGGL((march_exact_kernel<false, true, kResid, T, true>), grid, block, 0, ctx->stream, v, g,
+                               vol, r, pitch, proj, dist, tmin, tmax, epi, W, H);
+        else
+            hipLaunchKernelGGL((march_exact_kernel<false, false, kResid, T, true>), grid, block, 0, ctx->stream, v, g,
+                               vol, r, pitch, proj, dist, tmin, tmax, epi, W, H);
     } else {
         if (big)
             hipLaunchKernelGGL((march_exact_kernel<false, true, kResid, T>), grid, block, 0, ctx->stream, v, g, vol, r,
@@ -642,10 +658,12 @@ size_t slicer_stack_scratch_bytes(int W, int H, int nslab) {
     return npix * 4 * sizeof(int2) + (size_t)nslab * npix * 4 * sizeof(double);
 }
 
-void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, int W, int H, size_t pitch,
+void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g_in, VolData vol, int W, int H, size_t pitch,
                          float* stack, size_t stride, const float* bounds, const float* d_bounds, int nslab,
                          float step, void* scratch, size_t scratch_bytes) {
     if (nslab <= 0) return;
+    // the exact one-pass form takes the flat-beam geometry, as the per-slab launches below do
+    const TiltG g = ctx->numerics == ARTIA_NUMERICS_EXACT ? flat_beam(g_in) : g_in;
     const size_t npix = (size_t)W * (size_t)H;
     const size_t range_bytes = npix * 4 * sizeof(int2), slab_bytes = npix * 4 * sizeof(double);
     const int per_pass = scratch_bytes > range_bytes ? (int)std::min<size_t>((scratch_bytes - range_bytes) / slab_bytes,
@@ -694,8 +712,8 @@ void launch_vol_traversal(artia_ctx* ctx, const VolG& v, const TiltG& g, int W, 
     if (ctx->numerics == ARTIA_NUMERICS_REFERENCE)
         hipLaunchKernelGGL(vol_traversal_ref_kernel, grid_for(r), dim3(64, 4), 0, ctx->stream, v, g, r, pitch, dist);
     else
-        hipLaunchKernelGGL(vol_traversal_exact_kernel, grid_for(r), dim3(64, 4), 0, ctx->stream, v, g, r, pitch,
-                           dist);
+        hipLaunchKernelGGL(vol_traversal_exact_kernel, grid_for(r), dim3(64, 4), 0, ctx->stream, v, flat_beam(g), r,
+                           pitch, dist);
 }
 
 }  // namespace artia

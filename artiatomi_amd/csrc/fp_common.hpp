@@ -4,11 +4,24 @@
 // (ForwardProjectionRayMarcher_TL.cu:57-160 evaluated stably).
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 
 #include "device_common.hpp"
 
 namespace artia {
+
+// Exact numerics take a beam whose y component is a float residue of the rotation (|n_y| below
+// 1e-6 of |n|; every tilt about the y axis whose rotation leaves one) as parallel to the x-z plane:
+// n_y = 0, so the walk runs without the y axis (CellWalkerT<true>).  A ray then moves by less than
+// 1e-6 voxels over a 1000-voxel path.  Every exact forward-family launcher (march, slicer, traversal
+// length, adjoint, shift gradient, pair sets) takes the geometry through here, so they stay exact
+// transposes and bit-identical to one another.
+inline TiltG flat_beam(TiltG g) {
+    const float s = std::fabs(g.n[0]) + std::fabs(g.n[1]) + std::fabs(g.n[2]);
+    if (g.n[1] != 0.0f && std::fabs(g.n[1]) < 1e-6f * s) g.n[1] = 0.0f;
+    return g;
+}
 
 struct RayD {
     double c[3];
@@ -127,7 +140,12 @@ __device__ __forceinline__ bool subray_samples(const VolG& v, const TiltG& g, do
 // step() handles one cell (calling f(cell, k, m) when it holds samples k..k+m-1) and advances to
 // the next, active() says whether samples remain, so the walk can be resumed (the LDS-staged
 // projector runs it slab by slab).
-struct CellWalker {
+//
+// kFlat: the rays have no y component (dq[1] == 0 exactly, every tilt about the y axis whose beam
+// direction rounds to n_y = 0), so the walk never crosses a y face and the y axis drops out of the
+// step.  With dq[1] == 0 both forms visit the same cells with the same sample ranges.
+template <bool kFlat>
+struct CellWalkerT {
     int cell[3], sgn[3];
     float inv[3], off[3], kb[3];
     float fcount;
@@ -163,11 +181,28 @@ struct CellWalker {
         k = 0;
         guard = 0;
     }
+    // The walk of samples k0 .. n-1 of the same sub-ray, starting in the cell of sample k0 (crossing
+    // indices stay relative to q0, so moments use the sub-ray's own sample indices).
+    __device__ __forceinline__ void init_from(const float q0[3], const float dq[3], const int sgn_u[3],
+                                              const float inv_u[3], int k0, int n) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            cell[i] = (int)floorf(__builtin_fmaf((float)k0, dq[i], q0[i]));
+            sgn[i] = sgn_u[i];
+            inv[i] = inv_u[i];
+            off[i] = -q0[i] * inv[i];
+            kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]) : INFINITY;
+        }
+        count = n;
+        fcount = (float)n;
+        k = k0;
+        guard = 0;
+    }
     __device__ __forceinline__ bool active() const { return k < count && guard < 4 * count + 64; }
     // One step; moved(i) is called for every axis i the walk advances along.
     template <class F, class M>
     __device__ __forceinline__ void step(F&& f, M&& moved) {
-        const float kn = fminf(fminf(kb[0], kb[1]), kb[2]);
+        const float kn = kFlat ? fminf(kb[0], kb[2]) : fminf(fminf(kb[0], kb[1]), kb[2]);
         const int kend = kn < fcount ? max((int)ceilf(kn), 0) : count;
         if (kend > k) {
             f(cell, k, kend - k);
@@ -175,6 +210,7 @@ struct CellWalker {
         }
 #pragma unroll
         for (int i = 0; i < 3; i++) {
+            if (kFlat && i == 1) continue;
             if (kb[i] == kn) {
                 cell[i] += sgn[i];
                 kb[i] = __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]);
@@ -189,10 +225,12 @@ struct CellWalker {
     }
 };
 
+using CellWalker = CellWalkerT<false>;
+
 // The whole walk: f(cell, k, m) for every cell holding samples k..k+m-1, in order.
-template <class F>
+template <bool kFlat = false, class F>
 __device__ __forceinline__ void for_each_cell(const float q0[3], const float dq[3], int count, F&& f) {
-    CellWalker w;
+    CellWalkerT<kFlat> w;
     w.init(q0, dq, count);
     while (w.active()) w.step(f);
 }
@@ -207,19 +245,27 @@ struct CellMoments {
     float pab, pac, pbc, q;   // da db, da dc, db dc, a db dc + b da dc + c da db
 };
 
+// kFlat (dq[1] == 0): b is the same for every sample, da db = db dc = 0 and q = b da dc; the
+// values equal the general form's (fma(kc, 0, q0) = q0, fma(x, 0, y) = y for finite operands).
+template <bool kFlat = false>
 __device__ __forceinline__ CellMoments cell_moments(const float q0[3], const float dq[3], const int cell[3], int k,
                                                     int mi) {
     CellMoments cm;
     const float kc = (float)(2 * k + mi - 1) * 0.5f;   // k + (m - 1) / 2, exact
     cm.a = __builtin_fmaf(kc, dq[0], q0[0]) - (float)cell[0];
-    cm.b = __builtin_fmaf(kc, dq[1], q0[1]) - (float)cell[1];
+    cm.b = (kFlat ? q0[1] : __builtin_fmaf(kc, dq[1], q0[1])) - (float)cell[1];
     cm.c = __builtin_fmaf(kc, dq[2], q0[2]) - (float)cell[2];
     cm.m = (float)mi;
     cm.S2c = cm.m * __builtin_fmaf(cm.m, cm.m, -1.0f) * (1.0f / 12.0f);   // m, m^2 - 1 and their product exact
-    cm.pab = dq[0] * dq[1];
     cm.pac = dq[0] * dq[2];
-    cm.pbc = dq[1] * dq[2];
-    cm.q = __builtin_fmaf(cm.a, cm.pbc, __builtin_fmaf(cm.b, cm.pac, cm.c * cm.pab));
+    if (kFlat) {
+        cm.pab = cm.pbc = 0.0f;
+        cm.q = cm.b * cm.pac;
+    } else {
+        cm.pab = dq[0] * dq[1];
+        cm.pbc = dq[1] * dq[2];
+        cm.q = __builtin_fmaf(cm.a, cm.pbc, __builtin_fmaf(cm.b, cm.pac, cm.c * cm.pab));
+    }
     return cm;
 }
 
@@ -282,6 +328,9 @@ __device__ __forceinline__ float buffer_vox(const __amdgpu_buffer_rsrc_t rs, uin
 // Sum of one cell's m samples, m f(centre) + S2c corr, rounded once to float (relative error
 // 6e-8 per cell); the ray adds the cells in double, so per-cell roundings do not accumulate into a
 // float running sum.
+// kFlat: da db = db dc = 0, so the AB and BC terms vanish (corr = ABC q + AC da dc, the general
+// form's value up to the sign of a zero).
+template <bool kFlat = false>
 __device__ __forceinline__ double cell_sum_values(const float val[8], const CellMoments& cm, int mi) {
     const float v000 = val[0], v100 = val[1], v010 = val[2], v110 = val[3];
     const float v001 = val[4], v101 = val[5], v011 = val[6], v111 = val[7];
@@ -291,8 +340,13 @@ __device__ __forceinline__ double cell_sum_values(const float val[8], const Cell
     const float c01 = __builtin_fmaf(cm.a, x01, v001), c11 = __builtin_fmaf(cm.a, x11, v011);
     const float f = lerpf(lerpf(c00, c10, cm.b), lerpf(c01, c11, cm.b), cm.c);
     const float AB = x10 - x00, AC = x01 - x00, ABC = (x11 - x01) - AB;
-    const float BC = (v011 - v001) - (v010 - v000);
-    const float corr = __builtin_fmaf(ABC, cm.q, __builtin_fmaf(BC, cm.pbc, __builtin_fmaf(AC, cm.pac, AB * cm.pab)));
+    float corr;
+    if (kFlat) {
+        corr = __builtin_fmaf(ABC, cm.q, AC * cm.pac);
+    } else {
+        const float BC = (v011 - v001) - (v010 - v000);
+        corr = __builtin_fmaf(ABC, cm.q, __builtin_fmaf(BC, cm.pbc, __builtin_fmaf(AC, cm.pac, AB * cm.pab)));
+    }
     (void)mi;
     return (double)__builtin_fmaf(cm.m, f, cm.S2c * corr);
 }

@@ -209,6 +209,118 @@ __global__ void __launch_bounds__(256, 5) pair_march_kernel(PairArgs a) {
     }
 }
 
+// The transpose summed per sub-volume slab in LDS.  A workgroup owns cell layers [zs, zs + L) of
+// one sub-volume (voxel layers max(zs, 0) .. min(zs + L, nz - 1) in LDS) and walks every sub-ray
+// of every tilt of that sub-volume through them only: the samples between the sub-ray's crossings
+// of the slab's two z faces (the walker's own face-crossing formula, so neighbouring slabs split
+// each sub-ray's samples without overlap or gap).  The tilts' weights meet in LDS (shared-memory
+// atomics) and reach the gradient with one atomic per voxel and slab, instead of one per box voxel
+// and tube slab of every pair.  A corner outside the layers (a sample rounded across a slab face)
+// goes to global memory directly.
+constexpr int kAdjSlabFloats = 15360;   // 60 KiB of LDS: the x-y layers of a slab
+
+// Cell layers per slab of a sub-volume (0: its x-y layer pair does not fit).
+__host__ __device__ __forceinline__ int adj_slab_layers(int nx, int ny) {
+    const long long nxy = (long long)nx * (long long)ny;
+    const long long layers = (long long)kAdjSlabFloats / (nxy > 0 ? nxy : 1);
+    return layers >= 2 ? (int)(layers - 1) : 0;
+}
+
+struct AdjSlabArgs {
+    const int2* items;      // (sub-volume, first cell layer zs), per XCD in runs of per_xcd; x < 0: none
+    int per_xcd;
+    const int* vfirst;      // per sub-volume: first slot of its pairs in `order`, and their count
+    const int* vcount;
+};
+
+__global__ void __launch_bounds__(256) pair_adjoint_slab_kernel(PairArgs a, AdjSlabArgs sa, float scale,
+                                                                float* __restrict__ grads) {
+    __shared__ float box[kAdjSlabFloats];
+    const int pos = (int)(blockIdx.x >> 3);
+    if (pos >= sa.per_xcd) return;   // whole workgroup
+    const int2 it = sa.items[(int)(blockIdx.x & 7u) * sa.per_xcd + pos];
+    if (it.x < 0) return;   // whole workgroup
+    const int first = sa.vfirst[it.x], npair = sa.vcount[it.x];
+    const VolG v0 = a.pairs[a.order[first]].v;
+    const int L = adj_slab_layers(v0.nx, v0.ny);
+    const int zs = it.y, ze = min(zs + L, v0.nz);   // cell layers [zs, ze); layer c spans q_z in [c, c + 1)
+    const int bz0 = max(zs, 0), bz1 = min(ze, v0.nz - 1);
+    const int nxy = v0.nx * v0.ny, nbox = nxy * (bz1 - bz0 + 1);
+    for (int e = (int)threadIdx.x; e < nbox; e += 256) box[e] = 0.0f;
+    __syncthreads();
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int sub = lane & 3, pix = lane >> 2;
+    const int tx = (wave & 1) * 4 + (pix & 3), ty = (wave >> 1) * 4 + (pix >> 2);
+    for (int q = 0; q < npair; q++) {
+        const int p = a.order[first + q];
+        const PairDev& P = a.pairs[p];
+        const int x0 = P.x0, y0 = P.y0, w = P.w, h = P.h;
+        const long long io = P.img_offset;
+        const int ntx = (w + 7) >> 3, ntiles = ntx * ((h + 7) >> 3);
+        for (int t = 0; t < ntiles; t++) {
+            __asm__ volatile("" ::: "memory");   // geometry re-read per tile, as in pair_march_kernel
+            const VolG v = P.v;
+            const TiltG g = P.g;
+            float* __restrict__ gv = grads + P.vol_offset;
+            const double sx = a.shifts ? a.shifts[2 * p] : 0.0, sy = a.shifts ? a.shifts[2 * p + 1] : 0.0;
+            const double step = (double)v.vs[0] * 0.15;
+            const int i = (t % ntx) * 8 + tx, j = (t / ntx) * 8 + ty;
+            if (!(i < w && j < h)) continue;
+            // proj = (float)(0.25 * step * sum): the same factor per sample
+            const double r = (double)a.in[io + (long long)j * w + i] * (double)scale * (step * 0.25);
+            if (r == 0.0) continue;
+            SubraySamples s;
+            const double px = (double)(x0 + i) + ((sub & 2) ? 0.75 : 0.25);
+            const double py = (double)(y0 + j) + ((sub & 1) ? 0.75 : 0.25);
+            if (!subray_samples(v, g, px, py, step, s, sx, sy)) continue;
+            int sgn[3];
+            float inv[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                sgn[c] = s.dq[c] > 0.0f ? 1 : (s.dq[c] < 0.0f ? -1 : 0);
+                inv[c] = sgn[c] != 0 ? 1.0f / s.dq[c] : 0.0f;
+            }
+            // samples of this slab: from the crossing of its entry face to that of its exit face, as
+            // the walker computes face crossings (CellWalker::step), clamped to the sub-ray
+            const float offz = -s.q0[2] * inv[2];
+            const float fin = (float)(sgn[2] > 0 ? zs : ze), fout = (float)(sgn[2] > 0 ? ze : zs);
+            const int kA = max(0, min(s.count, (int)ceilf(__builtin_fmaf(fin, inv[2], offz))));
+            const int kB = max(0, min(s.count, (int)ceilf(__builtin_fmaf(fout, inv[2], offz))));
+            if (kB <= kA) continue;
+            CellWalker wk;
+            wk.init_from(s.q0, s.dq, sgn, inv, kA, kB);
+            while (wk.active()) {
+                wk.step([&](const int cell[3], int k, int mi) {
+                    double W[8];
+                    cell_weights(cell_moments(s.q0, s.dq, cell, k, mi), W);
+                    // clamp-addressed corners (cell_corners): 0 step where an index is clamped
+                    const int xa = clampi(cell[0], 0, v.nx - 1), ya = clampi(cell[1], 0, v.ny - 1);
+                    const int za = clampi(cell[2], 0, v.nz - 1);
+                    const int dx = (cell[0] + 1 > xa && xa + 1 < v.nx) ? 1 : 0;
+                    const int dy = (cell[1] + 1 > ya && ya + 1 < v.ny) ? v.nx : 0;
+                    const int zb = (cell[2] + 1 > za && za + 1 < v.nz) ? za + 1 : za;
+                    const int o0 = ya * v.nx + xa;
+#pragma unroll
+                    for (int c = 0; c < 8; c++) {
+                        const float wc = (float)(r * W[c]);
+                        if (wc == 0.0f) continue;
+                        const int zc = (c & 4) ? zb : za;
+                        const int o = o0 + ((c & 1) ? dx : 0) + ((c & 2) ? dy : 0);
+                        if (zc >= bz0 && zc <= bz1)
+                            atomicAdd(box + (zc - bz0) * nxy + o, wc);
+                        else
+                            unsafeAtomicAdd(gv + (size_t)zc * (size_t)nxy + (size_t)o, wc);
+                    }
+                });
+            }
+        }
+    }
+    __syncthreads();
+    float* __restrict__ gv = grads + a.pairs[a.order[first]].vol_offset + (size_t)bz0 * (size_t)nxy;
+    for (int e = (int)threadIdx.x; e < nbox; e += 256)
+        if (box[e] != 0.0f) unsafeAtomicAdd(gv + e, box[e]);
+}
+
 // vol_grads += scale * P_s^T in: the transpose of the per-cell closed form, one set of eight
 // atomics per cell a sub-ray crosses (as fp_adjoint_kernel).
 __global__ void __launch_bounds__(256) pair_adjoint_kernel(PairArgs a, float scale, float* __restrict__ grads) {
@@ -373,6 +485,12 @@ struct artia_pairs {
     PairDev* pairs = nullptr;
     int* order = nullptr;
     int npairs = 0, per_xcd = 0;
+    // the slab-summed adjoint (pair_adjoint_slab_kernel): work items and per-volume slot ranges,
+    // or none when a sub-volume's layers do not fit or a tilt runs parallel to its z layers
+    int2* slab_items = nullptr;
+    int* vfirst = nullptr;
+    int* vcount = nullptr;
+    int slab_per_xcd = 0;
 };
 
 namespace {
@@ -421,7 +539,7 @@ artia_status artia_pairs_create(artia_ctx* ctx, const artia_pair_volume* volumes
         if (!valid_geometry(&volumes[q.volume].geom, &geoms[p]))
             return fail(ctx, ARTIA_ERR_INVALID_ARGUMENT, "artia_pairs_create: invalid geometry of pair %d", p);
         PairDev& d = host[(size_t)p];
-        d.g = to_device(geoms[p]);
+        d.g = flat_beam(to_device(geoms[p]));   // as the exact forward projector takes it
         d.v = to_device(volumes[q.volume].geom);
         d.staged = ctx->march_lds && tube_plan(d.v, d.g, d.tp) ? 1 : 0;
         d.vol_offset = volumes[q.volume].offset;
@@ -439,6 +557,38 @@ artia_status artia_pairs_create(artia_ctx* ctx, const artia_pair_volume* volumes
     ps->ctx = ctx;
     ps->npairs = npairs;
     ps->per_xcd = (npairs + 7) / 8;
+    // slab work items of the summed adjoint: every sub-volume's cell layers -1 .. nz - 1 in runs of
+    // its slab depth, all on the XCD of the sub-volume (volume index mod 8), so the pairs' windows
+    // are read through one L2
+    std::vector<int> vfirst((size_t)nvol, 0), vcount((size_t)nvol, 0);
+    for (int s = 0; s < npairs; s++) {
+        const int vi = pairs[order[(size_t)s]].volume;
+        if (vcount[(size_t)vi]++ == 0) vfirst[(size_t)vi] = s;
+    }
+    // (the staging switch off keeps the per-pair scatter, as it keeps the per-pair walks direct)
+    bool slab_ok = ctx->march_lds;
+    for (int p = 0; p < npairs && slab_ok; p++)
+        slab_ok = std::fabs(host[(size_t)p].g.n[2]) > 1e-3f;   // rays cross the z layers
+    std::vector<std::vector<int2>> per((size_t)8);
+    for (int vi = 0; vi < nvol && slab_ok; vi++) {
+        if (vcount[(size_t)vi] == 0) continue;
+        const artia_vol_geom& g = volumes[vi].geom;
+        const int L = adj_slab_layers(g.nx, g.ny);
+        if (L <= 0) {
+            slab_ok = false;
+            break;
+        }
+        for (int zs = -1; zs < g.nz; zs += L) per[(size_t)(vi % 8)].push_back(make_int2(vi, zs));
+    }
+    std::vector<int2> items;
+    if (slab_ok) {
+        size_t longest = 0;
+        for (const auto& l : per) longest = std::max(longest, l.size());
+        ps->slab_per_xcd = (int)longest;
+        items.assign(8 * longest, make_int2(-1, 0));
+        for (size_t x = 0; x < 8; x++)
+            for (size_t k = 0; k < per[x].size(); k++) items[x * longest + k] = per[x][k];
+    }
     hipError_t e = hipSetDevice(ctx->device);
     if (e == hipSuccess) e = hipMalloc(&ps->pairs, sizeof(PairDev) * (size_t)npairs);
     if (e == hipSuccess) e = hipMalloc(&ps->order, sizeof(int) * (size_t)npairs);
@@ -447,6 +597,20 @@ artia_status artia_pairs_create(artia_ctx* ctx, const artia_pair_volume* volumes
                            ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(ps->order, order.data(), sizeof(int) * (size_t)npairs, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && !items.empty()) {
+        e = hipMalloc(&ps->slab_items, sizeof(int2) * items.size());
+        if (e == hipSuccess) e = hipMalloc(&ps->vfirst, sizeof(int) * (size_t)nvol);
+        if (e == hipSuccess) e = hipMalloc(&ps->vcount, sizeof(int) * (size_t)nvol);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ps->slab_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice,
+                               ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ps->vfirst, vfirst.data(), sizeof(int) * (size_t)nvol, hipMemcpyHostToDevice,
+                               ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ps->vcount, vcount.data(), sizeof(int) * (size_t)nvol, hipMemcpyHostToDevice,
+                               ctx->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         artia_pairs_destroy(ps);
@@ -460,6 +624,9 @@ artia_status artia_pairs_destroy(artia_pairs* ps) {
     if (!ps) return ARTIA_OK;
     if (ps->pairs) (void)hipFree(ps->pairs);
     if (ps->order) (void)hipFree(ps->order);
+    if (ps->slab_items) (void)hipFree(ps->slab_items);
+    if (ps->vfirst) (void)hipFree(ps->vfirst);
+    if (ps->vcount) (void)hipFree(ps->vcount);
     delete ps;
     return ARTIA_OK;
 }
@@ -528,7 +695,13 @@ artia_status artia_pairs_adjoint(artia_pairs* ps, const double* shifts, const fl
     a.shifts = shifts;
     a.in = images;
     Timing t(ctx, ms);
-    hipLaunchKernelGGL(pair_adjoint_kernel, pair_grid(ps), dim3(256), 0, ctx->stream, a, scale, vol_grads);
+    if (ps->slab_items) {
+        const AdjSlabArgs sa{ps->slab_items, ps->slab_per_xcd, ps->vfirst, ps->vcount};
+        hipLaunchKernelGGL(pair_adjoint_slab_kernel, dim3((unsigned)(8 * ps->slab_per_xcd)), dim3(256), 0, ctx->stream,
+                           a, sa, scale, vol_grads);
+    } else {
+        hipLaunchKernelGGL(pair_adjoint_kernel, pair_grid(ps), dim3(256), 0, ctx->stream, a, scale, vol_grads);
+    }
     artia_status st = check_hip(ctx, hipGetLastError(), "artia_pairs_adjoint launch");
     if (st != ARTIA_OK) return st;
     return t.finish();

@@ -212,7 +212,7 @@ __device__ __forceinline__ void stage_box(const VolG& v, const T* __restrict__ v
 //   scatter (kScatter true): f(float wgt[8], cell, k, m) returns eight corner weights, which are
 //     added to the slab's box in LDS (shared-memory atomics) and then to grad with one global
 //     atomic per box voxel, instead of eight global atomics per cell.
-template <bool kScatter, typename T, class F>
+template <bool kScatter, bool kFlat, typename T, class F>
 __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ vol, float* __restrict__ grad,
                                            float* __restrict__ chunk, int* zr, const TubePlan& tp, const TubeBox& tb,
                                            const TubeDir& td, const float q0[3], int count, F&& f) {
@@ -240,7 +240,7 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
         for (int i = 0; i < 8; i++)
             if (wgt[i] != 0.0f) unsafeAtomicAdd(grad + o[i], wgt[i]);
     };
-    CellWalker w;
+    CellWalkerT<kFlat> w;
     w.init_dir(q0, td.sgn, td.inv, count);
     const int nslab = zlo <= zhi ? (zhi - zlo) / kSlabCells + 1 : 0;
     for (int j = 0; j < nslab; j++) {
@@ -454,12 +454,12 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
     }
 }
 
-// The gather walk (see tube_slabs).
-template <typename T, class F>
+// The gather walk (see tube_slabs); kFlat: the rays have no y component (td.dq[1] == 0).
+template <bool kFlat = false, typename T, class F>
 __device__ __forceinline__ void tube_walk(const VolG& v, const T* __restrict__ vol, float* __restrict__ chunk,
                                           int* zr, const TubePlan& tp, const TubeBox& tb, const TubeDir& td,
                                           const float q0[3], int count, F&& f) {
-    tube_slabs<false>(v, vol, nullptr, chunk, zr, tp, tb, td, q0, count, f);
+    tube_slabs<false, kFlat>(v, vol, nullptr, chunk, zr, tp, tb, td, q0, count, f);
 }
 
 // The scatter walk (see tube_slabs): grad += the weights f returns.
@@ -467,7 +467,7 @@ template <class F>
 __device__ __forceinline__ void tube_scatter(const VolG& v, float* __restrict__ grad, float* __restrict__ chunk,
                                              int* zr, const TubePlan& tp, const TubeBox& tb, const TubeDir& td,
                                              const float q0[3], int count, F&& f) {
-    tube_slabs<true>(v, static_cast<const float*>(nullptr), grad, chunk, zr, tp, tb, td, q0, count, f);
+    tube_slabs<true, false>(v, static_cast<const float*>(nullptr), grad, chunk, zr, tp, tb, td, q0, count, f);
 }
 
 }  // namespace artia
