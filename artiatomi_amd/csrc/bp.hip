@@ -253,10 +253,11 @@ struct BPWinSet {
     int gated;   // 0: the dist gate passes every quad of the volume for every tilt (gate_passes_all)
 };
 
-// Block = 16 quads x 4 rows x 4 slices, as bp_exact_kernel; per tilt the block stages its
-// footprint (one pixel of margin) in LDS.  Blocks whose footprint lies inside the image apply the
-// quad footprint as a chord cap (or not at all where it covers the voxel boxes); blocks at the
-// image edge clamp each quad's lattice to its clamped footprint.
+// Block = 16 quads x 4 rows x 8 slices: each thread updates two quads, slices z and z + 4, so the
+// block's per-tilt set-up (footprint, tile staging) serves twice the voxels.  Per tilt the block
+// stages its footprint (one pixel of margin) in LDS.  Blocks whose footprint lies inside the image
+// apply the quad footprint as a chord cap (or not at all where it covers the voxel boxes); blocks at
+// the image edge clamp each quad's lattice to its clamped footprint.
 template <int kOS, int kNI, int kNJ, typename T>
 __global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __restrict__ vol, int W, int H,
                                                      size_t pitch) {
@@ -264,21 +265,28 @@ __global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __re
     const int tid = (int)threadIdx.x;
     const int xq = (int)blockIdx.x * 16 + (tid & 15);
     const int y = (int)blockIdx.y * 4 + ((tid >> 4) & 3);
-    const int z = (int)blockIdx.z * 4 + (tid >> 6);
-    const bool inVol = xq < v.nx / 4 && y < v.ny && z < v.nz;
+    const int zq[2] = {(int)blockIdx.z * kWinBlockZ + (tid >> 6), (int)blockIdx.z * kWinBlockZ + (tid >> 6) + 4};
     const unsigned x = (unsigned)xq * 4u;
-    T* vp = vol + ((size_t)(inVol ? z : 0) * (size_t)v.ny + (size_t)(inVol ? y : 0)) * (size_t)v.nx + (inVol ? x : 0u);
-    float out[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (inVol) {
-        const float4 vb = load_quad(vp);
-        out[0] = vb.x;
-        out[1] = vb.y;
-        out[2] = vb.z;
-        out[3] = vb.w;
+    bool inVol[2];
+    T* vp[2];
+    float out[2][4];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        inVol[q] = xq < v.nx / 4 && y < v.ny && zq[q] < v.nz;
+        vp[q] = vol + ((size_t)(inVol[q] ? zq[q] : 0) * (size_t)v.ny + (size_t)(inVol[q] ? y : 0)) * (size_t)v.nx +
+                (inVol[q] ? x : 0u);
+        out[q][0] = out[q][1] = out[q][2] = out[q][3] = 0.0f;
+        if (inVol[q]) {
+            const float4 vb = load_quad(vp[q]);
+            out[q][0] = vb.x;
+            out[q][1] = vb.y;
+            out[q][2] = vb.z;
+            out[q][3] = vb.w;
+        }
     }
-    bool any = false;
-    const double bx = (double)blockIdx.x * 64.0, by = (double)blockIdx.y * 4.0, bz = (double)blockIdx.z * 4.0;
-    const double qx = (double)x, qy = (double)y, qz = (double)z;
+    bool any[2] = {false, false};
+    const double bx = (double)blockIdx.x * 64.0, by = (double)blockIdx.y * 4.0,
+                 bz = (double)blockIdx.z * (double)kWinBlockZ;
     for (int t = 0; t < S.n; t++) {
         const BPWin& w = S.w[t];
         const float* __restrict__ img = S.img[t];
@@ -306,15 +314,21 @@ __global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __re
             }
         }
         __syncthreads();
-        if (!inVol) continue;
-        if (S.gated) {
-            const double gt = fabs(fma(w.gn[0], qx, fma(w.gn[1], qy, fma(w.gn[2], qz, w.g0))));
-            if (!(gt >= S.distMin && gt < S.distMax)) continue;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            if (!inVol[q]) continue;
+            const double qx = (double)x, qy = (double)y, qz = (double)zq[q];
+            if (S.gated) {
+                const double gt = fabs(fma(w.gn[0], qx, fma(w.gn[1], qy, fma(w.gn[2], qz, w.g0))));
+                if (!(gt >= S.distMin && gt < S.distMax)) continue;
+            }
+            any[q] |= window_quad<kOS, kNI, kNJ>(w, tile, tw, tx0, ty0, interior, W, H, x, (unsigned)y,
+                                                 (unsigned)zq[q], S.lambda, out[q]);
         }
-        any |= window_quad<kOS, kNI, kNJ>(w, tile, tw, tx0, ty0, interior, W, H, x, (unsigned)y, (unsigned)z,
-                                          S.lambda, out);
     }
-    if (any) store_quad(vp, out);
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+        if (any[q]) store_quad(vp[q], out[q]);
 }
 
 // The CTF backprojection of one tilt in one launch: every voxel takes its slab's filtered image
@@ -451,7 +465,8 @@ template <typename T>
 void launch_win_t(artia_ctx* ctx, const VolG& v, const BPWinSet& S, int os, int ni, int nj, T* vol, int W, int H,
                   size_t pitch) {
     const int nq = v.nx / 4;
-    const dim3 grid((unsigned)((nq + 15) / 16), (unsigned)((v.ny + 3) / 4), (unsigned)((v.nz + 3) / 4));
+    const dim3 grid((unsigned)((nq + 15) / 16), (unsigned)((v.ny + 3) / 4),
+                    (unsigned)((v.nz + kWinBlockZ - 1) / kWinBlockZ));
     if (os == 1)
         launch_win_os<1>(ctx, grid, v, S, ni, nj, vol, W, H, pitch);
     else

@@ -226,13 +226,14 @@ __device__ __forceinline__ void window_sums(const BPWin& w, const float* __restr
     den = d;
 }
 
-// The window form's plan for one tilt: the form applies, the footprint of a 64 x 4 x 4-voxel block
+// The window form's plan for one tilt: the form applies, the footprint of a 64 x 4 x 8-voxel block
 // fits the LDS tile and the window size has a compiled instance (OS 1: 1-3, OS 2: 2-4 lattice
 // points per axis).  The per-call and the batched sub-volume backprojections take the same decision,
 // so they stay bit-identical.
+constexpr int kWinBlockZ = 8;   // slices per block of the per-call kernel (two quads per thread)
 inline bool window_plan(const VolG& v, const TiltG& g, int os, BPWin& w) {
     if (os != 1 && os != 2) return false;
-    const BPExact e = prepare_exact(v, g);
+    const BPExact e = prepare_exact(v, g, 64, 4, kWinBlockZ);
     if ((long long)(e.tileW + 2) * (long long)(e.tileH + 2) > kTileFloats) return false;
     if (!prepare_window(v, g, e, os, os == 1 ? 3 : 4, w)) return false;
     return os == 1 ? true : (w.ni >= 2 && w.nj >= 2);
