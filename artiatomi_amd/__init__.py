@@ -27,6 +27,7 @@ from .core import (  # noqa: F401
     hitpoints,
     projection_index_list,
     real_length,
+    rot_matrix,
     sart_params,
     subvol_wbp_plain,
     subvolume_geometry,
@@ -42,5 +43,5 @@ __all__ = [
     "VolGeom", "hitpoints", "load", "projection_index_list", "real_length", "sart_params",
     "subvolume_geometry", "tilt_geometry", "volume_geometry", "FileInfo", "parse_file_header", "read_file",
     "volume_file_header", "marker_tilt_params", "ctf_file_defocus", "prep_params", "MotiveList", "shift_file_get",
-    "shift_file_set", "subvolume_job",
+    "shift_file_set", "subvolume_job", "rot_matrix",
 ]

@@ -230,7 +230,9 @@ SIGNATURES = {
     "artia_pairs_loss_grad": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _PF]),
     "artia_pairs_shift_grad": (_I, [_P, _P, _P, _P, _P, _PF]),
     "artia_pairs_adjoint": (_I, [_P, _P, _P, _F, _P, _PF]),
-    "artia_pairs_cc_search": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _PF]),
+    "artia_pairs_cc_search": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _PF]),
+    "artia_rot_matrix": (None, [_F, _F, _F, _PF]),
+    "artia_rot3d": (_I, [_P, _P, _I, _PF, _I, _P, _PF]),
     "artia_profile_enable": (_I, [_P, _I]),
     "artia_profile_read": (_I, [_P, _I, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "artia_fp_ctf": (_I, [_P, C.POINTER(VolGeom), C.POINTER(TiltGeom), _P, _I, _I, _I, _SZ, _F, _F,

@@ -226,15 +226,15 @@ def refine_pair_shifts(volumes: torch.Tensor, targets: torch.Tensor, pairs: "cor
 
 
 def cc_search_shifts(volumes: torch.Tensor, targets: torch.Tensor, pairs: "core.PairSet", square: int,
-                     max_shift: int, init: Optional[torch.Tensor] = None):
+                     max_shift: int, init: Optional[torch.Tensor] = None, multi_peak: bool = False):
     """EmSARTRefine's search (GetDisplacement, Reconstructor.cpp:1746-1959) for every pair at once:
     the models are the pairs projected at `init` (zero by default); a pair whose target matches its
     model moved by d gets the shift init + d (integer d within the max_shift disk).  Pairs whose
-    search fails keep `init`.  Returns (shifts (npairs, 2) float64, peak CC (npairs,), failed mask)."""
+    search fails keep `init`.  multi_peak selects MultiPeakDetection (Reconstructor.cpp:1887-1935).  Returns (shifts (npairs, 2) float64, peak CC (npairs,), failed mask)."""
     s0 = init.to(torch.float64).contiguous() if init is not None else torch.zeros(
         (pairs.npairs, 2), dtype=torch.float64, device=volumes.device)
     models = pairs.project(volumes, s0)
-    peak, cc, _ = pairs.cc_search(models, targets, square, max_shift)
+    peak, cc, _ = pairs.cc_search(models, targets, square, max_shift, multi_peak=multi_peak)
     failed = peak[:, 0] == -1000
     s = torch.where(failed[:, None], s0, s0 - peak.to(torch.float64))
     return s, cc, failed

@@ -27,6 +27,7 @@ SOURCES = [
     "subvol.hip",
     "adjoint.hip",
     "refine.hip",
+    "rotate.hip",
     "sart.hip",
     "prep.hip",
 ]

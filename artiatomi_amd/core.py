@@ -268,6 +268,15 @@ def shift_file_set(shift_data, proj: int, motive: int, value) -> None:
     _check(st, what="artia_shift_file_set")
 
 
+def rot_matrix(phi: float, psi: float, theta: float):
+    """computeRotMat's float matrix (artia_rot_matrix): 3 x 3 numpy float32, R = Rz(psi) Rx(theta) Rz(phi)."""
+    import numpy as np
+
+    R = (C.c_float * 9)()
+    _abi.load().artia_rot_matrix(float(phi), float(psi), float(theta), R)
+    return np.array(list(R), dtype=np.float32).reshape(3, 3)
+
+
 class MotiveList:
     """A motive list (MotiveListe.h:33-106) over the file's float array of shape (particles, 20),
     with the position and shift binnings EmSART's ScaleMotivelistPosition/-Shift apply."""
@@ -675,6 +684,24 @@ class ArtiaContext:
                           C.byref(ctf), _abi.farr(cut_length, 2), _abi.farr(dim_length, 2), float(lambda_),
                           int(oversampling), timed=timed)
 
+    def rot3d(self, vol, angles, out=None, timed=False):
+        """The size^3 reference cube rotated per motive (artia_rot3d, RotKernel / rot3d): angles is a
+        sequence of (phi, psi, theta) in degrees; returns out (n, size, size, size) float32."""
+        import torch
+
+        S = int(vol.shape[0])
+        if tuple(vol.shape) != (S, S, S):
+            raise ArtiaError(f"rot3d takes a cube, got shape {tuple(vol.shape)}")
+        flat = [float(a) for triple in angles for a in triple]
+        if len(flat) % 3:
+            raise ArtiaError("rot3d angles are (phi, psi, theta) triples")
+        n = len(flat) // 3
+        if out is None:
+            out = torch.empty((n, S, S, S), device=vol.device, dtype=torch.float32)
+        ms = self._call("artia_rot3d", _ptr(vol, "vol", S ** 3), S, _abi.farr(flat) if n else None, n,
+                        _ptr(out, "out", n * S ** 3), timed=timed)
+        return (out, ms) if timed else out
+
     def sart_run(self, vg: VolGeom, acq: Acquisition, projections, vol, params: SartParams, iterations=1,
                  volume_is_empty: bool = True, timed=False, noalloc=False):
         """Runs `iterations` SART iterations; returns (ms or None, volume_is_empty).  noalloc=True takes
@@ -906,8 +933,10 @@ class PairSet:
         return self._run("artia_pairs_adjoint", self._shifts(shifts), _ptr(images, "images", self.n_pixels),
                          float(scale), _ptr(vol_grads, "vol_grads", self.vol_floats), timed=timed)
 
-    def cc_search(self, models, targets, square: int, max_shift: int, want_map=False, timed=False):
-        """GetDisplacement's search per pair: (peak (npairs, 2) int32, cc (npairs,) float32, map or None)."""
+    def cc_search(self, models, targets, square: int, max_shift: int, want_map=False, timed=False,
+                  multi_peak=False):
+        """GetDisplacement's search per pair: (peak (npairs, 2) int32, cc (npairs,) float32, map or None).
+        multi_peak=True takes MultiPeakDetection's peak (the qualifying local maximum nearest to zero)."""
         import torch
 
         dev = models.device
@@ -916,7 +945,8 @@ class PairSet:
         side = 2 * int(max_shift) + 1
         ccmap = torch.empty((self.npairs, side, side), device=dev, dtype=torch.float32) if want_map else None
         ms = self._run("artia_pairs_cc_search", _ptr(models, "models", self.n_pixels),
-                       _ptr(targets, "targets", self.n_pixels), int(square), int(max_shift), peak.data_ptr(),
+                       _ptr(targets, "targets", self.n_pixels), int(square), int(max_shift), int(bool(multi_peak)),
+                       peak.data_ptr(),
                        cc.data_ptr(), None if ccmap is None else ccmap.data_ptr(), timed=timed)
         return (peak, cc, ccmap, ms) if timed else (peak, cc, ccmap)
 

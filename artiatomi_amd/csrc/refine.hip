@@ -388,10 +388,15 @@ constexpr int kMaxCcShift = 32;
 // mean and energy terms, the maxShift disk (wbpWeighting.cu:265-294) and nppiMaxIndx's first
 // maximum in the square's row-major order.  Wave w takes candidates w, w + 4, ...; lanes run
 // along window rows.
+// multi != 0: MultiPeakDetection (Reconstructor.cpp:1887-1935): among the peaks of the float map
+// (findPeak, wbpWeighting.cu:329-386: above the 8 neighbours, the two upper diagonal ones or equal,
+// and at least 0.9f of the maximum) the one nearest to zero shift (maxShiftWeighted,
+// wbpWeighting.cu:295-327: weight 1 / (dist + 0.0001f)), first in row-major order; the CC value and
+// the failure test keep the first search's maximum.  Candidates outside the disk are 0 in the map.
 __global__ void __launch_bounds__(256) pair_cc_kernel(const PairDev* __restrict__ pairs, int npairs,
                                                       const float* __restrict__ models,
-                                                      const float* __restrict__ targets, double n2, int M,
-                                                      int* __restrict__ peak, float* __restrict__ ccv,
+                                                      const float* __restrict__ targets, int N, int M,
+                                                      int multi, int* __restrict__ peak, float* __restrict__ ccv,
                                                       float* __restrict__ ccmap) {
     __shared__ double craw[(2 * kMaxCcShift + 1) * (2 * kMaxCcShift + 1)];
     const int p = (int)blockIdx.x;
@@ -439,7 +444,7 @@ __global__ void __launch_bounds__(256) pair_cc_kernel(const PairDev* __restrict_
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const double SA = tot[0], SB = tot[2];
+    const double SA = tot[0], SB = tot[2], n2 = (double)N * (double)N;
     const double sumA = tot[1] - SA * SA / n2, sumB = tot[3] - SB * SB / n2;
     const double den = sqrt(sumA * sumB), off = SA * SB / n2;
     const bool ok = sumA * sumB > 0.0;
@@ -468,6 +473,46 @@ __global__ void __launch_bounds__(256) pair_cc_kernel(const PairDev* __restrict_
             const int rx = c % side - M, ry = c / side - M;
             if (!(sqrtf((float)(rx * rx + ry * ry)) <= (float)M)) ccmap[(long long)p * ncand + c] = 0.0f;
         }
+    }
+    if (multi) {
+        // the float map value at shift (rx, ry), wrapped around the N x N square as findPeak's
+        // neighbours are: 0 outside the disk (maxShift)
+        auto map = [&](int rx, int ry) -> float {
+            rx = ((rx % N) + N) % N;
+            ry = ((ry % N) + N) % N;
+            rx = rx > N / 2 ? rx - N : rx;
+            ry = ry > N / 2 ? ry - N : ry;
+            if (rx < -M || rx > M || ry < -M || ry > M || !(sqrtf((float)(rx * rx + ry * ry)) <= (float)M))
+                return 0.0f;
+            return ok ? (float)((craw[(ry + M) * side + (rx + M)] - off) / den) : 0.0f;
+        };
+        const float thr = (float)best * 0.9f;
+        float bestw = 0.0f;
+        int mx = 0, my = 0;
+        bool seen = false;
+        for (int yy = 0; yy < side; yy++) {
+            const int ry = yy <= M ? yy : yy - side;
+            for (int xx = 0; xx < side; xx++) {
+                const int rx = xx <= M ? xx : xx - side;
+                const float d = sqrtf((float)(rx * rx + ry * ry));
+                float wv = 0.0f;
+                if (d <= (float)M) {
+                    const float val = map(rx, ry);
+                    const bool pk = map(rx - 1, ry) < val && map(rx + 1, ry) < val && map(rx, ry - 1) < val &&
+                                    map(rx, ry + 1) < val && map(rx - 1, ry - 1) <= val && map(rx - 1, ry + 1) < val &&
+                                    map(rx + 1, ry - 1) <= val && map(rx + 1, ry + 1) < val && val >= thr;
+                    wv = pk ? 1.0f / (d + 0.0001f) : 0.0f;
+                }
+                if (!seen || wv > bestw) {
+                    bestw = wv;
+                    mx = rx;
+                    my = ry;
+                    seen = true;
+                }
+            }
+        }
+        bx = mx;
+        by = my;
     }
     const bool fail = !ok || !(best > 0.0);
     peak[2 * p] = fail ? -1000 : bx;
@@ -708,18 +753,20 @@ artia_status artia_pairs_adjoint(artia_pairs* ps, const double* shifts, const fl
 }
 
 artia_status artia_pairs_cc_search(artia_pairs* ps, const float* models, const float* targets, int square,
-                                   int max_shift, int* peak, float* cc_value, float* cc_map, float* ms) {
+                                   int max_shift, int multi_peak, int* peak, float* cc_value, float* cc_map,
+                                   float* ms) {
     if (!ps) return ARTIA_ERR_INVALID_ARGUMENT;
     if (!models || !targets || !peak || !cc_value)
         return pairs_fail(ps, ARTIA_ERR_INVALID_ARGUMENT, "artia_pairs_cc_search: null pointer");
-    if (square <= 0 || max_shift < 0 || max_shift > kMaxCcShift)
+    if (max_shift < 0 || max_shift > kMaxCcShift || square < 2 * max_shift + 1)
         return fail(ps->ctx, ARTIA_ERR_INVALID_ARGUMENT,
-                    "artia_pairs_cc_search: square %d must be positive and max_shift %d in 0..%d", square, max_shift,
-                    kMaxCcShift);
+                    "artia_pairs_cc_search: max_shift %d must be in 0..%d and square %d at least 2 max_shift + 1",
+                    max_shift, kMaxCcShift, square);
     artia_ctx* ctx = ps->ctx;
     Timing t(ctx, ms);
     hipLaunchKernelGGL(pair_cc_kernel, dim3((unsigned)ps->npairs), dim3(256), 0, ctx->stream, ps->pairs, ps->npairs,
-                       models, targets, (double)square * (double)square, max_shift, peak, cc_value, cc_map);
+                       models, targets, square, max_shift, multi_peak ? 1 : 0, peak, cc_value,
+                       cc_map);
     artia_status st = check_hip(ctx, hipGetLastError(), "artia_pairs_cc_search launch");
     if (st != ARTIA_OK) return st;
     return t.finish();

@@ -402,9 +402,31 @@ artia_status artia_pairs_adjoint(artia_pairs* ps, const double* shifts, const fl
  * target matches the model moved by -r, so the shift estimate is s = -r.  Both are -1000 (the
  * reference's failure value) when the best CC is <= 0 or SumA SumB is 0.  cc_value[p] = the best
  * CC; cc_map (or NULL) receives (2 max_shift + 1)^2 floats per pair, [ry + max_shift][rx +
- * max_shift], 0 outside the disk.  0 <= max_shift <= 32. */
+ * max_shift], 0 outside the disk.  0 <= max_shift <= 32, square >= 2 max_shift + 1.
+ * multi_peak != 0 is GetDisplacement's MultiPeakDetection (Reconstructor.cpp:1887-1935): the peak is
+ * instead the qualifying local maximum nearest to zero shift.  On the float map (0 outside the disk)
+ * a pixel qualifies when it is above its left, right, upper, lower and lower-diagonal neighbours,
+ * at least equal to its upper-diagonal ones (neighbours wrap around the square; findPeak,
+ * wbpWeighting.cu:329-386) and at least 0.9f times the maximum; the winner maximises
+ * 1 / (sqrtf(rx^2 + ry^2) + 0.0001f) (maxShiftWeighted, wbpWeighting.cu:295-327), ties to the first
+ * in the square's row-major order.  cc_value and the failure test stay those of the maximum. */
 artia_status artia_pairs_cc_search(artia_pairs* ps, const float* models, const float* targets, int square,
-                                   int max_shift, int* peak, float* cc_value, float* cc_map, float* ms);
+                                   int max_shift, int multi_peak, int* peak, float* cc_value, float* cc_map,
+                                   float* ms);
+
+/* ---------------------------------------------------------------- motive rotation */
+/* RotKernel::computeRotMat (Kernels.cpp:1227-1327): R = Rz(psi) Rx(theta) Rz(phi) from the
+ * motive list's phi, psi, theta in degrees, row-major R[3 i + j] = rotMat[i][j], in float; the
+ * sixteen angles 0, 30, 45, 60, .., 330 take tabulated exact cosines and sines. */
+void artia_rot_matrix(float phi, float psi, float theta, float R[9]);
+/* The reference cube rotated per motive (EmSartRefine.cpp:805-865 with RotKernel::operator(),
+ * Kernels.cpp:1338-1381, and rot3d, wbpWeighting.cu:388-405), n rotations in one call:
+ * out + r size^3, [z][y][x], = in sampled at c + R_r^T ((x, y, z) - c), c = (float)(size / 2),
+ * trilinear with indices clamped to [0, size - 1] (the texture's CLAMP addressing), exact-float
+ * weights.  vol and out are device pointers; angles is a host array of n (phi, psi, theta)
+ * triples.  1 <= size <= 1024. */
+artia_status artia_rot3d(artia_ctx* ctx, const float* vol, int size, const float* angles, int n, float* out,
+                         float* ms);
 
 /* ---------------------------------------------------------------- per-tilt 3D-CTF steps
  * Reconstructor::ForwardProjectionCTF (Reconstructor.cpp:496-644) for one tilt: for every slab

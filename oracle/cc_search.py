@@ -19,6 +19,16 @@ relative to the uermel/Artiatomi source tree):
   8. nppiMaxIndx over the square: the first maximum in row-major order (:1866);
   9. wrap the index to a signed shift (x > N/2 -> x - N) and report (-1000, -1000) when the
      maximum is <= 0 (:1938-1956).
+With MultiPeakDetection (:1887-1935) step 8 is followed by:
+  8a. findPeak (wbpWeighting.cu:329-386) on the masked float map: a pixel is a peak when it
+      exceeds its left, right, upper and lower neighbours and its lower diagonal neighbours
+      strictly, its upper diagonal neighbours or equals them (the reference's <= for the two
+      pixels of row y - 1), and reaches maxVal * 0.9f (neighbours wrap around the square);
+  8b. nppiSet 1 everywhere, then 0 where the inverted mask is set: 1 at the peaks, 0 elsewhere;
+  8c. maxShiftWeighted (wbpWeighting.cu:295-327): 0 outside the disk, value / (dist + 0.0001f)
+      inside (in float);
+  8d. nppiMaxIndx again; the shift comes from this index, the failure test and the reported CC
+      value from step 8's maximum.
 The cuFFT pair computes, up to float rounding, the circular correlation
 CC(r) = sum_x A'(x) B'(x + r) / sqrt(SumA SumB) of the mean-free images, which is what this
 restatement evaluates (in double, through numpy's FFT).
@@ -40,9 +50,28 @@ def to_square(img: np.ndarray, N: int, x0: int = 0, y0: int = 0) -> np.ndarray:
     return sq
 
 
-def get_displacement(A_sq: np.ndarray, B_sq: np.ndarray, max_shift: int):
+def multi_peak_index(cc32: np.ndarray, max_val: float, max_shift: int) -> tuple:
+    """Steps 8a-8d on the masked float32 map: (x, y) square index of the nearest qualifying peak."""
+    N = cc32.shape[0]
+    v = cc32
+    L, R = np.roll(v, 1, axis=1), np.roll(v, -1, axis=1)       # [y][x - 1], [y][x + 1]
+    U, D = np.roll(v, 1, axis=0), np.roll(v, -1, axis=0)       # [y - 1][x], [y + 1][x]
+    UL, UR = np.roll(U, 1, axis=1), np.roll(U, -1, axis=1)     # [y - 1][x - 1], [y - 1][x + 1]
+    DL, DR = np.roll(D, 1, axis=1), np.roll(D, -1, axis=1)     # [y + 1][x - 1], [y + 1][x + 1]
+    thr = np.float32(np.float32(max_val) * np.float32(0.9))
+    peak = (L < v) & (R < v) & (U < v) & (D < v) & (UL <= v) & (DL < v) & (UR <= v) & (DR < v) & (v >= thr)
+    idx = np.arange(N)
+    m = np.where(idx > N // 2, N - idx, idx).astype(np.float32)
+    dist = np.sqrt(m[None, :] * m[None, :] + m[:, None] * m[:, None], dtype=np.float32)
+    w = np.where(peak, np.float32(1.0), np.float32(0.0))
+    w = np.where(dist > np.float32(max_shift), np.float32(0.0), (w / (dist + np.float32(0.0001))).astype(np.float32))
+    y, x = divmod(int(np.argmax(w)), N)
+    return x, y
+
+
+def get_displacement(A_sq: np.ndarray, B_sq: np.ndarray, max_shift: int, multi_peak: bool = False):
     """Steps 2-9 on two N x N squares: A the measured-minus-rest image, B the model.  Returns
-    (rx, ry, max value, masked correlation map)."""
+    (rx, ry, max value, masked correlation map); multi_peak takes steps 8a-8d."""
     N = A_sq.shape[0]
     A = A_sq - A_sq.mean()
     B = B_sq - B_sq.mean()
@@ -59,6 +88,9 @@ def get_displacement(A_sq: np.ndarray, B_sq: np.ndarray, max_shift: int):
     flat = int(np.argmax(cc))            # first maximum in row-major order
     y, x = divmod(flat, N)
     val = float(cc[y, x])
+    if multi_peak:
+        # the float map of the reference; its maximum is step 8's value
+        x, y = multi_peak_index(cc.astype(np.float32), float(np.float32(val)), max_shift)
     rx = x - N if x > N // 2 else x
     ry = y - N if y > N // 2 else y
     if val <= 0:

@@ -7,7 +7,8 @@
   - the batched adjoint passes the dot-product test with shifts (<= 1e-6) and matches the oracle's
     FP64 transpose per sub-volume (<= 1e-5);
   - the cross-correlation search returns the oracle's (oracle/cc_search.py, GetDisplacement)
-    peaks exactly, with its correlation values and maps to 1e-5;
+    peaks exactly, with its correlation values and maps to 1e-5, with and without
+    MultiPeakDetection (on projections, and on two-spot targets where the two searches differ);
   - the Gauss-Newton matrix J^T J of the same launch matches the oracle's FP64 Jacobian;
   - 12 fused evaluations (Levenberg-Marquardt) or 50 (Barzilai-Borwein) recover per-pair shifts
     from U[-3, 3] px, where the integer search is off by its quantisation.
@@ -231,25 +232,61 @@ def test_adjoint_dot_product_and_oracle(ctx, setup):
         assert e <= 1e-5
 
 
+def check_cc_search(ps, pairs, models, targets, max_shift, multi_peak):
+    """The device search against the oracle's GetDisplacement on each pair's square; returns the peaks."""
+    peak, cc, ccmap = ps.cc_search(models, targets, W, max_shift, want_map=True, multi_peak=multi_peak)
+    peak, cc, ccmap = host(peak), host(cc), host(ccmap)
+    th, mh = host(targets).astype(np.float64), host(models).astype(np.float64)
+    for p, pr in enumerate(pairs):
+        _, x0, y0, w, h = pr
+        a = cc_search.to_square(th[pr_slice(ps, p)].reshape(h, w), W, x0, y0)
+        b = cc_search.to_square(mh[pr_slice(ps, p)].reshape(h, w), W, x0, y0)
+        rx, ry, val, full = cc_search.get_displacement(a, b, max_shift, multi_peak=multi_peak)
+        assert (peak[p, 0], peak[p, 1]) == (rx, ry), (p, peak[p], (rx, ry))
+        assert abs(cc[p] - val) <= 1e-5
+        assert np.abs(ccmap[p] - cc_search.cc_map_of(full, max_shift)).max() <= 1e-5
+    return peak
+
+
+@pytest.mark.parametrize("multi_peak", [False, True])
 @pytest.mark.parametrize("max_shift", [0, 4, 7])
-def test_cc_search_matches_oracle(ctx, setup, max_shift):
+def test_cc_search_matches_oracle(ctx, setup, max_shift, multi_peak):
     ps = setup["ps"]
     rng = np.random.default_rng(11 + max_shift)
     s_true = rng.uniform(-3, 3, (ps.npairs, 2))
     targets = ps.project(setup["dflat"], dev(s_true, torch.float64))
     targets = targets + 0.02 * torch.from_numpy(rng.standard_normal(ps.n_pixels).astype(np.float32)).cuda()
     models = ps.project(setup["dflat"])
-    peak, cc, ccmap = ps.cc_search(models, targets, W, max_shift, want_map=True)
-    peak, cc, ccmap = host(peak), host(cc), host(ccmap)
-    th, mh = host(targets).astype(np.float64), host(models).astype(np.float64)
+    check_cc_search(ps, setup["pairs"], models, targets, max_shift, multi_peak)
+
+
+def spot(h, w, cx, cy, s=0.8):
+    y, x = np.mgrid[0:h, 0:w]
+    return np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * s * s))
+
+
+@pytest.mark.parametrize("amp", [0.95, 0.8])
+def test_cc_search_multi_peak_two_copies(ctx, setup, amp):
+    """Each target holds its model spot twice, moved by d1 (full strength) and by the nearer d2
+    (strength amp): MultiPeakDetection picks -d2 when amp >= 0.9, else -d1 like the plain search."""
+    ps = setup["ps"]
+    d1s = ((5, -3), (-4, 4), (6, 1), (-2, -6))
+    d2s = ((1, 2), (-1, 0), (0, -2), (2, 1))
+    models = np.zeros(ps.n_pixels, np.float32)
+    targets = np.zeros(ps.n_pixels, np.float32)
     for p, pr in enumerate(setup["pairs"]):
         _, x0, y0, w, h = pr
-        a = cc_search.to_square(th[pr_slice(ps, p)].reshape(h, w), W, x0, y0)
-        b = cc_search.to_square(mh[pr_slice(ps, p)].reshape(h, w), W, x0, y0)
-        rx, ry, val, full = cc_search.get_displacement(a, b, max_shift)
-        assert (peak[p, 0], peak[p, 1]) == (rx, ry), (p, peak[p], (rx, ry))
-        assert abs(cc[p] - val) <= 1e-5
-        assert np.abs(ccmap[p] - cc_search.cc_map_of(full, max_shift)).max() <= 1e-5
+        cx, cy = w // 2, h // 2
+        (ax, ay), (bx, by) = d1s[p % 4], d2s[p % 4]
+        models[pr_slice(ps, p)] = spot(h, w, cx, cy).ravel()
+        targets[pr_slice(ps, p)] = (spot(h, w, cx + ax, cy + ay) + amp * spot(h, w, cx + bx, cy + by)).ravel()
+    m, t = dev(models), dev(targets)
+    plain = check_cc_search(ps, setup["pairs"], m, t, 7, False)
+    multi = check_cc_search(ps, setup["pairs"], m, t, 7, True)
+    for p in range(ps.npairs):
+        d = d2s[p % 4] if amp >= 0.9 else d1s[p % 4]
+        assert tuple(plain[p]) == (-d1s[p % 4][0], -d1s[p % 4][1])
+        assert tuple(multi[p]) == (-d[0], -d[1]), (p, multi[p])
 
 
 def test_refinement_recovers_shifts(ctx, setup):

@@ -117,3 +117,58 @@ def test_window_sum_equals_square_correlation():
                     if 0 <= y - ry < h and 0 <= x - rx < w:
                         c += Bw[y, x] * Aw[y - ry, x - rx]
             assert abs((c - sA * sB / n2) / np.sqrt(sumA * sumB) - win[ry + M, rx + M]) < 1e-12
+
+
+def _map(N, entries):
+    cc = np.zeros((N, N), np.float32)
+    for x, y, v in entries:
+        cc[y % N, x % N] = v
+    return cc
+
+
+def test_multi_peak_takes_nearest_peak_above_nine_tenths():
+    N, M = 16, 6
+    assert cc_search.multi_peak_index(_map(N, [(5, 0, 1.0), (1, 1, 0.95)]), 1.0, M) == (1, 1)
+    assert cc_search.multi_peak_index(_map(N, [(5, 0, 1.0), (1, 1, 0.85)]), 1.0, M) == (5, 0)
+    assert cc_search.multi_peak_index(_map(N, [(5, 0, 1.0), (-1, -1, 0.95)]), 1.0, M) == (N - 1, N - 1)
+    # outside the disk a peak weighs nothing: sqrt(50) > 6
+    assert cc_search.multi_peak_index(_map(N, [(5, 0, 1.0), (5, 5, 0.99)]), 1.0, M) == (5, 0)
+    # the threshold is maxVal * 0.9f, reached with equality
+    t = np.float32(np.float32(1.0) * np.float32(0.9))
+    assert cc_search.multi_peak_index(_map(N, [(5, 0, 1.0), (1, 0, t)]), 1.0, M) == (1, 0)
+    below = np.nextafter(t, np.float32(0))
+    assert cc_search.multi_peak_index(_map(N, [(5, 0, 1.0), (1, 0, below)]), 1.0, M) == (5, 0)
+
+
+def test_multi_peak_neighbour_rule_is_asymmetric():
+    """findPeak compares the two pixels of row y - 1 at x -/+ 1 with <=, all other neighbours with <."""
+    N, M = 16, 6
+    # equal pair on the main diagonal: only the lower one, whose equal neighbour is upper-left, is a peak
+    assert cc_search.multi_peak_index(_map(N, [(3, 3, 1.0), (4, 4, 1.0)]), 1.0, M) == (4, 4)
+    # equal pair on the anti-diagonal: only (3, 4), whose equal neighbour is upper-right
+    assert cc_search.multi_peak_index(_map(N, [(4, 3, 1.0), (3, 4, 1.0)]), 1.0, M) == (3, 4)
+    # equal pair in a row or a column: neither is a peak, the farther single peak wins
+    assert cc_search.multi_peak_index(_map(N, [(1, 1, 1.0), (2, 1, 1.0), (0, 5, 0.95)]), 1.0, M) == (0, 5)
+    assert cc_search.multi_peak_index(_map(N, [(1, 1, 1.0), (1, 2, 1.0), (0, 5, 0.95)]), 1.0, M) == (0, 5)
+    # neighbours wrap around the square: (0, 2) sits right of (-1, 2)
+    assert cc_search.multi_peak_index(_map(N, [(5, 0, 1.0), (0, 2, 0.95), (-1, 2, 0.97)]), 1.0, M) == (N - 1, 2)
+    # equal distance: the first in row-major order
+    assert cc_search.multi_peak_index(_map(N, [(5, 0, 1.0), (0, 2, 0.95), (2, 0, 0.95)]), 1.0, M) == (2, 0)
+
+
+def test_get_displacement_multi_peak_on_two_copies():
+    """A target holding the model twice, moved by d1 (full strength) and by d2 (0.95): the plain
+    search finds -d1, MultiPeakDetection the nearer -d2; at 0.8 strength the second copy's peak
+    drops below 0.9 of the maximum."""
+    N, M = 48, 8
+    B = blob(N, 24, 24, 1.0)
+    d1, d2 = (5, -3), (1, 2)
+    for amp, want in ((0.95, d2), (0.8, d1)):
+        A = np.roll(np.roll(B, d1[1], axis=0), d1[0], axis=1) + amp * np.roll(np.roll(B, d2[1], axis=0), d2[0], axis=1)
+        rx, ry, val, _ = cc_search.get_displacement(A, B, M)
+        assert (rx, ry) == (-d1[0], -d1[1])
+        mx, my, mval, _ = cc_search.get_displacement(A, B, M, multi_peak=True)
+        assert (mx, my) == (-want[0], -want[1])
+        assert mval == val
+    rx, ry, _, _ = cc_search.get_displacement(-B, B, 4, multi_peak=True)
+    assert (rx, ry) == (-1000, -1000)
