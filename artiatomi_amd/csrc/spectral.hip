@@ -161,6 +161,27 @@ __global__ void __launch_bounds__(256) ctf_exact_stack_kernel(float2* __restrict
     for (int s = 0; s < ns; s++) q[(size_t)s * plane] = ctf_apply_at(q[(size_t)s * plane], p, b.k, b.def0[s], b.def1[s]);
 }
 
+// ns slabs of ONE image: spectrum s = base * CTF(slab s's defocus) * scale.  scale is 1, or 1 / N^2
+// (a power of two, so the product is exact) when the inverse transform writes the slab images
+// straight into their destination: C2R(scale X) = scale C2R(X) bit for bit, which is the
+// division by N^2 of the finishing pass.
+__global__ void __launch_bounds__(256) ctf_exact_fan_kernel(const float2* __restrict__ base,
+                                                            float2* __restrict__ spec, size_t pitch, CtfStack b,
+                                                            int ns, float scale) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= b.k.N / 2 + 1 || y >= b.k.N) return;
+    const CtfPoint p = ctf_point(x, y, b.k);
+    const size_t i = (size_t)y * pitch + x, plane = pitch * (size_t)b.k.N;
+    const float2 r0 = base[i];
+    for (int s = 0; s < ns; s++) {
+        float2 r = ctf_apply_at(r0, p, b.k, b.def0[s], b.def1[s]);
+        r.x = r.x * scale;
+        r.y = r.y * scale;
+        spec[(size_t)s * plane + i] = r;
+    }
+}
+
 // wbpWeighting.cu:48-144 (sin/cos of the angle as IEEE functions rather than __sinf/__cosf)
 __global__ void __launch_bounds__(256) wbp_weight_kernel(float2* __restrict__ spec, size_t pitch, int N, float ang,
                                                          int method, int exact) {
@@ -392,7 +413,7 @@ artia_status FftBatch::ensure(artia_ctx* ctx, int n) {
     }
     artia_status st = check_hip(ctx, hipMalloc(&square, (size_t)c * n * n * sizeof(float)), "hipMalloc(squares)");
     if (st == ARTIA_OK)
-        st = check_hip(ctx, hipMalloc(&spec, (size_t)c * (n / 2 + 1) * n * sizeof(float2)), "hipMalloc(spectra)");
+        st = check_hip(ctx, hipMalloc(&spec, (size_t)(c + 1) * (n / 2 + 1) * n * sizeof(float2)), "hipMalloc(spectra)");
     if (st == ARTIA_OK && work_bytes > 0)
         st = check_hip(ctx, hipMalloc(&work, work_bytes), "hipMalloc(fft work area)");
     if (st != ARTIA_OK) {
@@ -478,19 +499,37 @@ artia_status ctf_filter_slabs(artia_ctx* ctx, const float* src, size_t src_strid
     const CropParams cp = crop ? make_crop_params(cut, dim, corners) : CropParams{};
     const int bx = (N - W) / 2, by = (N - H) / 2;
     const size_t half = (size_t)(N / 2 + 1);
+    // Every slab filtering the same uncropped image (the sub-volume WBP): one forward transform,
+    // kept in the spare spectrum slot, serves all slabs.  When the slab images are whole squares
+    // of a power-of-two edge stored back to back, the inverse transform writes them in place with
+    // 1 / N^2 folded into the CTF factor.
+    const bool shared = src_stride == 0 && !crop;
+    const bool direct = shared && !forFP && W == N && H == N && pitch == (size_t)N &&
+                        out_stride == (size_t)N * (size_t)N && (N & (N - 1)) == 0;
+    float2* base = fb.spec + (size_t)fb.cap * half * (size_t)N;
+    if (shared) {
+        hipLaunchKernelGGL(slab_square_kernel, dim3((unsigned)((N + 63) / 64), (unsigned)((N + 3) / 4)), dim3(64, 4), 0,
+                           ctx->stream, src, (size_t)0, W, H, pitch, N, bx, by, fb.square, cp, 0, 0, 1);
+        if (hipfftSetStream(fb.r2c[0], ctx->stream) != HIPFFT_SUCCESS)
+            return fail(ctx, ARTIA_ERR_FFT, "hipfftSetStream failed");
+        if (hipfftExecR2C(fb.r2c[0], fb.square, (hipfftComplex*)base) != HIPFFT_SUCCESS)
+            return fail(ctx, ARTIA_ERR_FFT, "hipfftExecR2C failed");
+    }
     for (int k0 = 0; k0 < n;) {
         // the largest power-of-two batch that fits: at most log2(cap) + 1 chunks past the full ones
         int b = fb.cap, pi = 0;
         while (b > n - k0) b /= 2;
         for (int t = b; t > 1; t /= 2) pi++;
-        hipLaunchKernelGGL(slab_square_kernel, dim3((unsigned)((N + 63) / 64), (unsigned)((N + 3) / 4)), dim3(64, 4), 0,
-                           ctx->stream, src + (size_t)k0 * src_stride, src_stride, W, H, pitch, N, bx, by, fb.square,
-                           cp, !crop ? 0 : (forFP ? 1 : k0 + 1), crop && !forFP ? 1 : 0, b);
         if (hipfftSetStream(fb.r2c[pi], ctx->stream) != HIPFFT_SUCCESS ||
             hipfftSetStream(fb.c2r[pi], ctx->stream) != HIPFFT_SUCCESS)
             return fail(ctx, ARTIA_ERR_FFT, "hipfftSetStream failed");
-        if (hipfftExecR2C(fb.r2c[pi], fb.square, (hipfftComplex*)fb.spec) != HIPFFT_SUCCESS)
-            return fail(ctx, ARTIA_ERR_FFT, "hipfftExecR2C (batch %d) failed", b);
+        if (!shared) {
+            hipLaunchKernelGGL(slab_square_kernel, dim3((unsigned)((N + 63) / 64), (unsigned)((N + 3) / 4)),
+                               dim3(64, 4), 0, ctx->stream, src + (size_t)k0 * src_stride, src_stride, W, H, pitch, N,
+                               bx, by, fb.square, cp, !crop ? 0 : (forFP ? 1 : k0 + 1), crop && !forFP ? 1 : 0, b);
+            if (hipfftExecR2C(fb.r2c[pi], fb.square, (hipfftComplex*)fb.spec) != HIPFFT_SUCCESS)
+                return fail(ctx, ARTIA_ERR_FFT, "hipfftExecR2C (batch %d) failed", b);
+        }
         CtfStack cs;
         cs.k = make_ctf(N, p, dmin_nm[k0], dmax_nm[k0], angle_deg, forFP);
         for (int s = 0; s < b; s++) {
@@ -498,11 +537,18 @@ artia_status ctf_filter_slabs(artia_ctx* ctx, const float* src, size_t src_strid
             cs.def0[s] = ks.def0;
             cs.def1[s] = ks.def1;
         }
-        hipLaunchKernelGGL(ctf_exact_stack_kernel, dim3((unsigned)((N / 2 + 1 + 63) / 64), (unsigned)((N + 3) / 4)),
-                           dim3(64, 4), 0, ctx->stream, fb.spec, half, cs, b);
-        if (hipfftExecC2R(fb.c2r[pi], (hipfftComplex*)fb.spec, fb.square) != HIPFFT_SUCCESS)
+        const dim3 sgrid((unsigned)((N / 2 + 1 + 63) / 64), (unsigned)((N + 3) / 4));
+        if (shared)
+            hipLaunchKernelGGL(ctf_exact_fan_kernel, sgrid, dim3(64, 4), 0, ctx->stream, base, fb.spec, half, cs, b,
+                               direct ? 1.0f / ((float)N * (float)N) : 1.0f);
+        else
+            hipLaunchKernelGGL(ctf_exact_stack_kernel, sgrid, dim3(64, 4), 0, ctx->stream, fb.spec, half, cs, b);
+        float* inv = direct ? out + (size_t)k0 * out_stride : fb.square;
+        if (hipfftExecC2R(fb.c2r[pi], (hipfftComplex*)fb.spec, inv) != HIPFFT_SUCCESS)
             return fail(ctx, ARTIA_ERR_FFT, "hipfftExecC2R (batch %d) failed", b);
-        if (forFP)
+        if (direct) {
+            // the slab images are written
+        } else if (forFP)
             hipLaunchKernelGGL(finish_sum_kernel, grid2d(W, H), dim3(64, 4), 0, ctx->stream, fb.square, b, N, bx, by, W,
                                H, out, pitch, (float)(N * N), crop, cp);
         else

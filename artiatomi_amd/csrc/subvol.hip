@@ -12,8 +12,10 @@
 //          sub-volume size (largest first) and popped through a device counter.  A thread keeps its
 //          voxels in registers across all tilts and writes them once.  The per-tilt update is the
 //          plain form's arithmetic, added in the same order, so the two forms agree bit for bit.
-//          Without CTF the counter reset and the persistent kernel are captured in one hipGraph;
-//          with CTF each tilt's slab images are filtered first and one persistent launch follows.
+//          Without CTF the counter reset and the persistent kernel are captured in one hipGraph.
+//          With CTF the tilts go in chunks whose filtered slab images fit a stack budget: every
+//          tilt of a chunk is filtered (one forward transform per tilt, batched inverse transforms
+//          per slab), then one persistent launch adds the chunk's tilts in order.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -35,6 +37,10 @@ struct SubvolDev {
 // Per (s, t) pair.  mode 0: inactive; 1: LDS-tile eligible (affine footprint, POINT, no
 // magnification anisotropy); 2: global-memory path; 3: CTF slab stack, LDS-tile eligible for
 // LINEAR fetches (affine footprint, no magnification anisotropy).
+// Default bytes of filtered slab images a CTF plan keeps at once (a chunk of tilts); at most half
+// the free device memory, at least one tilt's stack.
+constexpr size_t kSubvolStackBytes = (size_t)16 << 30;
+
 struct PairDev {
     BPExact e;
     BPWin w;     // the window form (bp_window.hpp) when win != 0, as the per-call backprojection takes it
@@ -42,6 +48,12 @@ struct PairDev {
     VolG v;
     int mode;
     int win;
+};
+
+// CTF: tilt t's slab bounds and where its filtered slab images start in the stack (in slab images).
+struct TiltSlab {
+    SlabArgs s;
+    long long stack_off;
 };
 
 // One block's work: sub-volume s, tile origin (quad x, y, z) and tile shape (quads x rows x slices
@@ -144,15 +156,17 @@ __device__ __forceinline__ void quad_update_stack(const PairDev& p, const float*
     }
 }
 
-// Persistent backprojection over the work queue.  Tilts [t0, t1) of every item; image of tilt t at
-// images + (t - t0) * imgStride (no CTF), or slab k of the single tilt at images + k * imgStride.
+// Persistent backprojection over the work queue.  Tilts [t0, t1) of every item, added in tilt order
+// to the voxels held in registers; image of tilt t at images + (t - t0) * imgStride (no CTF), or
+// slab k of tilt t at images + (tslab[t].stack_off + k) * imgStride (CTF).
 template <int kOS, int kQPT>
 __global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__ items, int nitems,
                                                         unsigned* __restrict__ counter,
                                                         const SubvolDev* __restrict__ subs,
                                                         const PairDev* __restrict__ pairs, int ntilts, int t0, int t1,
                                                         const float* __restrict__ images, size_t imgStride, int W,
-                                                        int H, size_t pitch, int os, int filter, SlabArgs slabs) {
+                                                        int H, size_t pitch, int os, int filter, SlabArgs slabs,
+                                                        const TiltSlab* __restrict__ tslab) {
     __shared__ float tile[kTileFloats];
     __shared__ int cur;
     __shared__ int kr[2];
@@ -188,9 +202,11 @@ __global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__
         for (int t = t0; t < t1; t++) {
             const PairDev& p = pairs[(size_t)item.s * (size_t)ntilts + (size_t)t];
             if (p.mode == 0) continue;
-            // no CTF: tilt t's image; CTF (one tilt per launch): the tilt's slab stack
+            // no CTF: tilt t's image; CTF: the tilt's slab stack
             const float* img = images + (size_t)(t - t0) * imgStride;
             if (p.mode == 3) {
+                const TiltSlab ts = tslab[t];
+                img = images + (size_t)ts.stack_off * imgStride;
                 // slab of each quad, and the item's slab range
                 if (tid == 0) {
                     kr[0] = INT_MAX;
@@ -201,7 +217,7 @@ __global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__
 #pragma unroll
                 for (int q = 0; q < kQPT; q++) {
                     const unsigned z = (unsigned)(item.z0 + tz + q * item.tz);
-                    ks[q] = in[q] ? find_slab(slabs, gate_distance(p.e, p.v, p.g, x, (unsigned)y, z)) : -1;
+                    ks[q] = in[q] ? find_slab(ts.s, gate_distance(p.e, p.v, p.g, x, (unsigned)y, z)) : -1;
                     if (ks[q] >= 0) {
                         atomicMin(&kr[0], ks[q]);
                         atomicMax(&kr[1], ks[q]);
@@ -383,12 +399,12 @@ void build_items(const std::vector<SubvolDev>& subs, const std::vector<int>& ord
 void launch_subvol(artia_ctx* ctx, hipStream_t stream, int qpt, int grid, const Item* items, int nitems,
                    unsigned* counter, const SubvolDev* subs, const PairDev* pairs, int ntilts, int t0, int t1,
                    const float* images, size_t imgStride, int W, int H, size_t pitch, int os, int filter,
-                   SlabArgs slabs) {
+                   SlabArgs slabs, const TiltSlab* tslab) {
     (void)ctx;
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
 #define ARTIA_SUBVOL_LAUNCH(OS, QPT)                                                                                 \
     hipLaunchKernelGGL((subvol_bp_kernel<OS, QPT>), dim3((unsigned)grid), dim3(256), 0, stream, items, nitems,      \
-                       counter, subs, pairs, ntilts, t0, t1, images, imgStride, W, H, pitch, os, filter, slabs)
+                       counter, subs, pairs, ntilts, t0, t1, images, imgStride, W, H, pitch, os, filter, slabs, tslab)
     const int o = (os == 1 || os == 2 || os == 4) ? os : 0;
     if (qpt == 2) {
         if (o == 1) ARTIA_SUBVOL_LAUNCH(1, 2);
@@ -475,10 +491,12 @@ struct artia_subvol_plan {
     PairDev* d_pairs = nullptr;
     Item* d_items = nullptr;
     unsigned* d_counter = nullptr;
-    float* d_bounds = nullptr;   // CTF: slab starts of every tilt, concatenated
-    float* d_stack = nullptr;    // CTF: the current tilt's filtered slab images
-    std::vector<int> bound_off, bound_count;
-    std::vector<float> last_end;
+    float* d_bounds = nullptr;     // CTF: slab starts of every tilt, concatenated
+    float* d_stack = nullptr;      // CTF: the filtered slab images of the current chunk of tilts
+    TiltSlab* d_tslab = nullptr;   // CTF: per tilt, its slab bounds and stack offset
+    std::vector<int> bound_count;
+    std::vector<long long> stack_off;   // per tilt, within its chunk (in slab images)
+    std::vector<int> chunks;            // tilt chunk starts, then ntilts
     int max_slabs = 0;
     int nitems = 0;
     artia_subvol_tuning tuning{};
@@ -492,7 +510,7 @@ struct artia_subvol_plan {
         if (graph) (void)hipGraphDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
         for (void* p : {(void*)d_subs, (void*)d_pairs, (void*)d_items, (void*)d_counter, (void*)d_bounds,
-                        (void*)d_stack})
+                        (void*)d_stack, (void*)d_tslab})
             if (p) (void)hipFree(p);
     }
 };
@@ -515,7 +533,7 @@ artia_status set_items(artia_subvol_plan* p, int policy, int qpt) {
     return upload(p->ctx, (void**)&p->d_items, items.data(), items.size() * sizeof(Item), "sub-volume queue");
 }
 
-// CTF: tilt t's slab images, filtered into the stack (batched FFTs, no border crop).
+// CTF: tilt t's slab images, filtered into its place in the stack (batched FFTs, no border crop).
 artia_status filter_tilt(artia_subvol_plan* p, int t) {
     const Inputs& in = p->in;
     const int count = p->bound_count[(size_t)t];
@@ -527,19 +545,18 @@ artia_status filter_tilt(artia_subvol_plan* p, int t) {
         artia_geom_defocus_minmax(starts[(size_t)k], &p->defocus[(size_t)t], &p->ctfp, &dmin[(size_t)k],
                                   &dmax[(size_t)k]);
     const size_t slabStride = (size_t)in.H * in.pitch;
-    return ctf_filter_slabs(p->ctx, in.images + (size_t)t * slabStride, 0, count, in.W, in.H, in.pitch, p->d_stack,
-                            slabStride, nullptr, nullptr, nullptr, p->ctfp, dmin.data(), dmax.data(),
+    return ctf_filter_slabs(p->ctx, in.images + (size_t)t * slabStride, 0, count, in.W, in.H, in.pitch,
+                            p->d_stack + (size_t)p->stack_off[(size_t)t] * slabStride, slabStride, nullptr, nullptr, nullptr, p->ctfp, dmin.data(), dmax.data(),
                             p->defocus[(size_t)t].astig_angle_deg, 0);
 }
 
-// CTF: the persistent backprojection of tilt t from its filtered stack.
-void launch_tilt(artia_subvol_plan* p, int t, hipStream_t stream) {
+// CTF: the persistent backprojection of tilts [t0, t1) from their filtered stacks.
+void launch_tilts(artia_subvol_plan* p, int t0, int t1, hipStream_t stream) {
     const Inputs& in = p->in;
     const size_t slabStride = (size_t)in.H * in.pitch;
     launch_subvol(p->ctx, stream, p->tuning.quads_per_thread, p->grid, p->d_items, p->nitems, p->d_counter,
-                  p->d_subs, p->d_pairs, in.ntilts, t, t + 1, p->d_stack, slabStride, in.W, in.H, in.pitch, in.os,
-                  ARTIA_FILTER_LINEAR,
-                  SlabArgs{p->d_bounds + p->bound_off[(size_t)t], p->bound_count[(size_t)t], p->last_end[(size_t)t]});
+                  p->d_subs, p->d_pairs, in.ntilts, t0, t1, p->d_stack, slabStride, in.W, in.H, in.pitch, in.os,
+                  ARTIA_FILTER_LINEAR, SlabArgs{nullptr, 0, 0.0f}, p->d_tslab);
 }
 
 // Enqueues one full backprojection of the plan on `stream` (the graph body for the no-CTF case).
@@ -548,20 +565,27 @@ artia_status enqueue(artia_subvol_plan* p, hipStream_t stream) {
     if (!p->ctf) {
         launch_subvol(p->ctx, stream, p->tuning.quads_per_thread, p->grid, p->d_items, p->nitems, p->d_counter,
                       p->d_subs, p->d_pairs, in.ntilts, 0, in.ntilts, in.images, (size_t)in.H * in.pitch, in.W, in.H,
-                      in.pitch, in.os, ARTIA_FILTER_POINT, SlabArgs{nullptr, 0, 0.0f});
+                      in.pitch, in.os, ARTIA_FILTER_POINT, SlabArgs{nullptr, 0, 0.0f}, nullptr);
         return check_hip(p->ctx, hipGetLastError(), "sub-volume backprojection launch");
     }
+    // per chunk of tilts whose slab stacks fit together: filter them all, then one persistent
+    // launch adds the chunk's tilts to the voxels in registers
     artia_ctx* ctx = p->ctx;
     hipStream_t saved = ctx->stream;
     ctx->stream = stream;
-    for (int t = 0; t < in.ntilts; t++) {
-        if (p->bound_count[(size_t)t] == 0) continue;
-        const artia_status st = filter_tilt(p, t);
-        if (st != ARTIA_OK) {
-            ctx->stream = saved;
-            return st;
+    for (size_t c = 0; c + 1 < p->chunks.size(); c++) {
+        const int t0 = p->chunks[c], t1 = p->chunks[c + 1];
+        bool any = false;
+        for (int t = t0; t < t1; t++) {
+            if (p->bound_count[(size_t)t] == 0) continue;
+            any = true;
+            const artia_status st = filter_tilt(p, t);
+            if (st != ARTIA_OK) {
+                ctx->stream = saved;
+                return st;
+            }
         }
-        launch_tilt(p, t, stream);
+        if (any) launch_tilts(p, t0, t1, stream);
     }
     ctx->stream = saved;
     return check_hip(ctx, hipGetLastError(), "sub-volume CTF backprojection launch");
@@ -642,7 +666,7 @@ artia_status autotune(artia_subvol_plan* p) {
                     for (int rep = 0; rep < 2 && st == ARTIA_OK; rep++) {
                         (void)hipEventRecord(e0, ctx->stream);
                         if (tt >= 0) {
-                            launch_tilt(p, tt, ctx->stream);
+                            launch_tilts(p, tt, tt + 1, ctx->stream);
                             st = check_hip(ctx, hipGetLastError(), "autotune launch");
                         } else {
                             st = enqueue(p, ctx->stream);
@@ -790,20 +814,48 @@ artia_status artia_subvol_plan_create(artia_ctx* ctx, const artia_subvol* subs, 
     }
     if (ctf) {
         std::vector<float> all, starts;
+        std::vector<int> boff;
+        std::vector<float> last;
         const float step = ctf->slice_thickness_nm / ctf->pixelsize_nm;
         for (int t = 0; t < ntilts; t++) {
             slab_starts(slab_range[2 * t], slab_range[2 * t + 1], step, starts);
-            p->bound_off.push_back((int)all.size());
+            boff.push_back((int)all.size());
             p->bound_count.push_back((int)starts.size());
-            p->last_end.push_back(starts.empty() ? 0.0f : starts.back() + step);
+            last.push_back(starts.empty() ? 0.0f : starts.back() + step);
             all.insert(all.end(), starts.begin(), starts.end());
             p->max_slabs = std::max(p->max_slabs, (int)starts.size());
         }
+        // tilt chunks: consecutive tilts whose slab images fit the stack budget together
+        const size_t slabBytes = (size_t)height * pitch * sizeof(float);
+        size_t budget = ctx->stack_limit ? ctx->stack_limit : kSubvolStackBytes;
+        size_t freeB = 0, totalB = 0;
+        if (hipMemGetInfo(&freeB, &totalB) == hipSuccess && freeB / 2 < budget) budget = freeB / 2;
+        const long long capSlabs = std::max((long long)p->max_slabs, (long long)(budget / slabBytes));
+        long long used = 0, peak = 1;
+        p->chunks.push_back(0);
+        for (int t = 0; t < ntilts; t++) {
+            const long long c = p->bound_count[(size_t)t];
+            if (used > 0 && used + c > capSlabs) {
+                p->chunks.push_back(t);
+                used = 0;
+            }
+            p->stack_off.push_back(used);
+            used += c;
+            peak = std::max(peak, used);
+        }
+        p->chunks.push_back(ntilts);
         if ((st = upload(ctx, (void**)&p->d_bounds, all.data(), all.size() * sizeof(float), "slab bounds")) !=
+            ARTIA_OK) {
+            delete p;
+            return st;
+        }
+        std::vector<TiltSlab> ts((size_t)ntilts);
+        for (int t = 0; t < ntilts; t++)
+            ts[(size_t)t] = TiltSlab{SlabArgs{p->d_bounds + boff[(size_t)t], p->bound_count[(size_t)t], last[(size_t)t]},
+                                     p->stack_off[(size_t)t]};
+        if ((st = upload(ctx, (void**)&p->d_tslab, ts.data(), ts.size() * sizeof(TiltSlab), "tilt slab table")) !=
                 ARTIA_OK ||
-            (st = check_hip(ctx, hipMalloc(&p->d_stack, (size_t)std::max(1, p->max_slabs) * height * pitch *
-                                                            sizeof(float)),
-                            "slab image stack")) != ARTIA_OK) {
+            (st = check_hip(ctx, hipMalloc(&p->d_stack, (size_t)peak * slabBytes), "slab image stack")) != ARTIA_OK) {
             delete p;
             return st;
         }

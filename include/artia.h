@@ -149,7 +149,8 @@ artia_status artia_set_fp_staging(artia_ctx* ctx, int enable);
  * images of a tilt and the forward slabs' sub-ray sums.  0 (the default) allows 32 GiB of images and
  * 16 GiB of sums.  Within a smaller limit the forward slabs go in several volume passes, or slab by
  * slab, and without room for the images every slab is a separate launch, as in the reference's loop;
- * the results are the same. */
+ * the results are the same.  A CTF sub-volume plan created afterwards takes the same limit for the
+ * filtered slab images of its chunk of tilts (0: 16 GiB). */
 artia_status artia_set_stack_limit(artia_ctx* ctx, size_t bytes);
 artia_status artia_sync(artia_ctx* ctx);
 const char* artia_last_error(artia_ctx* ctx);
@@ -647,7 +648,10 @@ artia_status artia_subvol_wbp_plain(artia_ctx* ctx, const artia_subvol* subs, in
 
 /* Batched form of the same computation.  No reference counterpart: one persistent launch over a
  * device work queue of (sub-volume, voxel tile) items sorted by size, each thread accumulating all
- * tilts in registers; captured once in a hipGraph (without CTF) and replayed by execute.  Results
+ * tilts in registers; captured once in a hipGraph (without CTF) and replayed by execute.  With CTF
+ * the tilts run in chunks whose filtered slab images fit together (artia_set_stack_limit bytes, by
+ * default 16 GiB, at most half the free device memory, at least one tilt): each tilt of a chunk is
+ * transformed once and filtered per slab, then one persistent launch adds the chunk's tilts.  Results
  * equal artia_subvol_wbp_plain's bit for bit in ARTIA_NUMERICS_EXACT (the only numerics it
  * supports).  The plan keeps the pointers it was given; they must stay valid until destroy.
  * tuning NULL: the plan times the candidate tile shapes, quads per thread and persistent block

@@ -4,7 +4,8 @@ plain form: against the oracle's composition of lambda-1 backprojections (EmSart
   611-707, Reconstructor.cpp:894-1034) - exact numerics <= 1e-5 relative L2 against FP64,
   reference numerics <= 1e-6 against the float oracle.
 batched form: bit-identical to the plain form in exact numerics, for the autotuned and for fixed
-  tunings, with and without CTF, including repeated graph replays.
+  tunings, with and without CTF, including repeated graph replays; with CTF also for stack budgets
+  that split the tilts into chunks of about two tilts and of one tilt.
 """
 import numpy as np
 import pytest
@@ -110,8 +111,14 @@ def test_subvol_batched_equals_plain(gpu, ctf):
         A.subvol_wbp_plain(ctx, subs2, c["geoms"], images, os_, active=c["active"], **ctf_kwargs(c))
         torch.cuda.synchronize()
         plain2 = [v.cpu().numpy() for _, v in subs2]
-        for tu in TUNINGS:
-            print(f"subvol batched ctf={ctf} tuning={tu}", flush=True)
+        # (tuning, stack budget): with CTF, budgets of 1 MiB (16 slab images of 128^2, about two
+        # tilts per chunk) and 1 byte (one tilt per chunk) besides the default
+        runs = [(tu, 0) for tu in TUNINGS]
+        if ctf:
+            runs += [((0, 1, 2, 4), 1 << 20), ((0, 0, 1, 2), 1)]
+        for tu, limit in runs:
+            print(f"subvol batched ctf={ctf} tuning={tu} stack limit={limit}", flush=True)
+            ctx.set_stack_limit(limit)
             subs = device_subs(c)
             tuning = A.SubvolTuning(*tu) if tu is not None else None
             plan = A.SubvolPlan(ctx, subs, c["geoms"], images, os_, active=c["active"], tuning=tuning,
