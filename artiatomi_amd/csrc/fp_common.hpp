@@ -144,6 +144,15 @@ __device__ __forceinline__ bool subray_samples(const VolG& v, const TiltG& g, do
 // kFlat: the rays have no y component (dq[1] == 0 exactly, every tilt about the y axis whose beam
 // direction rounds to n_y = 0), so the walk never crosses a y face and the y axis drops out of the
 // step.  With dq[1] == 0 both forms visit the same cells with the same sample ranges.
+// Direction of one axis: the sign of dq and 1 / dq, or no crossings (sign 0) when dq is 0 or so
+// small that 1 / dq overflows (such an axis would cross no face within the walk's samples).
+__device__ __forceinline__ void walk_dir(float dq, int& sgn, float& inv) {
+    const float r = 1.0f / dq;
+    const bool ok = dq != 0.0f && __builtin_isfinite(r);
+    sgn = ok ? (dq > 0.0f ? 1 : -1) : 0;
+    inv = ok ? r : 0.0f;
+}
+
 template <bool kFlat>
 struct CellWalkerT {
     int cell[3], sgn[3];
@@ -155,8 +164,7 @@ struct CellWalkerT {
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             cell[i] = (int)floorf(q0[i]);
-            sgn[i] = dq[i] > 0.0f ? 1 : (dq[i] < 0.0f ? -1 : 0);
-            inv[i] = sgn[i] != 0 ? 1.0f / dq[i] : 0.0f;
+            walk_dir(dq[i], sgn[i], inv[i]);
             off[i] = -q0[i] * inv[i];
             kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]) : INFINITY;
         }
@@ -199,11 +207,26 @@ struct CellWalkerT {
         guard = 0;
     }
     __device__ __forceinline__ bool active() const { return k < count && guard < 4 * count + 64; }
-    // One step; moved(i) is called for every axis i the walk advances along.
+    // One step; moved(i) is called for every axis i the walk advances along.  The next crossing kn
+    // is the least of the axes' crossing indices (never NaN: finite, or INFINITY on an axis without
+    // crossings), found with compares, and every axis at kn advances.  The samples in the cell run
+    // up to kend = clamp(ceil(kn), 0, count).
     template <class F, class M>
     __device__ __forceinline__ void step(F&& f, M&& moved) {
-        const float kn = kFlat ? fminf(kb[0], kb[2]) : fminf(fminf(kb[0], kb[1]), kb[2]);
-        const int kend = kn < fcount ? max((int)ceilf(kn), 0) : count;
+        bool adv[3];
+        float kn;
+        if (kFlat) {
+            adv[0] = kb[0] <= kb[2];
+            adv[1] = false;
+            adv[2] = kb[2] <= kb[0];
+            kn = adv[0] ? kb[0] : kb[2];
+        } else {
+            const float kxy = kb[0] <= kb[1] ? kb[0] : kb[1];
+            kn = kxy <= kb[2] ? kxy : kb[2];
+#pragma unroll
+            for (int i = 0; i < 3; i++) adv[i] = kb[i] == kn;
+        }
+        const int kend = (int)__builtin_amdgcn_fmed3f(ceilf(kn), 0.0f, fcount);
         if (kend > k) {
             f(cell, k, kend - k);
             k = kend;
@@ -211,7 +234,7 @@ struct CellWalkerT {
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             if (kFlat && i == 1) continue;
-            if (kb[i] == kn) {
+            if (adv[i]) {
                 cell[i] += sgn[i];
                 kb[i] = __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]);
                 moved(i);
@@ -243,6 +266,7 @@ __device__ __forceinline__ void for_each_cell(const float q0[3], const float dq[
 struct CellMoments {
     float a, b, c, S2c, m;
     float pab, pac, pbc, q;   // da db, da dc, db dc, a db dc + b da dc + c da db
+    bool flat;                // db == 0: b is the same for every sample
 };
 
 // kFlat (dq[1] == 0): b is the same for every sample, da db = db dc = 0 and q = b da dc; the
@@ -251,13 +275,14 @@ template <bool kFlat = false>
 __device__ __forceinline__ CellMoments cell_moments(const float q0[3], const float dq[3], const int cell[3], int k,
                                                     int mi) {
     CellMoments cm;
-    const float kc = (float)(2 * k + mi - 1) * 0.5f;   // k + (m - 1) / 2, exact
+    const float kc = (float)(k + (k + mi) - 1) * 0.5f;   // k + (m - 1) / 2, exact
     cm.a = __builtin_fmaf(kc, dq[0], q0[0]) - (float)cell[0];
     cm.b = (kFlat ? q0[1] : __builtin_fmaf(kc, dq[1], q0[1])) - (float)cell[1];
     cm.c = __builtin_fmaf(kc, dq[2], q0[2]) - (float)cell[2];
     cm.m = (float)mi;
     cm.S2c = cm.m * __builtin_fmaf(cm.m, cm.m, -1.0f) * (1.0f / 12.0f);   // m, m^2 - 1 and their product exact
     cm.pac = dq[0] * dq[2];
+    cm.flat = kFlat || dq[1] == 0.0f;
     if (kFlat) {
         cm.pab = cm.pbc = 0.0f;
         cm.q = cm.b * cm.pac;
@@ -328,26 +353,30 @@ __device__ __forceinline__ float buffer_vox(const __amdgpu_buffer_rsrc_t rs, uin
 // Sum of one cell's m samples, m f(centre) + S2c corr, rounded once to float (relative error
 // 6e-8 per cell); the ray adds the cells in double, so per-cell roundings do not accumulate into a
 // float running sum.
-// kFlat: da db = db dc = 0, so the AB and BC terms vanish (corr = ABC q + AC da dc, the general
-// form's value up to the sign of a zero).
+// db = 0 (cm.flat; always under kFlat): b is the same for every sample and the cell is a
+// bilinear interpolant in (a, c) of its four y-blended edges w_xz = lerp(v_x0z, v_x1z, b): the
+// samples sum to m g(centre) + S2c (w11 - w01 - w10 + w00) da dc.  Every kernel takes this form
+// for such rays, so their sums agree bit for bit whatever walk they run.
 template <bool kFlat = false>
 __device__ __forceinline__ double cell_sum_values(const float val[8], const CellMoments& cm, int mi) {
     const float v000 = val[0], v100 = val[1], v010 = val[2], v110 = val[3];
     const float v001 = val[4], v101 = val[5], v011 = val[6], v111 = val[7];
+    (void)mi;
+    if (kFlat || cm.flat) {
+        const float w00 = lerpf(v000, v010, cm.b), w10 = lerpf(v100, v110, cm.b);
+        const float w01 = lerpf(v001, v011, cm.b), w11 = lerpf(v101, v111, cm.b);
+        const float d0 = w10 - w00, d1 = w11 - w01;
+        const float g = lerpf(__builtin_fmaf(cm.a, d0, w00), __builtin_fmaf(cm.a, d1, w01), cm.c);
+        return (double)__builtin_fmaf(cm.m, g, cm.S2c * ((d1 - d0) * cm.pac));
+    }
     // x differences, shared by the x lerps and the mixed differences
     const float x00 = v100 - v000, x10 = v110 - v010, x01 = v101 - v001, x11 = v111 - v011;
     const float c00 = __builtin_fmaf(cm.a, x00, v000), c10 = __builtin_fmaf(cm.a, x10, v010);
     const float c01 = __builtin_fmaf(cm.a, x01, v001), c11 = __builtin_fmaf(cm.a, x11, v011);
     const float f = lerpf(lerpf(c00, c10, cm.b), lerpf(c01, c11, cm.b), cm.c);
     const float AB = x10 - x00, AC = x01 - x00, ABC = (x11 - x01) - AB;
-    float corr;
-    if (kFlat) {
-        corr = __builtin_fmaf(ABC, cm.q, AC * cm.pac);
-    } else {
-        const float BC = (v011 - v001) - (v010 - v000);
-        corr = __builtin_fmaf(ABC, cm.q, __builtin_fmaf(BC, cm.pbc, __builtin_fmaf(AC, cm.pac, AB * cm.pab)));
-    }
-    (void)mi;
+    const float BC = (v011 - v001) - (v010 - v000);
+    const float corr = __builtin_fmaf(ABC, cm.q, __builtin_fmaf(BC, cm.pbc, __builtin_fmaf(AC, cm.pac, AB * cm.pab)));
     return (double)__builtin_fmaf(cm.m, f, cm.S2c * corr);
 }
 

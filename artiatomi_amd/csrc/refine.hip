@@ -276,10 +276,7 @@ __global__ void __launch_bounds__(256) pair_adjoint_slab_kernel(PairArgs a, AdjS
             int sgn[3];
             float inv[3];
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                sgn[c] = s.dq[c] > 0.0f ? 1 : (s.dq[c] < 0.0f ? -1 : 0);
-                inv[c] = sgn[c] != 0 ? 1.0f / s.dq[c] : 0.0f;
-            }
+            for (int c = 0; c < 3; c++) walk_dir(s.dq[c], sgn[c], inv[c]);
             // samples of this slab: from the crossing of its entry face to that of its exit face, as
             // the walker computes face crossings (CellWalker::step), clamped to the sub-ray
             const float offz = -s.q0[2] * inv[2];

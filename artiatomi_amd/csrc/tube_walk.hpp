@@ -75,8 +75,9 @@ __device__ __forceinline__ TubeDir tube_dir(const VolG& v, const TiltG& g, doubl
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         d.dq[i] = wave_uniform((float)(step * (double)g.n[i] * ((double)v.rcp[i] * (double)v.dim[i])));
-        d.sgn[i] = d.dq[i] > 0.0f ? 1 : (d.dq[i] < 0.0f ? -1 : 0);
-        d.inv[i] = wave_uniform(d.sgn[i] != 0 ? 1.0f / d.dq[i] : 0.0f);
+        float inv;
+        walk_dir(d.dq[i], d.sgn[i], inv);
+        d.inv[i] = wave_uniform(inv);
     }
     return d;
 }
