@@ -213,6 +213,17 @@ struct CellWalkerT {
     // up to kend = clamp(ceil(kn), 0, count).
     template <class F, class M>
     __device__ __forceinline__ void step(F&& f, M&& moved) {
+        step_impl<false>(f, moved);
+    }
+    // The same step with f called for every cell, also one holding no samples (m = 0): for callers
+    // whose callbacks add terms that are +-0 when m = 0 (cell sums and gradient sums: m = 0 and
+    // S2c = -0), which then run without a branch per step.
+    template <class F, class M>
+    __device__ __forceinline__ void step_all(F&& f, M&& moved) {
+        step_impl<true>(f, moved);
+    }
+    template <bool kAll, class F, class M>
+    __device__ __forceinline__ void step_impl(F&& f, M&& moved) {
         bool adv[3];
         float kn;
         if (kFlat) {
@@ -227,7 +238,11 @@ struct CellWalkerT {
             for (int i = 0; i < 3; i++) adv[i] = kb[i] == kn;
         }
         const int kend = (int)__builtin_amdgcn_fmed3f(ceilf(kn), 0.0f, fcount);
-        if (kend > k) {
+        if (kAll) {
+            const int mi = max(kend - k, 0);
+            f(cell, k, mi);
+            k += mi;
+        } else if (kend > k) {
             f(cell, k, kend - k);
             k = kend;
         }

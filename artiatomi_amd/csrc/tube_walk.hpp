@@ -209,7 +209,8 @@ __device__ __forceinline__ void stage_box(const VolG& v, const T* __restrict__ v
 // called by all 256 threads of the block (it synchronises); chunk holds kChunkFloats floats and
 // zr two ints of shared memory.
 //   gather (kScatter false): f(const float val[8], cell, k, m) for every cell holding samples
-//     k..k+m-1, with the cell's eight corner values read from vol (staged in LDS);
+//     k..k+m-1, with the cell's eight corner values read from vol (staged in LDS); f may also see
+//     cells with m = 0, to which it must add nothing but +-0;
 //   scatter (kScatter true): f(float wgt[8], cell, k, m) returns eight corner weights, which are
 //     added to the slab's box in LDS (shared-memory atomics) and then to grad with one global
 //     atomic per box voxel, instead of eight global atomics per cell.
@@ -288,22 +289,32 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
             int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
             // a fast lane enters with its cell in [zs, ze] and moves monotonically in z, so staying
             // in the slab is one range test
-            while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
-                w.step(
-                    [&](const int cell[3], int k, int mi) {
-                        float val[8];
-                        if (kScatter) {
+            if (kScatter) {
+                while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
+                    w.step(
+                        [&](const int cell[3], int k, int mi) {
+                            float val[8];
                             f(val, cell, k, mi);
 #pragma unroll
                             for (int i = 0; i < 8; i++)
                                 if (val[i] != 0.0f) atomicAdd(chunk + o + off[i], val[i]);
-                        } else {
+                        },
+                        [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
+                }
+            } else {
+                // every step reads its cell's corners and calls f, also for a cell without samples
+                // (f adds +-0 then: the gather callbacks add cell and gradient sums), so the loop
+                // body has no branch
+                while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
+                    w.step_all(
+                        [&](const int cell[3], int k, int mi) {
+                            float val[8];
 #pragma unroll
                             for (int i = 0; i < 8; i++) val[i] = chunk[o + off[i]];
                             f(val, cell, k, mi);
-                        }
-                    },
-                    [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
+                        },
+                        [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
+                }
             }
         } else {
             while (w.active() && (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
