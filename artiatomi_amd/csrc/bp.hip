@@ -297,6 +297,9 @@ __global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __re
         const bool interior = ux0 >= 0 && uy0 >= 0 && ux1 <= W - 1 && uy1 <= H - 1;
         const int tx0 = max(0, ux0), ty0 = max(0, uy0);
         const int tw = max(0, min(W - 1, ux1) - tx0 + 1), th = max(0, min(H - 1, uy1) - ty0 + 1);
+        // LDS row stride: odd where it fits, so that the four voxel rows of a wave, which read four
+        // consecutive tile rows at the same columns, fall in different banks
+        const int ts = (tw | 1) * th <= kTileFloats ? (tw | 1) : tw;
         if (t > 0) __syncthreads();   // the previous tilt's readers are done
         // tile entry idx = r * tw + c, strided by 256 over the block: one division per thread,
         // then the row and column are stepped (tiles are often a few columns wide and tall)
@@ -304,7 +307,7 @@ __global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __re
             int r = tid / tw, c = tid - r * tw;
             const int dr = 256 / tw, dc = 256 - dr * tw;
             for (int idx = tid; idx < tw * th; idx += 256) {
-                tile[idx] = img[(size_t)(ty0 + r) * pitch + (size_t)(tx0 + c)];
+                tile[r * ts + c] = img[(size_t)(ty0 + r) * pitch + (size_t)(tx0 + c)];
                 r += dr;
                 c += dc;
                 if (c >= tw) {
@@ -322,7 +325,7 @@ __global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __re
                 const double gt = fabs(fma(w.gn[0], qx, fma(w.gn[1], qy, fma(w.gn[2], qz, w.g0))));
                 if (!(gt >= S.distMin && gt < S.distMax)) continue;
             }
-            any[q] |= window_quad<kOS, kNI, kNJ>(w, tile, tw, tx0, ty0, interior, W, H, x, (unsigned)y,
+            any[q] |= window_quad<kOS, kNI, kNJ>(w, tile, ts, tx0, ty0, interior, W, H, x, (unsigned)y,
                                                  (unsigned)zq[q], S.lambda, out[q]);
         }
     }

@@ -300,10 +300,12 @@ __global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__
                 // the per-call backprojection's window form (POINT, lambda 1), from the LDS tile or,
                 // when the item's footprint does not fit it, from the image with clamped lattices
                 const bool interior = useTile && tx0 >= 1 && ty0 >= 1 && tx0 + tw <= W - 1 && ty0 + th <= H - 1;
+                // odd LDS row stride where it fits (rows of a wave in different banks)
+                const int ts = (tw | 1) * th <= kTileFloats ? (tw | 1) : tw;
                 if (useTile) {
                     for (int idx = tid; idx < tw * th; idx += 256) {
                         const int r = idx / tw, c = idx - r * tw;
-                        tile[idx] = img[(size_t)(ty0 + r) * pitch + (size_t)(tx0 + c)];
+                        tile[r * ts + c] = img[(size_t)(ty0 + r) * pitch + (size_t)(tx0 + c)];
                     }
                     __syncthreads();
                 }
@@ -312,7 +314,7 @@ __global__ void __launch_bounds__(256) subvol_bp_kernel(const Item* __restrict__
                     if (!in[q]) continue;
                     const unsigned z = (unsigned)(item.z0 + tz + q * item.tz);
                     if (useTile)
-                        window_quad_any<(kOS == 1 ? 1 : 2)>(p.w, tile, tw, tx0, ty0, interior, W, H, x, (unsigned)y, z,
+                        window_quad_any<(kOS == 1 ? 1 : 2)>(p.w, tile, ts, tx0, ty0, interior, W, H, x, (unsigned)y, z,
                                                            1.0f, acc[q]);
                     else
                         window_quad_any<(kOS == 1 ? 1 : 2)>(p.w, img, (int)pitch, 0, 0, false, W, H, x, (unsigned)y,

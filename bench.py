@@ -910,9 +910,15 @@ def main() -> int:
     # the profile covers the default configuration's kernels only
     default_kernels = args.numerics == "exact" and not args.no_fp_staging and not args.volume_f16
     # the driver's forward projection is the residual form of the march (compare and cropBorder in its
-    # epilogue); the backprojection runs the window form, one instance per window size
-    dom_kernel = "march_lds_kernel<false, true, float>" if dom == "fp" else "bp_win_kernel"
-    measured = measured_traffic(args.workload, dom_kernel) if default_kernels else None
+    # epilogue), the instance without the y axis for tilts about y; the backprojection runs the
+    # window form, one instance per window size
+    cands = (["march_lds_kernel<false, true, float, true>", "march_lds_kernel<false, true, float, false>"]
+             if dom == "fp" else ["bp_win_kernel"])
+    measured = None
+    for dom_kernel in cands if default_kernels else []:
+        measured = measured_traffic(args.workload, dom_kernel)
+        if measured:
+            break
     if measured:
         roofline["traffic"] = measured.pop("bytes")
         roofline["traffic_detail"] = measured
