@@ -392,13 +392,14 @@ def bench_ctf(args) -> int:
 
 
 def bench_subvol(args) -> int:
-    print(json.dumps(run_subvol(args.workload, args.steps)), flush=True)
+    print(json.dumps(run_subvol(args.workload, args.steps, plain=not args.no_plain)), flush=True)
     return 0
 
 
-def run_subvol(name: str, steps: int) -> dict:
+def run_subvol(name: str, steps: int, plain: bool = True) -> dict:
     """The batched sub-volume plan against the per-call plain schedule on one workload; returns the
-    benchmark line (value = steady-state batched Gvoxel-updates/s)."""
+    benchmark line (value = steady-state batched Gvoxel-updates/s).  plain=False runs the batched
+    plan alone (for profiling it)."""
     import numpy as np
     import torch
 
@@ -449,7 +450,8 @@ def run_subvol(name: str, steps: int) -> dict:
         f"inputs in {time.time() - t0:.1f} s")
     every = w.get("plain_every", 1)
     plain_info = {}
-    if every > 1:
+    t_plain, equal = None, None
+    if plain and every > 1:
         # plain on a tilt sample, one call per tilt; a plan over the same tilts for the bitwise check
         sample = list(range(0, T, every))
 
@@ -498,7 +500,7 @@ def run_subvol(name: str, steps: int) -> dict:
     log(f"plan + autotune in {time.time() - t0:.1f} s: policy {tu.tile_policy}, quads/thread "
         f"{tu.quads_per_thread}, blocks/CU {tu.blocks_per_cu}")
     torch.cuda.synchronize()
-    if every == 1:
+    if plain and every == 1:
         t0 = time.perf_counter()
         A.subvol_wbp_plain(ctx, subs_a, geoms, images, w["os"], **kw)
         torch.cuda.synchronize()
@@ -509,7 +511,7 @@ def run_subvol(name: str, steps: int) -> dict:
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - t0
     log(f"batched: {t_batched:.3f} s")
-    if every == 1:
+    if plain and every == 1:
         equal = all(torch.equal(a, b) for (_, a), (_, b) in zip(subs_a, subs_b))
     # further replays accumulate on top; timed for the steady-state rate
     reps = max(1, steps)
@@ -537,10 +539,11 @@ def run_subvol(name: str, steps: int) -> dict:
                    "cubes": len(sizes), "cube_sizes": n_classes, "cube_counts": counts, "oversampling": w["os"],
                    "ctf": w["ctf"], "weighting": w.get("weighting"), "numerics": "exact",
                    "parallelism": "single GPU"},
-        "plain": {"seconds": t_plain, "Gvoxel_updates_per_s": updates / t_plain / 1e9, **plain_info},
+        "plain": ({"seconds": t_plain, "Gvoxel_updates_per_s": updates / t_plain / 1e9, **plain_info}
+                  if t_plain is not None else None),
         "batched_first": {"seconds": t_batched},
-        "speedup_vs_plain": t_plain / t_rep,
-        "bitwise_equal_to_plain": bool(equal),
+        "speedup_vs_plain": t_plain / t_rep if t_plain is not None else None,
+        "bitwise_equal_to_plain": bool(equal) if equal is not None else None,
         "tuning": {"tile_policy": tu.tile_policy, "quads_per_thread": tu.quads_per_thread,
                    "blocks_per_cu": tu.blocks_per_cu, "autotuned": bool(tu.autotune)},
     }
@@ -820,6 +823,8 @@ def main() -> int:
                     help="store the SART volume as IEEE half (the reference's FP16Volume mode)")
     ap.add_argument("--no-fp-staging", action="store_true",
                     help="forward-project with direct volume reads instead of LDS-staged slabs")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="sub-volume workloads: run the batched plan alone, without the plain schedule")
     ap.add_argument("--no-extras", action="store_true",
                     help="SART line only: skip the reference-numerics iteration and the batched and adjoint "
                          "comparisons (for profiling runs)")
