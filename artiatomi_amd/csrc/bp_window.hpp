@@ -31,6 +31,9 @@
 namespace artia {
 
 constexpr float kWinBig = 1.2676506002282294e30f;   // 2^100
+// A y coordinate beyond every slab (|Y| - SY/2 stays above 1e35 for |Y| moved by any window step):
+// a row placed there has every chord 0.
+constexpr float kWinFar = 1e36f;
 
 // Per-tilt constants of the window form, prepared on the host in double.
 struct BPWin {
@@ -182,7 +185,9 @@ __device__ __forceinline__ void window_sums(const BPWin& w, const float* __restr
     // pixel of lattice point i: (i0 + di) / kOS = c0 + (ph + di) / kOS
     const int c0 = kOS == 1 ? i0 : (i0 >> 1), ph = kOS == 1 ? 0 : (i0 & 1);
     const int r0 = kOS == 1 ? j0 : (j0 >> 1), qh = kOS == 1 ? 0 : (j0 & 1);
-    const int base = (r0 - ty0) * tw + (c0 - tx0);
+    // tile rows are small: 24-bit multiplies (full rate; the 32-bit one is a quarter)
+    const int base = __mul24(r0 - ty0, tw) + (c0 - tx0);
+    const int qtw = qh ? tw : 0;
     // masked: the chord cap of each window column (m inside the footprint's columns, else 0)
     float cap[kNI];
 #pragma unroll
@@ -192,19 +197,22 @@ __device__ __forceinline__ void window_sums(const BPWin& w, const float* __restr
 #pragma unroll
     for (int dj = 0; dj < kNJ; dj++) {
         const float Pr = dj == 0 ? P0 : (dj == 1 ? P0 + w.PVd : __builtin_fmaf((float)dj, w.PVd, P0));
-        const float Yr = dj == 0 ? Y0 : (dj == 1 ? Y0 + w.YVd : __builtin_fmaf((float)dj, w.YVd, Y0));
-        const int rowOff = kOS == 1 ? dj * tw : ((dj >> 1) + ((dj & 1) ? qh : 0)) * tw;
+        float Yr = dj == 0 ? Y0 : (dj == 1 ? Y0 + w.YVd : __builtin_fmaf((float)dj, w.YVd, Y0));
+        const int rowOff = kOS == 1 ? dj * tw : (dj >> 1) * tw + ((dj & 1) ? qtw : 0);
         const int rowBase = base + rowOff;
         const int rowOdd = rowBase + ph;   // odd di of the 2x lattice: pixel c0 + ph + (di - 1) / 2
         bool rowOk = true;
         if (kMode != kWinOpen) rowOk = (unsigned)(j0 + dj - lj0) < (unsigned)(lj1 - lj0);
+        // masked: a row outside the footprint moves beyond every y slab, so its chords are 0 (the
+        // med3 of a negative value, 0 and the column cap), as a cap of 0 would make them
+        if (kMode == kWinMasked) Yr = rowOk ? Yr : kWinFar;
 #pragma unroll
         for (int di = 0; di < kNI; di++) {
             const float Pc = di == 0 ? Pr : (di == 1 ? Pr + w.PUd : __builtin_fmaf((float)di, w.PUd, Pr));
             const float Yc = di == 0 ? Yr : (di == 1 ? Yr + w.YUd : __builtin_fmaf((float)di, w.YUd, Yr));
             // min(m, P, SP - P, Y, SY - Y) clamped at 0 (cap 0: exactly 0)
             const float A = w.SPh - fabsf(Pc), Bq = w.SYh - fabsf(Yc);
-            const float mcap = kMode == kWinMasked ? (rowOk ? cap[di] : 0.0f) : w.m;
+            const float mcap = kMode == kWinMasked ? cap[di] : w.m;
             float chord = __builtin_amdgcn_fmed3f(fminf(A, Bq), 0.0f, mcap);
             int addr = kOS == 1 ? rowBase + di : ((di & 1) ? rowOdd + (di >> 1) : rowBase + (di >> 1));
             if (kMode == kWinClamp) {
