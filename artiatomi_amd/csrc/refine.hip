@@ -216,13 +216,15 @@ __global__ void __launch_bounds__(256, 5) pair_march_kernel(PairArgs a) {
 // each sub-ray's samples without overlap or gap).  The tilts' weights meet in LDS (shared-memory
 // atomics) and reach the gradient with one atomic per voxel and slab, instead of one per box voxel
 // and tube slab of every pair.  A corner outside the layers (a sample rounded across a slab face)
-// goes to global memory directly.
-constexpr int kAdjSlabFloats = 15360;   // 60 KiB of LDS: the x-y layers of a slab
+// goes to global memory directly.  The layers accumulate in double: a voxel receives the weights of
+// every sub-ray of every tilt near it (hundreds of terms of both signs for a signed image), whose
+// float sum would carry about 1e-6 of rounding and depend on the atomics' order.
+constexpr int kAdjSlabDoubles = 8192;   // 64 KiB of LDS: the x-y layers of a slab
 
 // Cell layers per slab of a sub-volume (0: its x-y layer pair does not fit).
 __host__ __device__ __forceinline__ int adj_slab_layers(int nx, int ny) {
     const long long nxy = (long long)nx * (long long)ny;
-    const long long layers = (long long)kAdjSlabFloats / (nxy > 0 ? nxy : 1);
+    const long long layers = (long long)kAdjSlabDoubles / (nxy > 0 ? nxy : 1);
     return layers >= 2 ? (int)(layers - 1) : 0;
 }
 
@@ -235,7 +237,7 @@ struct AdjSlabArgs {
 
 __global__ void __launch_bounds__(256) pair_adjoint_slab_kernel(PairArgs a, AdjSlabArgs sa, float scale,
                                                                 float* __restrict__ grads) {
-    __shared__ float box[kAdjSlabFloats];
+    __shared__ double box[kAdjSlabDoubles];
     const int pos = (int)(blockIdx.x >> 3);
     if (pos >= sa.per_xcd) return;   // whole workgroup
     const int2 it = sa.items[(int)(blockIdx.x & 7u) * sa.per_xcd + pos];
@@ -246,7 +248,7 @@ __global__ void __launch_bounds__(256) pair_adjoint_slab_kernel(PairArgs a, AdjS
     const int zs = it.y, ze = min(zs + L, v0.nz);   // cell layers [zs, ze); layer c spans q_z in [c, c + 1)
     const int bz0 = max(zs, 0), bz1 = min(ze, v0.nz - 1);
     const int nxy = v0.nx * v0.ny, nbox = nxy * (bz1 - bz0 + 1);
-    for (int e = (int)threadIdx.x; e < nbox; e += 256) box[e] = 0.0f;
+    for (int e = (int)threadIdx.x; e < nbox; e += 256) box[e] = 0.0;
     __syncthreads();
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
     const int sub = lane & 3, pix = lane >> 2;
@@ -299,14 +301,14 @@ __global__ void __launch_bounds__(256) pair_adjoint_slab_kernel(PairArgs a, AdjS
                     const int o0 = ya * v.nx + xa;
 #pragma unroll
                     for (int c = 0; c < 8; c++) {
-                        const float wc = (float)(r * W[c]);
-                        if (wc == 0.0f) continue;
+                        const double wc = r * W[c];
+                        if (wc == 0.0) continue;
                         const int zc = (c & 4) ? zb : za;
                         const int o = o0 + ((c & 1) ? dx : 0) + ((c & 2) ? dy : 0);
                         if (zc >= bz0 && zc <= bz1)
-                            atomicAdd(box + (zc - bz0) * nxy + o, wc);
+                            unsafeAtomicAdd(box + (zc - bz0) * nxy + o, wc);
                         else
-                            unsafeAtomicAdd(gv + (size_t)zc * (size_t)nxy + (size_t)o, wc);
+                            unsafeAtomicAdd(gv + (size_t)zc * (size_t)nxy + (size_t)o, (float)wc);
                     }
                 });
             }
@@ -315,7 +317,7 @@ __global__ void __launch_bounds__(256) pair_adjoint_slab_kernel(PairArgs a, AdjS
     __syncthreads();
     float* __restrict__ gv = grads + a.pairs[a.order[first]].vol_offset + (size_t)bz0 * (size_t)nxy;
     for (int e = (int)threadIdx.x; e < nbox; e += 256)
-        if (box[e] != 0.0f) unsafeAtomicAdd(gv + e, box[e]);
+        if (box[e] != 0.0) unsafeAtomicAdd(gv + e, (float)box[e]);
 }
 
 // vol_grads += scale * P_s^T in: the transpose of the per-cell closed form, one set of eight
