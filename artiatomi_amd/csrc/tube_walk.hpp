@@ -120,13 +120,13 @@ __device__ __forceinline__ TubeBox tube_box(const VolG& v, const TiltG& g, const
 template <typename T>
 __device__ __forceinline__ void chunk_corners(const VolG& v, const T* __restrict__ vol,
                                               const float* __restrict__ chunk, bool staged, int X0, int Y0, int Z0,
-                                              int CX, int CY, int CZ, int RX, const int cell[3], float val[8]) {
+                                              int CX, int CY, int CZ, const int cell[3], float val[8]) {
     const int cx = cell[0] - X0, cy = cell[1] - Y0, cz = cell[2] - Z0;
     const bool inside = staged && (unsigned)cx < (unsigned)(CX - 1) && (unsigned)cy < (unsigned)(CY - 1) &&
                         (unsigned)cz < (unsigned)(CZ - 1);
     // outside: every read at chunk[0] or chunk[1]
-    const int o = inside ? (cz * CY + cy) * RX + cx : 0;
-    const int dy = inside ? RX : 0, dz = inside ? RX * CY : 0;
+    const int o = inside ? (cz * CY + cy) * CX + cx : 0;
+    const int dy = inside ? CX : 0, dz = inside ? CX * CY : 0;
     val[0] = chunk[o];
     val[1] = chunk[o + 1];
     val[2] = chunk[o + dy];
@@ -172,8 +172,6 @@ struct BoxCursor {
             zz++;
         }
     }
-    // the entry's place in LDS, rows of RX floats
-    __device__ __forceinline__ int lds(int RX) const { return (zz * CY + yy) * RX + xx; }
     // global index of the entry (clamp-addressed: -1 and n map to the edge voxels)
     // (rows z * ny + y of the volume number below 2^31)
     __device__ __forceinline__ size_t voxel(const VolG& v, int X0, int Y0, int Z0) const {
@@ -183,28 +181,26 @@ struct BoxCursor {
     }
 };
 
-// Stages the padded box [X0, X0 + CX) x [Y0, ..) x [Z0, ..) (n = CX * CY * CZ entries) into chunk,
-// in rows of RX >= CX floats: four loads in flight per thread before their LDS writes.
+// Stages the padded box [X0, X0 + CX) x [Y0, ..) x [Z0, ..) (n = CX * CY * CZ entries) into chunk:
+// four loads in flight per thread before their LDS writes.
 template <typename T>
 __device__ __forceinline__ void stage_box(const VolG& v, const T* __restrict__ vol, float* __restrict__ chunk, int X0,
-                                          int Y0, int Z0, int CX, int CY, int RX, int n) {
+                                          int Y0, int Z0, int CX, int CY, int n) {
     int e = (int)threadIdx.x;
     if (e >= n) return;
     BoxCursor c(e, CX, CY);
     for (; e + 3 * 256 < n; e += 4 * 256) {
         float val[4];
-        int at[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             val[k] = vox(vol, c.voxel(v, X0, Y0, Z0));
-            at[k] = c.lds(RX);
             c.next();
         }
 #pragma unroll
-        for (int k = 0; k < 4; k++) chunk[at[k]] = val[k];
+        for (int k = 0; k < 4; k++) chunk[e + k * 256] = val[k];
     }
     for (; e < n; e += 256) {
-        chunk[c.lds(RX)] = vox(vol, c.voxel(v, X0, Y0, Z0));
+        chunk[e] = vox(vol, c.voxel(v, X0, Y0, Z0));
         c.next();
     }
 }
@@ -262,16 +258,14 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
         const int Y1 = min((int)floorf(tb.ayh + fmaxf(tp.by * z0, tp.by * z1)) + 2, v.ny);
         const int Z0 = max(zs, -1), Z1 = min(ze + 1, v.nz);
         const int CX = wave_uniform(X1 - X0 + 1), CY = wave_uniform(Y1 - Y0 + 1), CZ = Z1 - Z0 + 1;
-        // LDS row stride: odd, so that lanes in neighbouring rows of the box read different banks
-        const int RX = CX | 1;
-        const bool staged = CX > 1 && CY > 1 && CZ > 1 && RX * CY * CZ <= kChunkFloats;
+        const bool staged = CX > 1 && CY > 1 && CZ > 1 && CX * CY * CZ <= kChunkFloats;
         const int n = CX * CY * CZ;
         __syncthreads();   // the previous slab's reads (or flush) are done
         if (staged) {
             if (kScatter)
-                for (int e = (int)threadIdx.x; e < RX * CY * CZ; e += 256) chunk[e] = 0.0f;
+                for (int e = (int)threadIdx.x; e < n; e += 256) chunk[e] = 0.0f;
             else
-                stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, RX, n);
+                stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, n);
         }
         __syncthreads();
         // A lane whose cells in this slab provably lie in the box (its samples here run from the
@@ -289,10 +283,10 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
                    max(w.cell[1], yb) + 2 <= Y1 && zl >= Z0 && zh <= Z1;
         }
         if (fast) {
-            const int RXY = RX * CY;
-            const int st0 = td.sgn[0], st1 = td.sgn[1] * RX, st2 = td.sgn[2] * RXY;
-            const int off[8] = {0, 1, RX, RX + 1, RXY, RXY + 1, RXY + RX, RXY + RX + 1};
-            int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * RX + (w.cell[0] - X0);
+            const int CXY = CX * CY;
+            const int st0 = td.sgn[0], st1 = td.sgn[1] * CX, st2 = td.sgn[2] * CXY;
+            const int off[8] = {0, 1, CX, CX + 1, CXY, CXY + 1, CXY + CX, CXY + CX + 1};
+            int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
             // a fast lane enters with its cell in [zs, ze] and moves monotonically in z, so staying
             // in the slab is one range test
             if (kScatter) {
@@ -331,8 +325,8 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
                         const int cx = cell[0] - X0, cy = cell[1] - Y0, cz = cell[2] - Z0;
                         if (staged && (unsigned)cx < (unsigned)(CX - 1) && (unsigned)cy < (unsigned)(CY - 1) &&
                             (unsigned)cz < (unsigned)(CZ - 1)) {
-                            const int o = (cz * CY + cy) * RX + cx, RXY = RX * CY;
-                            const int off[8] = {0, 1, RX, RX + 1, RXY, RXY + 1, RXY + RX, RXY + RX + 1};
+                            const int o = (cz * CY + cy) * CX + cx, CXY = CX * CY;
+                            const int off[8] = {0, 1, CX, CX + 1, CXY, CXY + 1, CXY + CX, CXY + CX + 1};
 #pragma unroll
                             for (int i = 0; i < 8; i++)
                                 if (val[i] != 0.0f) atomicAdd(chunk + o + off[i], val[i]);
@@ -340,7 +334,7 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
                             scatter_global(cell, val);
                         }
                     } else {
-                        chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, RX, cell, val);
+                        chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
                         f(val, cell, k, mi);
                     }
                 });
@@ -351,7 +345,7 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
             if ((int)threadIdx.x < n) {
                 BoxCursor c((int)threadIdx.x, CX, CY);
                 for (int e = (int)threadIdx.x; e < n; e += 256) {
-                    const float a = chunk[c.lds(RX)];
+                    const float a = chunk[e];
                     if (a != 0.0f) unsafeAtomicAdd(grad + c.voxel(v, X0, Y0, Z0), a);
                     c.next();
                 }
@@ -418,12 +412,11 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
         const int Y1 = min((int)floorf(tb.ayh + fmaxf(tp.by * z0, tp.by * z1)) + 2, v.ny);
         const int Z0 = max(zs, -1), Z1 = min(ze + 1, v.nz);
         const int CX = wave_uniform(X1 - X0 + 1), CY = wave_uniform(Y1 - Y0 + 1), CZ = Z1 - Z0 + 1;
-        const int RX = CX | 1;   // odd LDS row stride, as in tube_slabs
-        const bool staged = !tail && CX > 1 && CY > 1 && CZ > 1 && RX * CY * CZ <= kChunkFloats;
+        const bool staged = !tail && CX > 1 && CY > 1 && CZ > 1 && CX * CY * CZ <= kChunkFloats;
         const int n = CX * CY * CZ;
         if (!tail) {
             __syncthreads();
-            if (staged) stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, RX, n);
+            if (staged) stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, n);
             __syncthreads();
         }
         for (;;) {
@@ -446,10 +439,10 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
                        max(w.cell[1], yb) + 2 <= Y1 && zl >= Z0 && zh <= Z1;
             }
             if (fast) {
-                const int RXY = RX * CY;
-                const int st0 = td.sgn[0], st1 = td.sgn[1] * RX, st2 = td.sgn[2] * RXY;
-                const int off[8] = {0, 1, RX, RX + 1, RXY, RXY + 1, RXY + RX, RXY + RX + 1};
-                int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * RX + (w.cell[0] - X0);
+                const int CXY = CX * CY;
+                const int st0 = td.sgn[0], st1 = td.sgn[1] * CX, st2 = td.sgn[2] * CXY;
+                const int off[8] = {0, 1, CX, CX + 1, CXY, CXY + 1, CXY + CX, CXY + CX + 1};
+                int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
                 while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
                     w.step(
                         [&](const int cell[3], int k, int mi) {
@@ -464,7 +457,7 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
                 while (w.active() && (tail || (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs))) {
                     w.step([&](const int cell[3], int k, int mi) {
                         float val[8];
-                        chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, RX, cell, val);
+                        chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
                         f(val, cell, k, mi);
                     });
                 }
