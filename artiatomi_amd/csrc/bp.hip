@@ -339,9 +339,10 @@ __global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __re
 // (Reconstructor.cpp:920-969).  A voxel lies in exactly one slab, so it receives the same single
 // update, in the same arithmetic, as in the per-slab launches.  For LINEAR sampling without
 // magnification anisotropy (affine footprint) the block stages its footprint of every slab its
-// quads fall in, when that fits the LDS tile, and fetches from it with tex2d's arithmetic.
+// quads fall in, when that fits the LDS tile, and fetches from it with tex2d's arithmetic.  Eight
+// waves per SIMD (61 registers, no spills) measured 5 % faster than the unbounded seven.
 template <int kPM, typename T>
-__global__ void __launch_bounds__(256) bp_stack_kernel(VolG v, TiltG g, BPExact e, T* __restrict__ vol,
+__global__ void __launch_bounds__(256, 8) bp_stack_kernel(VolG v, TiltG g, BPExact e, T* __restrict__ vol,
                                                        const float* __restrict__ stack, size_t stride, int W, int H,
                                                        size_t pitch, float lambda, int os, int filter, SlabArgs sl) {
     __shared__ float tile[kTileFloats];
