@@ -355,6 +355,7 @@ struct SlicerSegs {
     size_t pstride;
     double step, total;
     float step_f, px, py, tex;   // px, py: the sub-ray's pixel coordinates (exact in float)
+    float tnext;                 // bounds[k + 1], loaded one segment ahead of its use
     float q0[3];
     unsigned slot;
     int count;   // samples of the current slab; -1 when the ray misses it
@@ -368,7 +369,10 @@ struct SlicerSegs {
             part[(size_t)(k - kbeg) * pstride + slot] = acc;
         }
         if (++k > khi) return false;
-        const float tmin = bounds[k], tmax = tmin + step_f;
+        // the slab's start was loaded when the previous segment began; the next one's load is
+        // issued now, so that the walk of this segment covers its latency
+        const float tmin = tnext, tmax = tmin + step_f;
+        if (k < khi) tnext = bounds[k + 1];
         const double t_in = fmax(r.t_in, (double)tmin), t_last = fmin(r.t_out, (double)tmax);
         count = t_in <= t_last ? 0 : -1;
         total = 0.0;
@@ -398,7 +402,7 @@ struct SlicerSegs {
     }
 };
 
-template <typename T>
+template <typename T, bool kFlat = false>
 __global__ void __launch_bounds__(256, 6) slicer_segments_lds_kernel(VolG v, TiltG g, const T* __restrict__ vol,
                                                                   int W, int H, const float* __restrict__ bounds,
                                                                   int kbeg, int kend, float step_f,
@@ -427,6 +431,7 @@ __global__ void __launch_bounds__(256, 6) slicer_segments_lds_kernel(VolG v, Til
     seg.px = (float)x + ((sub & 2) ? 0.75f : 0.25f);
     seg.py = (float)y + ((sub & 1) ? 0.75f : 0.25f);
     seg.tex = 0.0f;
+    seg.tnext = 0.0f;
     seg.q0[0] = seg.q0[1] = seg.q0[2] = 0.0f;
     seg.slot = 4u * ((unsigned)y * (unsigned)W + (unsigned)x) + (unsigned)sub;
     seg.count = 0;
@@ -461,6 +466,7 @@ __global__ void __launch_bounds__(256, 6) slicer_segments_lds_kernel(VolG v, Til
             seg.klo = klo;
             seg.k = klo - 1;
             seg.khi = khi;
+            seg.tnext = bounds[klo];
             // cell layers of the ray between the slabs' ends (the walk adds a layer of margin)
             const double scale = (double)v.rcp[2] * (double)v.dim[2];
             const double ta = fmax(r.t_in, (double)bounds[klo]), tb = fmin(r.t_out, (double)(bounds[khi] + step_f));
@@ -473,10 +479,11 @@ __global__ void __launch_bounds__(256, 6) slicer_segments_lds_kernel(VolG v, Til
     const TubeDir td = tube_dir(v, g, step);
     const TubeBox tb = tube_box(v, g, tp, (double)tx0 + 0.25, (double)tx0 + 7.75, (double)ty0 + 0.25,
                                 (double)ty0 + 7.75, 0.0, 0.0);
-    tube_walk_segments(v, vol, chunk, zr, tp, tb, td, za, zb, seg,
-                       [&](const float val[8], const int cell[3], int kk, int mi) {
-                           seg.total += cell_sum_values(val, cell_moments(seg.q0, td.dq, cell, kk, mi), mi);
-                       });
+    tube_walk_segments<kFlat>(v, vol, chunk, zr, tp, tb, td, za, zb, seg,
+                              [&](const float val[8], const int cell[3], int kk, int mi) {
+                                  seg.total += cell_sum_values<kFlat>(
+                                      val, cell_moments<kFlat>(seg.q0, td.dq, cell, kk, mi), mi);
+                              });
 }
 
 // Slab images from the sub-ray sums: stack[k * stride + y * pitch + x] = 0 + the four sub-rays'
@@ -681,15 +688,25 @@ void launch_slicer_stack(artia_ctx* ctx, const VolG& v, const TiltG& g_in, VolDa
         int2* range = static_cast<int2*>(scratch);
         double* part = reinterpret_cast<double*>(static_cast<char*>(scratch) + range_bytes);
         const double dstep = (double)v.vs[0] * 0.15;
+        // rays without a y component: the walk without the y axis (the same cells and sums)
+        const bool flat = g.n[1] == 0.0f;
         for (int kb = 0; kb < nslab; kb += per_pass) {
             const int ke = std::min(nslab, kb + per_pass);
-            if (vol.half)
-                hipLaunchKernelGGL(slicer_segments_lds_kernel<vhalf>, grid1, dim3(256), 0, ctx->stream, v, g,
+            if (vol.half && flat)
+                hipLaunchKernelGGL((slicer_segments_lds_kernel<vhalf, true>), grid1, dim3(256), 0, ctx->stream, v, g,
                                    static_cast<const vhalf*>(vol.p), W, H, d_bounds, kb, ke, step, part, npix * 4,
                                    range, tp);
-            else
-                hipLaunchKernelGGL(slicer_segments_lds_kernel<float>, grid1, dim3(256), 0, ctx->stream, v, g,
+            else if (vol.half)
+                hipLaunchKernelGGL((slicer_segments_lds_kernel<vhalf, false>), grid1, dim3(256), 0, ctx->stream, v,
+                                   g, static_cast<const vhalf*>(vol.p), W, H, d_bounds, kb, ke, step, part, npix * 4,
+                                   range, tp);
+            else if (flat)
+                hipLaunchKernelGGL((slicer_segments_lds_kernel<float, true>), grid1, dim3(256), 0, ctx->stream, v, g,
                                    static_cast<const float*>(vol.p), W, H, d_bounds, kb, ke, step, part, npix * 4,
+                                   range, tp);
+            else
+                hipLaunchKernelGGL((slicer_segments_lds_kernel<float, false>), grid1, dim3(256), 0, ctx->stream, v,
+                                   g, static_cast<const float*>(vol.p), W, H, d_bounds, kb, ke, step, part, npix * 4,
                                    range, tp);
             hipLaunchKernelGGL(slicer_combine_kernel, dim3((unsigned)((W + 255) / 256), (unsigned)H), dim3(256), 0,
                                ctx->stream, part, npix * 4, range, W, H, pitch, stack, stride, kb, ke, dstep);

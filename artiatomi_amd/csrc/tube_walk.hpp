@@ -377,8 +377,8 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
 // each other along the rays, so one pass over the slabs of cell layers serves all of them; za..zb
 // are cell layers bounding every sample of the lane (za > zb: none).  Each segment is walked with
 // tube_slabs' CellWalker steps, so its cells, their order and their values are those of a
-// tube_walk of that segment alone.  Must be called by all 256 threads of the block.
-template <typename T, class Seg, class F>
+// tube_walk of that segment alone (kFlat: the walk without the y axis, as tube_walk<true>).  Must be called by all 256 threads of the block.
+template <bool kFlat, typename T, class Seg, class F>
 __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __restrict__ vol,
                                                    float* __restrict__ chunk, int* zr, const TubePlan& tp,
                                                    const TubeBox& tb, const TubeDir& td, int za, int zb, Seg& seg,
@@ -396,7 +396,7 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
     __syncthreads();
     const int zlo = max(zr[0], -1), zhi = min(zr[1], v.nz - 1);
 
-    CellWalker w;
+    CellWalkerT<kFlat> w;
     w.init_dir(seg.q0, td.sgn, td.inv, 0);   // no segment yet: the first pass below loads one
     bool more = true;
     const int nslab = zlo <= zhi ? (zhi - zlo) / kSlabCells + 1 : 0;
@@ -443,8 +443,10 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
                 const int st0 = td.sgn[0], st1 = td.sgn[1] * CX, st2 = td.sgn[2] * CXY;
                 const int off[8] = {0, 1, CX, CX + 1, CXY, CXY + 1, CXY + CX, CXY + CX + 1};
                 int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
+                // every step reads its cell's corners and calls f, also for a cell without
+                // samples (f adds +-0 then), so the loop body has no branch
                 while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
-                    w.step(
+                    w.step_all(
                         [&](const int cell[3], int k, int mi) {
                             float val[8];
 #pragma unroll
