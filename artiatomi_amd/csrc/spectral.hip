@@ -111,7 +111,10 @@ __device__ __forceinline__ float2 ctf_apply_at(float2 r, const CtfPoint& p, cons
     const double defocus = def0 + (1 - p.c2b) * (def1 - def0);
     const double lam = k.lambda, len = p.len;
     const double m = (double)(kPiF / 2.0f) * (p.lam4 - 2 * defocus * lam * len * len);
-    const double n = (double)k.pc * sin(m) + (double)k.ac * cos(m);
+    // sin and cos from one argument reduction (the same values as the two calls)
+    double sm, cm;
+    sincos(m, &sm, &cm);
+    const double n = (double)k.pc * sm + (double)k.ac * cm;
     double f = 1.0;
     if (p.damped) {
         if (k.forFP) {
