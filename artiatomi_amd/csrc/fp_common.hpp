@@ -4,6 +4,8 @@
 // (ForwardProjectionRayMarcher_TL.cu:57-160 evaluated stably).
 #pragma once
 
+#include <type_traits>
+
 #include <cmath>
 #include <cstdint>
 
@@ -153,12 +155,16 @@ __device__ __forceinline__ void walk_dir(float dq, int& sgn, float& inv) {
     inv = ok ? r : 0.0f;
 }
 
+// The walk without the y axis counts samples in float (k, and the m passed to f): the counts are
+// integers below 2^24, exact in float, and the cell moments take them without conversions.
 template <bool kFlat>
 struct CellWalkerT {
+    using Index = typename std::conditional<kFlat, float, int>::type;
     int cell[3], sgn[3];
     float inv[3], off[3], kb[3];
     float fcount;
-    int k, count, guard;
+    Index k;
+    int count, guard;
 
     __device__ __forceinline__ void init(const float q0[3], const float dq[3], int n) {
 #pragma unroll
@@ -170,7 +176,7 @@ struct CellWalkerT {
         }
         count = n;
         fcount = (float)n;
-        k = 0;
+        k = (Index)0;
         guard = 0;
     }
     // The same walk with the direction part (sgn, inv) precomputed: when every lane walks parallel
@@ -186,7 +192,7 @@ struct CellWalkerT {
         }
         count = n;
         fcount = (float)n;
-        k = 0;
+        k = (Index)0;
         guard = 0;
     }
     // The walk of samples k0 .. n-1 of the same sub-ray, starting in the cell of sample k0 (crossing
@@ -203,10 +209,12 @@ struct CellWalkerT {
         }
         count = n;
         fcount = (float)n;
-        k = k0;
+        k = (Index)k0;
         guard = 0;
     }
-    __device__ __forceinline__ bool active() const { return k < count && guard < 4 * count + 64; }
+    __device__ __forceinline__ bool active() const {
+        return (kFlat ? (float)k < fcount : (int)k < count) && guard < 4 * count + 64;
+    }
     // One step; moved(i) is called for every axis i the walk advances along.  The next crossing kn
     // is the least of the axes' crossing indices (never NaN: finite, or INFINITY on an axis without
     // crossings), found with compares, and every axis at kn advances.  The samples in the cell run
@@ -237,9 +245,9 @@ struct CellWalkerT {
 #pragma unroll
             for (int i = 0; i < 3; i++) adv[i] = kb[i] == kn;
         }
-        const int kend = (int)__builtin_amdgcn_fmed3f(ceilf(kn), 0.0f, fcount);
+        const Index kend = (Index)__builtin_amdgcn_fmed3f(ceilf(kn), 0.0f, fcount);
         if (kAll) {
-            const int mi = max(kend - k, 0);
+            const Index mi = kend - k > (Index)0 ? kend - k : (Index)0;
             f(cell, k, mi);
             k += mi;
         } else if (kend > k) {
@@ -286,11 +294,15 @@ struct CellMoments {
 
 // kFlat (dq[1] == 0): b is the same for every sample, da db = db dc = 0 and q = b da dc; the
 // values equal the general form's (fma(kc, 0, q0) = q0, fma(x, 0, y) = y for finite operands).
-template <bool kFlat = false>
-__device__ __forceinline__ CellMoments cell_moments(const float q0[3], const float dq[3], const int cell[3], int k,
-                                                    int mi) {
+// k and mi are ints, or (the walk without the y axis) floats holding the same integers.
+template <bool kFlat = false, typename K>
+__device__ __forceinline__ CellMoments cell_moments(const float q0[3], const float dq[3], const int cell[3], K k,
+                                                    K mi) {
     CellMoments cm;
-    const float kc = (float)(k + (k + mi) - 1) * 0.5f;   // k + (m - 1) / 2, exact
+    // k + (m - 1) / 2, exact: integer sums below 2^24 halved; in float as m / 2 + (k - 1/2), whose
+    // half-integer terms and sum are exact too
+    const float kc = std::is_same<K, float>::value ? __builtin_fmaf((float)mi, 0.5f, (float)k - 0.5f)
+                                                   : (float)(k + (k + mi) - 1) * 0.5f;
     cm.a = __builtin_fmaf(kc, dq[0], q0[0]) - (float)cell[0];
     cm.b = (kFlat ? q0[1] : __builtin_fmaf(kc, dq[1], q0[1])) - (float)cell[1];
     cm.c = __builtin_fmaf(kc, dq[2], q0[2]) - (float)cell[2];

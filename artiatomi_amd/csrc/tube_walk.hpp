@@ -292,7 +292,7 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
             if (kScatter) {
                 while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
                     w.step(
-                        [&](const int cell[3], int k, int mi) {
+                        [&](const int cell[3], auto k, auto mi) {
                             float val[8];
                             f(val, cell, k, mi);
 #pragma unroll
@@ -305,20 +305,37 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
                 // every step reads its cell's corners and calls f, also for a cell without samples
                 // (f adds +-0 then: the gather callbacks add cell and gradient sums), so the loop
                 // body has no branch
-                while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
-                    w.step_all(
-                        [&](const int cell[3], int k, int mi) {
-                            float val[8];
+                if (kFlat) {
+                    // y is fixed: the box offset is one 24-bit multiply-add of the x and z indices
+                    // per step instead of a select and add per axis
+                    const int K = (w.cell[1] - Y0) * CX - X0 - Z0 * CXY;
+                    while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
+                        w.step_all(
+                            [&](const int cell[3], auto k, auto mi) {
+                                const int oc = __mul24(cell[2], CXY) + (cell[0] + K);
+                                float val[8];
 #pragma unroll
-                            for (int i = 0; i < 8; i++) val[i] = chunk[o + off[i]];
-                            f(val, cell, k, mi);
-                        },
-                        [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
+                                for (int i = 0; i < 8; i++) val[i] = chunk[oc + off[i]];
+                                f(val, cell, k, mi);
+                            },
+                            [](int) {});
+                    }
+                } else {
+                    while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
+                        w.step_all(
+                            [&](const int cell[3], auto k, auto mi) {
+                                float val[8];
+#pragma unroll
+                                for (int i = 0; i < 8; i++) val[i] = chunk[o + off[i]];
+                                f(val, cell, k, mi);
+                            },
+                            [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
+                    }
                 }
             }
         } else {
             while (w.active() && (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
-                w.step([&](const int cell[3], int k, int mi) {
+                w.step([&](const int cell[3], auto k, auto mi) {
                     float val[8];
                     if (kScatter) {
                         f(val, cell, k, mi);
@@ -354,7 +371,7 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
     }
     // cells beyond the slabs (not expected): global memory
     while (w.active()) {
-        w.step([&](const int cell[3], int k, int mi) {
+        w.step([&](const int cell[3], auto k, auto mi) {
             float val[8];
             if (kScatter) {
                 f(val, cell, k, mi);
@@ -377,7 +394,8 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
 // each other along the rays, so one pass over the slabs of cell layers serves all of them; za..zb
 // are cell layers bounding every sample of the lane (za > zb: none).  Each segment is walked with
 // tube_slabs' CellWalker steps, so its cells, their order and their values are those of a
-// tube_walk of that segment alone (kFlat: the walk without the y axis, as tube_walk<true>).  Must be called by all 256 threads of the block.
+// tube_walk of that segment alone (kFlat: the walk without the y axis, as tube_walk<true>).  Must
+// be called by all 256 threads of the block.
 template <bool kFlat, typename T, class Seg, class F>
 __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __restrict__ vol,
                                                    float* __restrict__ chunk, int* zr, const TubePlan& tp,
@@ -445,19 +463,35 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
                 int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
                 // every step reads its cell's corners and calls f, also for a cell without
                 // samples (f adds +-0 then), so the loop body has no branch
-                while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
-                    w.step_all(
-                        [&](const int cell[3], int k, int mi) {
-                            float val[8];
+                if (kFlat) {
+                    // y is fixed: the box offset is one 24-bit multiply-add of the x and z indices
+                    const int K = (w.cell[1] - Y0) * CX - X0 - Z0 * CXY;
+                    while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
+                        w.step_all(
+                            [&](const int cell[3], auto k, auto mi) {
+                                const int oc = __mul24(cell[2], CXY) + (cell[0] + K);
+                                float val[8];
 #pragma unroll
-                            for (int i = 0; i < 8; i++) val[i] = chunk[o + off[i]];
-                            f(val, cell, k, mi);
-                        },
-                        [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
+                                for (int i = 0; i < 8; i++) val[i] = chunk[oc + off[i]];
+                                f(val, cell, k, mi);
+                            },
+                            [](int) {});
+                    }
+                } else {
+                    while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
+                        w.step_all(
+                            [&](const int cell[3], auto k, auto mi) {
+                                float val[8];
+#pragma unroll
+                                for (int i = 0; i < 8; i++) val[i] = chunk[o + off[i]];
+                                f(val, cell, k, mi);
+                            },
+                            [&](int i) { o += i == 0 ? st0 : (i == 1 ? st1 : st2); });
+                    }
                 }
             } else {
                 while (w.active() && (tail || (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs))) {
-                    w.step([&](const int cell[3], int k, int mi) {
+                    w.step([&](const int cell[3], auto k, auto mi) {
                         float val[8];
                         chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
                         f(val, cell, k, mi);
