@@ -120,9 +120,10 @@ __device__ __forceinline__ double march_sum(const VolG& v, const T* __restrict__
                                             const float dq[3], int count) {
     double total = 0.0;
     if (kBig) {
-        for_each_cell<kFlat>(q0, dq, count, [&](const int cell[3], auto k, auto mi) {
+        for_each_cell<kFlat>(q0, dq, count, [&](const auto* cell, auto k, auto mi) {
+            const int ic[3] = {(int)cell[0], (int)cell[1], (int)cell[2]};
             size_t o[8];
-            cell_corners<true>(v, cell, o);
+            cell_corners<true>(v, ic, o);
             float val[8];
 #pragma unroll
             for (int i = 0; i < 8; i++) val[i] = vox(vol, o[i]);
@@ -130,9 +131,10 @@ __device__ __forceinline__ double march_sum(const VolG& v, const T* __restrict__
         });
     } else {
         const __amdgpu_buffer_rsrc_t rs = volume_rsrc(vol, v);
-        for_each_cell<kFlat>(q0, dq, count, [&](const int cell[3], auto k, auto mi) {
+        for_each_cell<kFlat>(q0, dq, count, [&](const auto* cell, auto k, auto mi) {
+            const int ic[3] = {(int)cell[0], (int)cell[1], (int)cell[2]};
             uint32_t o[8];
-            cell_corners_bytes<(uint32_t)sizeof(T)>(v, cell, o);
+            cell_corners_bytes<(uint32_t)sizeof(T)>(v, ic, o);
             float val[8];
 #pragma unroll
             for (int i = 0; i < 8; i++) val[i] = buffer_vox(rs, o[i], vol);
@@ -328,7 +330,7 @@ __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, cons
                                 (double)ty0 + 7.75, 0.0, 0.0);
     double total = 0.0;   // summed as march_sum does, then added to the slicer's first sample
     tube_walk<kFlat>(v, vol, chunk, zr, tp, tb, td, L.q0, L.count,
-                     [&](const float val[8], const int cell[3], auto k, auto mi) {
+                     [&](const float val[8], const auto* cell, auto k, auto mi) {
                          total += cell_sum_values<kFlat>(val, cell_moments<kFlat>(L.q0, td.dq, cell, k, mi), mi);
                      });
     double acc = L.acc;
@@ -480,7 +482,7 @@ __global__ void __launch_bounds__(256, 6) slicer_segments_lds_kernel(VolG v, Til
     const TubeBox tb = tube_box(v, g, tp, (double)tx0 + 0.25, (double)tx0 + 7.75, (double)ty0 + 0.25,
                                 (double)ty0 + 7.75, 0.0, 0.0);
     tube_walk_segments<kFlat>(v, vol, chunk, zr, tp, tb, td, za, zb, seg,
-                              [&](const float val[8], const int cell[3], auto kk, auto mi) {
+                              [&](const float val[8], const auto* cell, auto kk, auto mi) {
                                   seg.total += cell_sum_values<kFlat>(
                                       val, cell_moments<kFlat>(seg.q0, td.dq, cell, kk, mi), mi);
                               });

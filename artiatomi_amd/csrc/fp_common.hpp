@@ -155,24 +155,34 @@ __device__ __forceinline__ void walk_dir(float dq, int& sgn, float& inv) {
     inv = ok ? r : 0.0f;
 }
 
-// The walk without the y axis counts samples in float (k, and the m passed to f): the counts are
-// integers below 2^24, exact in float, and the cell moments take them without conversions.
+// The walk without the y axis keeps its cell indices and sample counts in float (cell, k, and the m
+// passed to f): integers below 2^24, exact in float, which the cell moments and the crossing
+// indices take without conversions.
 template <bool kFlat>
 struct CellWalkerT {
     using Index = typename std::conditional<kFlat, float, int>::type;
-    int cell[3], sgn[3];
+    Index cell[3];
+    int sgn[3];
     float inv[3], off[3], kb[3];
     float fcount;
     Index k;
     int count, guard;
 
+    __device__ __forceinline__ int icell(int i) const { return (int)cell[i]; }
+    // whether the cell's z index lies in [zs, zs + n)
+    __device__ __forceinline__ bool z_in(int zs, int n) const {
+        if (kFlat) return cell[2] >= (Index)zs && cell[2] <= (Index)(zs + n - 1);
+        return (unsigned)((int)cell[2] - zs) <= (unsigned)(n - 1);
+    }
+
     __device__ __forceinline__ void init(const float q0[3], const float dq[3], int n) {
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            cell[i] = (int)floorf(q0[i]);
+            cell[i] = (Index)floorf(q0[i]);
             walk_dir(dq[i], sgn[i], inv[i]);
             off[i] = -q0[i] * inv[i];
-            kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]) : INFINITY;
+            kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (Index)(sgn[i] > 0 ? 1 : 0)), inv[i], off[i])
+                                : INFINITY;
         }
         count = n;
         fcount = (float)n;
@@ -184,11 +194,12 @@ struct CellWalkerT {
     __device__ __forceinline__ void init_dir(const float q0[3], const int sgn_u[3], const float inv_u[3], int n) {
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            cell[i] = (int)floorf(q0[i]);
+            cell[i] = (Index)floorf(q0[i]);
             sgn[i] = sgn_u[i];
             inv[i] = inv_u[i];
             off[i] = -q0[i] * inv[i];
-            kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]) : INFINITY;
+            kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (Index)(sgn[i] > 0 ? 1 : 0)), inv[i], off[i])
+                                : INFINITY;
         }
         count = n;
         fcount = (float)n;
@@ -201,11 +212,12 @@ struct CellWalkerT {
                                               const float inv_u[3], int k0, int n) {
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            cell[i] = (int)floorf(__builtin_fmaf((float)k0, dq[i], q0[i]));
+            cell[i] = (Index)floorf(__builtin_fmaf((float)k0, dq[i], q0[i]));
             sgn[i] = sgn_u[i];
             inv[i] = inv_u[i];
             off[i] = -q0[i] * inv[i];
-            kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]) : INFINITY;
+            kb[i] = sgn[i] != 0 ? __builtin_fmaf((float)(cell[i] + (Index)(sgn[i] > 0 ? 1 : 0)), inv[i], off[i])
+                                : INFINITY;
         }
         count = n;
         fcount = (float)n;
@@ -258,8 +270,8 @@ struct CellWalkerT {
         for (int i = 0; i < 3; i++) {
             if (kFlat && i == 1) continue;
             if (adv[i]) {
-                cell[i] += sgn[i];
-                kb[i] = __builtin_fmaf((float)(cell[i] + (sgn[i] > 0 ? 1 : 0)), inv[i], off[i]);
+                cell[i] += (Index)sgn[i];
+                kb[i] = __builtin_fmaf((float)(cell[i] + (Index)(sgn[i] > 0 ? 1 : 0)), inv[i], off[i]);
                 moved(i);
             }
         }
@@ -294,10 +306,9 @@ struct CellMoments {
 
 // kFlat (dq[1] == 0): b is the same for every sample, da db = db dc = 0 and q = b da dc; the
 // values equal the general form's (fma(kc, 0, q0) = q0, fma(x, 0, y) = y for finite operands).
-// k and mi are ints, or (the walk without the y axis) floats holding the same integers.
-template <bool kFlat = false, typename K>
-__device__ __forceinline__ CellMoments cell_moments(const float q0[3], const float dq[3], const int cell[3], K k,
-                                                    K mi) {
+// cell, k and mi are ints, or (the walk without the y axis) floats holding the same integers.
+template <bool kFlat = false, typename C, typename K>
+__device__ __forceinline__ CellMoments cell_moments(const float q0[3], const float dq[3], const C* cell, K k, K mi) {
     CellMoments cm;
     // k + (m - 1) / 2, exact: integer sums below 2^24 halved; in float as m / 2 + (k - 1/2), whose
     // half-integer terms and sum are exact too

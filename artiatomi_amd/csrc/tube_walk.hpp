@@ -36,7 +36,9 @@ struct TubePlan {
 // (steep rays).  The box of an 8x8-pixel tile spans the tile's extent in a_x = Q_x - bx Q_z
 // (affine in the pixel) plus the rays' drift over a slab.
 inline bool tube_plan(const VolG& v, const TiltG& g, TubePlan& tp) {
-    if (v.nx < 2 || v.ny < 2 || v.nz < 2) return false;
+    // (nz bounded so that the walk's float box offsets stay exact: |cell z * CXY| < 2^24, CXY being
+    // at most kChunkFloats / 2)
+    if (v.nx < 2 || v.ny < 2 || v.nz < 2 || (long long)(v.nz + 2) * (kChunkFloats / 2) >= (1LL << 24)) return false;
     double sc[3], N[3], dpx[3], dpy[3];
     for (int i = 0; i < 3; i++) {
         sc[i] = (double)v.rcp[i] * (double)v.dim[i];
@@ -214,6 +216,16 @@ __device__ __forceinline__ void stage_box(const VolG& v, const T* __restrict__ v
 //   scatter (kScatter true): f(float wgt[8], cell, k, m) returns eight corner weights, which are
 //     added to the slab's box in LDS (shared-memory atomics) and then to grad with one global
 //     atomic per box voxel, instead of eight global atomics per cell.
+// Box offset cell[2] * CXY + cell[0] + K of a cell of the walk without the y axis: float cells
+// (integers; tube_plan keeps |cell[2] * CXY| below 2^24, so every term and sum is exact) in one fma,
+// int cells in one 24-bit multiply-add.
+__device__ __forceinline__ int flat_box_offset(const float* cell, int CXY, int K) {
+    return (int)__builtin_fmaf(cell[2], (float)CXY, cell[0] + (float)K);
+}
+__device__ __forceinline__ int flat_box_offset(const int* cell, int CXY, int K) {
+    return __mul24(cell[2], CXY) + (cell[0] + K);
+}
+
 template <bool kScatter, bool kFlat, typename T, class F>
 __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ vol, float* __restrict__ grad,
                                            float* __restrict__ chunk, int* zr, const TubePlan& tp, const TubeBox& tb,
@@ -277,22 +289,22 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
             const float ke = fminf(fmaxf((zp - q0[2]) * td.inv[2], (float)w.k), (float)(w.count - 1));
             const int xb = (int)floorf(__builtin_fmaf(ke, td.dq[0], q0[0]));
             const int yb = (int)floorf(__builtin_fmaf(ke, td.dq[1], q0[1]));
-            const int zl = tp.zdir > 0 ? w.cell[2] : max(zs, -1);
-            const int zh = tp.zdir > 0 ? min(ze + 1, v.nz) : w.cell[2] + 1;
-            fast = min(w.cell[0], xb) - 1 >= X0 && max(w.cell[0], xb) + 2 <= X1 && min(w.cell[1], yb) - 1 >= Y0 &&
-                   max(w.cell[1], yb) + 2 <= Y1 && zl >= Z0 && zh <= Z1;
+            const int zl = tp.zdir > 0 ? w.icell(2) : max(zs, -1);
+            const int zh = tp.zdir > 0 ? min(ze + 1, v.nz) : w.icell(2) + 1;
+            fast = min(w.icell(0), xb) - 1 >= X0 && max(w.icell(0), xb) + 2 <= X1 && min(w.icell(1), yb) - 1 >= Y0 &&
+                   max(w.icell(1), yb) + 2 <= Y1 && zl >= Z0 && zh <= Z1;
         }
         if (fast) {
             const int CXY = CX * CY;
             const int st0 = td.sgn[0], st1 = td.sgn[1] * CX, st2 = td.sgn[2] * CXY;
             const int off[8] = {0, 1, CX, CX + 1, CXY, CXY + 1, CXY + CX, CXY + CX + 1};
-            int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
+            int o = ((w.icell(2) - Z0) * CY + (w.icell(1) - Y0)) * CX + (w.icell(0) - X0);
             // a fast lane enters with its cell in [zs, ze] and moves monotonically in z, so staying
             // in the slab is one range test
             if (kScatter) {
-                while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
+                while (w.active() && w.z_in(zs, kSlabCells)) {
                     w.step(
-                        [&](const int cell[3], auto k, auto mi) {
+                        [&](const auto* cell, auto k, auto mi) {
                             float val[8];
                             f(val, cell, k, mi);
 #pragma unroll
@@ -308,11 +320,11 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
                 if (kFlat) {
                     // y is fixed: the box offset is one 24-bit multiply-add of the x and z indices
                     // per step instead of a select and add per axis
-                    const int K = (w.cell[1] - Y0) * CX - X0 - Z0 * CXY;
-                    while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
+                    const int K = (w.icell(1) - Y0) * CX - X0 - Z0 * CXY;
+                    while (w.active() && w.z_in(zs, kSlabCells)) {
                         w.step_all(
-                            [&](const int cell[3], auto k, auto mi) {
-                                const int oc = __mul24(cell[2], CXY) + (cell[0] + K);
+                            [&](const auto* cell, auto k, auto mi) {
+                                const int oc = flat_box_offset(cell, CXY, K);
                                 float val[8];
 #pragma unroll
                                 for (int i = 0; i < 8; i++) val[i] = chunk[oc + off[i]];
@@ -321,9 +333,9 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
                             [](int) {});
                     }
                 } else {
-                    while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
+                    while (w.active() && w.z_in(zs, kSlabCells)) {
                         w.step_all(
-                            [&](const int cell[3], auto k, auto mi) {
+                            [&](const auto* cell, auto k, auto mi) {
                                 float val[8];
 #pragma unroll
                                 for (int i = 0; i < 8; i++) val[i] = chunk[o + off[i]];
@@ -334,12 +346,13 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
                 }
             }
         } else {
-            while (w.active() && (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs)) {
-                w.step([&](const int cell[3], auto k, auto mi) {
+            while (w.active() && (tp.zdir > 0 ? w.icell(2) <= ze : w.icell(2) >= zs)) {
+                w.step([&](const auto* cell, auto k, auto mi) {
                     float val[8];
                     if (kScatter) {
                         f(val, cell, k, mi);
-                        const int cx = cell[0] - X0, cy = cell[1] - Y0, cz = cell[2] - Z0;
+                        const int ic[3] = {(int)cell[0], (int)cell[1], (int)cell[2]};
+                        const int cx = ic[0] - X0, cy = ic[1] - Y0, cz = ic[2] - Z0;
                         if (staged && (unsigned)cx < (unsigned)(CX - 1) && (unsigned)cy < (unsigned)(CY - 1) &&
                             (unsigned)cz < (unsigned)(CZ - 1)) {
                             const int o = (cz * CY + cy) * CX + cx, CXY = CX * CY;
@@ -348,10 +361,11 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
                             for (int i = 0; i < 8; i++)
                                 if (val[i] != 0.0f) atomicAdd(chunk + o + off[i], val[i]);
                         } else {
-                            scatter_global(cell, val);
+                            scatter_global(ic, val);
                         }
                     } else {
-                        chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
+                        const int ic[3] = {(int)cell[0], (int)cell[1], (int)cell[2]};
+                        chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, ic, val);
                         f(val, cell, k, mi);
                     }
                 });
@@ -371,14 +385,15 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
     }
     // cells beyond the slabs (not expected): global memory
     while (w.active()) {
-        w.step([&](const int cell[3], auto k, auto mi) {
+        w.step([&](const auto* cell, auto k, auto mi) {
             float val[8];
+            const int ic[3] = {(int)cell[0], (int)cell[1], (int)cell[2]};
             if (kScatter) {
                 f(val, cell, k, mi);
-                scatter_global(cell, val);
+                scatter_global(ic, val);
             } else {
                 size_t o[8];
-                cell_corners<true>(v, cell, o);
+                cell_corners<true>(v, ic, o);
 #pragma unroll
                 for (int i = 0; i < 8; i++) val[i] = vox(vol, o[i]);
                 f(val, cell, k, mi);
@@ -444,32 +459,32 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
                 w.init_dir(seg.q0, td.sgn, td.inv, more ? seg.count : 0);
                 continue;
             }
-            if (!tail && (tp.zdir > 0 ? w.cell[2] > ze : w.cell[2] < zs)) break;   // resumes in a later slab
+            if (!tail && (tp.zdir > 0 ? w.icell(2) > ze : w.icell(2) < zs)) break;   // resumes in a later slab
             bool fast = false;
             if (staged) {
                 const float zp = tp.zdir > 0 ? (float)(ze + 1) : (float)zs;
                 const float ke = fminf(fmaxf((zp - seg.q0[2]) * td.inv[2], (float)w.k), (float)(w.count - 1));
                 const int xb = (int)floorf(__builtin_fmaf(ke, td.dq[0], seg.q0[0]));
                 const int yb = (int)floorf(__builtin_fmaf(ke, td.dq[1], seg.q0[1]));
-                const int zl = tp.zdir > 0 ? w.cell[2] : max(zs, -1);
-                const int zh = tp.zdir > 0 ? min(ze + 1, v.nz) : w.cell[2] + 1;
-                fast = min(w.cell[0], xb) - 1 >= X0 && max(w.cell[0], xb) + 2 <= X1 && min(w.cell[1], yb) - 1 >= Y0 &&
-                       max(w.cell[1], yb) + 2 <= Y1 && zl >= Z0 && zh <= Z1;
+                const int zl = tp.zdir > 0 ? w.icell(2) : max(zs, -1);
+                const int zh = tp.zdir > 0 ? min(ze + 1, v.nz) : w.icell(2) + 1;
+                fast = min(w.icell(0), xb) - 1 >= X0 && max(w.icell(0), xb) + 2 <= X1 && min(w.icell(1), yb) - 1 >= Y0 &&
+                       max(w.icell(1), yb) + 2 <= Y1 && zl >= Z0 && zh <= Z1;
             }
             if (fast) {
                 const int CXY = CX * CY;
                 const int st0 = td.sgn[0], st1 = td.sgn[1] * CX, st2 = td.sgn[2] * CXY;
                 const int off[8] = {0, 1, CX, CX + 1, CXY, CXY + 1, CXY + CX, CXY + CX + 1};
-                int o = ((w.cell[2] - Z0) * CY + (w.cell[1] - Y0)) * CX + (w.cell[0] - X0);
+                int o = ((w.icell(2) - Z0) * CY + (w.icell(1) - Y0)) * CX + (w.icell(0) - X0);
                 // every step reads its cell's corners and calls f, also for a cell without
                 // samples (f adds +-0 then), so the loop body has no branch
                 if (kFlat) {
                     // y is fixed: the box offset is one 24-bit multiply-add of the x and z indices
-                    const int K = (w.cell[1] - Y0) * CX - X0 - Z0 * CXY;
-                    while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
+                    const int K = (w.icell(1) - Y0) * CX - X0 - Z0 * CXY;
+                    while (w.active() && w.z_in(zs, kSlabCells)) {
                         w.step_all(
-                            [&](const int cell[3], auto k, auto mi) {
-                                const int oc = __mul24(cell[2], CXY) + (cell[0] + K);
+                            [&](const auto* cell, auto k, auto mi) {
+                                const int oc = flat_box_offset(cell, CXY, K);
                                 float val[8];
 #pragma unroll
                                 for (int i = 0; i < 8; i++) val[i] = chunk[oc + off[i]];
@@ -478,9 +493,9 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
                             [](int) {});
                     }
                 } else {
-                    while (w.active() && (unsigned)(w.cell[2] - zs) <= (unsigned)(kSlabCells - 1)) {
+                    while (w.active() && w.z_in(zs, kSlabCells)) {
                         w.step_all(
-                            [&](const int cell[3], auto k, auto mi) {
+                            [&](const auto* cell, auto k, auto mi) {
                                 float val[8];
 #pragma unroll
                                 for (int i = 0; i < 8; i++) val[i] = chunk[o + off[i]];
@@ -490,10 +505,11 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
                     }
                 }
             } else {
-                while (w.active() && (tail || (tp.zdir > 0 ? w.cell[2] <= ze : w.cell[2] >= zs))) {
-                    w.step([&](const int cell[3], auto k, auto mi) {
+                while (w.active() && (tail || (tp.zdir > 0 ? w.icell(2) <= ze : w.icell(2) >= zs))) {
+                    w.step([&](const auto* cell, auto k, auto mi) {
                         float val[8];
-                        chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, cell, val);
+                        const int ic[3] = {(int)cell[0], (int)cell[1], (int)cell[2]};
+                        chunk_corners(v, vol, chunk, staged, X0, Y0, Z0, CX, CY, CZ, ic, val);
                         f(val, cell, k, mi);
                     });
                 }
