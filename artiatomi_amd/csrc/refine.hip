@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(256, 5) pair_march_kernel(PairArgs a) {
             for (int c = 0; c < 3; c++) s.q0[c] = s.dq[c] = 0.0f;
         }
         double G[3] = {0.0, 0.0, 0.0};
-        auto cell_fn = [&](const float val[8], const int cell[3], int k, int mi) {
+        auto cell_fn = [&](const float val[8], const auto* cell, auto k, auto mi) {
             const CellMoments cm = cell_moments(s.q0, s.dq, cell, k, mi);
             if (kMode != kShiftGrad) acc += cell_sum_values(val, cm, mi);
             if (kMode != kProject) cell_grad_sum(val, cm, mi, G);
@@ -136,7 +136,12 @@ __global__ void __launch_bounds__(256, 5) pair_march_kernel(PairArgs a) {
             const TubeDir td = tube_dir(v, g, step);
             const double pi0 = (double)(x0 + (t % ntx) * 8), pj0 = (double)(y0 + (t / ntx) * 8);
             const TubeBox tb = tube_box(v, g, P.tp, pi0 + 0.25, pi0 + 7.75, pj0 + 0.25, pj0 + 7.75, sx, sy);
-            tube_walk(v, a.vols + P.vol_offset, chunk, zr, P.tp, tb, td, s.q0, s.count, cell_fn);
+            // rays without a y component (n_y == 0 after flat_beam): the walk without the y axis,
+            // the same cells and sums (block-uniform: one pair per block)
+            if (g.n[1] == 0.0f)
+                tube_walk<true>(v, a.vols + P.vol_offset, chunk, zr, P.tp, tb, td, s.q0, s.count, cell_fn);
+            else
+                tube_walk<false>(v, a.vols + P.vol_offset, chunk, zr, P.tp, tb, td, s.q0, s.count, cell_fn);
         } else if (ok) {
             for_each_cell(s.q0, s.dq, s.count, [&](const int cell[3], int k, int mi) {
                 float val[8];
