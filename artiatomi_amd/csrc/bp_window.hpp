@@ -292,10 +292,12 @@ __device__ __forceinline__ bool window_quad(const BPWin& w, const float* __restr
     for (int k = 0; k < 4; k++) {
         const float du = __builtin_fmaf((float)k, dui, fu);
         const float dv = __builtin_fmaf((float)k, dvi, fv);
-        const int i0 = (int)ceilf((du + w.uMin) * (float)kOS - 0.5f);
-        const int j0 = (int)ceilf((dv + w.vMin) * (float)kOS - 0.5f);
-        const float a0 = __builtin_fmaf((float)i0 + 0.5f, kDelta, -du);
-        const float b0 = __builtin_fmaf((float)j0 + 0.5f, kDelta, -dv);
+        // the window's first lattice point, as a float (an integer) and an int
+        const float fi0 = ceilf((du + w.uMin) * (float)kOS - 0.5f);
+        const float fj0 = ceilf((dv + w.vMin) * (float)kOS - 0.5f);
+        const int i0 = (int)fi0, j0 = (int)fj0;
+        const float a0 = __builtin_fmaf(fi0 + 0.5f, kDelta, -du);
+        const float b0 = __builtin_fmaf(fj0 + 0.5f, kDelta, -dv);
         float num, den;
         if (mode == kWinOpen)
             window_sums<kOS, kNI, kNJ, kWinOpen>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, 0, 0, 0, 0, num,
