@@ -442,9 +442,25 @@ __device__ __forceinline__ void cell_weights(const CellMoments& cm, double W[8])
 // + ABC b c sums to m (d f/da)(centre) + S2c ABC db dc, and likewise for b and c.  Where a corner
 // index is clamped its step vanishes, and with it the derivative along that axis, as for the
 // clamp-addressed read.  Accumulated into G.
+// db = 0 (cm.flat): the x and z derivatives are those of the bilinear interpolant in (a, c) of the
+// y-blended edges w_xz that the cell sum forms (shared with it), the y derivative interpolates the
+// y differences in (a, c), and with da db = db dc = 0 only the y derivative keeps a correction.
 __device__ __forceinline__ void cell_grad_sum(const float val[8], const CellMoments& cm, int mi, double G[3]) {
     const float v000 = val[0], v100 = val[1], v010 = val[2], v110 = val[3];
     const float v001 = val[4], v101 = val[5], v011 = val[6], v111 = val[7];
+    if (cm.flat) {
+        const float w00 = lerpf(v000, v010, cm.b), w10 = lerpf(v100, v110, cm.b);
+        const float w01 = lerpf(v001, v011, cm.b), w11 = lerpf(v101, v111, cm.b);
+        const float ga = lerpf(w10 - w00, w11 - w01, cm.c);
+        const float gc = lerpf(w01 - w00, w11 - w10, cm.a);
+        const float y00 = v010 - v000, y10 = v110 - v100, y01 = v011 - v001, y11 = v111 - v101;
+        const float gb = lerpf(lerpf(y00, y10, cm.a), lerpf(y01, y11, cm.a), cm.c);
+        const float ABCf = (y11 - y01) - (y10 - y00);
+        G[0] += (double)(cm.m * ga);
+        G[1] += (double)__builtin_fmaf(cm.m, gb, (cm.S2c * ABCf) * cm.pac);
+        G[2] += (double)(cm.m * gc);
+        return;
+    }
     const float ABC = ((v111 - v110) - (v101 - v100)) - ((v011 - v010) - (v001 - v000));
     // d f / d a at the centre: the x-differences interpolated in b, then c (likewise for b and c)
     const float ga = lerpf(lerpf(v100 - v000, v110 - v010, cm.b), lerpf(v101 - v001, v111 - v011, cm.b), cm.c);
