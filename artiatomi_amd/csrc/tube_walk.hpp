@@ -184,10 +184,76 @@ struct BoxCursor {
 };
 
 // Stages the padded box [X0, X0 + CX) x [Y0, ..) x [Z0, ..) (n = CX * CY * CZ entries) into chunk:
-// four loads in flight per thread before their LDS writes.
+// four loads in flight per thread before their LDS writes.  Boxes up to 256 entries wide go row by
+// row: thread t keeps column t mod CX and takes every (256 / CX)-th row from row t / CX on, so its
+// x index and clamp are fixed and a row step is two adds and a carry (rows z * ny + y of the
+// volume number below 2^23).
 template <typename T>
 __device__ __forceinline__ void stage_box(const VolG& v, const T* __restrict__ vol, float* __restrict__ chunk, int X0,
                                           int Y0, int Z0, int CX, int CY, int n) {
+    const int P = 256 / CX;   // rows per pass
+    if (P > 0) {
+        const int rows = n / CX;
+        const int t = (int)threadIdx.x, r0 = t / CX, x = t - r0 * CX;
+        if (r0 >= P || r0 >= rows) return;
+        const int dz = P / CY, dy = P - dz * CY;
+        int zz = r0 / CY, yy = r0 - zz * CY;
+        if (X0 >= 0 && X0 + CX <= v.nx && Y0 >= 0 && Y0 + CY <= v.ny && Z0 >= 0 && Z0 + rows / CY <= v.nz) {
+            // the box inside the volume (no clamps, block-uniform): a voxel pointer and an LDS
+            // pointer stepped per row, the voxel step one of two (with or without the y carry)
+            const long long nx = v.nx, nxy = nx * (long long)v.ny;
+            const T* __restrict__ pv = vol + ((long long)(Z0 + zz) * nxy + (long long)(Y0 + yy) * nx + (X0 + x));
+            const long long stepN = (long long)dy * nx + (long long)dz * nxy, stepC = stepN - (long long)CY * nx + nxy;
+            float* __restrict__ q = chunk + (r0 * CX + x);
+            const int qs = P * CX;
+            auto next = [&]() {
+                yy += dy;
+                const bool c = yy >= CY;
+                yy = c ? yy - CY : yy;
+                pv += c ? stepC : stepN;
+            };
+            int r = r0;
+            for (; r + 3 * P < rows; r += 4 * P) {
+                float val[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    val[k] = vox(pv, 0);
+                    next();
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) q[k * qs] = val[k];
+                q += 4 * qs;
+            }
+            for (; r < rows; r += P) {
+                *q = vox(pv, 0);
+                next();
+                q += qs;
+            }
+            return;
+        }
+        const size_t gx = (size_t)(unsigned)clampi(X0 + x, 0, v.nx - 1);
+        auto load = [&]() {
+            const int gy = clampi(Y0 + yy, 0, v.ny - 1), gz = clampi(Z0 + zz, 0, v.nz - 1);
+            const float val = vox(vol, (size_t)(unsigned)(__mul24(gz, v.ny) + gy) * (size_t)(unsigned)v.nx + gx);
+            yy += dy;
+            zz += dz;
+            if (yy >= CY) {
+                yy -= CY;
+                zz++;
+            }
+            return val;
+        };
+        int r = r0;
+        for (; r + 3 * P < rows; r += 4 * P) {
+            float val[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) val[k] = load();
+#pragma unroll
+            for (int k = 0; k < 4; k++) chunk[__mul24(r + k * P, CX) + x] = val[k];
+        }
+        for (; r < rows; r += P) chunk[__mul24(r, CX) + x] = load();
+        return;
+    }
     int e = (int)threadIdx.x;
     if (e >= n) return;
     BoxCursor c(e, CX, CY);
