@@ -186,8 +186,8 @@ struct BoxCursor {
 // Stages the padded box [X0, X0 + CX) x [Y0, ..) x [Z0, ..) (n = CX * CY * CZ entries) into chunk:
 // four loads in flight per thread before their LDS writes.  Boxes up to 256 entries wide go row by
 // row: thread t keeps column t mod CX and takes every (256 / CX)-th row from row t / CX on, so its
-// x index and clamp are fixed and a row step is two adds and a carry (rows z * ny + y of the
-// volume number below 2^23).
+// x index and clamp are fixed and a row step is two adds and a carry (the rows z * ny + y of the
+// volume number below 2^31, z and ny each below 2^23: 24-bit multiplies).
 template <typename T>
 __device__ __forceinline__ void stage_box(const VolG& v, const T* __restrict__ vol, float* __restrict__ chunk, int X0,
                                           int Y0, int Z0, int CX, int CY, int n) {
