@@ -110,6 +110,170 @@ __global__ void __launch_bounds__(256) vol_traversal_ref_kernel(VolG v, TiltG g,
     dist[(size_t)y * pitch + (size_t)x] += val;
 }
 
+// march_ref_kernel's forward projection (not the slicer) with the volume read from LDS-staged slab
+// boxes: the same samples in the same order, the same float arithmetic per sample (tex3d_linear's
+// clamps, weights and lerps) and the same voxel values, so the same bits.  A block is a 16x16-pixel
+// tile, a thread one pixel; its four sub-rays run one after another as in the reference (the float
+// running sum chains them), so the block walks the slabs of cell layers once per sub-ray offset,
+// staging for each slab the box the tile's sub-rays of that offset cross.  A sample whose corners
+// fall outside the box reads global memory.
+constexpr int kRefTile = 16;   // pixels per side of a block's tile
+constexpr int kRefSlab = 4;    // cell layers per slab
+
+// tex3d_linear with the eight voxels taken from the staged box [X0, X0 + CX) x [Y0, ..) x [Z0, ..)
+// (padded indices, clamp-addressed voxels) when every clamped corner lies in it, else from vol.
+template <typename T>
+__device__ __forceinline__ float tex3d_linear_box(const T* __restrict__ vol, int nx, int ny, int nz,
+                                                  const float* __restrict__ box, bool staged, int X0, int Y0,
+                                                  int Z0, int CX, int CY, int CZ, float x, float y, float z) {
+    x = fminf(fmaxf(x, -1.0f), (float)(nx + 1));
+    y = fminf(fmaxf(y, -1.0f), (float)(ny + 1));
+    z = fminf(fmaxf(z, -1.0f), (float)(nz + 1));
+    const float xb = x - 0.5f, yb = y - 0.5f, zb = z - 0.5f;
+    const float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
+    const float a = xb - fx, b = yb - fy, c = zb - fz;
+    int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+    const int i1 = clampi(i0 + 1, 0, nx - 1), j1 = clampi(j0 + 1, 0, ny - 1), k1 = clampi(k0 + 1, 0, nz - 1);
+    i0 = clampi(i0, 0, nx - 1);
+    j0 = clampi(j0, 0, ny - 1);
+    k0 = clampi(k0, 0, nz - 1);
+    float v000, v100, v010, v110, v001, v101, v011, v111;
+    if (staged && i0 >= X0 && i1 < X0 + CX && j0 >= Y0 && j1 < Y0 + CY && k0 >= Z0 && k1 < Z0 + CZ) {
+        const int r00 = ((k0 - Z0) * CY + (j0 - Y0)) * CX - X0, r10 = ((k0 - Z0) * CY + (j1 - Y0)) * CX - X0;
+        const int r01 = ((k1 - Z0) * CY + (j0 - Y0)) * CX - X0, r11 = ((k1 - Z0) * CY + (j1 - Y0)) * CX - X0;
+        v000 = box[r00 + i0];
+        v100 = box[r00 + i1];
+        v010 = box[r10 + i0];
+        v110 = box[r10 + i1];
+        v001 = box[r01 + i0];
+        v101 = box[r01 + i1];
+        v011 = box[r11 + i0];
+        v111 = box[r11 + i1];
+    } else {
+        const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
+        const T* p00 = vol + (size_t)k0 * sz + (size_t)j0 * sy;
+        const T* p10 = vol + (size_t)k0 * sz + (size_t)j1 * sy;
+        const T* p01 = vol + (size_t)k1 * sz + (size_t)j0 * sy;
+        const T* p11 = vol + (size_t)k1 * sz + (size_t)j1 * sy;
+        v000 = vox(p00, i0);
+        v100 = vox(p00, i1);
+        v010 = vox(p10, i0);
+        v110 = vox(p10, i1);
+        v001 = vox(p01, i0);
+        v101 = vox(p01, i1);
+        v011 = vox(p11, i0);
+        v111 = vox(p11, i1);
+    }
+    const float c00 = lerpf(v000, v100, a);
+    const float c10 = lerpf(v010, v110, a);
+    const float c01 = lerpf(v001, v101, a);
+    const float c11 = lerpf(v011, v111, a);
+    return lerpf(lerpf(c00, c10, b), lerpf(c01, c11, b), c);
+}
+
+// The sample's z cell as tex3d_linear finds it (floor of the clamped coordinate minus 1/2).
+__device__ __forceinline__ int ref_cell_z(const VolG& v, float qz) {
+    return (int)floorf(fminf(fmaxf(qz, -1.0f), (float)(v.nz + 1)) - 0.5f);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) march_ref_lds_kernel(VolG v, TiltG g, const T* __restrict__ vol, Roi roi,
+                                                            size_t pitch, float* __restrict__ proj,
+                                                            float* __restrict__ dist, TubePlan tp) {
+    __shared__ float chunk[kChunkFloats];
+    __shared__ int zr[2];
+    const int tx0 = roi.x0 + (int)blockIdx.x * kRefTile, ty0 = roi.y0 + (int)blockIdx.y * kRefTile;
+    const int x = tx0 + (int)(threadIdx.x & (kRefTile - 1)), y = ty0 + (int)(threadIdx.x / kRefTile);
+    const bool active = x < roi.x1 && y < roi.y1;
+    const float step = v.vs[0] * 0.15f;
+    float temp = 0.0f, hits = 0.0f, len = 0.0f;
+    // the sample's texel coordinates, as march_ref_kernel computes them
+    auto texel = [&](const float f[3], float q[3]) {
+        q[0] = ((f[0] - v.bbMin[0]) * v.rcp[0]) * v.dim[0];
+        q[1] = ((f[1] - v.bbMin[1]) * v.rcp[1]) * v.dim[1];
+        q[2] = ((f[2] - v.bbMin[2]) * v.rcp[2]) * v.dim[2] - v.zshift;
+    };
+    for (int s = 0; s < 4; s++) {
+        // the reference's sub-ray order: osx outer, osy inner, each 0.25 then 0.75
+        const float osx = (s & 2) ? 0.75f : 0.25f, osy = (s & 1) ? 0.75f : 0.25f;
+        float c[3] = {0.0f, 0.0f, 0.0f}, f[3] = {0.0f, 0.0f, 0.0f};
+        float t_in = 1.0f, t_out = 0.0f;
+        bool live = false;
+        if (active) {
+            sub_ray_origin(g, (float)x + osx, (float)y + osy, c);
+            slab_test(v, g, c, t_in, t_out);
+            live = (t_out - t_in) > 0.0f;
+            if (live) {
+                hits += 1.0f;
+                len += (t_out - t_in);
+                for (int i = 0; i < 3; i++) f[i] = c[i] + t_in * g.n[i];
+            }
+        }
+        int guard = 0;
+        // cell layers of the block's samples of this offset (first and last sample, one of margin)
+        __syncthreads();   // the previous offset's readers are done with zr and chunk
+        if (threadIdx.x == 0) {
+            zr[0] = INT_MAX;
+            zr[1] = INT_MIN;
+        }
+        __syncthreads();
+        if (live) {
+            float q[3], e[3], qe[3];
+            texel(f, q);
+            for (int i = 0; i < 3; i++) e[i] = c[i] + t_out * g.n[i];
+            texel(e, qe);
+            const int za = ref_cell_z(v, q[2]), zb = ref_cell_z(v, qe[2]);
+            atomicMin(&zr[0], min(za, zb) - 1);
+            atomicMax(&zr[1], max(za, zb) + 1);
+        }
+        __syncthreads();
+        const int zlo = max(zr[0], -1), zhi = min(zr[1], v.nz - 1);
+        const int nslab = zlo <= zhi ? (zhi - zlo) / kRefSlab + 1 : 0;
+        const TubeBox tb = tube_box(v, g, tp, (double)tx0 + osx, (double)tx0 + (kRefTile - 1) + osx,
+                                    (double)ty0 + osy, (double)ty0 + (kRefTile - 1) + osy, 0.0, 0.0);
+        for (int j = 0; j < nslab; j++) {
+            const int zs = tp.zdir > 0 ? zlo + j * kRefSlab : zhi - (j + 1) * kRefSlab + 1;
+            const int ze = zs + kRefSlab - 1;
+            const float z0 = (float)zs, z1 = (float)(ze + 1);
+            const int X0 = wave_uniform(max((int)floorf(tb.axl + fminf(tp.bx * z0, tp.bx * z1)) - 2, -1));
+            const int X1 = min((int)floorf(tb.axh + fmaxf(tp.bx * z0, tp.bx * z1)) + 3, v.nx);
+            const int Y0 = wave_uniform(max((int)floorf(tb.ayl + fminf(tp.by * z0, tp.by * z1)) - 2, -1));
+            const int Y1 = min((int)floorf(tb.ayh + fmaxf(tp.by * z0, tp.by * z1)) + 3, v.ny);
+            const int Z0 = max(zs, -1), Z1 = min(ze + 1, v.nz);
+            const int CX = wave_uniform(X1 - X0 + 1), CY = wave_uniform(Y1 - Y0 + 1), CZ = Z1 - Z0 + 1;
+            const bool staged = CX > 1 && CY > 1 && CZ > 1 && CX * CY * CZ <= kChunkFloats;
+            __syncthreads();   // the previous slab's readers are done
+            if (staged) stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, CX * CY * CZ);
+            __syncthreads();
+            while (live && t_in <= t_out && guard < kMaxSamplesPerRay) {
+                float q[3];
+                texel(f, q);
+                const int kz = ref_cell_z(v, q[2]);
+                if (tp.zdir > 0 ? kz > ze : kz < zs) break;   // a later slab's sample
+                const float val = tex3d_linear_box(vol, v.nx, v.ny, v.nz, chunk, staged, X0, Y0, Z0, CX, CY, CZ,
+                                                   q[0], q[1], q[2]);
+                temp += (val * v.vs[0]) * 0.15f;
+                guard++;
+                t_in += step;
+                for (int i = 0; i < 3; i++) f[i] = c[i] + t_in * g.n[i];
+            }
+        }
+        // samples beyond the slabs (not expected): global memory
+        while (live && t_in <= t_out && guard < kMaxSamplesPerRay) {
+            float q[3];
+            texel(f, q);
+            temp += (tex3d_linear(vol, v.nx, v.ny, v.nz, q[0], q[1], q[2]) * v.vs[0]) * 0.15f;
+            guard++;
+            t_in += step;
+            for (int i = 0; i < 3; i++) f[i] = c[i] + t_in * g.n[i];
+        }
+    }
+    if (!active) return;
+    const size_t i = (size_t)y * pitch + (size_t)x;
+    proj[i] += temp * 0.25f;
+    dist[i] = fmaxf(0.0f, len / hits);
+}
+
 // ---------------------------------------------------------------- exact numerics
 
 // Sum of the trilinear samples q0 + k*dq, k = 0..count-1 (coordinates shifted by -0.5, texel j
@@ -599,6 +763,14 @@ void launch_march_t(artia_ctx* ctx, const VolG& v, const TiltG& g, const T* vol,
                     float* proj, float* dist, bool slicer, float tmin, float tmax) {
     if (r.x1 <= r.x0 || r.y1 <= r.y0) return;
     if (ctx->numerics == ARTIA_NUMERICS_REFERENCE) {
+        TubePlan tp;
+        if (!slicer && ctx->march_lds && tube_plan(v, g, tp, (double)(kRefTile - 1), kRefSlab)) {
+            const dim3 grid((unsigned)((r.x1 - r.x0 + kRefTile - 1) / kRefTile),
+                            (unsigned)((r.y1 - r.y0 + kRefTile - 1) / kRefTile));
+            hipLaunchKernelGGL((march_ref_lds_kernel<T>), grid, dim3(256), 0, ctx->stream, v, g, vol, r, pitch, proj,
+                               dist, tp);
+            return;
+        }
         const dim3 grid = grid_for(r), block(64, 4);
         if (slicer)
             hipLaunchKernelGGL((march_ref_kernel<true, T>), grid, block, 0, ctx->stream, v, g, vol, r, pitch, proj,

@@ -35,7 +35,9 @@ struct TubePlan {
 // voxels on an axis, rays parallel to the z layers, or a tile's slab box larger than the LDS chunk
 // (steep rays).  The box of an 8x8-pixel tile spans the tile's extent in a_x = Q_x - bx Q_z
 // (affine in the pixel) plus the rays' drift over a slab.
-inline bool tube_plan(const VolG& v, const TiltG& g, TubePlan& tp) {
+// span: the tile's extent in pixels between its outermost sub-rays (7.5 for 8 pixels at offsets
+// 0.25 .. 7.75); slab: cell layers per slab.
+inline bool tube_plan(const VolG& v, const TiltG& g, TubePlan& tp, double span = 7.5, int slab = kSlabCells) {
     // (nz bounded so that the walk's float box offsets stay exact: |cell z * CXY| < 2^24, CXY being
     // at most kChunkFloats / 2)
     if (v.nx < 2 || v.ny < 2 || v.nz < 2 || (long long)(v.nz + 2) * (kChunkFloats / 2) >= (1LL << 24)) return false;
@@ -48,10 +50,10 @@ inline bool tube_plan(const VolG& v, const TiltG& g, TubePlan& tp) {
     }
     if (!(std::fabs(N[2]) > 1e-3 * (std::fabs(N[0]) + std::fabs(N[1]) + std::fabs(N[2])))) return false;
     const double bx = N[0] / N[2], by = N[1] / N[2];
-    const double wx = 7.5 * (std::fabs(dpx[0] - bx * dpx[2]) + std::fabs(dpy[0] - bx * dpy[2]));
-    const double wy = 7.5 * (std::fabs(dpx[1] - by * dpx[2]) + std::fabs(dpy[1] - by * dpy[2]));
-    const double cx = wx + std::fabs(bx) * (kSlabCells + 1) + 5.0, cy = wy + std::fabs(by) * (kSlabCells + 1) + 5.0;
-    if (cx * cy * (kSlabCells + 1) > (double)kChunkFloats) return false;
+    const double wx = span * (std::fabs(dpx[0] - bx * dpx[2]) + std::fabs(dpy[0] - bx * dpy[2]));
+    const double wy = span * (std::fabs(dpx[1] - by * dpx[2]) + std::fabs(dpy[1] - by * dpy[2]));
+    const double cx = wx + std::fabs(bx) * (slab + 1) + 5.0, cy = wy + std::fabs(by) * (slab + 1) + 5.0;
+    if (cx * cy * (slab + 1) > (double)kChunkFloats) return false;
     tp.bx = (float)bx;
     tp.by = (float)by;
     tp.zdir = N[2] > 0.0 ? 1 : -1;

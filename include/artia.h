@@ -138,10 +138,11 @@ artia_status artia_destroy(artia_ctx* ctx);
 artia_status artia_set_stream(artia_ctx* ctx, void* hip_stream);
 artia_status artia_get_stream(artia_ctx* ctx, void** hip_stream);
 artia_status artia_set_numerics(artia_ctx* ctx, int numerics);
-/* Exact forward projection (march and slicer) and its adjoint: enable != 0 (the default) walks
- * each 8x8-pixel tile's rays through LDS-staged slabs of the volume where the geometry allows it,
- * 0 always reads (or, for the adjoint, atomically adds to) the volume directly.  Projections are
- * bit-identical either way; adjoint sums differ only in the order of their float atomics.  Pair
+/* Exact forward projection (march and slicer), the reference-numerics march, and the exact
+ * adjoint: enable != 0 (the default) walks each tile's rays through LDS-staged slabs of the volume
+ * where the geometry allows it, 0 always reads (or, for the adjoint, atomically adds to) the volume
+ * directly.  Projections are bit-identical either way; adjoint sums differ only in the order of
+ * their float atomics.  Pair
  * sets take the setting in force when they are created.  No reference counterpart (a tuning
  * choice). */
 artia_status artia_set_fp_staging(artia_ctx* ctx, int enable);

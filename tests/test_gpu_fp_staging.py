@@ -132,3 +132,66 @@ def test_mag_anisotropy_staged_equals_direct(ctx):
     p1, p2 = project_both(ctx, run)
     assert float(p1.abs().max()) > 0
     assert torch.equal(p1, p2)
+
+
+@pytest.mark.parametrize("nx,ny,nz,W,H,tilt,psi,shift,roi", CASES)
+def test_reference_march_staged_equals_direct(ctx, nx, ny, nz, W, H, tilt, psi, shift, roi):
+    """Reference numerics: the LDS-staged march takes the reference kernel's samples, order and float
+    arithmetic, so it leaves the same bits as the direct-read kernel."""
+    print(f"reference march staged vs direct: {nx}x{ny}x{nz} tilt={tilt} psi={psi} roi={roi}", flush=True)
+    vg = A.volume_geometry(nx, ny, nz)
+    g = A.tilt_geometry(tilt, W, H, psi_deg=psi, shift=shift)
+    gen = torch.Generator(device="cuda").manual_seed(3 * nx + int(tilt))
+    vol = torch.rand((nz, ny, nx), generator=gen, device="cuda")
+
+    def run():
+        proj = torch.zeros((H, W), device="cuda")
+        dist = torch.zeros((H, W), device="cuda")
+        ctx.fp_march(vg, g, vol, proj, dist, roi=roi)
+        return proj, dist
+
+    ctx.set_numerics("reference")
+    try:
+        (p1, d1), (p2, d2) = project_both(ctx, run)
+    finally:
+        ctx.set_numerics("exact")
+    assert float(p1.abs().max()) > 0
+    assert torch.equal(p1, p2)
+    assert torch.equal(d1, d2)
+
+
+def test_reference_sart_iteration_staged_equals_direct(ctx):
+    vg = A.volume_geometry(48, 48, 16)
+    sv = A.subvolume_geometry(24, 1.0, (1.0, 1.0, 1.0), A.volume_geometry(96, 96, 40), (40, 52, 18),
+                              shift=(0.0, 0.0, 3.5))
+    W = H = 56
+    tilts = [-54.0, -27.0, 0.0, 27.0, 54.0]
+    acq = A.Acquisition.build(vg, W, H, tilts, psis_deg=[1.0, -2.0, 0.5, 3.0, -1.0])
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    projs = torch.rand((len(tilts), H, W), generator=gen, device="cuda")
+    vols = []
+    ctx.set_numerics("reference")
+    try:
+        for staged in (True, False):
+            ctx.set_fp_staging(staged)
+            vol = torch.zeros((16, 48, 48), device="cuda")
+            ctx.sart_run(vg, acq, projs, vol, A.sart_params(lambda_=0.3, oversampling=2), iterations=2)
+            vols.append(vol)
+        # a z-shifted sub-volume
+        g = A.tilt_geometry(-42.0, 96, 96, psi_deg=-2.0)
+        svol = torch.rand((24, 24, 24), generator=gen, device="cuda")
+        outs = []
+        for staged in (True, False):
+            ctx.set_fp_staging(staged)
+            proj = torch.zeros((96, 96), device="cuda")
+            dist = torch.zeros((96, 96), device="cuda")
+            ctx.fp_march(sv, g, svol, proj, dist)
+            outs.append(proj)
+    finally:
+        ctx.set_numerics("exact")
+        ctx.set_fp_staging(True)
+    torch.cuda.synchronize()
+    assert float(vols[0].abs().max()) > 0
+    assert torch.equal(vols[0], vols[1])
+    assert float(outs[0].abs().max()) > 0
+    assert torch.equal(outs[0], outs[1])
