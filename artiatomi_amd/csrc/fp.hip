@@ -116,7 +116,8 @@ __global__ void __launch_bounds__(256) vol_traversal_ref_kernel(VolG v, TiltG g,
 // tile, a thread one pixel; its four sub-rays run one after another as in the reference (the float
 // running sum chains them), so the block walks the slabs of cell layers once per sub-ray offset,
 // staging for each slab the box the tile's sub-rays of that offset cross.  A sample whose corners
-// fall outside the box reads global memory.
+// fall outside the box reads global memory.  Five waves per SIMD (96 registers, a few spilled)
+// measured 6 % faster than the unbounded four.
 constexpr int kRefTile = 16;   // pixels per side of a block's tile
 constexpr int kRefSlab = 4;    // cell layers per slab
 
@@ -177,7 +178,7 @@ __device__ __forceinline__ int ref_cell_z(const VolG& v, float qz) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) march_ref_lds_kernel(VolG v, TiltG g, const T* __restrict__ vol, Roi roi,
+__global__ void __launch_bounds__(256, 5) march_ref_lds_kernel(VolG v, TiltG g, const T* __restrict__ vol, Roi roi,
                                                             size_t pitch, float* __restrict__ proj,
                                                             float* __restrict__ dist, TubePlan tp) {
     __shared__ float chunk[kChunkFloats];
