@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(256, 5) march_ref_lds_kernel(VolG v, TiltG g, 
             const int CX = wave_uniform(X1 - X0 + 1), CY = wave_uniform(Y1 - Y0 + 1), CZ = Z1 - Z0 + 1;
             const bool staged = CX > 1 && CY > 1 && CZ > 1 && CX * CY * CZ <= kChunkFloats;
             __syncthreads();   // the previous slab's readers are done
-            if (staged) stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, CX * CY * CZ);
+            if (staged) stage_box_rows(v, vol, chunk, X0, Y0, Z0, CX, CY, CX * CY * CZ);
             __syncthreads();
             while (live && t_in <= t_out && guard < kMaxSamplesPerRay) {
                 float q[3];
@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(256, 6) march_lds_kernel(VolG v, TiltG g, cons
     const TubeBox tb = tube_box(v, g, tp, (double)tx0 + 0.25, (double)tx0 + 7.75, (double)ty0 + 0.25,
                                 (double)ty0 + 7.75, 0.0, 0.0);
     double total = 0.0;   // summed as march_sum does, then added to the slicer's first sample
-    tube_walk<kFlat>(v, vol, chunk, zr, tp, tb, td, L.q0, L.count,
+    tube_walk<kFlat, true>(v, vol, chunk, zr, tp, tb, td, L.q0, L.count,
                      [&](const float val[8], const auto* cell, auto k, auto mi) {
                          total += cell_sum_values<kFlat>(val, cell_moments<kFlat>(L.q0, td.dq, cell, k, mi), mi);
                      });
@@ -646,7 +646,7 @@ __global__ void __launch_bounds__(256, 6) slicer_segments_lds_kernel(VolG v, Til
     const TubeDir td = tube_dir(v, g, step);
     const TubeBox tb = tube_box(v, g, tp, (double)tx0 + 0.25, (double)tx0 + 7.75, (double)ty0 + 0.25,
                                 (double)ty0 + 7.75, 0.0, 0.0);
-    tube_walk_segments<kFlat>(v, vol, chunk, zr, tp, tb, td, za, zb, seg,
+    tube_walk_segments<kFlat, true>(v, vol, chunk, zr, tp, tb, td, za, zb, seg,
                               [&](const float val[8], const auto* cell, auto kk, auto mi) {
                                   seg.total += cell_sum_values<kFlat>(
                                       val, cell_moments<kFlat>(seg.q0, td.dq, cell, kk, mi), mi);

@@ -186,13 +186,36 @@ struct BoxCursor {
 };
 
 // Stages the padded box [X0, X0 + CX) x [Y0, ..) x [Z0, ..) (n = CX * CY * CZ entries) into chunk:
-// four loads in flight per thread before their LDS writes.  Boxes up to 256 entries wide go row by
-// row: thread t keeps column t mod CX and takes every (256 / CX)-th row from row t / CX on, so its
-// x index and clamp are fixed and a row step is two adds and a carry (the rows z * ny + y of the
-// volume number below 2^31, z and ny each below 2^23: 24-bit multiplies).
+// four loads in flight per thread before their LDS writes.
 template <typename T>
 __device__ __forceinline__ void stage_box(const VolG& v, const T* __restrict__ vol, float* __restrict__ chunk, int X0,
                                           int Y0, int Z0, int CX, int CY, int n) {
+    int e = (int)threadIdx.x;
+    if (e >= n) return;
+    BoxCursor c(e, CX, CY);
+    for (; e + 3 * 256 < n; e += 4 * 256) {
+        float val[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            val[k] = vox(vol, c.voxel(v, X0, Y0, Z0));
+            c.next();
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) chunk[e + k * 256] = val[k];
+    }
+    for (; e < n; e += 256) {
+        chunk[e] = vox(vol, c.voxel(v, X0, Y0, Z0));
+        c.next();
+    }
+}
+
+
+// stage_box row by row, for boxes up to 256 entries wide (wider ones go to stage_box): thread t keeps column t mod CX and takes every (256 / CX)-th row from row t / CX on, so its
+// x index and clamp are fixed and a row step is two adds and a carry (the rows z * ny + y of the
+// volume number below 2^31, z and ny each below 2^23: 24-bit multiplies).
+template <typename T>
+__device__ __forceinline__ void stage_box_rows(const VolG& v, const T* __restrict__ vol, float* __restrict__ chunk,
+                                               int X0, int Y0, int Z0, int CX, int CY, int n) {
     const int P = 256 / CX;   // rows per pass
     if (P > 0) {
         const int rows = n / CX;
@@ -256,23 +279,7 @@ __device__ __forceinline__ void stage_box(const VolG& v, const T* __restrict__ v
         for (; r < rows; r += P) chunk[__mul24(r, CX) + x] = load();
         return;
     }
-    int e = (int)threadIdx.x;
-    if (e >= n) return;
-    BoxCursor c(e, CX, CY);
-    for (; e + 3 * 256 < n; e += 4 * 256) {
-        float val[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            val[k] = vox(vol, c.voxel(v, X0, Y0, Z0));
-            c.next();
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) chunk[e + k * 256] = val[k];
-    }
-    for (; e < n; e += 256) {
-        chunk[e] = vox(vol, c.voxel(v, X0, Y0, Z0));
-        c.next();
-    }
+    stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, n);
 }
 
 // Walks every lane's samples q0 + k*dq (k < count) cell by cell, slab by slab through LDS.  Must be
@@ -294,7 +301,7 @@ __device__ __forceinline__ int flat_box_offset(const int* cell, int CXY, int K) 
     return __mul24(cell[2], CXY) + (cell[0] + K);
 }
 
-template <bool kScatter, bool kFlat, typename T, class F>
+template <bool kScatter, bool kFlat, bool kRows, typename T, class F>
 __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ vol, float* __restrict__ grad,
                                            float* __restrict__ chunk, int* zr, const TubePlan& tp, const TubeBox& tb,
                                            const TubeDir& td, const float q0[3], int count, F&& f) {
@@ -344,6 +351,8 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
         if (staged) {
             if (kScatter)
                 for (int e = (int)threadIdx.x; e < n; e += 256) chunk[e] = 0.0f;
+            else if (kRows)
+                stage_box_rows(v, vol, chunk, X0, Y0, Z0, CX, CY, n);
             else
                 stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, n);
         }
@@ -479,7 +488,7 @@ __device__ __forceinline__ void tube_slabs(const VolG& v, const T* __restrict__ 
 // tube_slabs' CellWalker steps, so its cells, their order and their values are those of a
 // tube_walk of that segment alone (kFlat: the walk without the y axis, as tube_walk<true>).  Must
 // be called by all 256 threads of the block.
-template <bool kFlat, typename T, class Seg, class F>
+template <bool kFlat, bool kRows, typename T, class Seg, class F>
 __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __restrict__ vol,
                                                    float* __restrict__ chunk, int* zr, const TubePlan& tp,
                                                    const TubeBox& tb, const TubeDir& td, int za, int zb, Seg& seg,
@@ -517,7 +526,10 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
         const int n = CX * CY * CZ;
         if (!tail) {
             __syncthreads();
-            if (staged) stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, n);
+            if (staged && kRows)
+                stage_box_rows(v, vol, chunk, X0, Y0, Z0, CX, CY, n);
+            else if (staged)
+                stage_box(v, vol, chunk, X0, Y0, Z0, CX, CY, n);
             __syncthreads();
         }
         for (;;) {
@@ -586,12 +598,13 @@ __device__ __forceinline__ void tube_walk_segments(const VolG& v, const T* __res
     }
 }
 
-// The gather walk (see tube_slabs); kFlat: the rays have no y component (td.dq[1] == 0).
-template <bool kFlat = false, typename T, class F>
+// The gather walk (see tube_slabs); kFlat: the rays have no y component (td.dq[1] == 0); kRows:
+// the boxes staged by stage_box_rows.
+template <bool kFlat = false, bool kRows = false, typename T, class F>
 __device__ __forceinline__ void tube_walk(const VolG& v, const T* __restrict__ vol, float* __restrict__ chunk,
                                           int* zr, const TubePlan& tp, const TubeBox& tb, const TubeDir& td,
                                           const float q0[3], int count, F&& f) {
-    tube_slabs<false, kFlat>(v, vol, nullptr, chunk, zr, tp, tb, td, q0, count, f);
+    tube_slabs<false, kFlat, kRows>(v, vol, nullptr, chunk, zr, tp, tb, td, q0, count, f);
 }
 
 // The scatter walk (see tube_slabs): grad += the weights f returns.
@@ -599,7 +612,7 @@ template <class F>
 __device__ __forceinline__ void tube_scatter(const VolG& v, float* __restrict__ grad, float* __restrict__ chunk,
                                              int* zr, const TubePlan& tp, const TubeBox& tb, const TubeDir& td,
                                              const float q0[3], int count, F&& f) {
-    tube_slabs<true, false>(v, static_cast<const float*>(nullptr), grad, chunk, zr, tp, tb, td, q0, count, f);
+    tube_slabs<true, false, false>(v, static_cast<const float*>(nullptr), grad, chunk, zr, tp, tb, td, q0, count, f);
 }
 
 }  // namespace artia
