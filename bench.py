@@ -183,6 +183,27 @@ def measured_traffic(workload: str, kernel: str):
     return None
 
 
+def measured_valu(workload: str, kernel: str):
+    """VALU counters of `kernel` from the committed rocprofv3 summary of `workload` (its VALU table:
+    instructions per work unit, wave-level issue rate per SIMD per cycle, share of wave time waiting
+    on an operand), or None; like measured_traffic, the last profiler run's figures."""
+    path = Path(__file__).resolve().parent / "profiles" / f"{workload}.md"
+    try:
+        text = path.read_text()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        name = cells[0].strip("`") if cells else ""
+        if len(cells) == 11 and (name == kernel or ("<" not in kernel and name.startswith(kernel + "<"))):
+            try:
+                return {"valu_per_unit": cells[4], "issue_rate": float(cells[6]),
+                        "dependency_wait": cells[7], "source": f"profiles/{workload}.md"}
+            except ValueError:
+                return None
+    return None
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -927,6 +948,18 @@ def main() -> int:
     if measured:
         roofline["traffic"] = measured.pop("bytes")
         roofline["traffic_detail"] = measured
+    # the kernels are bound by their VALU issue, not by HBM: their counters from the same profile
+    if default_kernels:
+        for kname, kcands in (("fp", ["march_lds_kernel<false, true, float, true>",
+                                      "march_lds_kernel<false, true, float, false>"]),
+                              ("bp", ["bp_win_kernel"])):
+            for kc in kcands:
+                valu = measured_valu(args.workload, kc)
+                if valu and kname in per_kernel:
+                    per_kernel[kname]["valu_detail"] = valu
+                    if kname == dom:
+                        roofline["valu_detail"] = valu
+                    break
 
     cpu = None
     if not args.no_cpu_baseline:
