@@ -438,6 +438,132 @@ __global__ void __launch_bounds__(256, 8) bp_stack_kernel(VolG v, TiltG g, BPExa
     if (any) store_quad(vp, out);
 }
 
+// bp_stack_kernel for LINEAR sampling at OS 1 or 2 with an affine footprint and no magnification
+// anisotropy, blocks of 4 quads x 4 rows x 16 slices: thin along x, so that a block spans few slabs
+// at every tilt.  For each slab its quads fall in, the block stages the LINEAR fetches at the
+// lattice points of its footprint (tile_linear over the slab image: the values the per-box loop
+// fetches), so a lattice point costs one LDS read instead of a bilinear fetch.  e holds the block
+// offsets of this shape.  Quads whose footprint leaves the staged lattice fetch from global memory.
+template <int kOS, int kPM, typename T>
+__global__ void __launch_bounds__(256, 8) bp_stack_lat_kernel(VolG v, TiltG g, BPExact e, T* __restrict__ vol,
+                                                           const float* __restrict__ stack, size_t stride, int W,
+                                                           int H, size_t pitch, float lambda, SlabArgs sl) {
+    __shared__ float tile[kTileFloats];
+    __shared__ int kr[2];
+    constexpr float kDelta = 1.0f / (float)kOS;
+    const int tid = (int)threadIdx.x;
+    const int xq = (int)blockIdx.x * 4 + (tid & 3);
+    const int y = (int)blockIdx.y * 4 + ((tid >> 2) & 3);
+    const int z = (int)blockIdx.z * 16 + (tid >> 4);
+    const bool inVol = xq < v.nx / 4 && y < v.ny && z < v.nz;
+    const unsigned x = (unsigned)xq * 4u;
+    if (tid == 0) {
+        kr[0] = INT_MAX;
+        kr[1] = INT_MIN;
+    }
+    __syncthreads();
+    const int slab = inVol ? find_slab(sl, gate_distance(e, v, g, x, (unsigned)y, (unsigned)z)) : -1;
+    if (slab >= 0) {
+        atomicMin(&kr[0], slab);
+        atomicMax(&kr[1], slab);
+    }
+    __syncthreads();
+    const int k0 = kr[0], k1 = kr[1];
+    bool tiled = false;
+    int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1, ls = 0, planeL = 0;
+    if (k0 <= k1) {
+        // the block's footprint (one pixel of margin for rounding): its lattice points
+        // [tx0 * OS, tx1 * OS) x [ty0 * OS, ty1 * OS) hold every quad footprint's lattice
+        const double bx = (double)blockIdx.x * 16.0, by = (double)blockIdx.y * 4.0, bz = (double)blockIdx.z * 16.0;
+        const double base0 = e.G[0][0] * bx + e.G[0][1] * by + e.G[0][2] * bz;
+        const double base1 = e.G[1][0] * bx + e.G[1][1] * by + e.G[1][2] * bz;
+        tx0 = max(0, (int)floor(base0 + e.tlo[0]) - 1);
+        ty0 = max(0, (int)floor(base1 + e.tlo[1]) - 1);
+        tx1 = min(W - 1, (int)ceil(base0 + e.thi[0]) + 1);
+        ty1 = min(H - 1, (int)ceil(base1 + e.thi[1]) + 1);
+        const int lw = max(0, tx1 - tx0) * kOS, lh = max(0, ty1 - ty0) * kOS;
+        ls = lw | 1;   // odd LDS row stride
+        planeL = ls * lh;
+        tiled = lw > 0 && lh > 0 && (long long)planeL * (long long)(k1 - k0 + 1) <= (long long)kTileFloats;
+        if (tiled) {
+            // lattice point (c, r) at pixel position tx0 + (c + 0.5) / OS, ty0 + (r + 0.5) / OS (exact
+            // in float, the per-box loop's px0 + (i + 0.5) / OS of the same point)
+            const int per = lw * lh, n = per * (k1 - k0 + 1);
+            for (int idx = tid; idx < n; idx += 256) {
+                const int s = idx / per, rem = idx - s * per;
+                const int r = rem / lw, c = rem - r * lw;
+                const float px = (float)tx0 + ((float)c + 0.5f) * kDelta, py = (float)ty0 + ((float)r + 0.5f) * kDelta;
+                tile[s * planeL + r * ls + c] =
+                    tile_linear(stack + (size_t)(k0 + s) * stride, (int)pitch, 0, 0, W, H, px, py);
+            }
+        }
+    }
+    __syncthreads();
+    if (slab < 0) return;
+    int px0, px1, py0, py1;
+    if (!footprint_affine(e, x, (unsigned)y, (unsigned)z, W, H, -1.0, INFINITY, px0, px1, py0, py1)) return;
+    const int nI = (px1 - px0) * kOS, nJ = (py1 - py0) * kOS;
+    if (nI <= 0 || nJ <= 0) return;
+    const bool fromLat = tiled && px0 >= tx0 && px1 <= tx1 && py0 >= ty0 && py1 <= ty1;
+    const float* __restrict__ lat = tile + (fromLat ? (slab - k0) * planeL : 0);
+    const int oi = (px0 - tx0) * kOS, oj = (py0 - ty0) * kOS;
+    const float* __restrict__ img = stack + (size_t)slab * stride;
+    T* vp = vol + ((size_t)z * (size_t)v.ny + (size_t)y) * (size_t)v.nx + x;
+    const float4 vb = load_quad(vp);
+    float out[4] = {vb.x, vb.y, vb.z, vb.w};
+    const float osf = (float)kOS;
+    const float du0 = (float)(e.pu0 + (double)x * e.pui + (double)y * e.puj + (double)z * e.puk - (double)px0);
+    const float dv0 = (float)(e.pv0 + (double)x * e.pvi + (double)y * e.pvj + (double)z * e.pvk - (double)py0);
+    const float dui = (float)e.pui, dvi = (float)e.pvi;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float du = __builtin_fmaf((float)k, dui, du0);
+        const float dv = __builtin_fmaf((float)k, dvi, dv0);
+        const int i0 = max(0, (int)ceilf((du + e.uMin) * osf - 0.5f));
+        const int i1 = min(nI - 1, (int)floorf((du + e.uMax) * osf - 0.5f));
+        const int j0 = max(0, (int)ceilf((dv + e.vMin) * osf - 0.5f));
+        const int j1 = min(nJ - 1, (int)floorf((dv + e.vMax) * osf - 0.5f));
+        float num = 0.0f, den = 0.0f;
+        if (fromLat)
+            voxel_sums_lattice<kPM>(e, lat, ls, oi, oj, kDelta, du, dv, i0, i1, j0, j1, num, den);
+        else
+            voxel_sums_global(e, g, img, W, H, pitch, ARTIA_FILTER_LINEAR, px0, py0, kOS, kDelta, du, dv, i0, i1,
+                              j0, j1, num, den);
+        den = fmaxf(0.0f, den);
+        if (den != 0.0f) {
+            out[k] += (lambda * num) / den;
+            any = true;
+        }
+    }
+    if (any) store_quad(vp, out);
+}
+
+template <int kOS, typename T>
+void launch_stack_lat_os(artia_ctx* ctx, dim3 grid, const VolG& v, const TiltG& g, const BPExact& e, T* vol,
+                         const float* stack, size_t stride, int W, int H, size_t pitch, float lambda,
+                         const SlabArgs& sl) {
+    if (e.parallel == 0)
+        hipLaunchKernelGGL((bp_stack_lat_kernel<kOS, 0, T>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, stack,
+                           stride, W, H, pitch, lambda, sl);
+    else if (e.parallel == 2)
+        hipLaunchKernelGGL((bp_stack_lat_kernel<kOS, 2, T>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, stack,
+                           stride, W, H, pitch, lambda, sl);
+    else
+        hipLaunchKernelGGL((bp_stack_lat_kernel<kOS, -1, T>), grid, dim3(256), 0, ctx->stream, v, g, e, vol, stack,
+                           stride, W, H, pitch, lambda, sl);
+}
+
+template <typename T>
+void launch_stack_lat(artia_ctx* ctx, const VolG& v, const TiltG& g, const BPExact& e, T* vol, const float* stack,
+                      size_t stride, int W, int H, size_t pitch, float lambda, int os, const SlabArgs& sl) {
+    const dim3 grid((unsigned)((v.nx / 4 + 3) / 4), (unsigned)((v.ny + 3) / 4), (unsigned)((v.nz + 15) / 16));
+    if (os == 1)
+        launch_stack_lat_os<1>(ctx, grid, v, g, e, vol, stack, stride, W, H, pitch, lambda, sl);
+    else
+        launch_stack_lat_os<2>(ctx, grid, v, g, e, vol, stack, stride, W, H, pitch, lambda, sl);
+}
+
 template <int kOS, int kNI, typename T>
 void launch_win_nj(artia_ctx* ctx, dim3 grid, const VolG& v, const BPWinSet& S, int nj, T* vol, int W, int H,
                    size_t pitch) {
@@ -552,10 +678,23 @@ void launch_stack_t(artia_ctx* ctx, dim3 grid, const VolG& v, const TiltG& g, co
 void launch_bp_stack(artia_ctx* ctx, const VolG& v, const TiltG& g, VolData vol, const float* stack, size_t stride,
                      int W, int H, size_t pitch, float lambda, int os, int filter, const float* bounds, int nslabs,
                      float last_end) {
+    const SlabArgs sl{bounds, nslabs, last_end};
+    if ((os == 1 || os == 2) && filter == ARTIA_FILTER_LINEAR && (size_t)H * pitch < ((size_t)1 << 31)) {
+        // the staged-lattice form, with the block offsets of its 16 x 4 x 16-voxel blocks
+        const BPExact e16 = prepare_exact(v, g, 16, 4, 16);
+        if (e16.affine && e16.identityAniso) {
+            if (vol.half)
+                launch_stack_lat(ctx, v, g, e16, static_cast<vhalf*>(vol.p), stack, stride, W, H, pitch, lambda, os,
+                                 sl);
+            else
+                launch_stack_lat(ctx, v, g, e16, static_cast<float*>(vol.p), stack, stride, W, H, pitch, lambda, os,
+                                 sl);
+            return;
+        }
+    }
     const BPExact e = prepare_exact(v, g);
     const int nq = v.nx / 4;
     const dim3 grid((unsigned)((nq + 15) / 16), (unsigned)((v.ny + 3) / 4), (unsigned)((v.nz + 3) / 4));
-    const SlabArgs sl{bounds, nslabs, last_end};
     if (vol.half)
         launch_stack_t(ctx, grid, v, g, e, static_cast<vhalf*>(vol.p), stack, stride, W, H, pitch, lambda, os, filter,
                        sl);

@@ -413,4 +413,29 @@ __device__ __forceinline__ void voxel_sums_linear_tile(const BPExact& e, const f
     }
 }
 
+// voxel_sums_linear_tile with each lattice point's LINEAR fetch read from a staged lattice plane:
+// lat[(oj + j) * ls + (oi + i)] holds tile_linear's value at the pixel position of column i and row
+// j of the quad's footprint lattice (px0 + (i + 0.5) delta, py0 + (j + 0.5) delta).  The same
+// chords, values and sums in the same order, so the same bits.
+template <int kPM>
+__device__ __forceinline__ void voxel_sums_lattice(const BPExact& e, const float* __restrict__ lat, int ls, int oi,
+                                                   int oj, float delta, float du, float dv, int i0, int i1, int j0,
+                                                   int j1, float& num, float& den) {
+    for (int j = j0; j <= j1; j++) {
+        const float b = __builtin_fmaf((float)j + 0.5f, delta, -dv);
+        const float bV0 = __builtin_fmaf(b, e.V[0], e.Smin[0]);
+        const float bV1 = __builtin_fmaf(b, e.V[1], e.Smin[1]);
+        const float bV2 = __builtin_fmaf(b, e.V[2], e.Smin[2]);
+        const float a0 = __builtin_fmaf((float)i0 + 0.5f, delta, -du);
+        const float* __restrict__ row = lat + (oj + j) * ls + oi;
+        for (int i = i0; i <= i1; i++) {
+            const float chord = chord_at<kPM>(e, lattice_a(i, i0, delta, a0), b, bV0, bV1, bV2);
+            if (chord > 0.0f) {
+                num = __builtin_fmaf(row[i], chord, num);
+                den += chord;
+            }
+        }
+    }
+}
+
 }  // namespace artia
