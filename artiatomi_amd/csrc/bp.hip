@@ -258,7 +258,7 @@ struct BPWinSet {
 // stages its footprint (one pixel of margin) in LDS.  Blocks whose footprint lies inside the image
 // apply the quad footprint as a chord cap (or not at all where it covers the voxel boxes); blocks at
 // the image edge clamp each quad's lattice to its clamped footprint.
-template <int kOS, int kNI, int kNJ, typename T>
+template <int kOS, int kNI, int kNJ, typename T, bool kYCol>
 __global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __restrict__ vol, int W, int H,
                                                      size_t pitch) {
     __shared__ float tile[kTileFloats];
@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(256) bp_win_kernel(VolG v, BPWinSet S, T* __re
                 const double gt = fabs(fma(w.gn[0], qx, fma(w.gn[1], qy, fma(w.gn[2], qz, w.g0))));
                 if (!(gt >= S.distMin && gt < S.distMax)) continue;
             }
-            any[q] |= window_quad<kOS, kNI, kNJ>(w, tile, ts, tx0, ty0, interior, W, H, x, (unsigned)y,
+            any[q] |= window_quad<kOS, kNI, kNJ, kYCol>(w, tile, ts, tx0, ty0, interior, W, H, x, (unsigned)y,
                                                  (unsigned)zq[q], S.lambda, out[q]);
         }
     }
@@ -564,43 +564,54 @@ void launch_stack_lat(artia_ctx* ctx, const VolG& v, const TiltG& g, const BPExa
         launch_stack_lat_os<2>(ctx, grid, v, g, e, vol, stack, stride, W, H, pitch, lambda, sl);
 }
 
-template <int kOS, int kNI, typename T>
+template <int kOS, int kNI, bool kYCol, typename T>
 void launch_win_nj(artia_ctx* ctx, dim3 grid, const VolG& v, const BPWinSet& S, int nj, T* vol, int W, int H,
                    size_t pitch) {
     constexpr int kLo = kOS == 1 ? 1 : 2;
     if (nj == kLo)
-        hipLaunchKernelGGL((bp_win_kernel<kOS, kNI, kLo, T>), grid, dim3(256), 0, ctx->stream, v, S, vol, W, H, pitch);
+        hipLaunchKernelGGL((bp_win_kernel<kOS, kNI, kLo, T, kYCol>), grid, dim3(256), 0, ctx->stream, v, S, vol, W,
+                           H, pitch);
     else if (nj == kLo + 1)
-        hipLaunchKernelGGL((bp_win_kernel<kOS, kNI, kLo + 1, T>), grid, dim3(256), 0, ctx->stream, v, S, vol, W, H,
-                           pitch);
+        hipLaunchKernelGGL((bp_win_kernel<kOS, kNI, kLo + 1, T, kYCol>), grid, dim3(256), 0, ctx->stream, v, S, vol,
+                           W, H, pitch);
     else
-        hipLaunchKernelGGL((bp_win_kernel<kOS, kNI, kLo + 2, T>), grid, dim3(256), 0, ctx->stream, v, S, vol, W, H,
-                           pitch);
+        hipLaunchKernelGGL((bp_win_kernel<kOS, kNI, kLo + 2, T, kYCol>), grid, dim3(256), 0, ctx->stream, v, S, vol,
+                           W, H, pitch);
 }
 
-template <int kOS, typename T>
+template <int kOS, bool kYCol, typename T>
 void launch_win_os(artia_ctx* ctx, dim3 grid, const VolG& v, const BPWinSet& S, int ni, int nj, T* vol, int W, int H,
                    size_t pitch) {
     constexpr int kLo = kOS == 1 ? 1 : 2;
     if (ni == kLo)
-        launch_win_nj<kOS, kLo>(ctx, grid, v, S, nj, vol, W, H, pitch);
+        launch_win_nj<kOS, kLo, kYCol>(ctx, grid, v, S, nj, vol, W, H, pitch);
     else if (ni == kLo + 1)
-        launch_win_nj<kOS, kLo + 1>(ctx, grid, v, S, nj, vol, W, H, pitch);
+        launch_win_nj<kOS, kLo + 1, kYCol>(ctx, grid, v, S, nj, vol, W, H, pitch);
     else
-        launch_win_nj<kOS, kLo + 2>(ctx, grid, v, S, nj, vol, W, H, pitch);
+        launch_win_nj<kOS, kLo + 2, kYCol>(ctx, grid, v, S, nj, vol, W, H, pitch);
 }
 
-// One launch of the window form over S.n tilts sharing the window size (ni, nj).
+// One launch of the window form over S.n tilts sharing the window size (ni, nj); the y slab term
+// is taken once per window column when no tilt's detector v axis has a y component.
 template <typename T>
 void launch_win_t(artia_ctx* ctx, const VolG& v, const BPWinSet& S, int os, int ni, int nj, T* vol, int W, int H,
                   size_t pitch) {
     const int nq = v.nx / 4;
     const dim3 grid((unsigned)((nq + 15) / 16), (unsigned)((v.ny + 3) / 4),
                     (unsigned)((v.nz + kWinBlockZ - 1) / kWinBlockZ));
-    if (os == 1)
-        launch_win_os<1>(ctx, grid, v, S, ni, nj, vol, W, H, pitch);
-    else
-        launch_win_os<2>(ctx, grid, v, S, ni, nj, vol, W, H, pitch);
+    bool ycol = true;
+    for (int t = 0; t < S.n; t++) ycol = ycol && S.w[t].YVd == 0.0f;
+    if (os == 1) {
+        if (ycol)
+            launch_win_os<1, true>(ctx, grid, v, S, ni, nj, vol, W, H, pitch);
+        else
+            launch_win_os<1, false>(ctx, grid, v, S, ni, nj, vol, W, H, pitch);
+    } else {
+        if (ycol)
+            launch_win_os<2, true>(ctx, grid, v, S, ni, nj, vol, W, H, pitch);
+        else
+            launch_win_os<2, false>(ctx, grid, v, S, ni, nj, vol, W, H, pitch);
+    }
 }
 
 template <int kOS, typename T>

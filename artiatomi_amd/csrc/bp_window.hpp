@@ -175,8 +175,14 @@ inline bool gate_passes_all(const VolG& v, const BPWin& w, double distMin, doubl
 //   kWinMasked: as a chord cap of 0 outside it (every window pixel lies in the tile);
 //   kWinClamp:  by skipping the point and its read (blocks at the image edges).
 // The three give the same sums for finite images: a point outside adds fma(val, 0, n) = n.
+// kYCol: the y slab term does not change down a window column (YVd == 0: every tilt about y whose
+// detector v axis has no y component, the u axis being the volume's y axis), so it is evaluated
+// once per column, and the points whose chord is 0 for a whole column (outside the voxel's y slab,
+// or outside the footprint's columns) or a whole row (outside the footprint's rows) are skipped
+// with their reads.  The sums start at -0, the additive identity, so that the first point taken
+// adds what the window's first point would.
 constexpr int kWinOpen = 0, kWinMasked = 1, kWinClamp = 2;
-template <int kOS, int kNI, int kNJ, int kMode>
+template <int kOS, int kNI, int kNJ, int kMode, bool kYCol = false>
 __device__ __forceinline__ void window_sums(const BPWin& w, const float* __restrict__ tile, int tw, int tx0, int ty0,
                                             int i0, int j0, float a0, float b0, int li0, int li1, int lj0, int lj1,
                                             float& num, float& den) {
@@ -193,7 +199,18 @@ __device__ __forceinline__ void window_sums(const BPWin& w, const float* __restr
 #pragma unroll
     for (int di = 0; di < kNI; di++)
         cap[di] = (kMode == kWinMasked && !((unsigned)(i0 + di - li0) < (unsigned)(li1 - li0))) ? 0.0f : w.m;
-    float n = 0.0f, d = 0.0f;
+    // kYCol: Yc = (Y0 + dj * 0) + di * YUd has the absolute value of Y0 + di * YUd at every point of
+    // column di; a chord is positive only where min(A, Bq) > 0 and the cap is positive
+    float Bcol[kNI];
+    bool live[kNI];
+#pragma unroll
+    for (int di = 0; di < kNI; di++) {
+        const float Yc = di == 0 ? Y0 : (di == 1 ? Y0 + w.YUd : __builtin_fmaf((float)di, w.YUd, Y0));
+        Bcol[di] = w.SYh - fabsf(Yc);
+        live[di] = Bcol[di] > 0.0f;
+        if (kMode != kWinOpen) live[di] = live[di] && (unsigned)(i0 + di - li0) < (unsigned)(li1 - li0);
+    }
+    float n = kYCol ? -0.0f : 0.0f, d = kYCol ? -0.0f : 0.0f;
 #pragma unroll
     for (int dj = 0; dj < kNJ; dj++) {
         const float Pr = dj == 0 ? P0 : (dj == 1 ? P0 + w.PVd : __builtin_fmaf((float)dj, w.PVd, P0));
@@ -206,22 +223,24 @@ __device__ __forceinline__ void window_sums(const BPWin& w, const float* __restr
         // masked: a row outside the footprint moves beyond every y slab, so its chords are 0 (the
         // med3 of a negative value, 0 and the column cap), as a cap of 0 would make them
         if (kMode == kWinMasked) Yr = rowOk ? Yr : kWinFar;
+        if (kYCol && !rowOk) continue;
 #pragma unroll
         for (int di = 0; di < kNI; di++) {
+            if (kYCol && !live[di]) continue;
             const float Pc = di == 0 ? Pr : (di == 1 ? Pr + w.PUd : __builtin_fmaf((float)di, w.PUd, Pr));
             const float Yc = di == 0 ? Yr : (di == 1 ? Yr + w.YUd : __builtin_fmaf((float)di, w.YUd, Yr));
             // min(m, P, SP - P, Y, SY - Y) clamped at 0 (cap 0: exactly 0)
-            const float A = w.SPh - fabsf(Pc), Bq = w.SYh - fabsf(Yc);
-            const float mcap = kMode == kWinMasked ? cap[di] : w.m;
+            const float A = w.SPh - fabsf(Pc), Bq = kYCol ? Bcol[di] : w.SYh - fabsf(Yc);
+            const float mcap = (kMode == kWinMasked && !kYCol) ? cap[di] : w.m;
             float chord = __builtin_amdgcn_fmed3f(fminf(A, Bq), 0.0f, mcap);
             int addr = kOS == 1 ? rowBase + di : ((di & 1) ? rowOdd + (di >> 1) : rowBase + (di >> 1));
-            if (kMode == kWinClamp) {
+            if (kMode == kWinClamp && !kYCol) {
                 const bool ok = rowOk && (unsigned)(i0 + di - li0) < (unsigned)(li1 - li0);
                 chord = ok ? chord : 0.0f;
                 addr = ok ? addr : 0;
             }
             const float val = tile[addr];
-            if (di == 0 && dj == 0) {
+            if (!kYCol && di == 0 && dj == 0) {
                 n = val * chord;
                 d = chord;
             } else {
@@ -254,7 +273,7 @@ inline bool window_plan(const VolG& v, const TiltG& g, int os, BPWin& w) {
 // footprint lies in the image with a pixel to spare and the tile holds it with a one-pixel margin;
 // then the footprint test is skipped (w.covers) or applied as a chord cap (w.masked), otherwise
 // each point outside is skipped with its read.  Returns whether a voxel changed.
-template <int kOS, int kNI, int kNJ>
+template <int kOS, int kNI, int kNJ, bool kYCol = false>
 __device__ __forceinline__ bool window_quad(const BPWin& w, const float* __restrict__ tile, int tw, int tx0, int ty0,
                                             bool interior, int W, int H, unsigned x, unsigned y, unsigned z,
                                             float lambda, float out[4]) {
@@ -300,13 +319,13 @@ __device__ __forceinline__ bool window_quad(const BPWin& w, const float* __restr
         const float b0 = __builtin_fmaf(fj0 + 0.5f, kDelta, -dv);
         float num, den;
         if (mode == kWinOpen)
-            window_sums<kOS, kNI, kNJ, kWinOpen>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, 0, 0, 0, 0, num,
+            window_sums<kOS, kNI, kNJ, kWinOpen, kYCol>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, 0, 0, 0, 0, num,
                                                  den);
         else if (mode == kWinMasked)
-            window_sums<kOS, kNI, kNJ, kWinMasked>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, li0, li1, lj0,
+            window_sums<kOS, kNI, kNJ, kWinMasked, kYCol>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, li0, li1, lj0,
                                                    lj1, num, den);
         else
-            window_sums<kOS, kNI, kNJ, kWinClamp>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, li0, li1, lj0,
+            window_sums<kOS, kNI, kNJ, kWinClamp, kYCol>(w, tile, tw, tx0, ty0, iq + i0, jq + j0, a0, b0, li0, li1, lj0,
                                                   lj1, num, den);
         if (den > 0.0f) {
             out[k] += (lambda * num) * __builtin_amdgcn_rcpf(den);
